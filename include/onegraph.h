@@ -1,0 +1,247 @@
+/*
+ * onegraph.h -- C ABI of the MI355X-native out-of-GPU-memory graph engine.
+ *
+ * The reference (NKU-EmbeddedSystem/OneGraph) has no library or FFI: each of
+ * its 15 programs reads a binary CSR file and calls one free function from
+ * main().  Every entry point below names the reference function or code block
+ * it replaces (paths relative to the reference repository root).
+ *
+ * Conventions shared by every entry point:
+ *   - plain pointers and sizes only; the caller owns every output array;
+ *   - every call returns an og_status; nothing throws or exits across the ABI;
+ *   - og_last_error() returns a thread-local message for the last failure;
+ *   - vertex ids are uint32, edge offsets uint64 (the reference's
+ *     EDGE_POINTER_TYPE, BFS/oneGraph_BFS/globals.h:14);
+ *   - BFS/SSSP results use the reference CPU convention: the source holds 1,
+ *     a vertex at hop h (distance d) holds h+1 (d+1), unreached holds
+ *     0xFFFFFFFF (BFS/cpu_BFS/main.cpp:141-145, SSSP/oneGraph/GraphMeta.h:374-381);
+ *   - CC labels: min-label propagation from value[i]=i
+ *     (CC/serial/serial.cpp:164-213).
+ */
+#ifndef ONEGRAPH_H
+#define ONEGRAPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OG_UNREACHED 0xFFFFFFFFu
+
+typedef enum og_status {
+    OG_OK = 0,
+    OG_EINVAL = 1,          /* bad argument: null pointer, source >= V, unknown enum */
+    OG_EIO = 2,             /* open / read / write failed or file truncated        */
+    OG_EFORMAT = 3,         /* CSR header or offsets inconsistent                  */
+    OG_EBUDGET = 4,         /* edge budget cannot hold what the request needs      */
+    OG_ENOMEM_HOST = 5,     /* host or pinned allocation failed                    */
+    OG_ENOMEM_DEVICE = 6,   /* device allocation failed                            */
+    OG_EHIP = 7,            /* HIP runtime error, including "no GPU present"       */
+    OG_EBUSY = 8,           /* object already in use / server queue full           */
+    OG_EUNSUPPORTED = 9,    /* algorithm and graph kind do not fit together        */
+    OG_ESTATE = 10          /* call order violated (e.g. run before og_prepare)    */
+} og_status;
+
+/* Graph kinds = the three on-disk CSR layouts of the reference. */
+typedef enum og_graph_kind {
+    OG_KIND_PLAIN = 0,      /* u64 V, u64 E, u64 ptr[V], u32 col[E]
+                               (BFS/oneGraph_BFS/GraphMeta.h:125-145)              */
+    OG_KIND_WEIGHTED = 1,   /* u64 V, u64 E, u64 ptr[V], {u32 to,u32 w}[E]
+                               (SSSP/subway/subway.cpp:144-152, globals.h:18-21)   */
+    OG_KIND_PULL = 2        /* u64 V, u64 E, u32 outDegree[V], u64 ptr[V],
+                               u32 inNeighbour[E]  (PR/cpu/Graph.h readDataFromFile) */
+} og_graph_kind;
+
+/* The reference's three out-of-memory approaches. */
+typedef enum og_approach {
+    OG_ASCETIC = 0,         /* static prefix + dynamic region (BFS/oneGraph_BFS)   */
+    OG_SUBWAY = 1,          /* active-subgraph generation (BFS/sbw_BFS)             */
+    OG_USM = 2              /* graph in managed memory (BFS/usm_BFS)                */
+} og_approach;
+
+typedef struct og_graph og_graph;
+
+typedef struct og_graph_info {
+    uint64_t num_vertices;
+    uint64_t num_edges;
+    uint32_t kind;          /* og_graph_kind */
+    uint32_t max_degree;
+} og_graph_info;
+
+/* ---------------------------------------------------------------- errors */
+
+/* Thread-local text of the last failing call on this thread ("" if none). */
+const char *og_last_error(void);
+/* Library version string. */
+const char *og_version(void);
+/* Number of visible HIP devices (0 on a host without a GPU). */
+og_status og_device_count(int *count);
+
+/* ---------------------------------------------------------------- graphs */
+
+/* Replaces GraphMeta::readDataFromFile (BFS/oneGraph_BFS/GraphMeta.h:125-145),
+ * its weighted twin (SSSP/oneGraph/GraphMeta.h) and the PR reader
+ * (PR/cpu/Graph.h).  Unlike the reference it checks sizes, offsets and
+ * column ids (OG_EIO / OG_EFORMAT) and loads into pinned host memory. */
+og_status og_graph_load(const char *path, og_graph_kind kind, og_graph **out);
+
+/* Writes the graph in the reference byte format of its kind. */
+og_status og_graph_save(const og_graph *g, const char *path);
+
+/* Builds a graph from caller arrays in the reference layout: ptr has V
+ * entries (no sentinel; the last vertex ends at E, GraphMeta.h:222).
+ * weights may be NULL unless kind == OG_KIND_WEIGHTED; out_degree is used
+ * only for OG_KIND_PULL. The arrays are copied. */
+og_status og_graph_from_csr(uint64_t num_vertices, uint64_t num_edges,
+                            const uint64_t *ptr, const uint32_t *cols,
+                            const uint32_t *weights, const uint32_t *out_degree,
+                            og_graph_kind kind, og_graph **out);
+
+/* Synthetic inputs (no reference counterpart: the reference relies on the
+ * external Ascetic converter, README.md:7).  Directed R-MAT with quadrant
+ * probabilities a,b,c (d = 1-a-b-c), self-loops and duplicates removed,
+ * rows sorted.  weight_seed != 0 attaches u32 weights uniform in [1, 63]. */
+og_status og_graph_generate_rmat(uint32_t scale, uint32_t edge_factor,
+                                 double a, double b, double c, uint64_t seed,
+                                 uint64_t weight_seed, og_graph **out);
+
+/* Undirected Erdos-Renyi G(n, m = n*avg_degree/2), symmetrised, no
+ * self-loops or duplicates; returned as OG_KIND_PLAIN out-edge CSR. */
+og_status og_graph_generate_er(uint64_t num_vertices, uint32_t avg_degree,
+                               uint64_t seed, og_graph **out);
+
+/* Builds the pull layout used by PageRank (outDegree + in-edge CSR) from a
+ * plain or weighted out-edge graph (weights dropped). */
+og_status og_graph_to_pull(const og_graph *g, og_graph **out);
+
+/* Adds u32 weights uniform in [1, 63] (seeded) to a plain graph. */
+og_status og_graph_add_weights(const og_graph *g, uint64_t weight_seed, og_graph **out);
+
+og_status og_graph_get_info(const og_graph *g, og_graph_info *info);
+
+/* Copies the CSR out in the reference layout (any pointer may be NULL). */
+og_status og_graph_export(const og_graph *g, uint64_t *ptr, uint32_t *cols,
+                          uint32_t *weights, uint32_t *out_degree);
+
+/* Reference findbignode (BFS/oneGraph_BFS/GraphMeta.h:149-165): first vertex
+ * after `last` whose degree exceeds 3000, else `last`. */
+og_status og_graph_find_big_node(const og_graph *g, uint32_t last, uint32_t *out);
+
+void og_graph_free(og_graph *g);
+
+/* ------------------------------------------------------------- planning */
+
+typedef struct og_plan_params {
+    uint64_t edge_budget_bytes; /* device bytes for static region + staging;
+                                   0 = reference default: 12 GiB minus
+                                   paramSize*4*V (GraphMeta.h:304-309)        */
+    float advisek;              /* static share of the budget; <= 0 selects
+                                   the reference auto rule (GraphMeta.h:311-324) */
+    uint32_t param_size;        /* reference paramSize (17 BFS/SSSP/CC, 19 PR);
+                                   only used when edge_budget_bytes == 0      */
+    uint32_t host_threads;      /* gather threads; 0 = hardware concurrency   */
+    uint32_t flags;             /* OG_PLAN_* bits                             */
+} og_plan_params;
+
+#define OG_PLAN_NO_OVERLAP 1u   /* serialise copies and relax (for A/B timing) */
+
+typedef struct og_plan_info {
+    uint64_t total_budget_edges;  /* reference total_gpu_size               */
+    uint64_t static_edges;        /* reference max_partition_size           */
+    uint64_t staging_edges;       /* reference partOverloadSize             */
+    uint64_t overload_edges;      /* reference overloadSize                 */
+    uint64_t static_vertices;     /* number of vertices with isInStatic     */
+    uint64_t vsplit;              /* vertices [0, vsplit) are static        */
+    uint64_t device_bytes;        /* device bytes allocated by the plan     */
+    float advisek_used;
+    uint32_t approach;
+} og_plan_info;
+
+/* Restatement of GraphMeta::getMaxPartitionSize (GraphMeta.h:299-360), pure
+ * host computation, no device needed.  ptr has V entries (reference layout,
+ * no sentinel).  edge_bytes = sizeof(Edge) (4 or 8). */
+og_status og_plan_compute(uint64_t num_vertices, uint64_t num_edges,
+                          const uint64_t *ptr, uint32_t edge_bytes,
+                          const og_plan_params *params, og_plan_info *info);
+
+/* Places the graph on the device for one approach: replaces
+ * GraphMeta::initGraphHost/initGraphDevice (GraphMeta.h:203-291) for
+ * Ascetic, the set-up of runGPUbfs_subway (BFS/sbw_BFS/main.cpp:323-392) for
+ * Subway and the malloc_shared graph of BFS/usm_BFS/Graph.h:58-62 for USM.
+ * Frees any previous preparation of this graph. */
+og_status og_prepare(og_graph *g, og_approach approach,
+                     const og_plan_params *params, og_plan_info *info);
+
+/* Releases the device state of og_prepare. */
+og_status og_release(og_graph *g);
+
+/* --------------------------------------------------------------- runs */
+
+typedef struct og_run_stats {
+    uint32_t iterations;          /* frontier rounds                             */
+    uint32_t approach;
+    uint64_t reached;             /* vertices with a value != OG_UNREACHED        */
+    uint64_t traversed_edges;     /* sum of out-degree over reached vertices      */
+    uint64_t work_edges;          /* edges relaxed, summed over iterations (D)    */
+    uint64_t updates;             /* successful improvements (U)                  */
+    uint64_t node_sum;            /* reference nodeSum: sum of frontier sizes     */
+    uint64_t static_node_sum;     /* reference staticnodeSum                      */
+    uint64_t static_work_edges;   /* edges relaxed out of the static region       */
+    uint64_t h2d_bytes;           /* dynamic-region payload copied host->device   */
+    uint64_t chunks;              /* dynamic-region copies issued                 */
+    double total_ms;              /* reference totalProcess                       */
+    double pre_ms;                /* reference preProcess (classify/compact/scan) */
+    double static_ms;             /* reference staticProcess                      */
+    double overload_ms;           /* reference overloadProcess                    */
+    double overload_move_ms;      /* reference overloadMoveProcess (copy stream)  */
+    double gather_ms;             /* host gather of active dynamic edges          */
+    /* Kernel timing from HIP events on the launching stream. */
+    double relax_static_kernel_ms;  uint64_t relax_static_launches;  uint64_t relax_static_bytes;
+    double relax_dynamic_kernel_ms; uint64_t relax_dynamic_launches; uint64_t relax_dynamic_bytes;
+    double frontier_kernel_ms;      uint64_t frontier_launches;      uint64_t frontier_bytes;
+    double copy_ms;               /* sum of copy-stream durations (HIP events)    */
+} og_run_stats;
+
+/* Replaces bfs_opt (BFS/oneGraph_BFS/CalculateOpt.h:9-211),
+ * runGPUbfs_subway (BFS/sbw_BFS/main.cpp:323-506) and runGPUBFS
+ * (BFS/usm_BFS/main.cpp:120-217) for one source, using the approach given
+ * to og_prepare.  levels (V entries) may be NULL.  stats may be NULL. */
+og_status og_bfs(og_graph *g, uint32_t source, uint32_t *levels, og_run_stats *stats);
+
+/* Replaces sssp_opt (SSSP/oneGraph/CalculateOpt.h:12-192),
+ * runGPUbfs_subway4 (SSSP/subway/subway.cpp:314-647) and runGPUsssp3
+ * (SSSP/usm/sssp_usm.cpp:140-284).  Needs an OG_KIND_WEIGHTED graph. */
+og_status og_sssp(og_graph *g, uint32_t source, uint32_t *dist, og_run_stats *stats);
+
+/* Replaces cc_opt (CC/onegraph/CalculateOpt.h:6-144), the Subway CC main
+ * (CC/sbw/sbw.cpp:305-490) and the USM CC main (CC/usm/usm.cpp:127-257). */
+og_status og_cc(og_graph *g, uint32_t *labels, og_run_stats *stats);
+
+typedef struct og_pr_params {
+    double damping;          /* 0.85                                   */
+    double tolerance;        /* stop when L1(new - old) < tolerance     */
+    uint32_t max_iterations; /* 100                                    */
+    uint32_t fp64;           /* 0: FP32 ranks with Kahan pull sums; 1: FP64 */
+} og_pr_params;
+
+typedef struct og_pr_stats {
+    og_run_stats run;
+    uint32_t iterations;
+    double final_l1;         /* L1 residual of the last iteration (FP64 reduction) */
+} og_pr_stats;
+
+/* Replaces pr_opt (PR/OneGraph/CalculateOpt.h:13-185), pr_calQ of Subway
+ * (PR/sbw/sbw.cpp:159-283) and USM (PR/usm/pr.cpp:189-255), with the
+ * normalised PageRank definition: r = (1-d)/n + d*(sum_in r[u]/outdeg[u] +
+ * dangling/n).  Needs an OG_KIND_PULL graph.  Exactly one of ranks_f32 /
+ * ranks_f64 is written, chosen by params->fp64 (the other may be NULL). */
+og_status og_pagerank(og_graph *g, const og_pr_params *params, float *ranks_f32,
+                      double *ranks_f64, og_pr_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ONEGRAPH_H */

@@ -1,0 +1,299 @@
+"""Python binding of libonegraph's C ABI (include/onegraph.h) over ctypes.
+
+The engine itself is C++/HIP; this module only marshals numpy arrays and the
+ABI structs so tests and bench.py can drive it.  Loading fails loudly when
+the shared library has not been built: there is no Python fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libonegraph.so")
+CLI_PATH = os.path.join(_HERE, "onegraph")
+
+KIND_PLAIN, KIND_WEIGHTED, KIND_PULL = 0, 1, 2
+ASCETIC, SUBWAY, USM = 0, 1, 2
+APPROACHES = {"ascetic": ASCETIC, "subway": SUBWAY, "usm": USM}
+UNREACHED = 0xFFFFFFFF
+PLAN_NO_OVERLAP = 1
+
+STATUS = {
+    0: "OG_OK", 1: "OG_EINVAL", 2: "OG_EIO", 3: "OG_EFORMAT", 4: "OG_EBUDGET",
+    5: "OG_ENOMEM_HOST", 6: "OG_ENOMEM_DEVICE", 7: "OG_EHIP", 8: "OG_EBUSY",
+    9: "OG_EUNSUPPORTED", 10: "OG_ESTATE",
+}
+
+
+class OgError(RuntimeError):
+    def __init__(self, status, message):
+        self.status = status
+        self.name = STATUS.get(status, str(status))
+        super().__init__(f"{self.name}: {message}")
+
+
+class PlanParams(C.Structure):
+    _fields_ = [("edge_budget_bytes", C.c_uint64), ("advisek", C.c_float), ("param_size", C.c_uint32),
+                ("host_threads", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("total_budget_edges", C.c_uint64), ("static_edges", C.c_uint64), ("staging_edges", C.c_uint64),
+                ("overload_edges", C.c_uint64), ("static_vertices", C.c_uint64), ("vsplit", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("advisek_used", C.c_float), ("approach", C.c_uint32)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("approach", C.c_uint32), ("reached", C.c_uint64),
+                ("traversed_edges", C.c_uint64), ("work_edges", C.c_uint64), ("updates", C.c_uint64),
+                ("node_sum", C.c_uint64), ("static_node_sum", C.c_uint64), ("static_work_edges", C.c_uint64),
+                ("h2d_bytes", C.c_uint64), ("chunks", C.c_uint64), ("total_ms", C.c_double),
+                ("pre_ms", C.c_double), ("static_ms", C.c_double), ("overload_ms", C.c_double),
+                ("overload_move_ms", C.c_double), ("gather_ms", C.c_double),
+                ("relax_static_kernel_ms", C.c_double), ("relax_static_launches", C.c_uint64),
+                ("relax_static_bytes", C.c_uint64), ("relax_dynamic_kernel_ms", C.c_double),
+                ("relax_dynamic_launches", C.c_uint64), ("relax_dynamic_bytes", C.c_uint64),
+                ("frontier_kernel_ms", C.c_double), ("frontier_launches", C.c_uint64),
+                ("frontier_bytes", C.c_uint64), ("copy_ms", C.c_double)]
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint64), ("num_edges", C.c_uint64), ("kind", C.c_uint32),
+                ("max_degree", C.c_uint32)]
+
+
+class PrParams(C.Structure):
+    _fields_ = [("damping", C.c_double), ("tolerance", C.c_double), ("max_iterations", C.c_uint32),
+                ("fp64", C.c_uint32)]
+
+
+class PrStats(C.Structure):
+    _fields_ = [("run", RunStats), ("iterations", C.c_uint32), ("final_l1", C.c_double)]
+
+
+def as_dict(s):
+    out = {}
+    for name, _ in s._fields_:
+        v = getattr(s, name)
+        out[name] = as_dict(v) if isinstance(v, C.Structure) else v
+    return out
+
+
+_lib = None
+_P = C.c_void_p
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+
+
+def lib():
+    """Loads libonegraph.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m onegraph_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "og_last_error": (C.c_char_p, []),
+        "og_version": (C.c_char_p, []),
+        "og_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+        "og_graph_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_P)]),
+        "og_graph_save": (C.c_int, [_P, C.c_char_p]),
+        "og_graph_from_csr": (C.c_int, [C.c_uint64, C.c_uint64, _u64p, _u32p, _u32p, _u32p, C.c_int, C.POINTER(_P)]),
+        "og_graph_generate_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                             C.c_uint64, C.c_uint64, C.POINTER(_P)]),
+        "og_graph_generate_er": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(_P)]),
+        "og_graph_to_pull": (C.c_int, [_P, C.POINTER(_P)]),
+        "og_graph_add_weights": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
+        "og_graph_get_info": (C.c_int, [_P, C.POINTER(GraphInfo)]),
+        "og_graph_export": (C.c_int, [_P, _u64p, _u32p, _u32p, _u32p]),
+        "og_graph_find_big_node": (C.c_int, [_P, C.c_uint32, _u32p]),
+        "og_graph_free": (None, [_P]),
+        "og_plan_compute": (C.c_int, [C.c_uint64, C.c_uint64, _u64p, C.c_uint32, C.POINTER(PlanParams),
+                                      C.POINTER(PlanInfo)]),
+        "og_prepare": (C.c_int, [_P, C.c_int, C.POINTER(PlanParams), C.POINTER(PlanInfo)]),
+        "og_release": (C.c_int, [_P]),
+        "og_bfs": (C.c_int, [_P, C.c_uint32, _u32p, C.POINTER(RunStats)]),
+        "og_sssp": (C.c_int, [_P, C.c_uint32, _u32p, C.POINTER(RunStats)]),
+        "og_cc": (C.c_int, [_P, _u32p, C.POINTER(RunStats)]),
+        "og_pagerank": (C.c_int, [_P, C.POINTER(PrParams), C.POINTER(C.c_float), C.POINTER(C.c_double),
+                                  C.POINTER(PrStats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(st):
+    if st != 0:
+        raise OgError(st, lib().og_last_error().decode(errors="replace"))
+
+
+def _u32(a):
+    return a.ctypes.data_as(_u32p) if a is not None else None
+
+
+def _u64(a):
+    return a.ctypes.data_as(_u64p) if a is not None else None
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(lib().og_device_count(C.byref(n)))
+    return n.value
+
+
+def version():
+    return lib().og_version().decode()
+
+
+def plan_compute(ptr, num_edges, edge_bytes, edge_budget_bytes=0, advisek=0.0, param_size=0):
+    """Static-region plan for a CSR with V-entry offsets (reference layout)."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.uint64)
+    p = PlanParams(edge_budget_bytes, advisek, param_size, 0, 0)
+    info = PlanInfo()
+    _check(lib().og_plan_compute(len(ptr), num_edges, _u64(ptr), edge_bytes, C.byref(p), C.byref(info)))
+    return as_dict(info)
+
+
+class Graph:
+    """Owns an og_graph handle (host CSR in pinned memory + device state)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    # ------------------------------------------------------------ creation
+    @classmethod
+    def _make(cls, fn, *args):
+        h = _P()
+        _check(fn(*args, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load(cls, path, kind=KIND_PLAIN):
+        return cls._make(lib().og_graph_load, path.encode(), kind)
+
+    @classmethod
+    def from_csr(cls, ptr, cols, weights=None, out_degree=None, kind=None):
+        """ptr has V entries (reference layout: no sentinel, row V-1 ends at E)."""
+        ptr = np.ascontiguousarray(ptr, dtype=np.uint64)
+        cols = np.ascontiguousarray(cols, dtype=np.uint32)
+        E = len(cols)
+        if kind is None:
+            kind = KIND_PULL if out_degree is not None else (KIND_WEIGHTED if weights is not None else KIND_PLAIN)
+        w = np.ascontiguousarray(weights, dtype=np.uint32) if weights is not None else None
+        od = np.ascontiguousarray(out_degree, dtype=np.uint32) if out_degree is not None else None
+        V = len(ptr)
+        return cls._make(lib().og_graph_from_csr, V, E, _u64(ptr), _u32(cols) if E else _u32(np.zeros(1, np.uint32)),
+                         _u32(w), _u32(od), kind)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, weight_seed=0):
+        return cls._make(lib().og_graph_generate_rmat, scale, edge_factor, a, b, c, seed, weight_seed)
+
+    @classmethod
+    def er(cls, n, avg_degree, seed=1):
+        return cls._make(lib().og_graph_generate_er, n, avg_degree, seed)
+
+    def to_pull(self):
+        return Graph._make(lib().og_graph_to_pull, self._h)
+
+    def add_weights(self, seed):
+        return Graph._make(lib().og_graph_add_weights, self._h, seed)
+
+    # ------------------------------------------------------------ queries
+    def info(self):
+        i = GraphInfo()
+        _check(lib().og_graph_get_info(self._h, C.byref(i)))
+        return as_dict(i)
+
+    @property
+    def num_vertices(self):
+        return self.info()["num_vertices"]
+
+    @property
+    def num_edges(self):
+        return self.info()["num_edges"]
+
+    def export(self):
+        """Returns (ptr[V], cols[E], weights or None, out_degree or None)."""
+        inf = self.info()
+        V, E, kind = inf["num_vertices"], inf["num_edges"], inf["kind"]
+        ptr = np.empty(V, np.uint64)
+        cols = np.empty(max(E, 1), np.uint32)
+        w = np.empty(max(E, 1), np.uint32) if kind == KIND_WEIGHTED else None
+        od = np.empty(V, np.uint32) if kind == KIND_PULL else None
+        _check(lib().og_graph_export(self._h, _u64(ptr), _u32(cols), _u32(w), _u32(od)))
+        return ptr, cols[:E], (w[:E] if w is not None else None), od
+
+    def save(self, path):
+        _check(lib().og_graph_save(self._h, path.encode()))
+
+    def find_big_node(self, last=0):
+        out = C.c_uint32()
+        _check(lib().og_graph_find_big_node(self._h, last, C.byref(out)))
+        return out.value
+
+    # ------------------------------------------------------------ device
+    def prepare(self, approach=ASCETIC, edge_budget_bytes=0, advisek=0.0, param_size=0, host_threads=0, flags=0):
+        if isinstance(approach, str):
+            approach = APPROACHES[approach]
+        p = PlanParams(edge_budget_bytes, advisek, param_size, host_threads, flags)
+        info = PlanInfo()
+        _check(lib().og_prepare(self._h, approach, C.byref(p), C.byref(info)))
+        return as_dict(info)
+
+    def release(self):
+        _check(lib().og_release(self._h))
+
+    def bfs(self, source, want_values=True):
+        out = np.empty(self.num_vertices, np.uint32) if want_values else None
+        st = RunStats()
+        _check(lib().og_bfs(self._h, source, _u32(out), C.byref(st)))
+        return out, as_dict(st)
+
+    def sssp(self, source, want_values=True):
+        out = np.empty(self.num_vertices, np.uint32) if want_values else None
+        st = RunStats()
+        _check(lib().og_sssp(self._h, source, _u32(out), C.byref(st)))
+        return out, as_dict(st)
+
+    def cc(self, want_values=True):
+        out = np.empty(self.num_vertices, np.uint32) if want_values else None
+        st = RunStats()
+        _check(lib().og_cc(self._h, _u32(out), C.byref(st)))
+        return out, as_dict(st)
+
+    def pagerank(self, damping=0.85, tolerance=1e-7, max_iterations=100, fp64=False, want_values=True):
+        V = self.num_vertices
+        f32 = np.empty(V, np.float32) if (want_values and not fp64) else None
+        f64 = np.empty(V, np.float64) if (want_values and fp64) else None
+        p = PrParams(damping, tolerance, max_iterations, 1 if fp64 else 0)
+        st = PrStats()
+        _check(lib().og_pagerank(self._h, C.byref(p),
+                                 f32.ctypes.data_as(C.POINTER(C.c_float)) if f32 is not None else None,
+                                 f64.ctypes.data_as(C.POINTER(C.c_double)) if f64 is not None else None,
+                                 C.byref(st)))
+        return (f64 if fp64 else f32), as_dict(st)
+
+    # ------------------------------------------------------------ lifetime
+    def close(self):
+        if self._h and self._h.value:
+            lib().og_graph_free(self._h)
+            self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

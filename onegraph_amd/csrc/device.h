@@ -1,0 +1,247 @@
+// Device-side state and helpers shared by engine.hip and pagerank.hip.
+// gfx950 only: 64-lane wavefronts, 160 KiB LDS per CU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "og_internal.h"
+
+#define OG_HIP(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            ::og::raise(e_ == hipErrorOutOfMemory ? OG_ENOMEM_DEVICE : OG_EHIP, "%s: %s", \
+                        #x, hipGetErrorString(e_));                                        \
+    } while (0)
+
+namespace og {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;             // threads per block of the frontier kernels
+constexpr int kFlagVecs = 2;            // uint4 flag loads per thread
+constexpr int kFlagsPerThread = 16 * kFlagVecs;
+constexpr int kVertsPerBlock = kBlock * kFlagsPerThread;   // 8192
+constexpr int kDynEdgesPerThread = 4;   // dynamic relax: edges per lane
+constexpr int kDynEdgesPerBlock = kBlock * kDynEdgesPerThread;
+constexpr int kSlots = 2;               // dynamic staging double buffer
+
+enum Algo { ALGO_BFS = 0, ALGO_SSSP = 1, ALGO_CC = 2 };
+
+// Degree-bin thresholds and shapes for the static relax (autotunable).
+struct Tuning {
+    uint32_t t1 = 32;      // bin 0: 1 <= deg <= t1, lpv0 lanes per vertex
+    uint32_t lpv0 = 8;     // 1, 2, 4, 8, 16, 32 or 64
+    uint32_t t2 = 4096;    // bin 1: t1 < deg <= t2, one wave per vertex
+    uint32_t tile = 2048;  // bin 2: deg > t2, tiles of `tile` edges, one workgroup each
+};
+
+// Frontier summary produced on the device each iteration and read by the host.
+struct FrontierInfo {
+    uint32_t n[4];          // bin0 vertices, bin1 vertices, bin2 tiles, dynamic vertices
+    uint32_t nzero;         // active vertices with no out-edge
+    uint32_t nhub;          // bin2 vertices
+    uint64_t static_deg;    // sum of degrees over static active vertices
+    uint64_t dyn_deg;       // sum of degrees over dynamic active vertices
+};
+
+// Timing of one class of launches: pairs of events on the launching stream.
+struct TimedClass {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> marks;
+    uint64_t launches = 0;
+};
+
+class EventPool {
+  public:
+    ~EventPool() {
+        for (auto e : all_) (void)hipEventDestroy(e);
+    }
+    hipEvent_t get() {
+        if (next_ == all_.size()) {
+            hipEvent_t e;
+            OG_HIP(hipEventCreate(&e));
+            all_.push_back(e);
+        }
+        return all_[next_++];
+    }
+    void reset() { next_ = 0; }
+
+  private:
+    std::vector<hipEvent_t> all_;
+    size_t next_ = 0;
+};
+
+struct DeviceState {
+    og_approach approach = OG_ASCETIC;
+    int device = 0;
+    uint64_t V = 0, E = 0, Vpad = 0;
+    uint32_t eb = 4;                 // edge bytes
+    og_plan_info plan{};
+    Tuning tune;
+    unsigned host_threads = 0;
+    uint32_t flags = 0;
+
+    // graph on the device
+    uint64_t *d_ptr = nullptr;       // V + 1 (device copy; USM: managed)
+    void *d_static = nullptr;        // static-region edges (USM: managed, whole graph)
+    uint64_t static_used = 0;        // edges of the static vertices, = ptr[vsplit]
+    void *d_staging = nullptr;       // kSlots * slot_edges edges
+    uint64_t slot_edges = 0;
+    bool managed = false;
+    uint64_t *m_ptr = nullptr;       // USM managed copies
+    void *m_edges = nullptr;
+
+    // per-vertex state
+    uint32_t *d_val = nullptr;       // V
+    uint8_t *d_flag = nullptr;       // Vpad next-frontier flags
+    uint32_t *d_list = nullptr;      // V: bin0 | bin1 | dynamic lists
+    uint64_t *d_hub = nullptr;       // hub tiles (tile << 32 | vertex)
+    uint64_t hub_cap = 0;
+    uint64_t *d_doff = nullptr;      // V + 1 degree prefix of the dynamic list
+    uint32_t *d_bcnt = nullptr;      // 5 * nblk per-block counts
+    uint64_t *d_bdeg = nullptr;      // 2 * nblk per-block degree sums
+    uint32_t nblk = 0;
+    FrontierInfo *d_info = nullptr;
+    unsigned long long *d_counters = nullptr;  // [0] updates static, [1] updates dynamic, [2..] scratch
+
+    // pinned host mirrors
+    FrontierInfo *h_info = nullptr;
+    uint32_t *h_dlist = nullptr;     // V
+    uint64_t *h_doff = nullptr;      // V + 1
+    void *h_stage[kSlots] = {nullptr, nullptr};
+    unsigned long long *h_counters = nullptr;
+
+    hipStream_t s_compute = nullptr, s_copy = nullptr;
+    hipEvent_t ev_h2d[kSlots] = {nullptr, nullptr};    // copy into slot finished
+    hipEvent_t ev_relax[kSlots] = {nullptr, nullptr};  // relax reading slot finished
+    bool slot_h2d_pending[kSlots] = {false, false};
+    bool slot_relax_pending[kSlots] = {false, false};
+    EventPool events;
+    std::unique_ptr<WorkerPool> pool;
+
+    // PageRank state (allocated on first use)
+    void *d_rank[2] = {nullptr, nullptr};
+    void *d_contrib = nullptr;
+    uint32_t *d_outdeg = nullptr;
+    double *d_partial = nullptr;     // per-block residual / dangling partials
+    double *d_scalars = nullptr;     // [0] dangling, [1] l1
+    double *h_scalars = nullptr;
+    uint32_t pr_nblk = 0;
+    bool pr_fp64 = false;
+    std::shared_ptr<void> pr_plan;   // PageRank bins and chunk ranges (pagerank.hip)
+
+    std::vector<void *> allocations; // device allocations to free
+    uint64_t device_bytes = 0;
+
+    void *dalloc(size_t bytes) {
+        void *p = nullptr;
+        if (bytes == 0) bytes = 256;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            raise(OG_ENOMEM_DEVICE, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        }
+        allocations.push_back(p);
+        device_bytes += bytes;
+        return p;
+    }
+};
+
+// ------------------------------------------------------------ wave helpers
+
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t x, int d) {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    lo = __shfl_up(lo, d, kWave);
+    hi = __shfl_up(hi, d, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m) {
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    lo = __shfl_xor(lo, m, kWave);
+    hi = __shfl_xor(hi, m, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Exclusive scan over the block of x (u64); *total receives the block sum.
+// `lds` must hold blockDim.x / 64 entries.  Contains __syncthreads.
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t x, uint64_t *total, uint64_t *lds) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    uint64_t inc = x;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint64_t y = shfl_up_u64(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == kWave - 1) lds[wid] = inc;
+    __syncthreads();
+    uint64_t before = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+        uint64_t s = lds[w];
+        before += (w < wid) ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return before + inc - x;
+}
+
+// Sum over the block; result valid in every thread.  Contains __syncthreads.
+__device__ __forceinline__ uint64_t block_sum(uint64_t x, uint64_t *lds) {
+    uint64_t t;
+    (void)block_exclusive_scan(x, &t, lds);
+    return t;
+}
+
+// Adds the wave's per-lane counts to *ctr with one atomic per wave.
+__device__ __forceinline__ void wave_add(unsigned long long *ctr, uint32_t v) {
+    uint64_t s = v;
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) s += shfl_xor_u64(s, m);
+    if ((threadIdx.x & (kWave - 1)) == 0 && s) atomicAdd(ctr, (unsigned long long)s);
+}
+
+// ------------------------------------------------------------- host timing
+
+struct Stopwatch {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+// Records a start/stop event pair around a launch sequence on `s`.
+struct EventSpan {
+    DeviceState *d;
+    TimedClass *cls;
+    hipStream_t s;
+    hipEvent_t a;
+    EventSpan(DeviceState *d_, TimedClass *c, hipStream_t s_) : d(d_), cls(c), s(s_) {
+        a = d->events.get();
+        OG_HIP(hipEventRecord(a, s));
+    }
+    ~EventSpan() noexcept(false) {
+        hipEvent_t b = d->events.get();
+        OG_HIP(hipEventRecord(b, s));
+        cls->marks.push_back({a, b});
+        cls->launches++;
+    }
+};
+
+inline double sum_ms(const TimedClass &c) {
+    double t = 0;
+    for (auto &m : c.marks) {
+        float ms = 0;
+        OG_HIP(hipEventElapsedTime(&ms, m.first, m.second));
+        t += ms;
+    }
+    return t;
+}
+
+void ensure_device(DeviceState *d);
+
+}  // namespace og

@@ -1,0 +1,814 @@
+// Frontier engine for BFS, SSSP and CC on gfx950: wave64 ballot/scan frontier
+// compaction, degree-binned relax kernels over the HBM-resident static region,
+// and the dynamic-region loader that streams active edges from pinned host
+// memory through a double-buffered staging area on a copy stream.
+//
+// One iteration (reference loop: BFS/oneGraph_BFS/CalculateOpt.h:75-188):
+//   1. k_frontier<count> / k_frontier_scan / k_frontier<write>: one pass over
+//      the next-frontier flags builds the static bin lists (ordered), the hub
+//      tile list, the sorted dynamic list and its degree prefix, and clears
+//      the flags.  Replaces K1-K6 (gpu_kernels.h:243-379), the host prefix H4
+//      (CalculateOpt.h:110-123) and every per-iteration .wait() but one.
+//   2. static relax: bin 0 (lpv0 lanes/vertex), bin 1 (wave/vertex), bin 2
+//      (workgroup per `tile` edges, neighbour list staged in LDS).  Replaces
+//      bfs_kernelStatic (gpu_kernels.h:211-241), sssp_kernel
+//      (SSSP/oneGraph/gpu_kernels.h:177-203), cc_kernelStaticSwap
+//      (CC/onegraph/gpu_kernels.h:133-157).
+//   3. dynamic region: the host gathers the active edges of the sorted dynamic
+//      list into a pinned slot (H5, GraphMeta.h:470-506), hipMemcpyAsync moves
+//      the slot on the copy stream while the other slot is being gathered, and
+//      an edge-parallel relax consumes it (K8/K10/K12).
+#include <algorithm>
+#include <cstdio>
+
+#include "device.h"
+
+namespace og {
+
+// ================================================================= kernels
+
+__global__ void k_fill_u32(uint32_t *a, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+__global__ void k_iota(uint32_t *a, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = (uint32_t)i;
+}
+
+__global__ void k_fill_u8(uint8_t *a, uint64_t n, uint8_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+__global__ void k_set_source(uint32_t *val, uint8_t *flag, uint32_t src) {
+    val[src] = 1;
+    flag[src] = 1;
+}
+
+// Per-thread flag window: kFlagsPerThread consecutive vertices.
+__device__ __forceinline__ void load_flags(const uint8_t *flag, uint64_t base, uint64_t Vpad, uint4 (&w)[kFlagVecs]) {
+#pragma unroll
+    for (int k = 0; k < kFlagVecs; ++k) {
+        uint64_t at = base + 16ull * k;
+        w[k] = at < Vpad ? *(const uint4 *)(flag + at) : make_uint4(0, 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ bool flag_at(const uint4 (&w)[kFlagVecs], int j) {
+    const uint4 &q = w[j >> 4];
+    const int b = j & 15;
+    uint32_t word = (b < 4) ? q.x : (b < 8) ? q.y : (b < 12) ? q.z : q.w;
+    return (word >> (8 * (b & 3))) & 0xFFu;
+}
+
+__device__ __forceinline__ bool any_flag(const uint4 (&w)[kFlagVecs]) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int k = 0; k < kFlagVecs; ++k) o |= w[k].x | w[k].y | w[k].z | w[k].w;
+    return o != 0;
+}
+
+// Bin of an active vertex: 0..2 static bins, 3 dynamic, 4 no out-edges.
+__device__ __forceinline__ int bin_of(uint64_t v, uint64_t vsplit, uint64_t deg, const Tuning &tn) {
+    if (deg == 0) return 4;
+    if (v >= vsplit) return 3;
+    if (deg <= tn.t1) return 0;
+    if (deg <= tn.t2) return 1;
+    return 2;
+}
+
+// Pass 1 (WRITE = false): per-block counts.  Pass 3 (WRITE = true): emits
+// lists in vertex order, the dynamic degree prefix, and clears the flags.
+template <bool WRITE>
+__global__ __launch_bounds__(kBlock) void k_frontier(uint8_t *__restrict__ flag, uint64_t V, uint64_t Vpad,
+                                                     uint64_t vsplit, const uint64_t *__restrict__ ptr, Tuning tn,
+                                                     uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
+                                                     uint32_t nblk, const FrontierInfo *__restrict__ info,
+                                                     uint32_t *__restrict__ list, uint64_t *__restrict__ hub,
+                                                     uint64_t *__restrict__ doff) {
+    __shared__ uint64_t lds[kBlock / kWave];
+    const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kFlagsPerThread;
+    uint4 w[kFlagVecs];
+    load_flags(flag, base, Vpad, w);
+    const bool any = any_flag(w);
+
+    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, cz = 0, ch = 0;
+    uint64_t sdeg = 0, ddeg = 0;
+    if (any) {
+        for (int j = 0; j < kFlagsPerThread; ++j) {
+            if (!flag_at(w, j)) continue;
+            const uint64_t v = base + j;
+            const uint64_t deg = ptr[v + 1] - ptr[v];
+            switch (bin_of(v, vsplit, deg, tn)) {
+                case 0: ++c0; sdeg += deg; break;
+                case 1: ++c1; sdeg += deg; break;
+                case 2: c2 += (uint32_t)((deg + tn.tile - 1) / tn.tile); ++ch; sdeg += deg; break;
+                case 3: ++c3; ddeg += deg; break;
+                default: ++cz; break;
+            }
+        }
+    }
+    if (!WRITE) {
+        uint64_t p01 = block_sum((uint64_t)c0 | ((uint64_t)c1 << 32), lds);
+        uint64_t p23 = block_sum((uint64_t)c2 | ((uint64_t)c3 << 32), lds);
+        uint64_t pzh = block_sum((uint64_t)cz | ((uint64_t)ch << 32), lds);
+        uint64_t ts = block_sum(sdeg, lds);
+        uint64_t td = block_sum(ddeg, lds);
+        if (threadIdx.x == 0) {
+            bcnt[0 * nblk + blockIdx.x] = (uint32_t)p01;
+            bcnt[1 * nblk + blockIdx.x] = (uint32_t)(p01 >> 32);
+            bcnt[2 * nblk + blockIdx.x] = (uint32_t)p23;
+            bcnt[3 * nblk + blockIdx.x] = (uint32_t)(p23 >> 32);
+            bcnt[4 * nblk + blockIdx.x] = (uint32_t)pzh | ((uint32_t)(pzh >> 32) << 16);
+            bdeg[0 * nblk + blockIdx.x] = ts;
+            bdeg[1 * nblk + blockIdx.x] = td;
+        }
+        return;
+    }
+    uint64_t t01, t23, tdd;
+    const uint64_t e01 = block_exclusive_scan((uint64_t)c0 | ((uint64_t)c1 << 32), &t01, lds);
+    const uint64_t e23 = block_exclusive_scan((uint64_t)c2 | ((uint64_t)c3 << 32), &t23, lds);
+    const uint64_t edd = block_exclusive_scan(ddeg, &tdd, lds);
+    if (!any) return;
+    const uint32_t n0 = info->n[0], n1 = info->n[1];
+    uint64_t p0 = (uint64_t)bcnt[0 * nblk + blockIdx.x] + (uint32_t)e01;
+    uint64_t p1 = (uint64_t)n0 + bcnt[1 * nblk + blockIdx.x] + (uint32_t)(e01 >> 32);
+    uint64_t p2 = (uint64_t)bcnt[2 * nblk + blockIdx.x] + (uint32_t)e23;
+    uint64_t p3 = (uint64_t)bcnt[3 * nblk + blockIdx.x] + (uint32_t)(e23 >> 32);
+    uint64_t pd = bdeg[1 * nblk + blockIdx.x] + edd;
+    const uint64_t dyn_base = (uint64_t)n0 + n1;
+    for (int j = 0; j < kFlagsPerThread; ++j) {
+        if (!flag_at(w, j)) continue;
+        const uint64_t v = base + j;
+        const uint64_t deg = ptr[v + 1] - ptr[v];
+        switch (bin_of(v, vsplit, deg, tn)) {
+            case 0: list[p0++] = (uint32_t)v; break;
+            case 1: list[p1++] = (uint32_t)v; break;
+            case 2: {
+                const uint32_t nt = (uint32_t)((deg + tn.tile - 1) / tn.tile);
+                for (uint32_t t = 0; t < nt; ++t) hub[p2++] = ((uint64_t)t << 32) | v;
+                break;
+            }
+            case 3:
+                list[dyn_base + p3] = (uint32_t)v;
+                doff[p3] = pd;
+                ++p3;
+                pd += deg;
+                break;
+            default: break;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kFlagVecs; ++k) {
+        uint64_t at = base + 16ull * k;
+        if (at < Vpad) *(uint4 *)(flag + at) = make_uint4(0, 0, 0, 0);
+    }
+}
+
+// Pass 2: one workgroup scans the per-block counts (exclusive, in place) and
+// publishes the totals.
+__global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *bcnt, uint64_t *bdeg, uint32_t nblk,
+                                                        FrontierInfo *info, uint64_t *doff) {
+    __shared__ uint64_t lds[1024 / kWave];
+    uint64_t tot4[4];
+    for (int a = 0; a < 4; ++a) {
+        uint64_t carry = 0;
+        for (uint32_t b = 0; b < nblk; b += blockDim.x) {
+            const uint32_t i = b + threadIdx.x;
+            const uint64_t x = i < nblk ? bcnt[a * nblk + i] : 0;
+            uint64_t t;
+            const uint64_t ex = block_exclusive_scan(x, &t, lds);
+            if (i < nblk) bcnt[a * nblk + i] = (uint32_t)(carry + ex);
+            carry += t;
+        }
+        tot4[a] = carry;
+    }
+    uint64_t zsum = 0, hsum = 0, ssum = 0, dcarry = 0;
+    for (uint32_t b = 0; b < nblk; b += blockDim.x) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t zh = i < nblk ? bcnt[4 * nblk + i] : 0;
+        zsum += block_sum(zh & 0xFFFFu, lds);
+        hsum += block_sum(zh >> 16, lds);
+        ssum += block_sum(i < nblk ? bdeg[i] : 0, lds);
+        const uint64_t x = i < nblk ? bdeg[nblk + i] : 0;
+        uint64_t t;
+        const uint64_t ex = block_exclusive_scan(x, &t, lds);
+        if (i < nblk) bdeg[nblk + i] = dcarry + ex;
+        dcarry += t;
+    }
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 4; ++a) info->n[a] = (uint32_t)tot4[a];
+        info->nzero = (uint32_t)zsum;
+        info->nhub = (uint32_t)hsum;
+        info->static_deg = ssum;
+        info->dyn_deg = dcarry;
+        doff[tot4[3]] = dcarry;
+    }
+}
+
+// ---------------------------------------------------------------- relax
+
+template <bool W>
+__device__ __forceinline__ void load_edge(const void *__restrict__ edges, uint64_t i, uint32_t &to, uint32_t &wt) {
+    if (W) {
+        const uint2 e = ((const uint2 *)edges)[i];
+        to = e.x;
+        wt = e.y;
+    } else {
+        to = ((const uint32_t *)edges)[i];
+        wt = 1;
+    }
+}
+
+template <int ALGO>
+__device__ __forceinline__ uint32_t offer(uint32_t du, uint32_t wt) {
+    if (ALGO == ALGO_BFS) return du + 1;
+    if (ALGO == ALGO_SSSP) {
+        const uint32_t s = du + wt;
+        return s < du ? 0xFFFFFFFEu : s;
+    }
+    return du;
+}
+
+// Lowers val[v] to nd; marks v for the next frontier when it improved.
+// BFS writes the same level from every writer in an iteration, so a plain
+// store is race-free; SSSP/CC use atomicMin so no smaller offer is lost
+// (the reference's check-then-write in K7/K11 can lose one).
+template <int ALGO>
+__device__ __forceinline__ uint32_t try_relax(uint32_t *__restrict__ val, uint8_t *__restrict__ flag, uint32_t v,
+                                              uint32_t nd) {
+    const uint32_t cur = val[v];
+    if (nd >= cur) return 0;
+    if (ALGO == ALGO_BFS) {
+        val[v] = nd;
+        flag[v] = 1;
+        return 1;
+    }
+    const uint32_t old = atomicMin(val + v, nd);
+    if (nd < old) {
+        flag[v] = 1;
+        return 1;
+    }
+    return 0;
+}
+
+// Bins 0 and 1: LPV lanes per frontier vertex (LPV = 64: one wave).
+template <int ALGO, bool W, int LPV>
+__global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restrict__ list, uint32_t n,
+                                                        const uint64_t *__restrict__ ptr,
+                                                        const void *__restrict__ edges, uint32_t *val,
+                                                        uint8_t *flag, unsigned long long *upd) {
+    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t item = gid / LPV;
+    const uint32_t sub = (uint32_t)(gid % LPV);
+    uint32_t cnt = 0;
+    if (item < n) {
+        const uint32_t u = list[item];
+        const uint64_t b = ptr[u], e = ptr[u + 1];
+        const uint32_t du = val[u];
+        for (uint64_t i = b + sub; i < e; i += LPV) {
+            uint32_t to, wt;
+            load_edge<W>(edges, i, to, wt);
+            cnt += try_relax<ALGO>(val, flag, to, offer<ALGO>(du, wt));
+        }
+    }
+    wave_add(upd, cnt);
+}
+
+// Bin 2: one workgroup per tile of a hub's neighbour list; the tile is
+// staged in LDS with coalesced loads before the relax pass.
+template <int ALGO, bool W>
+__global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict__ hub, uint32_t tile,
+                                                      const uint64_t *__restrict__ ptr,
+                                                      const void *__restrict__ edges, uint32_t *val, uint8_t *flag,
+                                                      unsigned long long *upd) {
+    extern __shared__ uint32_t s_edges[];
+    const uint64_t it = hub[blockIdx.x];
+    const uint32_t u = (uint32_t)it, t = (uint32_t)(it >> 32);
+    const uint64_t b = ptr[u] + (uint64_t)t * tile;
+    const uint64_t e = min(ptr[u + 1], b + tile);
+    const uint32_t cnt = (uint32_t)(e - b);
+    const uint32_t du = val[u];
+    for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+        if (W)
+            ((uint2 *)s_edges)[k] = ((const uint2 *)edges)[b + k];
+        else
+            s_edges[k] = ((const uint32_t *)edges)[b + k];
+    }
+    __syncthreads();
+    uint32_t c = 0;
+    for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+        uint32_t to, wt;
+        if (W) {
+            const uint2 x = ((const uint2 *)s_edges)[k];
+            to = x.x;
+            wt = x.y;
+        } else {
+            to = s_edges[k];
+            wt = 1;
+        }
+        c += try_relax<ALGO>(val, flag, to, offer<ALGO>(du, wt));
+    }
+    wave_add(upd, c);
+}
+
+// Dynamic chunk: edges [c0, c1) of the concatenated active dynamic edge
+// space sit in `stage`.  Each block takes kDynEdgesPerBlock consecutive edges,
+// loads the covering slice of the degree prefix into LDS and maps each edge
+// to its source with a binary search there (every listed vertex has >= 1
+// edge, so a block covers at most kDynEdgesPerBlock vertices).
+template <int ALGO, bool W>
+__global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict__ dlist,
+                                                      const uint64_t *__restrict__ doff, uint32_t ndyn,
+                                                      const void *__restrict__ stage, uint64_t c0, uint64_t c1,
+                                                      uint32_t *val, uint8_t *flag, unsigned long long *upd) {
+    __shared__ uint64_t s_off[kDynEdgesPerBlock + 2];
+    __shared__ uint32_t s_i[2];
+    const uint64_t q0 = c0 + (uint64_t)blockIdx.x * kDynEdgesPerBlock;
+    const uint64_t q1 = min(q0 + kDynEdgesPerBlock, c1);
+    if (threadIdx.x < 2) {
+        const uint64_t q = threadIdx.x == 0 ? q0 : q1 - 1;
+        uint32_t lo = 0, hi = ndyn;  // largest i with doff[i] <= q; doff[0] = 0
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (doff[mid] <= q) lo = mid; else hi = mid;
+        }
+        s_i[threadIdx.x] = lo;
+    }
+    __syncthreads();
+    const uint32_t i0 = s_i[0];
+    const uint32_t nv = s_i[1] - i0 + 1;
+    for (uint32_t k = threadIdx.x; k <= nv; k += kBlock) s_off[k] = doff[i0 + k];
+    __syncthreads();
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < kDynEdgesPerThread; ++j) {
+        const uint64_t q = q0 + threadIdx.x + (uint64_t)j * kBlock;
+        if (q < q1) {
+            uint32_t lo = 0, hi = nv;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_off[mid] <= q) lo = mid; else hi = mid;
+            }
+            const uint32_t u = dlist[i0 + lo];
+            const uint32_t du = val[u];
+            uint32_t to, wt;
+            load_edge<W>(stage, q - c0, to, wt);
+            c += try_relax<ALGO>(val, flag, to, offer<ALGO>(du, wt));
+        }
+    }
+    wave_add(upd, c);
+}
+
+// reached = #{v : val[v] != UNREACHED}, traversed = sum of their degrees.
+__global__ __launch_bounds__(kBlock) void k_reach_stats(const uint32_t *__restrict__ val,
+                                                        const uint64_t *__restrict__ ptr, uint64_t V,
+                                                        unsigned long long *out) {
+    __shared__ uint64_t lds[kBlock / kWave];
+    uint64_t r = 0, t = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
+        if (val[v] != OG_UNREACHED) {
+            ++r;
+            t += ptr[v + 1] - ptr[v];
+        }
+    }
+    r = block_sum(r, lds);
+    t = block_sum(t, lds);
+    if (threadIdx.x == 0) {
+        atomicAdd(out + 0, (unsigned long long)r);
+        atomicAdd(out + 1, (unsigned long long)t);
+    }
+}
+
+// ============================================================ host driver
+
+static unsigned grid_for(uint64_t threads, unsigned block = kBlock) {
+    uint64_t g = (threads + block - 1) / block;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, 0x7FFFFFFFull));
+}
+
+static unsigned fill_grid(uint64_t n) { return (unsigned)std::min<uint64_t>(8192, std::max<uint64_t>(1, (n + 255) / 256)); }
+
+void device_state_free(DeviceState *d) {
+    if (!d) return;
+    if (d->s_compute) (void)hipStreamSynchronize(d->s_compute);
+    if (d->s_copy) (void)hipStreamSynchronize(d->s_copy);
+    for (void *p : d->allocations) (void)hipFree(p);
+    d->allocations.clear();
+    for (int s = 0; s < kSlots; ++s) {
+        if (d->h_stage[s]) (void)hipHostFree(d->h_stage[s]);
+        if (d->ev_h2d[s]) (void)hipEventDestroy(d->ev_h2d[s]);
+        if (d->ev_relax[s]) (void)hipEventDestroy(d->ev_relax[s]);
+    }
+    if (d->h_info) (void)hipHostFree(d->h_info);
+    if (d->h_dlist) (void)hipHostFree(d->h_dlist);
+    if (d->h_doff) (void)hipHostFree(d->h_doff);
+    if (d->h_counters) (void)hipHostFree(d->h_counters);
+    if (d->h_scalars) (void)hipHostFree(d->h_scalars);
+    if (d->s_compute) (void)hipStreamDestroy(d->s_compute);
+    if (d->s_copy) (void)hipStreamDestroy(d->s_copy);
+    delete d;
+}
+
+static void *pinned(size_t bytes) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        raise(OG_ENOMEM_HOST, "pinned host allocation of %zu bytes failed", bytes);
+    }
+    return p;
+}
+
+// Largest number of hub tiles any frontier can produce for the current tuning.
+static uint64_t hub_capacity(const og_graph *g, uint64_t vsplit, const Tuning &tn) {
+    uint64_t cap = 0;
+    for (uint64_t v = 0; v < vsplit; ++v) {
+        uint64_t deg = g->ptr[v + 1] - g->ptr[v];
+        if (deg > tn.t2) cap += (deg + tn.tile - 1) / tn.tile;
+    }
+    return cap;
+}
+
+static void alloc_hub(DeviceState *d, const og_graph *g) {
+    uint64_t cap = hub_capacity(g, d->plan.vsplit, d->tune);
+    if (cap > d->hub_cap || !d->d_hub) {
+        d->d_hub = (uint64_t *)d->dalloc(std::max<uint64_t>(cap, 1) * sizeof(uint64_t));
+        d->hub_cap = std::max<uint64_t>(cap, 1);
+    }
+}
+
+static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params *params, og_plan_info *info) {
+    if (!hip_available()) raise(OG_EHIP, "no HIP device is visible");
+    if (approach != OG_ASCETIC && approach != OG_SUBWAY && approach != OG_USM)
+        raise(OG_EINVAL, "unknown approach %d", (int)approach);
+    og_plan_params p{};
+    if (params) p = *params;
+    if (g->dev) {
+        device_state_free(g->dev);
+        g->dev = nullptr;
+    }
+    std::unique_ptr<DeviceState, void (*)(DeviceState *)> d(new DeviceState, device_state_free);
+    OG_HIP(hipGetDevice(&d->device));
+    d->approach = approach;
+    d->V = g->V;
+    d->E = g->E;
+    d->Vpad = (g->V + 15) / 16 * 16;
+    d->eb = g->edge_bytes();
+    d->flags = p.flags;
+    d->host_threads = p.host_threads ? p.host_threads : default_threads();
+    const uint64_t V = g->V, E = g->E, eb = d->eb;
+
+    // ---- plan
+    og_plan_info &pl = d->plan;
+    if (approach == OG_ASCETIC) {
+        og_status st = og_plan_compute(V, E, g->ptr, (uint32_t)eb, &p, &pl);
+        if (st != OG_OK) raise(st, "%s", og_last_error());
+    } else if (approach == OG_SUBWAY) {
+        // Subway keeps no static region; the whole budget is the transfer
+        // buffer (reference: 0.9 * (12 GiB - 32 V) / 4, BFS/sbw_BFS/main.cpp:334).
+        uint64_t total = p.edge_budget_bytes ? p.edge_budget_bytes / eb
+                                             : (uint64_t)(0.9 * (double)((12ull << 30) - 32ull * V) / (double)eb);
+        std::memset(&pl, 0, sizeof pl);
+        pl.total_budget_edges = total;
+        pl.static_edges = 0;
+        pl.staging_edges = total;
+        pl.overload_edges = E;
+        pl.static_vertices = 0;
+        pl.vsplit = 0;
+        pl.advisek_used = 0.0f;
+    } else {
+        std::memset(&pl, 0, sizeof pl);
+        pl.total_budget_edges = E;
+        pl.static_edges = E;
+        pl.static_vertices = V;
+        pl.vsplit = V;
+        pl.advisek_used = 1.0f;
+    }
+    pl.approach = (uint32_t)approach;
+
+    OG_HIP(hipStreamCreateWithFlags(&d->s_compute, hipStreamNonBlocking));
+    OG_HIP(hipStreamCreateWithFlags(&d->s_copy, hipStreamNonBlocking));
+    for (int s = 0; s < kSlots; ++s) {
+        OG_HIP(hipEventCreateWithFlags(&d->ev_h2d[s], hipEventDisableTiming));
+        OG_HIP(hipEventCreateWithFlags(&d->ev_relax[s], hipEventDisableTiming));
+    }
+
+    // ---- graph placement
+    if (approach == OG_USM) {
+        // Graph arrays stay in pinned host memory and the kernels read them
+        // through the device mapping: what SYCL malloc_shared gives on a GPU
+        // without XNACK page migration.
+        if (!g->pinned_ptr || !(eb == 8 ? g->pinned_wedges : g->pinned_cols))
+            raise(OG_EUNSUPPORTED, "USM approach needs the graph in pinned host memory");
+        void *dp = nullptr, *de = nullptr;
+        OG_HIP(hipHostGetDevicePointer(&dp, g->ptr, 0));
+        OG_HIP(hipHostGetDevicePointer(&de, (void *)g->edge_data(), 0));
+        d->d_ptr = (uint64_t *)dp;
+        d->d_static = de;
+        d->static_used = E;
+        d->managed = true;
+    } else {
+        d->d_ptr = (uint64_t *)d->dalloc((V + 1) * sizeof(uint64_t));
+        OG_HIP(hipMemcpyAsync(d->d_ptr, g->ptr, (V + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, d->s_compute));
+        if (pl.static_edges) {
+            d->d_static = d->dalloc(pl.static_edges * eb);
+            d->static_used = g->ptr[pl.vsplit];
+            if (d->static_used)
+                OG_HIP(hipMemcpyAsync(d->d_static, g->edge_data(), d->static_used * eb, hipMemcpyHostToDevice,
+                                      d->s_compute));
+        }
+        if (pl.vsplit < V) {
+            d->slot_edges = pl.staging_edges / kSlots;
+            if (d->slot_edges == 0)
+                raise(OG_EBUDGET, "edge budget leaves %llu staging edges for %llu dynamic-region edges",
+                      (unsigned long long)pl.staging_edges, (unsigned long long)(E - g->ptr[pl.vsplit]));
+            d->d_staging = d->dalloc(kSlots * d->slot_edges * eb);
+            for (int s = 0; s < kSlots; ++s) d->h_stage[s] = pinned(d->slot_edges * eb);
+            d->h_dlist = (uint32_t *)pinned(V * sizeof(uint32_t));
+            d->h_doff = (uint64_t *)pinned((V + 1) * sizeof(uint64_t));
+            d->pool.reset(new WorkerPool(d->host_threads));
+        }
+    }
+
+    // ---- vertex state
+    d->d_val = (uint32_t *)d->dalloc(V * sizeof(uint32_t));
+    d->d_flag = (uint8_t *)d->dalloc(d->Vpad);
+    d->d_list = (uint32_t *)d->dalloc(V * sizeof(uint32_t));
+    d->d_doff = (uint64_t *)d->dalloc((V + 1) * sizeof(uint64_t));
+    d->nblk = (uint32_t)((d->Vpad + kVertsPerBlock - 1) / kVertsPerBlock);
+    d->d_bcnt = (uint32_t *)d->dalloc(5ull * d->nblk * sizeof(uint32_t));
+    d->d_bdeg = (uint64_t *)d->dalloc(2ull * d->nblk * sizeof(uint64_t));
+    d->d_info = (FrontierInfo *)d->dalloc(sizeof(FrontierInfo));
+    d->d_counters = (unsigned long long *)d->dalloc(16 * sizeof(unsigned long long));
+    d->h_info = (FrontierInfo *)pinned(sizeof(FrontierInfo));
+    d->h_counters = (unsigned long long *)pinned(16 * sizeof(unsigned long long));
+    alloc_hub(d.get(), g);
+    hipLaunchKernelGGL(k_fill_u8, dim3(fill_grid(d->Vpad)), dim3(256), 0, d->s_compute, d->d_flag, d->Vpad, (uint8_t)0);
+    OG_HIP(hipGetLastError());
+    OG_HIP(hipStreamSynchronize(d->s_compute));
+    pl.device_bytes = d->device_bytes;
+    if (info) *info = pl;
+    g->dev = d.release();
+}
+
+// ------------------------------------------------------- frontier driver
+
+struct RunTimers {
+    TimedClass front, stat, dyn, copy;
+};
+
+// Host gather of dynamic edges [c0, c1) into a pinned slot (reference
+// fillEdgeArrByMultiThread, GraphMeta.h:470-506, with the copy offsets taken
+// from the device-built degree prefix instead of a serial host scan).
+static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0, uint64_t c1, uint8_t *dst) {
+    const uint64_t eb = d->eb;
+    const uint8_t *src = (const uint8_t *)g->edge_data();
+    const uint64_t n = c1 - c0;
+    const unsigned T = d->pool->size();
+    const unsigned use = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(T, n / 16384));
+    const uint64_t L = (n + use - 1) / use;
+    d->pool->run([&](unsigned t) {
+        if (t >= use) return;
+        const uint64_t p0 = c0 + (uint64_t)t * L;
+        const uint64_t p1 = std::min(c1, p0 + L);
+        if (p0 >= p1) return;
+        uint64_t i = (uint64_t)(std::upper_bound(d->h_doff, d->h_doff + ndyn + 1, p0) - d->h_doff) - 1;
+        uint64_t p = p0;
+        while (p < p1) {
+            const uint32_t v = d->h_dlist[i];
+            const uint64_t vb = d->h_doff[i], ve = d->h_doff[i + 1];
+            const uint64_t stop = std::min(ve, p1);
+            std::memcpy(dst + (p - c0) * eb, src + (g->ptr[v] + (p - vb)) * eb, (stop - p) * eb);
+            p = stop;
+            ++i;
+        }
+    });
+}
+
+template <int ALGO, bool W>
+static void launch_static(DeviceState *d, const FrontierInfo &fi, RunTimers &tm) {
+    const void *edges = d->d_static;
+    unsigned long long *upd = d->d_counters + 0;
+    const uint32_t n0 = fi.n[0], n1 = fi.n[1], n2 = fi.n[2];
+    if (!(n0 | n1 | n2)) return;
+    EventSpan span(d, &tm.stat, d->s_compute);
+    if (n0) {
+        const uint64_t thr = (uint64_t)n0 * d->tune.lpv0;
+        const dim3 gr(grid_for(thr)), bl(kBlock);
+        switch (d->tune.lpv0) {
+#define OG_LPV(L) \
+    case L: hipLaunchKernelGGL((k_relax_group<ALGO, W, L>), gr, bl, 0, d->s_compute, d->d_list, n0, d->d_ptr, edges, d->d_val, d->d_flag, upd); break;
+            OG_LPV(1) OG_LPV(2) OG_LPV(4) OG_LPV(8) OG_LPV(16) OG_LPV(32)
+            default: OG_LPV(64)
+#undef OG_LPV
+        }
+    }
+    if (n1)
+        hipLaunchKernelGGL((k_relax_group<ALGO, W, 64>), dim3(grid_for((uint64_t)n1 * 64)), dim3(kBlock), 0,
+                           d->s_compute, d->d_list + n0, n1, d->d_ptr, edges, d->d_val, d->d_flag, upd);
+    if (n2)
+        hipLaunchKernelGGL((k_relax_hub<ALGO, W>), dim3(n2), dim3(kBlock), d->tune.tile * d->eb, d->s_compute,
+                           d->d_hub, d->tune.tile, d->d_ptr, edges, d->d_val, d->d_flag, upd);
+    OG_HIP(hipGetLastError());
+}
+
+template <int ALGO, bool W>
+static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &fi, RunTimers &tm, og_run_stats &st,
+                        double &gather_ms) {
+    const uint32_t ndyn = fi.n[3];
+    const uint64_t D = fi.dyn_deg;
+    if (!ndyn || !D) return;
+    const uint32_t *d_dlist = d->d_list + fi.n[0] + fi.n[1];
+    OG_HIP(hipMemcpyAsync(d->h_dlist, d_dlist, (size_t)ndyn * 4, hipMemcpyDeviceToHost, d->s_copy));
+    OG_HIP(hipMemcpyAsync(d->h_doff, d->d_doff, ((size_t)ndyn + 1) * 8, hipMemcpyDeviceToHost, d->s_copy));
+    OG_HIP(hipStreamSynchronize(d->s_copy));
+    const uint64_t C = d->slot_edges, eb = d->eb;
+    const bool serial = (d->flags & OG_PLAN_NO_OVERLAP) != 0;
+    for (uint64_t c0 = 0, c = 0; c0 < D; c0 += C, ++c) {
+        const uint64_t c1 = std::min(D, c0 + C);
+        const int slot = (int)(c % kSlots);
+        if (d->slot_h2d_pending[slot]) OG_HIP(hipEventSynchronize(d->ev_h2d[slot]));
+        Stopwatch sw;
+        gather(d, g, ndyn, c0, c1, (uint8_t *)d->h_stage[slot]);
+        gather_ms += sw.ms();
+        uint8_t *dslot = (uint8_t *)d->d_staging + (uint64_t)slot * C * eb;
+        if (d->slot_relax_pending[slot]) OG_HIP(hipStreamWaitEvent(d->s_copy, d->ev_relax[slot], 0));
+        {
+            EventSpan span(d, &tm.copy, d->s_copy);
+            OG_HIP(hipMemcpyAsync(dslot, d->h_stage[slot], (c1 - c0) * eb, hipMemcpyHostToDevice, d->s_copy));
+        }
+        OG_HIP(hipEventRecord(d->ev_h2d[slot], d->s_copy));
+        d->slot_h2d_pending[slot] = true;
+        OG_HIP(hipStreamWaitEvent(d->s_compute, d->ev_h2d[slot], 0));
+        {
+            EventSpan span(d, &tm.dyn, d->s_compute);
+            hipLaunchKernelGGL((k_relax_dyn<ALGO, W>), dim3(grid_for(c1 - c0, kDynEdgesPerBlock)), dim3(kBlock), 0,
+                               d->s_compute, d_dlist, d->d_doff, ndyn, (const void *)dslot, c0, c1, d->d_val,
+                               d->d_flag, d->d_counters + 1);
+            OG_HIP(hipGetLastError());
+        }
+        OG_HIP(hipEventRecord(d->ev_relax[slot], d->s_compute));
+        d->slot_relax_pending[slot] = true;
+        if (serial) OG_HIP(hipStreamSynchronize(d->s_compute));
+        st.h2d_bytes += (c1 - c0) * eb;
+        st.chunks++;
+    }
+}
+
+template <int ALGO, bool W>
+static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats *stats) {
+    DeviceState *d = g->dev;
+    const uint64_t V = d->V;
+    og_run_stats st{};
+    st.approach = (uint32_t)d->approach;
+    RunTimers tm;
+    d->events.reset();
+    double gather_ms = 0, overload_wall = 0;
+    hipStream_t s = d->s_compute;
+
+    OG_HIP(hipStreamSynchronize(d->s_copy));
+    OG_HIP(hipStreamSynchronize(s));
+    Stopwatch total;
+    OG_HIP(hipMemsetAsync(d->d_counters, 0, 16 * sizeof(unsigned long long), s));
+    if (ALGO == ALGO_CC) {
+        hipLaunchKernelGGL(k_iota, dim3(fill_grid(V)), dim3(256), 0, s, d->d_val, V);
+        hipLaunchKernelGGL(k_fill_u8, dim3(fill_grid(V)), dim3(256), 0, s, d->d_flag, V, (uint8_t)1);
+    } else {
+        hipLaunchKernelGGL(k_fill_u8, dim3(fill_grid(d->Vpad)), dim3(256), 0, s, d->d_flag, d->Vpad, (uint8_t)0);
+        hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid(V)), dim3(256), 0, s, d->d_val, V, OG_UNREACHED);
+        hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d->d_val, d->d_flag, src);
+    }
+    OG_HIP(hipGetLastError());
+
+    for (;;) {
+        {
+            EventSpan span(d, &tm.front, s);
+            hipLaunchKernelGGL(k_frontier<false>, dim3(d->nblk), dim3(kBlock), 0, s, d->d_flag, V, d->Vpad,
+                               d->plan.vsplit, d->d_ptr, d->tune, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
+                               d->d_list, d->d_hub, d->d_doff);
+            hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
+                               d->d_doff);
+            hipLaunchKernelGGL(k_frontier<true>, dim3(d->nblk), dim3(kBlock), 0, s, d->d_flag, V, d->Vpad,
+                               d->plan.vsplit, d->d_ptr, d->tune, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
+                               d->d_list, d->d_hub, d->d_doff);
+            OG_HIP(hipGetLastError());
+        }
+        OG_HIP(hipMemcpyAsync(d->h_info, d->d_info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
+        OG_HIP(hipStreamSynchronize(s));
+        const FrontierInfo fi = *d->h_info;
+        const uint64_t active = (uint64_t)fi.n[0] + fi.n[1] + fi.nhub + fi.n[3] + fi.nzero;
+        if (active == 0) break;
+        if (fi.n[2] > d->hub_cap) raise(OG_EHIP, "hub tile list overflow (%u > %llu)", fi.n[2], (unsigned long long)d->hub_cap);
+        st.iterations++;
+        st.node_sum += active;
+        st.static_node_sum += (uint64_t)fi.n[0] + fi.n[1] + fi.nhub;
+        st.work_edges += fi.static_deg + fi.dyn_deg;
+        st.static_work_edges += fi.static_deg;
+        st.relax_static_bytes += 24ull * ((uint64_t)fi.n[0] + fi.n[1] + fi.nhub) + (W && ALGO == ALGO_SSSP ? 12ull : 8ull) * fi.static_deg;
+        st.relax_dynamic_bytes += 24ull * fi.n[3] + (W && ALGO == ALGO_SSSP ? 12ull : 8ull) * fi.dyn_deg;
+        st.frontier_bytes += 2ull * d->Vpad + 4ull * active;
+        launch_static<ALGO, W>(d, fi, tm);
+        if (fi.n[3]) {
+            Stopwatch ow;
+            run_dynamic<ALGO, W>(d, g, fi, tm, st, gather_ms);
+            overload_wall += ow.ms();
+        }
+    }
+    OG_HIP(hipStreamSynchronize(s));
+    st.total_ms = total.ms();
+
+    // statistics and results
+    hipLaunchKernelGGL(k_reach_stats, dim3(std::min<uint64_t>(2048, (V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       d->d_val, d->d_ptr, V, d->d_counters + 2);
+    OG_HIP(hipGetLastError());
+    OG_HIP(hipMemcpyAsync(d->h_counters, d->d_counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    if (out) OG_HIP(hipMemcpyAsync(out, d->d_val, V * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    OG_HIP(hipStreamSynchronize(s));
+    st.updates = d->h_counters[0] + d->h_counters[1];
+    st.reached = d->h_counters[2];
+    st.traversed_edges = d->h_counters[3];
+    st.relax_static_bytes += 8ull * d->h_counters[0];
+    st.relax_dynamic_bytes += 8ull * d->h_counters[1];
+    st.frontier_kernel_ms = sum_ms(tm.front);
+    st.frontier_launches = tm.front.launches;
+    st.relax_static_kernel_ms = sum_ms(tm.stat);
+    st.relax_static_launches = tm.stat.launches;
+    st.relax_dynamic_kernel_ms = sum_ms(tm.dyn);
+    st.relax_dynamic_launches = tm.dyn.launches;
+    st.copy_ms = sum_ms(tm.copy);
+    st.pre_ms = st.frontier_kernel_ms;
+    st.static_ms = st.relax_static_kernel_ms;
+    st.overload_ms = overload_wall;
+    st.overload_move_ms = st.copy_ms;
+    st.gather_ms = gather_ms;
+    if (stats) *stats = st;
+}
+
+static og_status run_algo(og_graph *g, int algo, uint32_t src, uint32_t *out, og_run_stats *stats) {
+    clear_error();
+    if (!g) return fail(OG_EINVAL, "graph is NULL");
+    if (!g->dev) return fail(OG_ESTATE, "og_prepare has not been called for this graph");
+    if (algo != ALGO_CC && src >= g->V) return fail(OG_EINVAL, "source %u >= V=%llu", src, (unsigned long long)g->V);
+    if (g->kind == OG_KIND_PULL) return fail(OG_EUNSUPPORTED, "pull-layout graphs serve PageRank only");
+    if (algo == ALGO_SSSP && g->kind != OG_KIND_WEIGHTED) return fail(OG_EUNSUPPORTED, "SSSP needs a weighted graph");
+    int expect = 0;
+    if (!g->busy.compare_exchange_strong(expect, 1)) return fail(OG_EBUSY, "graph is running another call");
+    og_status st = guard([&]() -> og_status {
+        OG_HIP(hipSetDevice(g->dev->device));
+        const bool w = g->kind == OG_KIND_WEIGHTED;
+        if (algo == ALGO_BFS) {
+            if (w) run_frontier<ALGO_BFS, true>(g, src, out, stats);
+            else run_frontier<ALGO_BFS, false>(g, src, out, stats);
+        } else if (algo == ALGO_SSSP) {
+            run_frontier<ALGO_SSSP, true>(g, src, out, stats);
+        } else {
+            if (w) run_frontier<ALGO_CC, true>(g, 0, out, stats);
+            else run_frontier<ALGO_CC, false>(g, 0, out, stats);
+        }
+        return OG_OK;
+    });
+    g->busy.store(0);
+    return st;
+}
+
+}  // namespace og
+
+using namespace og;
+
+extern "C" og_status og_prepare(og_graph *g, og_approach approach, const og_plan_params *params, og_plan_info *info) {
+    clear_error();
+    if (!g) return fail(OG_EINVAL, "graph is NULL");
+    int expect = 0;
+    if (!g->busy.compare_exchange_strong(expect, 1)) return fail(OG_EBUSY, "graph is running another call");
+    og_status st = guard([&]() -> og_status {
+        prepare_impl(g, approach, params, info);
+        return OG_OK;
+    });
+    g->busy.store(0);
+    return st;
+}
+
+extern "C" og_status og_release(og_graph *g) {
+    clear_error();
+    if (!g) return fail(OG_EINVAL, "graph is NULL");
+    int expect = 0;
+    if (!g->busy.compare_exchange_strong(expect, 1)) return fail(OG_EBUSY, "graph is running another call");
+    device_state_free(g->dev);
+    g->dev = nullptr;
+    g->busy.store(0);
+    return OG_OK;
+}
+
+extern "C" og_status og_bfs(og_graph *g, uint32_t source, uint32_t *levels, og_run_stats *stats) {
+    return run_algo(g, ALGO_BFS, source, levels, stats);
+}
+
+extern "C" og_status og_sssp(og_graph *g, uint32_t source, uint32_t *dist, og_run_stats *stats) {
+    return run_algo(g, ALGO_SSSP, source, dist, stats);
+}
+
+extern "C" og_status og_cc(og_graph *g, uint32_t *labels, og_run_stats *stats) {
+    return run_algo(g, ALGO_CC, 0, labels, stats);
+}
