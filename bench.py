@@ -84,11 +84,26 @@ def main():
     ap.add_argument("--approach", default="ascetic", choices=["ascetic", "subway", "usm"])
     ap.add_argument("--compare", action="store_true", help="also time subway and usm")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); --gpus {args.gpus} refused",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration_mib:
+        # A copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel give the
+        # factor between the counters and bytes actually moved.
+        import torch
+
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        a = torch.arange(n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "reps": 3}))
+        return
 
     import onegraph_amd as og
 
@@ -140,6 +155,14 @@ def main():
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(r["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_us": round(r["avg_us"], 3), "bytes_per_launch": round(r["bytes_per_launch"], 1)}
+        # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+        # (calibrated against a plain copy) for this kernel and workload.
+        tpath = os.path.join(HERE, "profiles", "k_relax_dyn_sssp_rmat24_traffic.json")
+        if dom == "relax_dynamic" and args.scale == 24 and args.approach == "ascetic" and os.path.exists(tpath):
+            with open(tpath) as f:
+                tr = json.load(f)
+            roof["traffic"] = tr["traffic_bytes_per_launch"]
+            roof["traffic_source"] = tr["source"]
     link = None
     if agg["copy_ms"] > 0:
         gbs = agg["h2d_bytes"] / (agg["copy_ms"] * 1e-3) / 1e9
