@@ -202,6 +202,8 @@ typedef struct og_run_stats {
     double relax_dynamic_kernel_ms; uint64_t relax_dynamic_launches; uint64_t relax_dynamic_bytes;
     double frontier_kernel_ms;      uint64_t frontier_launches;      uint64_t frontier_bytes;
     double copy_ms;               /* sum of copy-stream durations (HIP events)    */
+    uint64_t checksum;            /* digest of the result array (og_levels_checksum);
+                                     multi-source: sum over the sources' digests    */
 } og_run_stats;
 
 /* Replaces bfs_opt (BFS/oneGraph_BFS/CalculateOpt.h:9-211),
@@ -218,6 +220,21 @@ og_status og_sssp(og_graph *g, uint32_t source, uint32_t *dist, og_run_stats *st
 /* Replaces cc_opt (CC/onegraph/CalculateOpt.h:6-144), the Subway CC main
  * (CC/sbw/sbw.cpp:305-490) and the USM CC main (CC/usm/usm.cpp:127-257). */
 og_status og_cc(og_graph *g, uint32_t *labels, og_run_stats *stats);
+
+/* Multi-source BFS (no reference counterpart; the reference runs one source
+ * per bfs_opt iteration, BFS/oneGraph_BFS/CalculateOpt.h:55-209).  Sources
+ * go in batches of 64, each batch one frontier traversal carrying a u64
+ * mask per vertex, on the approach given to og_prepare.  Outputs (each may
+ * be NULL): levels[i*V + v] in the og_bfs convention, checksums[i] =
+ * og_levels_checksum of source i's levels, reached[i] = vertices reached. */
+og_status og_msbfs(og_graph *g, const uint32_t *sources, uint32_t num_sources,
+                   uint32_t *levels, uint64_t *checksums, uint64_t *reached,
+                   og_run_stats *stats);
+
+/* Digest of a BFS/SSSP result array: sum mod 2^64 of mix64(v << 32 | x[v])
+ * over entries x[v] != OG_UNREACHED (mix64 = SplitMix64 finaliser).  The
+ * device computes the same digest into og_run_stats.checksum. */
+og_status og_levels_checksum(const uint32_t *values, uint64_t num_vertices, uint64_t *checksum);
 
 typedef struct og_pr_params {
     double damping;          /* 0.85                                   */

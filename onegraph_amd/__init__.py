@@ -55,7 +55,7 @@ class RunStats(C.Structure):
                 ("relax_static_bytes", C.c_uint64), ("relax_dynamic_kernel_ms", C.c_double),
                 ("relax_dynamic_launches", C.c_uint64), ("relax_dynamic_bytes", C.c_uint64),
                 ("frontier_kernel_ms", C.c_double), ("frontier_launches", C.c_uint64),
-                ("frontier_bytes", C.c_uint64), ("copy_ms", C.c_double)]
+                ("frontier_bytes", C.c_uint64), ("copy_ms", C.c_double), ("checksum", C.c_uint64)]
 
 
 class GraphInfo(C.Structure):
@@ -117,6 +117,8 @@ def lib():
         "og_bfs": (C.c_int, [_P, C.c_uint32, _u32p, C.POINTER(RunStats)]),
         "og_sssp": (C.c_int, [_P, C.c_uint32, _u32p, C.POINTER(RunStats)]),
         "og_cc": (C.c_int, [_P, _u32p, C.POINTER(RunStats)]),
+        "og_msbfs": (C.c_int, [_P, _u32p, C.c_uint32, _u32p, _u64p, _u64p, C.POINTER(RunStats)]),
+        "og_levels_checksum": (C.c_int, [_u32p, C.c_uint64, _u64p]),
         "og_pagerank": (C.c_int, [_P, C.POINTER(PrParams), C.POINTER(C.c_float), C.POINTER(C.c_double),
                                   C.POINTER(PrStats)]),
     }
@@ -145,6 +147,14 @@ def device_count():
     n = C.c_int(0)
     _check(lib().og_device_count(C.byref(n)))
     return n.value
+
+
+def levels_checksum(values):
+    """Digest of a BFS/SSSP result array (same as og_run_stats.checksum)."""
+    v = np.ascontiguousarray(values, dtype=np.uint32)
+    out = C.c_uint64()
+    _check(lib().og_levels_checksum(_u32(v), len(v), C.byref(out)))
+    return out.value
 
 
 def version():
@@ -261,6 +271,17 @@ class Graph:
         st = RunStats()
         _check(lib().og_sssp(self._h, source, _u32(out), C.byref(st)))
         return out, as_dict(st)
+
+    def msbfs(self, sources, want_levels=True):
+        """Multi-source BFS; returns (levels [n, V] or None, checksums, reached, stats)."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        n, V = len(src), self.num_vertices
+        lv = np.empty((n, V), np.uint32) if want_levels else None
+        cs = np.empty(n, np.uint64)
+        rc = np.empty(n, np.uint64)
+        st = RunStats()
+        _check(lib().og_msbfs(self._h, _u32(src), n, _u32(lv), _u64(cs), _u64(rc), C.byref(st)))
+        return lv, cs, rc, as_dict(st)
 
     def cc(self, want_values=True):
         out = np.empty(self.num_vertices, np.uint32) if want_values else None

@@ -29,7 +29,18 @@ constexpr int kDynEdgesPerThread = 4;   // dynamic relax: edges per lane
 constexpr int kDynEdgesPerBlock = kBlock * kDynEdgesPerThread;
 constexpr int kSlots = 2;               // dynamic staging double buffer
 
-enum Algo { ALGO_BFS = 0, ALGO_SSSP = 1, ALGO_CC = 2 };
+enum Algo { ALGO_BFS = 0, ALGO_SSSP = 1, ALGO_CC = 2, ALGO_MSBFS = 3 };
+
+constexpr int kMsWidth = 64;            // sources per multi-source batch (bits of a u64 mask)
+
+// Per-vertex state the relax kernels touch.
+struct RelaxArgs {
+    uint32_t *val;        // BFS level / SSSP distance / CC label
+    uint8_t *flag;        // next-frontier flags
+    uint64_t *visit;      // multi-source: sources that reached v
+    const uint64_t *front;// multi-source: sources whose frontier holds v
+    uint64_t *next;       // multi-source: sources offered to v this round
+};
 
 // Degree-bin thresholds and shapes for the static relax (autotunable).
 struct Tuning {
@@ -121,6 +132,13 @@ struct DeviceState {
     bool slot_relax_pending[kSlots] = {false, false};
     EventPool events;
     std::unique_ptr<WorkerPool> pool;
+
+    // multi-source BFS state (allocated on first use)
+    uint64_t *d_visit = nullptr, *d_front = nullptr, *d_next = nullptr;  // V each
+    uint8_t *d_mlvl = nullptr;       // V x 64 levels, vertex-major (0 = unreached)
+    uint32_t *d_msrc = nullptr;      // 64 sources of the batch
+    unsigned long long *d_msum = nullptr;  // 2 x 64: checksum, reached per source
+    uint32_t *d_extract = nullptr;   // V scratch for one source's u32 levels
 
     // PageRank state (allocated on first use)
     void *d_rank[2] = {nullptr, nullptr};

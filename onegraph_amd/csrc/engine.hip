@@ -254,12 +254,40 @@ __device__ __forceinline__ uint32_t try_relax(uint32_t *__restrict__ val, uint8_
     return 0;
 }
 
+// What a frontier vertex carries to its out-edges: its value, or for the
+// multi-source BFS the mask of sources whose frontier holds it.
+template <int ALGO> struct SrcOf { using T = uint32_t; };
+template <> struct SrcOf<ALGO_MSBFS> { using T = uint64_t; };
+
+template <int ALGO>
+__device__ __forceinline__ typename SrcOf<ALGO>::T load_src(const RelaxArgs &a, uint32_t u) {
+    if constexpr (ALGO == ALGO_MSBFS) return a.front[u];
+    else return a.val[u];
+}
+
+// Relaxes edge (u -> to, wt) given u's carried state; returns 1 on update.
+// Multi-source: the sources of u's frontier that have not reached `to` are
+// OR-ed into next[to] (one 64-bit atomic for all 64 searches).
+template <int ALGO>
+__device__ __forceinline__ uint32_t relax_edge(const RelaxArgs &a, typename SrcOf<ALGO>::T s, uint32_t to,
+                                               uint32_t wt) {
+    if constexpr (ALGO == ALGO_MSBFS) {
+        const uint64_t m = s & ~a.visit[to];
+        if (!m) return 0;
+        atomicOr((unsigned long long *)(a.next + to), (unsigned long long)m);
+        a.flag[to] = 1;
+        return 1;
+    } else {
+        return try_relax<ALGO>(a.val, a.flag, to, offer<ALGO>(s, wt));
+    }
+}
+
 // Bins 0 and 1: LPV lanes per frontier vertex (LPV = 64: one wave).
 template <int ALGO, bool W, int LPV>
 __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restrict__ list, uint32_t n,
                                                         const uint64_t *__restrict__ ptr,
-                                                        const void *__restrict__ edges, uint32_t *val,
-                                                        uint8_t *flag, unsigned long long *upd) {
+                                                        const void *__restrict__ edges, RelaxArgs a,
+                                                        unsigned long long *upd) {
     const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     const uint64_t item = gid / LPV;
     const uint32_t sub = (uint32_t)(gid % LPV);
@@ -267,11 +295,11 @@ __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restri
     if (item < n) {
         const uint32_t u = list[item];
         const uint64_t b = ptr[u], e = ptr[u + 1];
-        const uint32_t du = val[u];
+        const auto s = load_src<ALGO>(a, u);
         for (uint64_t i = b + sub; i < e; i += LPV) {
             uint32_t to, wt;
             load_edge<W>(edges, i, to, wt);
-            cnt += try_relax<ALGO>(val, flag, to, offer<ALGO>(du, wt));
+            cnt += relax_edge<ALGO>(a, s, to, wt);
         }
     }
     wave_add(upd, cnt);
@@ -282,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restri
 template <int ALGO, bool W>
 __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict__ hub, uint32_t tile,
                                                       const uint64_t *__restrict__ ptr,
-                                                      const void *__restrict__ edges, uint32_t *val, uint8_t *flag,
+                                                      const void *__restrict__ edges, RelaxArgs a,
                                                       unsigned long long *upd) {
     extern __shared__ uint32_t s_edges[];
     const uint64_t it = hub[blockIdx.x];
@@ -290,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
     const uint64_t b = ptr[u] + (uint64_t)t * tile;
     const uint64_t e = min(ptr[u + 1], b + tile);
     const uint32_t cnt = (uint32_t)(e - b);
-    const uint32_t du = val[u];
+    const auto s = load_src<ALGO>(a, u);
     for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
         if (W)
             ((uint2 *)s_edges)[k] = ((const uint2 *)edges)[b + k];
@@ -309,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
             to = s_edges[k];
             wt = 1;
         }
-        c += try_relax<ALGO>(val, flag, to, offer<ALGO>(du, wt));
+        c += relax_edge<ALGO>(a, s, to, wt);
     }
     wave_add(upd, c);
 }
@@ -323,7 +351,7 @@ template <int ALGO, bool W>
 __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict__ dlist,
                                                       const uint64_t *__restrict__ doff, uint32_t ndyn,
                                                       const void *__restrict__ stage, uint64_t c0, uint64_t c1,
-                                                      uint32_t *val, uint8_t *flag, unsigned long long *upd) {
+                                                      RelaxArgs a, unsigned long long *upd) {
     __shared__ uint64_t s_off[kDynEdgesPerBlock + 2];
     __shared__ uint32_t s_i[2];
     const uint64_t q0 = c0 + (uint64_t)blockIdx.x * kDynEdgesPerBlock;
@@ -353,32 +381,116 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
                 if (s_off[mid] <= q) lo = mid; else hi = mid;
             }
             const uint32_t u = dlist[i0 + lo];
-            const uint32_t du = val[u];
+            const auto s = load_src<ALGO>(a, u);
             uint32_t to, wt;
             load_edge<W>(stage, q - c0, to, wt);
-            c += try_relax<ALGO>(val, flag, to, offer<ALGO>(du, wt));
+            c += relax_edge<ALGO>(a, s, to, wt);
         }
     }
     wave_add(upd, c);
 }
 
-// reached = #{v : val[v] != UNREACHED}, traversed = sum of their degrees.
+// Order-independent digest of a result array: the sum (mod 2^64) of
+// mix64(v << 32 | value[v]) over reached vertices.
+__device__ __forceinline__ uint64_t d_mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// reached = #{v : val[v] != UNREACHED}, traversed = sum of their degrees,
+// checksum = digest of val.
 __global__ __launch_bounds__(kBlock) void k_reach_stats(const uint32_t *__restrict__ val,
                                                         const uint64_t *__restrict__ ptr, uint64_t V,
                                                         unsigned long long *out) {
     __shared__ uint64_t lds[kBlock / kWave];
-    uint64_t r = 0, t = 0;
+    uint64_t r = 0, t = 0, h = 0;
     for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
-        if (val[v] != OG_UNREACHED) {
+        const uint32_t x = val[v];
+        if (x != OG_UNREACHED) {
             ++r;
             t += ptr[v + 1] - ptr[v];
+            h += d_mix64((v << 32) | x);
         }
     }
     r = block_sum(r, lds);
     t = block_sum(t, lds);
+    h = block_sum(h, lds);
     if (threadIdx.x == 0) {
         atomicAdd(out + 0, (unsigned long long)r);
         atomicAdd(out + 1, (unsigned long long)t);
+        atomicAdd(out + 2, (unsigned long long)h);
+    }
+}
+
+// ------------------------------------------------------- multi-source BFS
+
+// Seeds up to 64 sources: bit j of visit/front at source j, level 1.
+__global__ void k_ms_init(const uint32_t *__restrict__ src, uint32_t n, uint64_t *visit, uint64_t *front,
+                          uint8_t *flag, uint8_t *lvl) {
+    const uint32_t j = threadIdx.x;
+    if (j < n) {
+        const uint32_t s = src[j];
+        atomicOr((unsigned long long *)(visit + s), 1ull << j);
+        atomicOr((unsigned long long *)(front + s), 1ull << j);
+        flag[s] = 1;
+        lvl[(uint64_t)s * kMsWidth + j] = 1;
+    }
+}
+
+// After a round: for every flagged vertex, the offered sources it had not
+// seen become its new frontier mask and get `level`; vertices with nothing
+// new leave the frontier.
+__global__ __launch_bounds__(kBlock) void k_ms_commit(uint8_t *__restrict__ flag, uint64_t V, uint64_t *visit,
+                                                      uint64_t *front, uint64_t *next, uint8_t *lvl, uint8_t level) {
+    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
+        if (!flag[v]) continue;
+        const uint64_t vis = visit[v];
+        const uint64_t nw = next[v] & ~vis;
+        next[v] = 0;
+        front[v] = nw;
+        if (nw) {
+            visit[v] = vis | nw;
+            uint64_t m = nw;
+            while (m) {
+                const int j = __builtin_ctzll(m);
+                m &= m - 1;
+                lvl[v * kMsWidth + j] = level;
+            }
+        } else {
+            flag[v] = 0;
+        }
+    }
+}
+
+// Per-source reached count, traversed edges and digest over the vertex-major
+// level array: lane j of each wave handles source j (one 64-byte line per
+// vertex), partial sums leave with one atomic per lane per block.
+__global__ __launch_bounds__(kBlock) void k_ms_digest(const uint8_t *__restrict__ lvl,
+                                                      const uint64_t *__restrict__ ptr, uint64_t V,
+                                                      unsigned long long *out /* 3 x 64 */) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t wave = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const uint64_t nwaves = (uint64_t)gridDim.x * kBlock / kWave;
+    uint64_t r = 0, t = 0, h = 0;
+    for (uint64_t v = wave; v < V; v += nwaves) {
+        const uint32_t x = lvl[v * kMsWidth + lane];
+        if (x) {
+            ++r;
+            t += ptr[v + 1] - ptr[v];
+            h += d_mix64((v << 32) | x);
+        }
+    }
+    atomicAdd(out + lane, (unsigned long long)h);
+    atomicAdd(out + kMsWidth + lane, (unsigned long long)r);
+    atomicAdd(out + 2 * kMsWidth + lane, (unsigned long long)t);
+}
+
+// Levels of source j as u32 in the single-source convention.
+__global__ void k_ms_extract(const uint8_t *__restrict__ lvl, uint64_t V, uint32_t j, uint32_t *out) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = lvl[v * kMsWidth + j];
+        out[v] = x ? x : OG_UNREACHED;
     }
 }
 
@@ -587,10 +699,15 @@ static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0
     });
 }
 
+static RelaxArgs relax_args(DeviceState *d) {
+    return RelaxArgs{d->d_val, d->d_flag, d->d_visit, d->d_front, d->d_next};
+}
+
 template <int ALGO, bool W>
 static void launch_static(DeviceState *d, const FrontierInfo &fi, RunTimers &tm) {
     const void *edges = d->d_static;
     unsigned long long *upd = d->d_counters + 0;
+    const RelaxArgs a = relax_args(d);
     const uint32_t n0 = fi.n[0], n1 = fi.n[1], n2 = fi.n[2];
     if (!(n0 | n1 | n2)) return;
     EventSpan span(d, &tm.stat, d->s_compute);
@@ -599,7 +716,7 @@ static void launch_static(DeviceState *d, const FrontierInfo &fi, RunTimers &tm)
         const dim3 gr(grid_for(thr)), bl(kBlock);
         switch (d->tune.lpv0) {
 #define OG_LPV(L) \
-    case L: hipLaunchKernelGGL((k_relax_group<ALGO, W, L>), gr, bl, 0, d->s_compute, d->d_list, n0, d->d_ptr, edges, d->d_val, d->d_flag, upd); break;
+    case L: hipLaunchKernelGGL((k_relax_group<ALGO, W, L>), gr, bl, 0, d->s_compute, d->d_list, n0, d->d_ptr, edges, a, upd); break;
             OG_LPV(1) OG_LPV(2) OG_LPV(4) OG_LPV(8) OG_LPV(16) OG_LPV(32)
             default: OG_LPV(64)
 #undef OG_LPV
@@ -607,10 +724,10 @@ static void launch_static(DeviceState *d, const FrontierInfo &fi, RunTimers &tm)
     }
     if (n1)
         hipLaunchKernelGGL((k_relax_group<ALGO, W, 64>), dim3(grid_for((uint64_t)n1 * 64)), dim3(kBlock), 0,
-                           d->s_compute, d->d_list + n0, n1, d->d_ptr, edges, d->d_val, d->d_flag, upd);
+                           d->s_compute, d->d_list + n0, n1, d->d_ptr, edges, a, upd);
     if (n2)
         hipLaunchKernelGGL((k_relax_hub<ALGO, W>), dim3(n2), dim3(kBlock), d->tune.tile * d->eb, d->s_compute,
-                           d->d_hub, d->tune.tile, d->d_ptr, edges, d->d_val, d->d_flag, upd);
+                           d->d_hub, d->tune.tile, d->d_ptr, edges, a, upd);
     OG_HIP(hipGetLastError());
 }
 
@@ -620,6 +737,7 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
     const uint32_t ndyn = fi.n[3];
     const uint64_t D = fi.dyn_deg;
     if (!ndyn || !D) return;
+    const RelaxArgs a = relax_args(d);
     const uint32_t *d_dlist = d->d_list + fi.n[0] + fi.n[1];
     OG_HIP(hipMemcpyAsync(d->h_dlist, d_dlist, (size_t)ndyn * 4, hipMemcpyDeviceToHost, d->s_copy));
     OG_HIP(hipMemcpyAsync(d->h_doff, d->d_doff, ((size_t)ndyn + 1) * 8, hipMemcpyDeviceToHost, d->s_copy));
@@ -645,8 +763,8 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
         {
             EventSpan span(d, &tm.dyn, d->s_compute);
             hipLaunchKernelGGL((k_relax_dyn<ALGO, W>), dim3(grid_for(c1 - c0, kDynEdgesPerBlock)), dim3(kBlock), 0,
-                               d->s_compute, d_dlist, d->d_doff, ndyn, (const void *)dslot, c0, c1, d->d_val,
-                               d->d_flag, d->d_counters + 1);
+                               d->s_compute, d_dlist, d->d_doff, ndyn, (const void *)dslot, c0, c1, a,
+                               d->d_counters + 1);
             OG_HIP(hipGetLastError());
         }
         OG_HIP(hipEventRecord(d->ev_relax[slot], d->s_compute));
@@ -655,6 +773,78 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
         st.h2d_bytes += (c1 - c0) * eb;
         st.chunks++;
     }
+}
+
+// Algorithmic bytes per relaxed edge: column (+ weight) and the target state.
+template <int ALGO, bool W>
+constexpr uint64_t edge_bytes_alg() {
+    return ALGO == ALGO_MSBFS ? 12ull : (ALGO == ALGO_SSSP ? 12ull : 8ull);
+}
+
+// Iterates compaction -> static relax -> dynamic relax -> after(round) until
+// the frontier is empty.  One host synchronisation per round (the frontier
+// summary).  Accumulates into st / tm.
+template <int ALGO, bool W, class After>
+static void frontier_loop(og_graph *g, og_run_stats &st, RunTimers &tm, double &gather_ms, double &overload_wall,
+                          After after) {
+    DeviceState *d = g->dev;
+    const uint64_t V = d->V;
+    hipStream_t s = d->s_compute;
+    for (;;) {
+        {
+            EventSpan span(d, &tm.front, s);
+            hipLaunchKernelGGL(k_frontier<false>, dim3(d->nblk), dim3(kBlock), 0, s, d->d_flag, V, d->Vpad,
+                               d->plan.vsplit, d->d_ptr, d->tune, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
+                               d->d_list, d->d_hub, d->d_doff);
+            hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
+                               d->d_doff);
+            hipLaunchKernelGGL(k_frontier<true>, dim3(d->nblk), dim3(kBlock), 0, s, d->d_flag, V, d->Vpad,
+                               d->plan.vsplit, d->d_ptr, d->tune, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
+                               d->d_list, d->d_hub, d->d_doff);
+            OG_HIP(hipGetLastError());
+        }
+        OG_HIP(hipMemcpyAsync(d->h_info, d->d_info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
+        OG_HIP(hipStreamSynchronize(s));
+        const FrontierInfo fi = *d->h_info;
+        const uint64_t active = (uint64_t)fi.n[0] + fi.n[1] + fi.nhub + fi.n[3] + fi.nzero;
+        if (active == 0) break;
+        if (fi.n[2] > d->hub_cap)
+            raise(OG_EHIP, "hub tile list overflow (%u > %llu)", fi.n[2], (unsigned long long)d->hub_cap);
+        st.iterations++;
+        st.node_sum += active;
+        st.static_node_sum += (uint64_t)fi.n[0] + fi.n[1] + fi.nhub;
+        st.work_edges += fi.static_deg + fi.dyn_deg;
+        st.static_work_edges += fi.static_deg;
+        st.relax_static_bytes += 24ull * ((uint64_t)fi.n[0] + fi.n[1] + fi.nhub) + edge_bytes_alg<ALGO, W>() * fi.static_deg;
+        st.relax_dynamic_bytes += 24ull * fi.n[3] + edge_bytes_alg<ALGO, W>() * fi.dyn_deg;
+        st.frontier_bytes += 2ull * d->Vpad + 4ull * active;
+        launch_static<ALGO, W>(d, fi, tm);
+        if (fi.n[3]) {
+            Stopwatch ow;
+            run_dynamic<ALGO, W>(d, g, fi, tm, st, gather_ms);
+            overload_wall += ow.ms();
+        }
+        after(st.iterations);
+    }
+}
+
+static void finish_stats(og_run_stats &st, RunTimers &tm, double gather_ms, double overload_wall,
+                         const unsigned long long *ctr, uint64_t upd_bytes) {
+    st.updates = ctr[0] + ctr[1];
+    st.relax_static_bytes += upd_bytes * ctr[0];
+    st.relax_dynamic_bytes += upd_bytes * ctr[1];
+    st.frontier_kernel_ms = sum_ms(tm.front);
+    st.frontier_launches = tm.front.launches;
+    st.relax_static_kernel_ms = sum_ms(tm.stat);
+    st.relax_static_launches = tm.stat.launches;
+    st.relax_dynamic_kernel_ms = sum_ms(tm.dyn);
+    st.relax_dynamic_launches = tm.dyn.launches;
+    st.copy_ms = sum_ms(tm.copy);
+    st.pre_ms = st.frontier_kernel_ms;
+    st.static_ms = st.relax_static_kernel_ms;
+    st.overload_ms = overload_wall;
+    st.overload_move_ms = st.copy_ms;
+    st.gather_ms = gather_ms;
 }
 
 template <int ALGO, bool W>
@@ -681,68 +871,98 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
         hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d->d_val, d->d_flag, src);
     }
     OG_HIP(hipGetLastError());
-
-    for (;;) {
-        {
-            EventSpan span(d, &tm.front, s);
-            hipLaunchKernelGGL(k_frontier<false>, dim3(d->nblk), dim3(kBlock), 0, s, d->d_flag, V, d->Vpad,
-                               d->plan.vsplit, d->d_ptr, d->tune, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
-                               d->d_list, d->d_hub, d->d_doff);
-            hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
-                               d->d_doff);
-            hipLaunchKernelGGL(k_frontier<true>, dim3(d->nblk), dim3(kBlock), 0, s, d->d_flag, V, d->Vpad,
-                               d->plan.vsplit, d->d_ptr, d->tune, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
-                               d->d_list, d->d_hub, d->d_doff);
-            OG_HIP(hipGetLastError());
-        }
-        OG_HIP(hipMemcpyAsync(d->h_info, d->d_info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
-        OG_HIP(hipStreamSynchronize(s));
-        const FrontierInfo fi = *d->h_info;
-        const uint64_t active = (uint64_t)fi.n[0] + fi.n[1] + fi.nhub + fi.n[3] + fi.nzero;
-        if (active == 0) break;
-        if (fi.n[2] > d->hub_cap) raise(OG_EHIP, "hub tile list overflow (%u > %llu)", fi.n[2], (unsigned long long)d->hub_cap);
-        st.iterations++;
-        st.node_sum += active;
-        st.static_node_sum += (uint64_t)fi.n[0] + fi.n[1] + fi.nhub;
-        st.work_edges += fi.static_deg + fi.dyn_deg;
-        st.static_work_edges += fi.static_deg;
-        st.relax_static_bytes += 24ull * ((uint64_t)fi.n[0] + fi.n[1] + fi.nhub) + (W && ALGO == ALGO_SSSP ? 12ull : 8ull) * fi.static_deg;
-        st.relax_dynamic_bytes += 24ull * fi.n[3] + (W && ALGO == ALGO_SSSP ? 12ull : 8ull) * fi.dyn_deg;
-        st.frontier_bytes += 2ull * d->Vpad + 4ull * active;
-        launch_static<ALGO, W>(d, fi, tm);
-        if (fi.n[3]) {
-            Stopwatch ow;
-            run_dynamic<ALGO, W>(d, g, fi, tm, st, gather_ms);
-            overload_wall += ow.ms();
-        }
-    }
+    frontier_loop<ALGO, W>(g, st, tm, gather_ms, overload_wall, [](uint32_t) {});
     OG_HIP(hipStreamSynchronize(s));
     st.total_ms = total.ms();
 
-    // statistics and results
     hipLaunchKernelGGL(k_reach_stats, dim3(std::min<uint64_t>(2048, (V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                        d->d_val, d->d_ptr, V, d->d_counters + 2);
     OG_HIP(hipGetLastError());
     OG_HIP(hipMemcpyAsync(d->h_counters, d->d_counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     if (out) OG_HIP(hipMemcpyAsync(out, d->d_val, V * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     OG_HIP(hipStreamSynchronize(s));
-    st.updates = d->h_counters[0] + d->h_counters[1];
     st.reached = d->h_counters[2];
     st.traversed_edges = d->h_counters[3];
-    st.relax_static_bytes += 8ull * d->h_counters[0];
-    st.relax_dynamic_bytes += 8ull * d->h_counters[1];
-    st.frontier_kernel_ms = sum_ms(tm.front);
-    st.frontier_launches = tm.front.launches;
-    st.relax_static_kernel_ms = sum_ms(tm.stat);
-    st.relax_static_launches = tm.stat.launches;
-    st.relax_dynamic_kernel_ms = sum_ms(tm.dyn);
-    st.relax_dynamic_launches = tm.dyn.launches;
-    st.copy_ms = sum_ms(tm.copy);
-    st.pre_ms = st.frontier_kernel_ms;
-    st.static_ms = st.relax_static_kernel_ms;
-    st.overload_ms = overload_wall;
-    st.overload_move_ms = st.copy_ms;
-    st.gather_ms = gather_ms;
+    st.checksum = d->h_counters[4];
+    finish_stats(st, tm, gather_ms, overload_wall, d->h_counters, 8);
+    if (stats) *stats = st;
+}
+
+// Multi-source BFS: sources in batches of 64; per vertex, u64 masks of the
+// sources that reached it (visit), whose frontier holds it (front) and that
+// were offered to it this round (next); levels as u8 per (vertex, source).
+template <bool W>
+static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t *levels, uint64_t *checksums,
+                      uint64_t *reached, og_run_stats *stats) {
+    DeviceState *d = g->dev;
+    const uint64_t V = d->V;
+    hipStream_t s = d->s_compute;
+    if (!d->d_visit) {
+        d->d_visit = (uint64_t *)d->dalloc(V * 8);
+        d->d_front = (uint64_t *)d->dalloc(V * 8);
+        d->d_next = (uint64_t *)d->dalloc(V * 8);
+        d->d_mlvl = (uint8_t *)d->dalloc(V * kMsWidth);
+        d->d_msrc = (uint32_t *)d->dalloc(kMsWidth * 4);
+        d->d_msum = (unsigned long long *)d->dalloc(3 * kMsWidth * 8);
+    }
+    std::vector<unsigned long long> hsum(3 * kMsWidth);
+    og_run_stats st{};
+    st.approach = (uint32_t)d->approach;
+    RunTimers tm;
+    d->events.reset();
+    double gather_ms = 0, overload_wall = 0;
+    OG_HIP(hipStreamSynchronize(d->s_copy));
+    OG_HIP(hipStreamSynchronize(s));
+    Stopwatch total;
+    OG_HIP(hipMemsetAsync(d->d_counters, 0, 16 * sizeof(unsigned long long), s));
+    double device_ms = 0;
+    for (uint32_t b0 = 0; b0 < n; b0 += kMsWidth) {
+        const uint32_t nb = std::min<uint32_t>(kMsWidth, n - b0);
+        OG_HIP(hipMemsetAsync(d->d_visit, 0, V * 8, s));
+        OG_HIP(hipMemsetAsync(d->d_front, 0, V * 8, s));
+        OG_HIP(hipMemsetAsync(d->d_next, 0, V * 8, s));
+        OG_HIP(hipMemsetAsync(d->d_mlvl, 0, V * kMsWidth, s));
+        OG_HIP(hipMemsetAsync(d->d_flag, 0, d->Vpad, s));
+        OG_HIP(hipMemcpyAsync(d->d_msrc, sources + b0, nb * 4, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_ms_init, dim3(1), dim3(kMsWidth), 0, s, d->d_msrc, nb, d->d_visit, d->d_front,
+                           d->d_flag, d->d_mlvl);
+        OG_HIP(hipGetLastError());
+        uint32_t level = 1;
+        const unsigned cgrid = (unsigned)std::min<uint64_t>(4096, (V + kBlock - 1) / kBlock);
+        frontier_loop<ALGO_MSBFS, W>(g, st, tm, gather_ms, overload_wall, [&](uint32_t) {
+            if (++level > 255) raise(OG_EUNSUPPORTED, "multi-source BFS deeper than 254 levels");
+            hipLaunchKernelGGL(k_ms_commit, dim3(cgrid), dim3(kBlock), 0, s, d->d_flag, V, d->d_visit, d->d_front,
+                               d->d_next, d->d_mlvl, (uint8_t)level);
+            OG_HIP(hipGetLastError());
+        });
+        OG_HIP(hipMemsetAsync(d->d_msum, 0, 3 * kMsWidth * 8, s));
+        hipLaunchKernelGGL(k_ms_digest, dim3((unsigned)std::min<uint64_t>(1024, (V + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, d->d_mlvl, d->d_ptr, V, d->d_msum);
+        OG_HIP(hipGetLastError());
+        OG_HIP(hipMemcpyAsync(hsum.data(), d->d_msum, 3 * kMsWidth * 8, hipMemcpyDeviceToHost, s));
+        OG_HIP(hipStreamSynchronize(s));
+        device_ms = total.ms();
+        for (uint32_t j = 0; j < nb; ++j) {
+            if (checksums) checksums[b0 + j] = hsum[j];
+            if (reached) reached[b0 + j] = hsum[kMsWidth + j];
+            st.reached += hsum[kMsWidth + j];
+            st.traversed_edges += hsum[2 * kMsWidth + j];
+            st.checksum += hsum[j];
+        }
+        if (levels) {
+            if (!d->d_extract) d->d_extract = (uint32_t *)d->dalloc(V * 4);
+            for (uint32_t j = 0; j < nb; ++j) {
+                hipLaunchKernelGGL(k_ms_extract, dim3(fill_grid(V)), dim3(256), 0, s, d->d_mlvl, V, j, d->d_extract);
+                OG_HIP(hipGetLastError());
+                OG_HIP(hipMemcpyAsync(levels + (uint64_t)(b0 + j) * V, d->d_extract, V * 4, hipMemcpyDeviceToHost, s));
+            }
+            OG_HIP(hipStreamSynchronize(s));
+        }
+    }
+    st.total_ms = device_ms;
+    OG_HIP(hipMemcpyAsync(d->h_counters, d->d_counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    OG_HIP(hipStreamSynchronize(s));
+    finish_stats(st, tm, gather_ms, overload_wall, d->h_counters, 9);
     if (stats) *stats = st;
 }
 
@@ -811,4 +1031,35 @@ extern "C" og_status og_sssp(og_graph *g, uint32_t source, uint32_t *dist, og_ru
 
 extern "C" og_status og_cc(og_graph *g, uint32_t *labels, og_run_stats *stats) {
     return run_algo(g, ALGO_CC, 0, labels, stats);
+}
+
+extern "C" og_status og_msbfs(og_graph *g, const uint32_t *sources, uint32_t num_sources, uint32_t *levels,
+                              uint64_t *checksums, uint64_t *reached, og_run_stats *stats) {
+    clear_error();
+    if (!g || (!sources && num_sources)) return fail(OG_EINVAL, "graph/sources is NULL");
+    if (!g->dev) return fail(OG_ESTATE, "og_prepare has not been called for this graph");
+    if (g->kind == OG_KIND_PULL) return fail(OG_EUNSUPPORTED, "pull-layout graphs serve PageRank only");
+    for (uint32_t i = 0; i < num_sources; ++i)
+        if (sources[i] >= g->V) return fail(OG_EINVAL, "source %u >= V", sources[i]);
+    int expect = 0;
+    if (!g->busy.compare_exchange_strong(expect, 1)) return fail(OG_EBUSY, "graph is running another call");
+    og_status st = guard([&]() -> og_status {
+        OG_HIP(hipSetDevice(g->dev->device));
+        if (g->kind == OG_KIND_WEIGHTED) run_msbfs<true>(g, sources, num_sources, levels, checksums, reached, stats);
+        else run_msbfs<false>(g, sources, num_sources, levels, checksums, reached, stats);
+        return OG_OK;
+    });
+    g->busy.store(0);
+    return st;
+}
+
+// Host digest of a result array, equal to the one the device computes:
+// sum (mod 2^64) of mix64(v << 32 | value[v]) over v with value != UNREACHED.
+extern "C" og_status og_levels_checksum(const uint32_t *values, uint64_t num_vertices, uint64_t *checksum) {
+    if (!values || !checksum) return fail(OG_EINVAL, "null argument");
+    uint64_t h = 0;
+    for (uint64_t v = 0; v < num_vertices; ++v)
+        if (values[v] != OG_UNREACHED) h += mix64((v << 32) | values[v]);
+    *checksum = h;
+    return OG_OK;
 }

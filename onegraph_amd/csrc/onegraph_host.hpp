@@ -1,0 +1,45 @@
+// C++ host layer above the C ABI, mirroring the reference's driver functions
+// (same names and argument meaning):
+//   bfs_opt(path, sourceNode, adviseRate)   BFS/oneGraph_BFS/CalculateOpt.h:9
+//   sssp_opt(path, sourceNode, adviseRate)  SSSP/oneGraph/CalculateOpt.h:12
+//   cc_opt(path, adviseRate)                CC/onegraph/CalculateOpt.h:6
+//   pr_opt(path, adviseRate)                PR/OneGraph/CalculateOpt.h:13
+// Differences: they return an og_status instead of exiting, honour the
+// source (the BFS reference replaces it with findbignode, CalculateOpt.h:
+// 26-32,60, kept behind Options::bignode_runs), take the approach and the
+// explicit edge budget from Options, and can write the result array.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "onegraph.h"
+
+namespace onegraph {
+
+struct Options {
+    og_approach approach = OG_ASCETIC;
+    uint64_t edge_budget_bytes = 0;  // 0: reference default budget
+    double budget_fraction = 0;      // > 0: budget = fraction * CSR edge bytes
+    uint32_t bignode_runs = 0;       // > 0: sources from findbignode, like the reference BFS
+    uint32_t repeat = 1;
+    bool json = false;
+    bool pr_fp64 = false;
+    double pr_tolerance = 1e-7;
+    uint32_t pr_max_iterations = 100;
+    std::string out_path;            // optional binary dump of the result array
+    std::string generate;            // "rmat:SCALE:EF:SEED[:WSEED]" or "er:N:AVG:SEED" instead of -f
+};
+
+// Prints the reference's labelled lines (TimeRecord "<name> time is X ms",
+// nodeSum, staticnodeSum, "move overload size") and optionally one JSON line.
+void print_report(const char *algo, const og_run_stats &st, const og_plan_info &plan, bool json, FILE *f = stdout);
+
+og_status bfs_opt(const std::string &path, uint32_t sourceNode, float adviseRate, const Options &opt = Options());
+og_status sssp_opt(const std::string &path, uint32_t sourceNode, double adviseRate, const Options &opt = Options());
+og_status cc_opt(const std::string &path, float adviseRate, const Options &opt = Options());
+og_status pr_opt(const std::string &path, double adviseRate, const Options &opt = Options());
+
+}  // namespace onegraph

@@ -1,0 +1,81 @@
+"""The drop-in command line: reference flag spellings, errors as exit codes,
+and (on the GPU) result dumps equal to the oracle."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import onegraph_amd as og
+import oracle
+
+
+def run(*args, timeout=120):
+    return subprocess.run([og.CLI_PATH, *args], capture_output=True, text=True, timeout=timeout)
+
+
+def test_usage_and_errors(tmp_path):
+    r = run("--help")
+    assert r.returncode == 0 and "--type" in r.stdout
+    assert run("--bogus").returncode == 2
+    assert run("--type", "bfs").returncode == 2          # no -f
+    assert run("--type", "bfs", "-f").returncode == 2    # flag without value
+    assert run("--type", "xx", "-f", "x").returncode == 2
+    r = run("--type", "bfs", "-f", str(tmp_path / "missing.bcsr"))
+    assert r.returncode == 1 and "OG_EIO" not in r.stdout  # message goes to stderr
+    assert "cannot open" in r.stderr
+
+
+def test_generate_and_save(tmp_path):
+    f = str(tmp_path / "g.bwcsr")
+    r = run("-type", "sssp", "--generate", "rmat:9:8:3:4", "--save", f)
+    assert r.returncode == 0, r.stderr
+    g = og.Graph.load(f, og.KIND_WEIGHTED)
+    ref = og.Graph.rmat(9, 8, seed=3, weight_seed=4)
+    assert np.array_equal(g.export()[1], ref.export()[1]) and np.array_equal(g.export()[2], ref.export()[2])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("algo,flags", [
+    ("bfs", ["--advisek", "0.8", "--budget-fraction", "0.25"]),
+    ("sssp", ["-adviseK", "0.8", "--budget-fraction", "0.25"]),
+    ("cc", ["--adviseK", "0.5", "--budget-fraction", "0.1", "--approach", "subway"]),
+    ("bfs", ["--approach", "usm", "--method", "1"]),
+])
+def test_cli_runs_match_oracle(gpu, tmp_path, algo, flags):
+    f = str(tmp_path / "g.csr")
+    kind = og.KIND_WEIGHTED if algo == "sssp" else og.KIND_PLAIN
+    g = og.Graph.rmat(12, 8, seed=5, weight_seed=6 if kind == og.KIND_WEIGHTED else 0)
+    g.save(f)
+    ptr, to, w, _ = g.export()
+    out = str(tmp_path / "out.bin")
+    src = int(np.flatnonzero(np.diff(np.append(ptr.astype(np.int64), len(to))) > 0)[3])
+    r = run("--type", algo, "-f", f, "-r", str(src), "--out", out, "--json", *flags)
+    assert r.returncode == 0, r.stderr
+    assert "totalProcess time is" in r.stdout and "nodeSum :" in r.stdout
+    got = np.fromfile(out, np.uint32)
+    if algo == "bfs":
+        ref = oracle.bfs(ptr, to, src)[0]
+    elif algo == "sssp":
+        ref = oracle.sssp(ptr, to, w, src)
+    else:
+        ref = oracle.cc(ptr, to)[0]
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.gpu
+def test_cli_bignode_and_pagerank(gpu, tmp_path):
+    f = str(tmp_path / "g.bcsr")
+    g = og.Graph.rmat(14, 16, seed=9)
+    g.save(f)
+    r = run("--type", "bfs", "-f", f, "--bignode-runs", "3")
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.count("testIndex") == 3
+    pf = str(tmp_path / "g.prcsr")
+    p = g.to_pull()
+    p.save(pf)
+    out = str(tmp_path / "pr.bin")
+    r = run("--type", "pr", "-f", pf, "--advisek", "0.8", "--budget-fraction", "0.25", "--fp64", "--out", out)
+    assert r.returncode == 0, r.stderr
+    pptr, pin, _, od = p.export()
+    ref, _, _ = oracle.pagerank(od, pptr, pin)
+    assert np.abs(np.fromfile(out, np.float64) - ref).max() < 1e-9

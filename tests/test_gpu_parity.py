@@ -146,3 +146,28 @@ def test_errors(gpu):
         g.sssp(0)
     assert e.value.name == "OG_EUNSUPPORTED"
     g.close()
+
+
+@pytest.mark.parametrize("approach,frac,k", [("ascetic", 1.0, 1.0), ("ascetic", 0.25, 0.8), ("ascetic", 1 / 64, 0.5),
+                                              ("subway", 0.25, 0.0), ("usm", 0.0, 0.0)])
+def test_msbfs(gpu, rmat_w, approach, frac, k):
+    ptr, to, w, _ = rmat_w.export()
+    _say("msbfs rmat13", approach, frac, k)
+    _prepare(rmat_w, approach, frac, k, 8)
+    deg = np.diff(np.append(ptr.astype(np.int64), len(to)))
+    rng = np.random.default_rng(41)
+    srcs = rng.choice(np.flatnonzero(deg > 0), 100, replace=False).astype(np.uint32)
+    srcs[5] = srcs[6]  # a repeated source inside one batch
+    srcs[70] = 0       # a source with out-degree 0 in the second batch
+    lv, cs, rc, st = rmat_w.msbfs(srcs)
+    for i, s in enumerate(srcs):
+        ref, _, _ = oracle.bfs(ptr, to, int(s))
+        assert np.array_equal(lv[i], ref), (approach, i, int(s))
+        assert cs[i] == og.levels_checksum(ref)
+        assert rc[i] == int((ref != og.UNREACHED).sum())
+    # single-source runs agree through the device digest alone
+    for s in srcs[:3]:
+        _, st1 = rmat_w.bfs(int(s), want_values=False)
+        assert st1["checksum"] == cs[list(srcs).index(s)]
+    _, cs2, _, _ = rmat_w.msbfs(srcs[:64], want_levels=False)
+    assert np.array_equal(cs2, cs[:64])

@@ -300,3 +300,14 @@ def test_no_device_errors_are_statuses():
     with pytest.raises(og.OgError) as e:
         g.bfs(0)
     assert e.value.name == "OG_ESTATE"
+
+
+def test_levels_checksum_matches_restatement():
+    rng = np.random.default_rng(2)
+    x = rng.integers(1, 40, 5000).astype(np.uint32)
+    x[rng.random(5000) < 0.3] = og.UNREACHED
+    keep = x != og.UNREACHED
+    v = np.arange(5000, dtype=np.uint64)[keep]
+    with np.errstate(over="ignore"):
+        ref = int(mix64((v << np.uint64(32)) | x[keep].astype(np.uint64)).sum(dtype=np.uint64))
+    assert og.levels_checksum(x) == ref
