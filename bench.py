@@ -72,12 +72,238 @@ def cpu_info():
     return model
 
 
+def base_record(metric_value, unit, steps, warmup, ms, dtype, config, higher=True):
+    return {"metric": METRIC, "value": metric_value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": round(ms, 3), "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def roofline_of(stats_list):
+    agg = {}
+    for key in ["relax_static_kernel_ms", "relax_static_launches", "relax_static_bytes", "relax_dynamic_kernel_ms",
+                "relax_dynamic_launches", "relax_dynamic_bytes", "frontier_kernel_ms", "frontier_launches",
+                "frontier_bytes", "copy_ms", "h2d_bytes"]:
+        agg[key] = sum(s[key] for s in stats_list)
+    kt = kernel_table(agg)
+    roof = None
+    if kt:
+        dom = max(kt, key=lambda c: kt[c]["ms"])
+        r = kt[dom]
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(r["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_us": round(r["avg_us"], 3),
+                "bytes_per_launch": round(r["bytes_per_launch"], 1)}
+    link = None
+    if agg["copy_ms"] > 0:
+        gbs = agg["h2d_bytes"] / (agg["copy_ms"] * 1e-3) / 1e9
+        link = {"achieved": round(gbs, 2), "peak": LINK_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LINK_PEAK_GBS, 4)}
+    return roof, link, kt
+
+
+def timed(fn, steps, warmup, label):
+    for i in range(warmup):
+        r = fn()
+        log(f"{label}: warmup {i} done")
+    stats = []
+    t = time.perf_counter()
+    for _ in range(steps):
+        stats.append(fn())
+    wall = time.perf_counter() - t
+    log(f"{label}: {steps} steps in {wall:.3f}s")
+    return stats, wall
+
+
+def wl_bfs_hbm(args):
+    """Plain BFS, whole graph in HBM (advisek 1.0), R-MAT scale 20 ef 16 seed
+    1001, source = lowest vertex with out-degree > 0; bit-exact vs cpu BFS."""
+    import onegraph_amd as og
+    import oracle
+
+    g = og.Graph.rmat(args.scale, args.edge_factor, seed=1001)
+    ptr, cols, _, _ = g.export()
+    V, E = len(ptr), len(cols)
+    deg = np.diff(np.append(ptr.astype(np.int64), E))
+    src = int(np.flatnonzero(deg > 0)[0])
+    plan = g.prepare("ascetic", edge_budget_bytes=E * 4, advisek=1.0)
+    stats, wall = timed(lambda: g.bfs(src, want_values=False)[1], args.steps, args.warmup, "bfs-hbm")
+    lv, _ = g.bfs(src)
+    ms = 1e3 * wall / args.steps
+    trav = stats[-1]["traversed_edges"]
+    t = time.perf_counter()
+    ref, _, _ = oracle.bfs(ptr, cols, src)
+    cpu_s = time.perf_counter() - t
+    roof, link, kt = roofline_of(stats)
+    out = base_record(round(trav / (ms * 1e-3) / 1e9, 4), "GTEPS", args.steps, args.warmup, ms, "u32",
+                      {"workload": f"bfs whole graph in HBM rmat{args.scale} ef{args.edge_factor}", "V": V, "E": E,
+                       "source": src, "advisek": 1.0, "approach": "ascetic", "static_edges": plan["static_edges"]})
+    out.update({"roofline": roof, "kernels": kt, "iterations": stats[-1]["iterations"],
+                "parity": {"bit_exact": bool(np.array_equal(lv, ref)), "reached": int((ref != og.UNREACHED).sum())},
+                "cpu_baseline": {"value": round(trav / cpu_s / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+                                 "sample": f"oracle level-synchronous BFS (runCPUBFS restated), full run {cpu_s:.2f} s",
+                                 "cpu": cpu_info()}})
+    print(json.dumps(out), flush=True)
+
+
+def er_graph(args):
+    import onegraph_amd as og
+
+    t0 = time.time()
+    g = og.Graph.er(1 << args.scale, 32, seed=1003)
+    log(f"ER n=2^{args.scale} E={g.num_edges} generated in {time.time() - t0:.1f}s")
+    return g
+
+
+def wl_cc_ooc(args):
+    """CC out-of-memory: ER n=2^25 avg degree 32 (seed 1003), Ascetic
+    advisek 0.8, edge budget 25% of the CSR edge bytes; bit-exact vs serial."""
+    import onegraph_amd as og
+    import oracle
+
+    g = er_graph(args)
+    ptr, cols, _, _ = g.export()
+    V, E = len(ptr), len(cols)
+    plan = g.prepare("ascetic", edge_budget_bytes=E, advisek=0.8)
+    stats, wall = timed(lambda: g.cc(want_values=False)[1], args.steps, args.warmup, "cc-ooc")
+    lab, _ = g.cc()
+    ms = 1e3 * wall / args.steps
+    cmp = {}
+    if args.compare:
+        for ap_name in ["subway", "usm"]:
+            g.prepare(ap_name, edge_budget_bytes=E if ap_name == "subway" else 0)
+            s2, w2 = timed(lambda: g.cc(want_values=False)[1], 1, 0, f"cc-{ap_name}")
+            cmp[ap_name] = {"ms": round(1e3 * w2, 2), "gteps": round(E / w2 / 1e9, 4), "h2d_bytes": s2[0]["h2d_bytes"]}
+    t = time.perf_counter()
+    ref, _ = oracle.cc(ptr, cols)
+    cpu_s = time.perf_counter() - t
+    roof, link, kt = roofline_of(stats)
+    out = base_record(round(E / (ms * 1e-3) / 1e9, 4), "GTEPS", args.steps, args.warmup, ms, "u32",
+                      {"workload": f"cc out-of-memory er n=2^{args.scale} avg 32", "V": V, "E": E,
+                       "edge_budget_bytes": E, "advisek": 0.8, "approach": "ascetic",
+                       "static_edges": plan["static_edges"], "staging_edges": plan["staging_edges"]})
+    out.update({"roofline": roof, "host_link": link, "kernels": kt, "iterations": stats[-1]["iterations"],
+                "h2d_bytes_per_step": stats[-1]["h2d_bytes"], "compare": cmp,
+                "parity": {"bit_exact": bool(np.array_equal(lab, ref)),
+                           "components": int(len(np.unique(ref)))},
+                "cpu_baseline": {"value": round(E / cpu_s / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+                                 "sample": f"oracle serial CC (CC/serial restated), full run {cpu_s:.2f} s",
+                                 "cpu": cpu_info()}})
+    print(json.dumps(out), flush=True)
+
+
+def wl_pr_ooc(args):
+    """PageRank out-of-memory on the same ER graph (pull layout), Ascetic
+    advisek 0.8, budget 25%; FP32-Kahan and FP64, errors vs the FP64 oracle."""
+    import onegraph_amd as og
+    import oracle
+
+    g0 = er_graph(args)
+    g = g0.to_pull()
+    g0.close()
+    ptr, cols, _, od = g.export()
+    V, E = len(ptr), len(cols)
+    plan = g.prepare("ascetic", edge_budget_bytes=E, advisek=0.8)
+    res = {}
+    threads = min(16, os.cpu_count() or 1)
+    t = time.perf_counter()
+    ref, ref_it, ref_l1 = oracle.pagerank(od, ptr, cols, threads=threads)
+    cpu_s = time.perf_counter() - t
+    rec = None
+    for fp64 in (False, True):
+        name = "fp64" if fp64 else "fp32"
+        stats, wall = timed(lambda: g.pagerank(fp64=fp64, want_values=False)[1], args.steps, args.warmup, f"pr-{name}")
+        r, st = g.pagerank(fp64=fp64)
+        ms = 1e3 * wall / args.steps
+        it = stats[-1]["iterations"]
+        err = np.abs(r.astype(np.float64) - ref)
+        roof, link, kt = roofline_of([s["run"] for s in stats])
+        res[name] = {"ms": round(ms, 3), "iterations": it, "gteps": round(E * it / (ms * 1e-3) / 1e9, 4),
+                     "max_abs": float(err.max()), "max_rel": float((err / ref).max()), "l1": float(err.sum()),
+                     "final_l1": stats[-1]["final_l1"], "roofline": roof, "host_link": link, "kernels": kt,
+                     "h2d_bytes_per_step": stats[-1]["run"]["h2d_bytes"]}
+        if not fp64:
+            rec = base_record(res[name]["gteps"], "GTEPS", args.steps, args.warmup, ms, "f32",
+                              {"workload": f"pagerank out-of-memory er n=2^{args.scale} avg 32", "V": V, "E": E,
+                               "edge_budget_bytes": E, "advisek": 0.8, "approach": "ascetic",
+                               "static_edges": plan["static_edges"]})
+            rec.update({"roofline": roof, "host_link": link})
+    rec.update({"pagerank": res,
+                "cpu_baseline": {"value": round(E * ref_it / cpu_s / 1e9, 5), "unit": "GTEPS", "cores": threads,
+                                 "kind": "port",
+                                 "sample": f"oracle FP64 PageRank on {threads} threads, {ref_it} iterations, {cpu_s:.2f} s",
+                                 "cpu": cpu_info()},
+                "parity": {"tolerance_max_abs": 1e-6, "fp32_ok": res["fp32"]["max_abs"] < 1e-6,
+                           "fp64_ok": res["fp64"]["max_abs"] < 1e-6}})
+    print(json.dumps(rec), flush=True)
+
+
+def wl_msbfs(args):
+    """Batched multi-source BFS: R-MAT scale 26 ef 16 (seed 1004), edge budget
+    50%, Ascetic advisek 0.8, 1024 sources (seed 3004) in batches of 64,
+    against single-source launches (a timed sample, extrapolated)."""
+    import onegraph_amd as og
+    import oracle
+
+    t0 = time.time()
+    g = og.Graph.rmat(args.scale, args.edge_factor, seed=1004)
+    V, E = g.num_vertices, g.num_edges
+    log(f"rmat scale {args.scale} E={E} generated in {time.time() - t0:.1f}s")
+    ptr, cols, _, _ = g.export()
+    deg = np.diff(np.append(ptr.astype(np.int64), E))
+    srcs = np.random.default_rng(3004).choice(np.flatnonzero(deg > 0), args.sources, replace=False).astype(np.uint32)
+    plan = g.prepare("ascetic", edge_budget_bytes=E * 4 // 2, advisek=0.8)
+    log(f"plan {plan}")
+    if args.warmup:
+        g.msbfs(srcs[:64], want_levels=False)
+    t = time.perf_counter()
+    _, cs, rc, st = g.msbfs(srcs, want_levels=False)
+    ms_wall = time.perf_counter() - t
+    log(f"msbfs {len(srcs)} sources in {ms_wall:.2f}s")
+    sample = srcs[: args.plain_sample]
+    plain = []
+    t = time.perf_counter()
+    for s in sample:
+        _, s1 = g.bfs(int(s), want_values=False)
+        plain.append(s1)
+    plain_wall = time.perf_counter() - t
+    per_plain = plain_wall / len(sample)
+    digests_equal = all(p["checksum"] == int(cs[i]) for i, p in enumerate(plain))
+    # full oracle check of two sources
+    ok = True
+    for i in range(2):
+        ref, _, _ = oracle.bfs(ptr, cols, int(srcs[i]))
+        ok &= og.levels_checksum(ref) == int(cs[i])
+    trav = st["traversed_edges"]
+    qps = len(srcs) / ms_wall
+    out = base_record(round(trav / ms_wall / 1e9, 4), "GTEPS", 1, args.warmup, 1e3 * ms_wall, "u64-mask",
+                      {"workload": f"multi-source bfs rmat{args.scale} ef{args.edge_factor} {len(srcs)} sources",
+                       "V": V, "E": E, "edge_budget_bytes": E * 4 // 2, "advisek": 0.8, "approach": "ascetic",
+                       "batch": 64})
+    roof, link, kt = roofline_of([st])
+    out.update({"queries_per_s": round(qps, 2), "plain_queries_per_s": round(1 / per_plain, 3),
+                "speedup_vs_plain": round(qps * per_plain, 2),
+                "plain_sample": len(sample), "plain_gteps": round(np.mean([p["traversed_edges"] for p in plain]) /
+                                                                   per_plain / 1e9, 4),
+                "roofline": roof, "host_link": link, "kernels": kt, "iterations": st["iterations"],
+                "h2d_bytes": st["h2d_bytes"],
+                "parity": {"digests_equal_plain_sample": digests_equal, "oracle_two_sources": bool(ok)}})
+    print(json.dumps(out), flush=True)
+
+
+def wl_serve(args):
+    from onegraph_amd import serve_bench
+
+    serve_bench.run(args)
+
+
+WORKLOADS = {"bfs-hbm": wl_bfs_hbm, "cc-ooc": wl_cc_ooc, "pr-ooc": wl_pr_ooc, "msbfs": wl_msbfs, "serve": wl_serve}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--scale", type=int, default=None,
+                    help="log2 V (default: 24 sssp-ooc, 20 bfs-hbm, 25 cc/pr-ooc, 26 msbfs)")
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--budget", type=float, default=0.25, help="edge budget / CSR edge bytes")
     ap.add_argument("--advisek", type=float, default=0.8)
@@ -86,7 +312,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--workload", default="sssp-ooc",
+                    choices=["sssp-ooc", "bfs-hbm", "cc-ooc", "pr-ooc", "msbfs", "serve"],
+                    help="sssp-ooc is the headline line; the others measure the remaining configurations")
+    ap.add_argument("--sources", type=int, default=1024, help="msbfs: number of sources")
+    ap.add_argument("--plain-sample", type=int, default=32, help="msbfs: single-source launches timed")
+    ap.add_argument("--rates", default="200,1000,5000", help="serve: arrival rates (queries/s)")
+    ap.add_argument("--duration", type=float, default=4.0, help="serve: seconds of arrivals per rate")
     args = ap.parse_args()
+    if args.scale is None:
+        args.scale = {"sssp-ooc": 24, "bfs-hbm": 20, "cc-ooc": 25, "pr-ooc": 25, "msbfs": 26, "serve": 0}[args.workload]
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); --gpus {args.gpus} refused",
               file=sys.stderr)
@@ -110,6 +345,9 @@ def main():
     if og.device_count() < 1:
         print("bench.py needs a GPU: no HIP device is visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload != "sssp-ooc":
+        WORKLOADS[args.workload](args)
+        return
 
     t0 = time.time()
     g = og.Graph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=1002, weight_seed=2002)

@@ -38,6 +38,8 @@ def lib():
         L.og_oracle_pagerank.argtypes = [C.c_uint64, C.c_uint64, _u32p, _u64p, _u32p, C.c_double, C.c_double,
                                          C.c_uint32, _f64p, _u32p, _f64p]
         L.og_oracle_pagerank.restype = None
+        L.og_oracle_pagerank_par.argtypes = L.og_oracle_pagerank.argtypes + [C.c_int]
+        L.og_oracle_pagerank_par.restype = None
         _lib = L
     return _lib
 
@@ -90,14 +92,19 @@ def cc(ptr, cols):
     return out, n
 
 
-def pagerank(out_degree, ptr, in_cols, damping=0.85, tol=1e-7, max_it=100):
-    """FP64 normalised PageRank over the pull layout; returns (ranks, iterations, l1)."""
+def pagerank(out_degree, ptr, in_cols, damping=0.85, tol=1e-7, max_it=100, threads=1):
+    """FP64 normalised PageRank over the pull layout; returns (ranks, iterations, l1).
+    threads > 1 splits the vertex loops (same per-vertex summation order)."""
     ptr, in_cols = _prep(ptr, in_cols)
     od = np.ascontiguousarray(out_degree, dtype=np.uint32)
     V, E = len(ptr), len(in_cols)
     out = np.empty(V, np.float64)
     it = C.c_uint32()
     l1 = C.c_double()
-    lib().og_oracle_pagerank(V, E, _p32(od), _p64(ptr), _p32(in_cols if E else np.zeros(1, np.uint32)),
-                             damping, tol, max_it, out.ctypes.data_as(_f64p), C.byref(it), C.byref(l1))
+    args = (V, E, _p32(od), _p64(ptr), _p32(in_cols if E else np.zeros(1, np.uint32)),
+            damping, tol, max_it, out.ctypes.data_as(_f64p), C.byref(it), C.byref(l1))
+    if threads > 1:
+        lib().og_oracle_pagerank_par(*args, threads)
+    else:
+        lib().og_oracle_pagerank(*args)
     return out, it.value, l1.value

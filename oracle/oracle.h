@@ -17,6 +17,9 @@ uint64_t og_oracle_cc(uint64_t V, uint64_t E, const uint64_t *ptr, const uint32_
 void og_oracle_pagerank(uint64_t V, uint64_t E, const uint32_t *outdeg, const uint64_t *ptr, const uint32_t *in_cols,
                         double damping, double tol, uint32_t max_it, double *rank, uint32_t *iterations,
                         double *final_l1);
+void og_oracle_pagerank_par(uint64_t V, uint64_t E, const uint32_t *outdeg, const uint64_t *ptr,
+                            const uint32_t *in_cols, double damping, double tol, uint32_t max_it, double *rank,
+                            uint32_t *iterations, double *final_l1, int threads);
 
 #ifdef __cplusplus
 }
