@@ -288,10 +288,137 @@ def wl_msbfs(args):
     print(json.dumps(out), flush=True)
 
 
-def wl_serve(args):
-    from onegraph_amd import serve_bench
+def _pct(xs, p):
+    return float(np.percentile(np.asarray(xs), p)) if len(xs) else None
 
-    serve_bench.run(args)
+
+def wl_serve(args):
+    """Served BFS/SSSP queries on three device-resident R-MAT graphs (scales
+    22, 24, 26; ef 16; weights [1,63]; seeds 1005+i / 2005+i).  Poisson
+    arrivals at each rate (seed 3005), 70% BFS / 30% SSSP, graph uniform,
+    source uniform over vertices with out-degree > 0.  The server batches
+    continuously; the plain path serves one query at a time with og_bfs /
+    og_sssp, its latency obtained by replaying the same arrival trace through
+    a FIFO queue with service times measured on a sample of its queries."""
+    import threading
+
+    import onegraph_amd as og
+
+    scales = [int(x) for x in (args.serve_scales.split(","))]
+    graphs, cands = [], []
+    for i, sc in enumerate(scales):
+        t0 = time.time()
+        g = og.Graph.rmat(sc, args.edge_factor, seed=1005 + i, weight_seed=2005 + i)
+        g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0)
+        ptr, _, _, _ = g.export()
+        deg = np.diff(np.append(ptr.astype(np.int64), g.num_edges))
+        cands.append(np.flatnonzero(deg > 0))
+        graphs.append(g)
+        log(f"graph {i}: rmat scale {sc} V={g.num_vertices} E={g.num_edges} ready in {time.time() - t0:.1f}s")
+    rates = [float(r) for r in args.rates.split(",")]
+
+    def trace(rate, seed):
+        rng = np.random.default_rng(seed)
+        n = max(1, int(rate * args.duration))
+        t = np.cumsum(rng.exponential(1.0 / rate, n))
+        gi = rng.integers(0, len(graphs), n)
+        kind = (rng.random(n) >= 0.7).astype(int)  # 70% BFS (0), 30% SSSP (1)
+        src = np.array([int(cands[g][rng.integers(len(cands[g]))]) for g in gi])
+        return t, gi, kind, src
+
+    # plain one-at-a-time path: service times on a sample, per (graph, kind)
+    service = {}
+    plain_ck = {}
+    rng = np.random.default_rng(3005)
+    for gi, g in enumerate(graphs):
+        for kind in (0, 1):
+            ts = []
+            for _ in range(args.plain_sample // 2 if kind else args.plain_sample):
+                s = int(cands[gi][rng.integers(len(cands[gi]))])
+                t = time.perf_counter()
+                _, st = g.bfs(s, want_values=False) if kind == 0 else g.sssp(s, want_values=False)
+                ts.append(time.perf_counter() - t)
+                plain_ck[(gi, kind, s)] = st["checksum"]
+            service[(gi, kind)] = np.array(ts)
+            log(f"plain g{gi} {'sssp' if kind else 'bfs'}: mean {1e3 * np.mean(ts):.2f} ms")
+
+    per_slot = sum(g.num_vertices * 5 for g in graphs)
+    budget = int(args.serve_budget_gib * (1 << 30)) if args.serve_budget_gib else 64 * per_slot
+    results = {}
+    verify = []
+    for ri, rate in enumerate(rates):
+        t_arr, gis, kinds, srcs = trace(rate, 3005 + ri)
+        # plain: FIFO replay with sampled service times
+        srng = np.random.default_rng(7 + ri)
+        free_at, plat = 0.0, []
+        for k in range(len(t_arr)):
+            sv = service[(int(gis[k]), int(kinds[k]))]
+            start = max(free_at, t_arr[k])
+            free_at = start + float(sv[srng.integers(len(sv))])
+            plat.append(1e3 * (free_at - t_arr[k]))
+        plain_qps = len(t_arr) / max(free_at, t_arr[-1])
+        srv = og.Server(graphs, state_budget_bytes=budget)
+        info0 = srv.info()
+        lat, done = [], {}
+        submit_at = {}
+        t0 = time.perf_counter()
+
+        def submitter():
+            for k in range(len(t_arr)):
+                dt = t0 + t_arr[k] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                qid = srv.submit(int(gis[k]), int(kinds[k]), int(srcs[k]))
+                submit_at[qid] = (k, time.perf_counter() - t0 - t_arr[k])
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        last = time.perf_counter()
+        while len(done) < len(t_arr):
+            for r in srv.poll(4096, timeout_ms=100):
+                done[r["id"]] = r
+                last = time.perf_counter()
+            if time.perf_counter() - last > 120:
+                raise RuntimeError("server stalled")
+        th.join()
+        wall = last - t0
+        info = srv.info()
+        srv.close()
+        bad = 0
+        for qid, r in done.items():
+            k, lag = submit_at[qid]
+            lat.append(r["latency_ms"] + 1e3 * lag)
+            if r["status"] != 0:
+                bad += 1
+        # spot-check against the single-query path (graphs are free again)
+        ids = list(done)
+        pick = np.random.default_rng(11 + ri).choice(len(ids), min(args.verify, len(ids)), replace=False)
+        mism = 0
+        for p in pick:
+            r = done[ids[p]]
+            g = graphs[r["graph"]]
+            _, st = g.bfs(r["source"], want_values=False) if r["kind"] == 0 else g.sssp(r["source"], want_values=False)
+            mism += st["checksum"] != r["checksum"]
+        verify.append({"rate": rate, "checked": int(len(pick)), "mismatches": int(mism)})
+        qps = len(t_arr) / wall
+        results[str(int(rate))] = {
+            "queries": len(t_arr), "served_qps": round(qps, 1), "p50_ms": round(_pct(lat, 50), 3),
+            "p99_ms": round(_pct(lat, 99), 3), "plain_qps": round(plain_qps, 1),
+            "plain_p50_ms": round(_pct(plat, 50), 3), "plain_p99_ms": round(_pct(plat, 99), 3),
+            "speedup_qps": round(qps / plain_qps, 2), "errors": bad, "rounds": info["rounds"],
+            "slots": {"bfs": info0["bfs_slots"], "sssp": info0["sssp_slots"]}, "state_bytes": info0["state_bytes"]}
+        log(f"rate {rate}: {results[str(int(rate))]}")
+    best = max(results.values(), key=lambda x: x["served_qps"] if x["p99_ms"] <= 50 else 0)
+    out = base_record(best["served_qps"], "queries/s", len(rates), 0, 1e3 / max(best["served_qps"], 1e-9),
+                      "u32", {"workload": "served bfs/sssp queries, rmat " + "/".join(map(str, scales)) + " ef16",
+                              "rates": rates, "mix": "70% bfs / 30% sssp", "duration_s": args.duration,
+                              "state_budget_bytes": budget})
+    out.update({"rates": results, "parity": verify,
+                "plain_service_ms": {f"g{k[0]}_{'sssp' if k[1] else 'bfs'}": round(1e3 * float(np.mean(v)), 3)
+                                     for k, v in service.items()}})
+    print(json.dumps(out), flush=True)
+    for g in graphs:
+        g.close()
 
 
 WORKLOADS = {"bfs-hbm": wl_bfs_hbm, "cc-ooc": wl_cc_ooc, "pr-ooc": wl_pr_ooc, "msbfs": wl_msbfs, "serve": wl_serve}
@@ -318,7 +445,10 @@ def main():
     ap.add_argument("--sources", type=int, default=1024, help="msbfs: number of sources")
     ap.add_argument("--plain-sample", type=int, default=32, help="msbfs: single-source launches timed")
     ap.add_argument("--rates", default="200,1000,5000", help="serve: arrival rates (queries/s)")
-    ap.add_argument("--duration", type=float, default=4.0, help="serve: seconds of arrivals per rate")
+    ap.add_argument("--duration", type=float, default=3.0, help="serve: seconds of arrivals per rate")
+    ap.add_argument("--serve-scales", default="22,24,26", help="serve: R-MAT scales of the graphs")
+    ap.add_argument("--serve-budget-gib", type=float, default=0.0, help="serve: per-query state budget")
+    ap.add_argument("--verify", type=int, default=100, help="serve: served results re-run on the plain path")
     args = ap.parse_args()
     if args.scale is None:
         args.scale = {"sssp-ooc": 24, "bfs-hbm": 20, "cc-ooc": 25, "pr-ooc": 25, "msbfs": 26, "serve": 0}[args.workload]

@@ -257,6 +257,66 @@ typedef struct og_pr_stats {
 og_status og_pagerank(og_graph *g, const og_pr_params *params, float *ranks_f32,
                       double *ranks_f64, og_pr_stats *stats);
 
+/* --------------------------------------------------------------- serving */
+
+/* Query server over device-resident graphs (no reference counterpart: the
+ * reference answers one query per process, BFS/oneGraph_BFS/main.cpp:3-14).
+ * Queries are batched continuously: each graph has a BFS engine and, when
+ * weighted, an SSSP engine with up to 64 query slots; queries that arrive
+ * while others run join at the next round and share its kernels.  Results
+ * equal og_bfs / og_sssp on the same graph and source. */
+
+#define OG_SERVER_MAX_GRAPHS 8
+#define OG_QUERY_BFS 0u
+#define OG_QUERY_SSSP 1u
+#define OG_QUERY_KEEP_VALUES 1u   /* submit flag: keep the result array for og_server_take_values */
+
+typedef struct og_server og_server;
+
+typedef struct og_server_params {
+    uint64_t state_budget_bytes;  /* HBM for per-query state (one slot column per
+                                     query: V bytes for BFS, 4V for SSSP), shared
+                                     evenly by the engines; 0 = a quarter of free HBM */
+    uint32_t max_pending;         /* queued queries before submit returns OG_EBUSY; 0 = 2^20 */
+    uint32_t flags;
+} og_server_params;
+
+typedef struct og_query_result {
+    uint64_t id;
+    uint32_t graph, kind, source, status;   /* status: og_status of the query */
+    uint64_t reached, traversed_edges, checksum;  /* checksum = og_levels_checksum of the result */
+    double queue_ms;              /* submit -> admitted into a slot      */
+    double latency_ms;            /* submit -> result ready              */
+    uint32_t rounds;              /* engine rounds the query took part in */
+    uint32_t reserved;
+} og_query_result;
+
+typedef struct og_server_info {
+    uint32_t num_graphs;
+    uint32_t bfs_slots[OG_SERVER_MAX_GRAPHS];
+    uint32_t sssp_slots[OG_SERVER_MAX_GRAPHS];
+    uint64_t state_bytes;         /* HBM held for per-query state */
+    uint64_t submitted, completed, rejected, pending, in_flight, rounds;
+} og_server_info;
+
+/* Graphs must be prepared with the whole graph in device memory (og_prepare
+ * with a budget holding every edge and advisek 1, or OG_USM); they stay
+ * reserved (OG_EBUSY for other calls) until og_server_destroy.  Slot counts
+ * follow params->state_budget_bytes (OG_EBUDGET when a graph gets none). */
+og_status og_server_create(og_graph *const *graphs, uint32_t num_graphs,
+                           const og_server_params *params, og_server **out);
+og_status og_server_submit(og_server *srv, uint32_t graph, uint32_t kind, uint32_t source,
+                           uint32_t flags, uint64_t *id);
+/* Returns up to max_results finished queries; waits up to timeout_ms for the
+ * first one (0: no wait, < 0: wait forever). */
+og_status og_server_poll(og_server *srv, og_query_result *results, uint32_t max_results,
+                         uint32_t *count, int32_t timeout_ms);
+/* Copies the V result values of a finished OG_QUERY_KEEP_VALUES query. */
+og_status og_server_take_values(og_server *srv, uint64_t id, uint32_t *values);
+og_status og_server_get_info(og_server *srv, og_server_info *info);
+/* Stops the engines (queries still queued or running are dropped). */
+og_status og_server_destroy(og_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

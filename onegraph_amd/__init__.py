@@ -72,11 +72,38 @@ class PrStats(C.Structure):
     _fields_ = [("run", RunStats), ("iterations", C.c_uint32), ("final_l1", C.c_double)]
 
 
+SERVER_MAX_GRAPHS = 8
+QUERY_BFS, QUERY_SSSP = 0, 1
+QUERY_KEEP_VALUES = 1
+
+
+class ServerParams(C.Structure):
+    _fields_ = [("state_budget_bytes", C.c_uint64), ("max_pending", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class QueryResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("graph", C.c_uint32), ("kind", C.c_uint32), ("source", C.c_uint32),
+                ("status", C.c_uint32), ("reached", C.c_uint64), ("traversed_edges", C.c_uint64),
+                ("checksum", C.c_uint64), ("queue_ms", C.c_double), ("latency_ms", C.c_double),
+                ("rounds", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("num_graphs", C.c_uint32), ("bfs_slots", C.c_uint32 * SERVER_MAX_GRAPHS),
+                ("sssp_slots", C.c_uint32 * SERVER_MAX_GRAPHS), ("state_bytes", C.c_uint64),
+                ("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
+                ("pending", C.c_uint64), ("in_flight", C.c_uint64), ("rounds", C.c_uint64)]
+
+
 def as_dict(s):
     out = {}
     for name, _ in s._fields_:
         v = getattr(s, name)
-        out[name] = as_dict(v) if isinstance(v, C.Structure) else v
+        if isinstance(v, C.Structure):
+            v = as_dict(v)
+        elif isinstance(v, C.Array):
+            v = list(v)
+        out[name] = v
     return out
 
 
@@ -119,6 +146,12 @@ def lib():
         "og_cc": (C.c_int, [_P, _u32p, C.POINTER(RunStats)]),
         "og_msbfs": (C.c_int, [_P, _u32p, C.c_uint32, _u32p, _u64p, _u64p, C.POINTER(RunStats)]),
         "og_levels_checksum": (C.c_int, [_u32p, C.c_uint64, _u64p]),
+        "og_server_create": (C.c_int, [C.POINTER(_P), C.c_uint32, C.POINTER(ServerParams), C.POINTER(_P)]),
+        "og_server_submit": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u64p]),
+        "og_server_poll": (C.c_int, [_P, C.POINTER(QueryResult), C.c_uint32, _u32p, C.c_int32]),
+        "og_server_take_values": (C.c_int, [_P, C.c_uint64, _u32p]),
+        "og_server_get_info": (C.c_int, [_P, C.POINTER(ServerInfo)]),
+        "og_server_destroy": (C.c_int, [_P]),
         "og_pagerank": (C.c_int, [_P, C.POINTER(PrParams), C.POINTER(C.c_float), C.POINTER(C.c_double),
                                   C.POINTER(PrStats)]),
     }
@@ -306,6 +339,61 @@ class Graph:
         if self._h and self._h.value:
             lib().og_graph_free(self._h)
             self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """Continuous-batching query server over device-resident graphs."""
+
+    def __init__(self, graphs, state_budget_bytes=0, max_pending=0):
+        self._graphs = list(graphs)
+        arr = (_P * len(self._graphs))(*[g._h.value for g in self._graphs])
+        p = ServerParams(state_budget_bytes, max_pending, 0)
+        h = _P()
+        _check(lib().og_server_create(arr, len(self._graphs), C.byref(p), C.byref(h)))
+        self._h = h
+        self._V = [g.num_vertices for g in self._graphs]
+
+    def submit(self, graph, kind, source, keep=False):
+        qid = C.c_uint64()
+        _check(lib().og_server_submit(self._h, graph, kind, source, QUERY_KEEP_VALUES if keep else 0,
+                                      C.byref(qid)))
+        return qid.value
+
+    def poll(self, max_results=1024, timeout_ms=0):
+        buf = (QueryResult * max_results)()
+        n = C.c_uint32()
+        _check(lib().og_server_poll(self._h, buf, max_results, C.byref(n), timeout_ms))
+        return [as_dict(buf[i]) for i in range(n.value)]
+
+    def take_values(self, qid, graph):
+        out = np.empty(self._V[graph], np.uint32)
+        _check(lib().og_server_take_values(self._h, qid, _u32(out)))
+        return out
+
+    def info(self):
+        i = ServerInfo()
+        _check(lib().og_server_get_info(self._h, C.byref(i)))
+        d = as_dict(i)
+        d["bfs_slots"] = d["bfs_slots"][: d["num_graphs"]]
+        d["sssp_slots"] = d["sssp_slots"][: d["num_graphs"]]
+        return d
+
+    def close(self):
+        if self._h and self._h.value:
+            _check(lib().og_server_destroy(self._h))
+            self._h = _P()
 
     def __del__(self):
         try:

@@ -59,6 +59,25 @@ struct FrontierInfo {
     uint64_t dyn_deg;       // sum of degrees over dynamic active vertices
 };
 
+// Buffers of one frontier traversal: next-frontier flags, the compacted bin
+// lists, the dynamic degree prefix and the per-block scan scratch.  A graph's
+// single-source runs use DeviceState::fr; each serving engine owns its own so
+// that engines can iterate concurrently on different streams.
+struct Frontier {
+    uint8_t *flag = nullptr;         // Vpad next-frontier flags
+    uint32_t *list = nullptr;        // V: bin0 | bin1 | dynamic lists
+    uint64_t *hub = nullptr;         // hub tiles (tile << 32 | vertex)
+    uint64_t hub_cap = 0;
+    uint64_t *doff = nullptr;        // V + 1 degree prefix of the dynamic list
+    uint32_t *bcnt = nullptr;        // 5 * nblk per-block counts
+    uint64_t *bdeg = nullptr;        // 2 * nblk per-block degree sums
+    uint32_t nblk = 0;
+    FrontierInfo *info = nullptr;
+    FrontierInfo *h_info = nullptr;  // pinned
+    unsigned long long *counters = nullptr;    // [0] updates static, [1] updates dynamic, [2..] scratch
+    unsigned long long *h_counters = nullptr;  // pinned
+};
+
 // Timing of one class of launches: pairs of events on the launching stream.
 struct TimedClass {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> marks;
@@ -85,6 +104,44 @@ class EventPool {
     size_t next_ = 0;
 };
 
+// Owner of device and pinned host allocations; frees them all at once.
+struct MemPool {
+    std::vector<void *> dev, pin;
+    uint64_t device_bytes = 0;
+    MemPool() = default;
+    MemPool(const MemPool &) = delete;
+    MemPool &operator=(const MemPool &) = delete;
+    ~MemPool() { release(); }
+    void *dalloc(size_t bytes) {
+        void *p = nullptr;
+        if (bytes == 0) bytes = 256;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            raise(OG_ENOMEM_DEVICE, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+        }
+        dev.push_back(p);
+        device_bytes += bytes;
+        return p;
+    }
+    void *palloc(size_t bytes) {
+        void *p = nullptr;
+        if (hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            raise(OG_ENOMEM_HOST, "pinned host allocation of %zu bytes failed", bytes);
+        }
+        pin.push_back(p);
+        return p;
+    }
+    void release() {
+        for (void *p : dev) (void)hipFree(p);
+        for (void *p : pin) (void)hipHostFree(p);
+        dev.clear();
+        pin.clear();
+        device_bytes = 0;
+    }
+};
+
 struct DeviceState {
     og_approach approach = OG_ASCETIC;
     int device = 0;
@@ -107,23 +164,12 @@ struct DeviceState {
 
     // per-vertex state
     uint32_t *d_val = nullptr;       // V
-    uint8_t *d_flag = nullptr;       // Vpad next-frontier flags
-    uint32_t *d_list = nullptr;      // V: bin0 | bin1 | dynamic lists
-    uint64_t *d_hub = nullptr;       // hub tiles (tile << 32 | vertex)
-    uint64_t hub_cap = 0;
-    uint64_t *d_doff = nullptr;      // V + 1 degree prefix of the dynamic list
-    uint32_t *d_bcnt = nullptr;      // 5 * nblk per-block counts
-    uint64_t *d_bdeg = nullptr;      // 2 * nblk per-block degree sums
-    uint32_t nblk = 0;
-    FrontierInfo *d_info = nullptr;
-    unsigned long long *d_counters = nullptr;  // [0] updates static, [1] updates dynamic, [2..] scratch
+    Frontier fr;                     // frontier buffers of single-source / batch runs
 
-    // pinned host mirrors
-    FrontierInfo *h_info = nullptr;
+    // pinned host mirrors of the dynamic list
     uint32_t *h_dlist = nullptr;     // V
     uint64_t *h_doff = nullptr;      // V + 1
     void *h_stage[kSlots] = {nullptr, nullptr};
-    unsigned long long *h_counters = nullptr;
 
     hipStream_t s_compute = nullptr, s_copy = nullptr;
     hipEvent_t ev_h2d[kSlots] = {nullptr, nullptr};    // copy into slot finished
@@ -151,21 +197,11 @@ struct DeviceState {
     bool pr_fp64 = false;
     std::shared_ptr<void> pr_plan;   // PageRank bins and chunk ranges (pagerank.hip)
 
-    std::vector<void *> allocations; // device allocations to free
-    uint64_t device_bytes = 0;
+    MemPool mem;                     // device + pinned allocations of this preparation
 
-    void *dalloc(size_t bytes) {
-        void *p = nullptr;
-        if (bytes == 0) bytes = 256;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            raise(OG_ENOMEM_DEVICE, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
-        }
-        allocations.push_back(p);
-        device_bytes += bytes;
-        return p;
-    }
+    void *dalloc(size_t bytes) { return mem.dalloc(bytes); }
+    void *palloc(size_t bytes) { return mem.palloc(bytes); }
+    uint64_t device_bytes() const { return mem.device_bytes; }
 };
 
 // ------------------------------------------------------------ wave helpers
@@ -260,6 +296,17 @@ inline double sum_ms(const TimedClass &c) {
     return t;
 }
 
-void ensure_device(DeviceState *d);
+// Allocates a frontier (device buffers + pinned summaries) sized for the
+// graph and the current tuning; freed with the DeviceState.
+void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m);
+
+// One compaction: flags -> bin lists / hub tiles / sorted dynamic list with
+// its degree prefix, flags cleared; the summary is copied to f.h_info
+// (stream-ordered, not synchronised).
+void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s);
+
+// Launches the static-region relax of the bins in `fi` for op ALGO.
+template <int ALGO, bool W>
+void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo &fi, const RelaxArgs &a, hipStream_t s);
 
 }  // namespace og

@@ -507,30 +507,14 @@ void device_state_free(DeviceState *d) {
     if (!d) return;
     if (d->s_compute) (void)hipStreamSynchronize(d->s_compute);
     if (d->s_copy) (void)hipStreamSynchronize(d->s_copy);
-    for (void *p : d->allocations) (void)hipFree(p);
-    d->allocations.clear();
     for (int s = 0; s < kSlots; ++s) {
-        if (d->h_stage[s]) (void)hipHostFree(d->h_stage[s]);
         if (d->ev_h2d[s]) (void)hipEventDestroy(d->ev_h2d[s]);
         if (d->ev_relax[s]) (void)hipEventDestroy(d->ev_relax[s]);
     }
-    if (d->h_info) (void)hipHostFree(d->h_info);
-    if (d->h_dlist) (void)hipHostFree(d->h_dlist);
-    if (d->h_doff) (void)hipHostFree(d->h_doff);
-    if (d->h_counters) (void)hipHostFree(d->h_counters);
-    if (d->h_scalars) (void)hipHostFree(d->h_scalars);
+    d->mem.release();
     if (d->s_compute) (void)hipStreamDestroy(d->s_compute);
     if (d->s_copy) (void)hipStreamDestroy(d->s_copy);
     delete d;
-}
-
-static void *pinned(size_t bytes) {
-    void *p = nullptr;
-    if (hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault) != hipSuccess || !p) {
-        (void)hipGetLastError();
-        raise(OG_ENOMEM_HOST, "pinned host allocation of %zu bytes failed", bytes);
-    }
-    return p;
 }
 
 // Largest number of hub tiles any frontier can produce for the current tuning.
@@ -543,12 +527,33 @@ static uint64_t hub_capacity(const og_graph *g, uint64_t vsplit, const Tuning &t
     return cap;
 }
 
-static void alloc_hub(DeviceState *d, const og_graph *g) {
-    uint64_t cap = hub_capacity(g, d->plan.vsplit, d->tune);
-    if (cap > d->hub_cap || !d->d_hub) {
-        d->d_hub = (uint64_t *)d->dalloc(std::max<uint64_t>(cap, 1) * sizeof(uint64_t));
-        d->hub_cap = std::max<uint64_t>(cap, 1);
-    }
+void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) {
+    const uint64_t V = d->V;
+    f.flag = (uint8_t *)m.dalloc(d->Vpad);
+    f.list = (uint32_t *)m.dalloc(V * sizeof(uint32_t));
+    f.doff = (uint64_t *)m.dalloc((V + 1) * sizeof(uint64_t));
+    f.nblk = (uint32_t)((d->Vpad + kVertsPerBlock - 1) / kVertsPerBlock);
+    f.bcnt = (uint32_t *)m.dalloc(5ull * f.nblk * sizeof(uint32_t));
+    f.bdeg = (uint64_t *)m.dalloc(2ull * f.nblk * sizeof(uint64_t));
+    f.info = (FrontierInfo *)m.dalloc(sizeof(FrontierInfo));
+    f.counters = (unsigned long long *)m.dalloc(16 * sizeof(unsigned long long));
+    f.h_info = (FrontierInfo *)m.palloc(sizeof(FrontierInfo));
+    f.h_counters = (unsigned long long *)m.palloc(16 * sizeof(unsigned long long));
+    const uint64_t cap = std::max<uint64_t>(hub_capacity(g, d->plan.vsplit, d->tune), 1);
+    f.hub = (uint64_t *)m.dalloc(cap * sizeof(uint64_t));
+    f.hub_cap = cap;
+    OG_HIP(hipMemset(f.flag, 0, d->Vpad));
+}
+
+void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s) {
+    const uint64_t V = d->V;
+    hipLaunchKernelGGL(k_frontier<false>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, V, d->Vpad, d->plan.vsplit,
+                       d->d_ptr, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.doff);
+    hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff);
+    hipLaunchKernelGGL(k_frontier<true>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, V, d->Vpad, d->plan.vsplit,
+                       d->d_ptr, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.doff);
+    OG_HIP(hipGetLastError());
+    OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
 }
 
 static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params *params, og_plan_info *info) {
@@ -637,30 +642,18 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
                 raise(OG_EBUDGET, "edge budget leaves %llu staging edges for %llu dynamic-region edges",
                       (unsigned long long)pl.staging_edges, (unsigned long long)(E - g->ptr[pl.vsplit]));
             d->d_staging = d->dalloc(kSlots * d->slot_edges * eb);
-            for (int s = 0; s < kSlots; ++s) d->h_stage[s] = pinned(d->slot_edges * eb);
-            d->h_dlist = (uint32_t *)pinned(V * sizeof(uint32_t));
-            d->h_doff = (uint64_t *)pinned((V + 1) * sizeof(uint64_t));
+            for (int s = 0; s < kSlots; ++s) d->h_stage[s] = d->palloc(d->slot_edges * eb);
+            d->h_dlist = (uint32_t *)d->palloc(V * sizeof(uint32_t));
+            d->h_doff = (uint64_t *)d->palloc((V + 1) * sizeof(uint64_t));
             d->pool.reset(new WorkerPool(d->host_threads));
         }
     }
 
     // ---- vertex state
     d->d_val = (uint32_t *)d->dalloc(V * sizeof(uint32_t));
-    d->d_flag = (uint8_t *)d->dalloc(d->Vpad);
-    d->d_list = (uint32_t *)d->dalloc(V * sizeof(uint32_t));
-    d->d_doff = (uint64_t *)d->dalloc((V + 1) * sizeof(uint64_t));
-    d->nblk = (uint32_t)((d->Vpad + kVertsPerBlock - 1) / kVertsPerBlock);
-    d->d_bcnt = (uint32_t *)d->dalloc(5ull * d->nblk * sizeof(uint32_t));
-    d->d_bdeg = (uint64_t *)d->dalloc(2ull * d->nblk * sizeof(uint64_t));
-    d->d_info = (FrontierInfo *)d->dalloc(sizeof(FrontierInfo));
-    d->d_counters = (unsigned long long *)d->dalloc(16 * sizeof(unsigned long long));
-    d->h_info = (FrontierInfo *)pinned(sizeof(FrontierInfo));
-    d->h_counters = (unsigned long long *)pinned(16 * sizeof(unsigned long long));
-    alloc_hub(d.get(), g);
-    hipLaunchKernelGGL(k_fill_u8, dim3(fill_grid(d->Vpad)), dim3(256), 0, d->s_compute, d->d_flag, d->Vpad, (uint8_t)0);
-    OG_HIP(hipGetLastError());
     OG_HIP(hipStreamSynchronize(d->s_compute));
-    pl.device_bytes = d->device_bytes;
+    frontier_alloc(d.get(), g, d->fr, d->mem);
+    pl.device_bytes = d->device_bytes();
     if (info) *info = pl;
     g->dev = d.release();
 }
@@ -700,35 +693,41 @@ static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0
 }
 
 static RelaxArgs relax_args(DeviceState *d) {
-    return RelaxArgs{d->d_val, d->d_flag, d->d_visit, d->d_front, d->d_next};
+    return RelaxArgs{d->d_val, d->fr.flag, d->d_visit, d->d_front, d->d_next};
 }
 
 template <int ALGO, bool W>
-static void launch_static(DeviceState *d, const FrontierInfo &fi, RunTimers &tm) {
+void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo &fi, const RelaxArgs &a, hipStream_t s) {
     const void *edges = d->d_static;
-    unsigned long long *upd = d->d_counters + 0;
-    const RelaxArgs a = relax_args(d);
+    unsigned long long *upd = f.counters + 0;
     const uint32_t n0 = fi.n[0], n1 = fi.n[1], n2 = fi.n[2];
-    if (!(n0 | n1 | n2)) return;
-    EventSpan span(d, &tm.stat, d->s_compute);
     if (n0) {
         const uint64_t thr = (uint64_t)n0 * d->tune.lpv0;
         const dim3 gr(grid_for(thr)), bl(kBlock);
         switch (d->tune.lpv0) {
 #define OG_LPV(L) \
-    case L: hipLaunchKernelGGL((k_relax_group<ALGO, W, L>), gr, bl, 0, d->s_compute, d->d_list, n0, d->d_ptr, edges, a, upd); break;
+    case L: hipLaunchKernelGGL((k_relax_group<ALGO, W, L>), gr, bl, 0, s, f.list, n0, d->d_ptr, edges, a, upd); break;
             OG_LPV(1) OG_LPV(2) OG_LPV(4) OG_LPV(8) OG_LPV(16) OG_LPV(32)
             default: OG_LPV(64)
 #undef OG_LPV
         }
     }
     if (n1)
-        hipLaunchKernelGGL((k_relax_group<ALGO, W, 64>), dim3(grid_for((uint64_t)n1 * 64)), dim3(kBlock), 0,
-                           d->s_compute, d->d_list + n0, n1, d->d_ptr, edges, a, upd);
+        hipLaunchKernelGGL((k_relax_group<ALGO, W, 64>), dim3(grid_for((uint64_t)n1 * 64)), dim3(kBlock), 0, s,
+                           f.list + n0, n1, d->d_ptr, edges, a, upd);
     if (n2)
-        hipLaunchKernelGGL((k_relax_hub<ALGO, W>), dim3(n2), dim3(kBlock), d->tune.tile * d->eb, d->s_compute,
-                           d->d_hub, d->tune.tile, d->d_ptr, edges, a, upd);
+        hipLaunchKernelGGL((k_relax_hub<ALGO, W>), dim3(n2), dim3(kBlock), d->tune.tile * d->eb, s, f.hub,
+                           d->tune.tile, d->d_ptr, edges, a, upd);
     OG_HIP(hipGetLastError());
+}
+template void launch_static_relax<ALGO_MSBFS, false>(DeviceState *, Frontier &, const FrontierInfo &, const RelaxArgs &, hipStream_t);
+template void launch_static_relax<ALGO_MSBFS, true>(DeviceState *, Frontier &, const FrontierInfo &, const RelaxArgs &, hipStream_t);
+
+template <int ALGO, bool W>
+static void launch_static(DeviceState *d, const FrontierInfo &fi, RunTimers &tm) {
+    if (!(fi.n[0] | fi.n[1] | fi.n[2])) return;
+    EventSpan span(d, &tm.stat, d->s_compute);
+    launch_static_relax<ALGO, W>(d, d->fr, fi, relax_args(d), d->s_compute);
 }
 
 template <int ALGO, bool W>
@@ -738,9 +737,9 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
     const uint64_t D = fi.dyn_deg;
     if (!ndyn || !D) return;
     const RelaxArgs a = relax_args(d);
-    const uint32_t *d_dlist = d->d_list + fi.n[0] + fi.n[1];
+    const uint32_t *d_dlist = d->fr.list + fi.n[0] + fi.n[1];
     OG_HIP(hipMemcpyAsync(d->h_dlist, d_dlist, (size_t)ndyn * 4, hipMemcpyDeviceToHost, d->s_copy));
-    OG_HIP(hipMemcpyAsync(d->h_doff, d->d_doff, ((size_t)ndyn + 1) * 8, hipMemcpyDeviceToHost, d->s_copy));
+    OG_HIP(hipMemcpyAsync(d->h_doff, d->fr.doff, ((size_t)ndyn + 1) * 8, hipMemcpyDeviceToHost, d->s_copy));
     OG_HIP(hipStreamSynchronize(d->s_copy));
     const uint64_t C = d->slot_edges, eb = d->eb;
     const bool serial = (d->flags & OG_PLAN_NO_OVERLAP) != 0;
@@ -763,8 +762,8 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
         {
             EventSpan span(d, &tm.dyn, d->s_compute);
             hipLaunchKernelGGL((k_relax_dyn<ALGO, W>), dim3(grid_for(c1 - c0, kDynEdgesPerBlock)), dim3(kBlock), 0,
-                               d->s_compute, d_dlist, d->d_doff, ndyn, (const void *)dslot, c0, c1, a,
-                               d->d_counters + 1);
+                               d->s_compute, d_dlist, d->fr.doff, ndyn, (const void *)dslot, c0, c1, a,
+                               d->fr.counters + 1);
             OG_HIP(hipGetLastError());
         }
         OG_HIP(hipEventRecord(d->ev_relax[slot], d->s_compute));
@@ -788,28 +787,18 @@ template <int ALGO, bool W, class After>
 static void frontier_loop(og_graph *g, og_run_stats &st, RunTimers &tm, double &gather_ms, double &overload_wall,
                           After after) {
     DeviceState *d = g->dev;
-    const uint64_t V = d->V;
     hipStream_t s = d->s_compute;
     for (;;) {
         {
             EventSpan span(d, &tm.front, s);
-            hipLaunchKernelGGL(k_frontier<false>, dim3(d->nblk), dim3(kBlock), 0, s, d->d_flag, V, d->Vpad,
-                               d->plan.vsplit, d->d_ptr, d->tune, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
-                               d->d_list, d->d_hub, d->d_doff);
-            hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
-                               d->d_doff);
-            hipLaunchKernelGGL(k_frontier<true>, dim3(d->nblk), dim3(kBlock), 0, s, d->d_flag, V, d->Vpad,
-                               d->plan.vsplit, d->d_ptr, d->tune, d->d_bcnt, d->d_bdeg, d->nblk, d->d_info,
-                               d->d_list, d->d_hub, d->d_doff);
-            OG_HIP(hipGetLastError());
+            frontier_compact(d, d->fr, s);
         }
-        OG_HIP(hipMemcpyAsync(d->h_info, d->d_info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
         OG_HIP(hipStreamSynchronize(s));
-        const FrontierInfo fi = *d->h_info;
+        const FrontierInfo fi = *d->fr.h_info;
         const uint64_t active = (uint64_t)fi.n[0] + fi.n[1] + fi.nhub + fi.n[3] + fi.nzero;
         if (active == 0) break;
-        if (fi.n[2] > d->hub_cap)
-            raise(OG_EHIP, "hub tile list overflow (%u > %llu)", fi.n[2], (unsigned long long)d->hub_cap);
+        if (fi.n[2] > d->fr.hub_cap)
+            raise(OG_EHIP, "hub tile list overflow (%u > %llu)", fi.n[2], (unsigned long long)d->fr.hub_cap);
         st.iterations++;
         st.node_sum += active;
         st.static_node_sum += (uint64_t)fi.n[0] + fi.n[1] + fi.nhub;
@@ -861,14 +850,14 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
     OG_HIP(hipStreamSynchronize(d->s_copy));
     OG_HIP(hipStreamSynchronize(s));
     Stopwatch total;
-    OG_HIP(hipMemsetAsync(d->d_counters, 0, 16 * sizeof(unsigned long long), s));
+    OG_HIP(hipMemsetAsync(d->fr.counters, 0, 16 * sizeof(unsigned long long), s));
     if (ALGO == ALGO_CC) {
         hipLaunchKernelGGL(k_iota, dim3(fill_grid(V)), dim3(256), 0, s, d->d_val, V);
-        hipLaunchKernelGGL(k_fill_u8, dim3(fill_grid(V)), dim3(256), 0, s, d->d_flag, V, (uint8_t)1);
+        hipLaunchKernelGGL(k_fill_u8, dim3(fill_grid(V)), dim3(256), 0, s, d->fr.flag, V, (uint8_t)1);
     } else {
-        hipLaunchKernelGGL(k_fill_u8, dim3(fill_grid(d->Vpad)), dim3(256), 0, s, d->d_flag, d->Vpad, (uint8_t)0);
+        hipLaunchKernelGGL(k_fill_u8, dim3(fill_grid(d->Vpad)), dim3(256), 0, s, d->fr.flag, d->Vpad, (uint8_t)0);
         hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid(V)), dim3(256), 0, s, d->d_val, V, OG_UNREACHED);
-        hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d->d_val, d->d_flag, src);
+        hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d->d_val, d->fr.flag, src);
     }
     OG_HIP(hipGetLastError());
     frontier_loop<ALGO, W>(g, st, tm, gather_ms, overload_wall, [](uint32_t) {});
@@ -876,15 +865,15 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
     st.total_ms = total.ms();
 
     hipLaunchKernelGGL(k_reach_stats, dim3(std::min<uint64_t>(2048, (V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       d->d_val, d->d_ptr, V, d->d_counters + 2);
+                       d->d_val, d->d_ptr, V, d->fr.counters + 2);
     OG_HIP(hipGetLastError());
-    OG_HIP(hipMemcpyAsync(d->h_counters, d->d_counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    OG_HIP(hipMemcpyAsync(d->fr.h_counters, d->fr.counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     if (out) OG_HIP(hipMemcpyAsync(out, d->d_val, V * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     OG_HIP(hipStreamSynchronize(s));
-    st.reached = d->h_counters[2];
-    st.traversed_edges = d->h_counters[3];
-    st.checksum = d->h_counters[4];
-    finish_stats(st, tm, gather_ms, overload_wall, d->h_counters, 8);
+    st.reached = d->fr.h_counters[2];
+    st.traversed_edges = d->fr.h_counters[3];
+    st.checksum = d->fr.h_counters[4];
+    finish_stats(st, tm, gather_ms, overload_wall, d->fr.h_counters, 8);
     if (stats) *stats = st;
 }
 
@@ -914,7 +903,7 @@ static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t
     OG_HIP(hipStreamSynchronize(d->s_copy));
     OG_HIP(hipStreamSynchronize(s));
     Stopwatch total;
-    OG_HIP(hipMemsetAsync(d->d_counters, 0, 16 * sizeof(unsigned long long), s));
+    OG_HIP(hipMemsetAsync(d->fr.counters, 0, 16 * sizeof(unsigned long long), s));
     double device_ms = 0;
     for (uint32_t b0 = 0; b0 < n; b0 += kMsWidth) {
         const uint32_t nb = std::min<uint32_t>(kMsWidth, n - b0);
@@ -922,16 +911,16 @@ static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t
         OG_HIP(hipMemsetAsync(d->d_front, 0, V * 8, s));
         OG_HIP(hipMemsetAsync(d->d_next, 0, V * 8, s));
         OG_HIP(hipMemsetAsync(d->d_mlvl, 0, V * kMsWidth, s));
-        OG_HIP(hipMemsetAsync(d->d_flag, 0, d->Vpad, s));
+        OG_HIP(hipMemsetAsync(d->fr.flag, 0, d->Vpad, s));
         OG_HIP(hipMemcpyAsync(d->d_msrc, sources + b0, nb * 4, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_ms_init, dim3(1), dim3(kMsWidth), 0, s, d->d_msrc, nb, d->d_visit, d->d_front,
-                           d->d_flag, d->d_mlvl);
+                           d->fr.flag, d->d_mlvl);
         OG_HIP(hipGetLastError());
         uint32_t level = 1;
         const unsigned cgrid = (unsigned)std::min<uint64_t>(4096, (V + kBlock - 1) / kBlock);
         frontier_loop<ALGO_MSBFS, W>(g, st, tm, gather_ms, overload_wall, [&](uint32_t) {
             if (++level > 255) raise(OG_EUNSUPPORTED, "multi-source BFS deeper than 254 levels");
-            hipLaunchKernelGGL(k_ms_commit, dim3(cgrid), dim3(kBlock), 0, s, d->d_flag, V, d->d_visit, d->d_front,
+            hipLaunchKernelGGL(k_ms_commit, dim3(cgrid), dim3(kBlock), 0, s, d->fr.flag, V, d->d_visit, d->d_front,
                                d->d_next, d->d_mlvl, (uint8_t)level);
             OG_HIP(hipGetLastError());
         });
@@ -960,9 +949,9 @@ static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t
         }
     }
     st.total_ms = device_ms;
-    OG_HIP(hipMemcpyAsync(d->h_counters, d->d_counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    OG_HIP(hipMemcpyAsync(d->fr.h_counters, d->fr.counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     OG_HIP(hipStreamSynchronize(s));
-    finish_stats(st, tm, gather_ms, overload_wall, d->h_counters, 9);
+    finish_stats(st, tm, gather_ms, overload_wall, d->fr.h_counters, 9);
     if (stats) *stats = st;
 }
 

@@ -276,10 +276,7 @@ static void run_pr(og_graph *g, const og_pr_params &prm, T *out, og_pr_stats *st
         d->d_outdeg = (uint32_t *)d->dalloc(V * 4);
         OG_HIP(hipMemcpy(d->d_outdeg, g->out_degree, V * 4, hipMemcpyHostToDevice));
         d->d_scalars = (double *)d->dalloc(4 * sizeof(double));
-        d->h_scalars = (double *)nullptr;
-        void *hp = nullptr;
-        OG_HIP(hipHostMalloc(&hp, 4 * sizeof(double), hipHostMallocDefault));
-        d->h_scalars = (double *)hp;
+        d->h_scalars = (double *)d->palloc(4 * sizeof(double));
         d->d_partial = (double *)d->dalloc(pp.partial_cap * sizeof(double));
     }
     if (!d->d_rank[0] || d->pr_fp64 != f64) {

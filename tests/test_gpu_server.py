@@ -1,0 +1,133 @@
+"""Continuous-batching server: each query's result is the same alone and in
+the mix, equals the single-query path and the oracle; admission and
+reservation rules hold."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import onegraph_amd as og
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def graphs():
+    gs = [og.Graph.rmat(12, 16, seed=51, weight_seed=52), og.Graph.er(3000, 8, seed=53).add_weights(54),
+          og.Graph.rmat(11, 8, seed=55)]
+    for g in gs:
+        g.prepare("ascetic", edge_budget_bytes=g.num_edges * (8 if g.info()["kind"] == og.KIND_WEIGHTED else 4),
+                  advisek=1.0)
+    yield gs
+    for g in gs:
+        g.close()
+
+
+def _queries(graphs, n, seed):
+    rng = np.random.default_rng(seed)
+    qs = []
+    for _ in range(n):
+        gi = int(rng.integers(len(graphs)))
+        g = graphs[gi]
+        weighted = g.info()["kind"] == og.KIND_WEIGHTED
+        kind = og.QUERY_SSSP if (weighted and rng.random() < 0.3) else og.QUERY_BFS
+        ptr, to, _, _ = g.export()
+        deg = np.diff(np.append(ptr.astype(np.int64), len(to)))
+        qs.append((gi, kind, int(rng.choice(np.flatnonzero(deg > 0)))))
+    return qs
+
+
+def _wait_all(srv, ids, timeout=60):
+    got = {}
+    t0 = time.time()
+    while len(got) < len(ids):
+        for r in srv.poll(256, timeout_ms=200):
+            got[r["id"]] = r
+        assert time.time() - t0 < timeout, "server did not finish"
+    return got
+
+
+def _oracle(graphs, gi, kind, src):
+    ptr, to, w, _ = graphs[gi].export()
+    return oracle.bfs(ptr, to, src)[0] if kind == og.QUERY_BFS else oracle.sssp(ptr, to, w, src)
+
+
+def test_alone_and_in_the_mix(gpu, graphs):
+    qs = _queries(graphs, 60, 7)
+    _say("server mix of", len(qs))
+    # small budget: 8 slots per engine, so queries queue and join mid-flight
+    per_slot = sum(g.num_vertices * (1 + (4 if g.info()["kind"] == og.KIND_WEIGHTED else 0)) for g in graphs)
+    with og.Server(graphs, state_budget_bytes=8 * per_slot) as srv:
+        info = srv.info()
+        assert info["bfs_slots"] == [8, 8, 8] and info["sssp_slots"] == [8, 8, 0]
+        ids = [srv.submit(gi, k, s, keep=True) for gi, k, s in qs]
+        res = _wait_all(srv, ids)
+        mix = {i: srv.take_values(i, qs[n][0]) for n, i in enumerate(ids)}
+        alone = {}
+        for n, (gi, k, s) in enumerate(qs[:20]):
+            i = srv.submit(gi, k, s, keep=True)
+            r = _wait_all(srv, [i])[i]
+            assert r["status"] == 0
+            alone[n] = (r, srv.take_values(i, gi))
+        info = srv.info()
+        assert info["completed"] == len(qs) + 20 and info["in_flight"] == 0
+    for n, (gi, k, s) in enumerate(qs):
+        r = res[ids[n]]
+        assert r["status"] == 0
+        ref = _oracle(graphs, gi, k, s)
+        assert np.array_equal(mix[ids[n]], ref), (n, gi, k, s)
+        assert r["checksum"] == og.levels_checksum(ref)
+        assert r["reached"] == int((ref != og.UNREACHED).sum())
+        if n in alone:
+            assert np.array_equal(alone[n][1], mix[ids[n]])
+            assert alone[n][0]["checksum"] == r["checksum"]
+
+
+def test_matches_single_query_path(gpu, graphs):
+    qs = _queries(graphs, 30, 9)
+    with og.Server(graphs) as srv:
+        ids = [srv.submit(gi, k, s) for gi, k, s in qs]
+        res = _wait_all(srv, ids)
+    for n, (gi, k, s) in enumerate(qs):  # graphs are free again after destroy
+        g = graphs[gi]
+        _, st = (g.bfs(s, want_values=False) if k == og.QUERY_BFS else g.sssp(s, want_values=False))
+        assert st["checksum"] == res[ids[n]]["checksum"]
+
+
+def test_admission_and_reservation(gpu, graphs):
+    g = graphs[0]
+    with pytest.raises(og.OgError) as e:
+        og.Server([g], state_budget_bytes=16)
+    assert e.value.name == "OG_EBUDGET"
+    srv = og.Server([g], state_budget_bytes=g.num_vertices * 5 * 2, max_pending=4)
+    assert srv.info()["bfs_slots"] == [2] and srv.info()["sssp_slots"] == [2]
+    with pytest.raises(og.OgError) as e:  # reserved while served
+        g.bfs(0)
+    assert e.value.name == "OG_EBUSY"
+    with pytest.raises(og.OgError) as e:
+        srv.submit(0, og.QUERY_BFS, g.num_vertices)
+    assert e.value.name == "OG_EINVAL"
+    rejected = 0
+    for _ in range(200):
+        try:
+            srv.submit(0, og.QUERY_SSSP, 0)
+        except og.OgError as ex:
+            assert ex.name == "OG_EBUSY"
+            rejected += 1
+    assert rejected > 0 and srv.info()["rejected"] == rejected
+    srv.close()
+    g.bfs(0)  # released
+    # a graph whose edges are not all on the device is refused
+    h = og.Graph.rmat(10, 8, seed=3)
+    h.prepare("ascetic", edge_budget_bytes=h.num_edges, advisek=0.5)
+    with pytest.raises(og.OgError) as e:
+        og.Server([h])
+    assert e.value.name == "OG_EUNSUPPORTED"
+    h.close()
