@@ -705,7 +705,10 @@ extern "C" og_status og_plan_compute(uint64_t V, uint64_t E, const uint64_t *ptr
     if (k < 0) k = 0.5f;
     if (k > 1) k = 1.0f;
     if (params->advisek > 0) k = params->advisek;
-    uint64_t stat = (uint64_t)(k * (float)total);
+    // The reference forms this product in float (GraphMeta.h:326), which drops
+    // up to ulp(total) edges once total > 2^24; double keeps advisek = 1 with a
+    // budget of E edges fully static.
+    uint64_t stat = (uint64_t)((double)k * (double)total);
     if (stat > E) stat = E;
     if (stat > total) stat = total;
     // ptr has V entries in the reference layout; the last row ends at E

@@ -236,7 +236,12 @@ def test_planner_hand_cases():
     assert abs(p["advisek_used"] - 1.0) < 1e-7 and p["static_edges"] == 10 and p["vsplit"] == 4
     # budget 2x smaller than E: (10 - 2)/9 = 0.888..
     p = og.plan_compute(ptr, 10, 4, edge_budget_bytes=20)
-    assert abs(p["advisek_used"] - 8 / 9) < 1e-6 and p["static_edges"] == int(np.float32(8 / 9) * np.float32(5))
+    assert abs(p["advisek_used"] - 8 / 9) < 1e-6 and p["static_edges"] == int(float(np.float32(8 / 9)) * 5)
+    # advisek = 1 with a budget of exactly E edges keeps every edge static even
+    # when E is not representable in float (E = 2^24 + 3 here)
+    big = np.array([0, (1 << 24) + 1], np.uint64)
+    p = og.plan_compute(big, (1 << 24) + 3, 4, edge_budget_bytes=((1 << 24) + 3) * 4, advisek=1.0)
+    assert p["static_edges"] == (1 << 24) + 3 and p["vsplit"] == 2 and p["staging_edges"] == 0
     # E/total = 10: the rule gives exactly 0
     p = og.plan_compute(ptr, 10, 4, edge_budget_bytes=4)
     assert p["advisek_used"] == 0.0 and p["static_edges"] == 0
