@@ -437,6 +437,12 @@ def main():
     ap.add_argument("--approach", default="ascetic", choices=["ascetic", "subway", "usm"])
     ap.add_argument("--compare", action="store_true", help="also time subway and usm")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--delta", type=int, default=32, help="SSSP near-far bucket width (0: plain rounds)")
+    ap.add_argument("--autotune", action="store_true", help="run the autotuner before timing")
+    ap.add_argument("--loader", default="zerocopy", choices=["zerocopy", "copy"],
+                    help="dynamic region: relax reads pinned host memory in place, or host gather + copy stream")
+    ap.add_argument("--no-pack", action="store_true", help="8-byte weighted edges on the link")
+    ap.add_argument("--sweep-delta", default="", help="comma list: time SSSP at each delta and report")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--workload", default="sssp-ooc",
@@ -489,8 +495,25 @@ def main():
 
     def run_approach(approach, steps, warmup):
         k = args.advisek if approach == "ascetic" else 0.0
-        plan = g.prepare(approach, edge_budget_bytes=budget if approach != "usm" else 0, advisek=k)
+        flags = (og.PLAN_ZERO_COPY if args.loader == "zerocopy" else 0) | (og.PLAN_NO_PACK if args.no_pack else 0)
+        plan = g.prepare(approach, edge_budget_bytes=budget if approach != "usm" else 0, advisek=k,
+                         flags=flags if approach != "usm" else 0)
         log(f"{approach}: plan {plan}")
+        if args.delta is not None:
+            g.set_tuning(delta=args.delta)
+        if args.autotune:
+            best, best_ms = g.autotune(og.TUNE_SSSP, src, runs=1)
+            log(f"{approach}: autotuned {best} ({best_ms:.1f} ms)")
+        if args.sweep_delta:
+            for dl in [int(x) for x in args.sweep_delta.split(",")]:
+                g.set_tuning(delta=dl)
+                ts = [g.sssp(src, want_values=False)[1] for _ in range(3)]
+                log(f"{approach}: delta {dl}: {min(t['total_ms'] for t in ts):.2f} ms, "
+                    f"{ts[-1]['iterations']} rounds, h2d {ts[-1]['h2d_bytes'] / 1e9:.2f} GB, "
+                    f"work {ts[-1]['work_edges'] / 1e6:.0f} M")
+            if args.delta is not None:
+                g.set_tuning(delta=args.delta)
+        log(f"{approach}: tuning {g.tuning()}")
         for i in range(warmup):
             _, st = g.sssp(src, want_values=False)
             log(f"{approach}: warmup {i} {st['total_ms']:.1f} ms, {st['iterations']} iterations")
@@ -524,18 +547,26 @@ def main():
                 "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_us": round(r["avg_us"], 3), "bytes_per_launch": round(r["bytes_per_launch"], 1)}
         # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-        # (calibrated against a plain copy) for this kernel and workload.
+        # (calibrated against a plain copy) for this kernel, workload and setting.
         tpath = os.path.join(HERE, "profiles", "k_relax_dyn_sssp_rmat24_traffic.json")
-        if dom == "relax_dynamic" and args.scale == 24 and args.approach == "ascetic" and os.path.exists(tpath):
+        if os.path.exists(tpath):
             with open(tpath) as f:
                 tr = json.load(f)
-            roof["traffic"] = tr["traffic_bytes_per_launch"]
-            roof["traffic_source"] = tr["source"]
+            want = {"scale": args.scale, "approach": args.approach, "loader": args.loader, "delta": args.delta,
+                    "kernel_class": dom}
+            if tr.get("config") == want:
+                roof["traffic"] = tr["traffic_bytes_per_launch"]
+                roof["traffic_source"] = tr["source"]
     link = None
     if agg["copy_ms"] > 0:
         gbs = agg["h2d_bytes"] / (agg["copy_ms"] * 1e-3) / 1e9
         link = {"achieved": round(gbs, 2), "peak": LINK_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LINK_PEAK_GBS, 4),
-                "bytes_per_step": agg["h2d_bytes"] / args.steps}
+                "bytes_per_step": agg["h2d_bytes"] / args.steps, "measured_on": "copy stream (hipMemcpyAsync)"}
+    elif agg["h2d_bytes"] and agg["relax_dynamic_kernel_ms"] > 0:
+        # zero-copy loader: the dynamic relax kernel itself reads the link
+        gbs = agg["h2d_bytes"] / (agg["relax_dynamic_kernel_ms"] * 1e-3) / 1e9
+        link = {"achieved": round(gbs, 2), "peak": LINK_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LINK_PEAK_GBS, 4),
+                "bytes_per_step": agg["h2d_bytes"] / args.steps, "measured_on": "k_relax_dyn (direct reads)"}
 
     out = {
         "metric": METRIC,
@@ -557,7 +588,8 @@ def main():
             "V": V, "E": E, "source": src,
             "edge_budget_bytes": budget, "budget_fraction": args.budget, "advisek": args.advisek,
             "approach": args.approach, "static_edges": plan["static_edges"], "staging_edges": plan["staging_edges"],
-            "vsplit": plan["vsplit"],
+            "vsplit": plan["vsplit"], "loader": args.loader, "delta": args.delta,
+            "link_format": "8-byte {to,w}" if args.no_pack else "4-byte to<<6|w",
         },
         "roofline": roof,
         "host_link": link,

@@ -146,6 +146,11 @@ typedef struct og_plan_params {
 } og_plan_params;
 
 #define OG_PLAN_NO_OVERLAP 1u   /* serialise copies and relax (for A/B timing) */
+#define OG_PLAN_ZERO_COPY 2u    /* dynamic region: the relax reads active edges
+                                   straight from pinned host memory over the link
+                                   (no staging copies, no host gather)          */
+#define OG_PLAN_NO_PACK 4u      /* keep 8-byte weighted edges on the link (no
+                                   4-byte (to << wbits | w) transfer format)     */
 
 typedef struct og_plan_info {
     uint64_t total_budget_edges;  /* reference total_gpu_size               */
@@ -220,6 +225,36 @@ og_status og_sssp(og_graph *g, uint32_t source, uint32_t *dist, og_run_stats *st
 /* Replaces cc_opt (CC/onegraph/CalculateOpt.h:6-144), the Subway CC main
  * (CC/sbw/sbw.cpp:305-490) and the USM CC main (CC/usm/usm.cpp:127-257). */
 og_status og_cc(og_graph *g, uint32_t *labels, og_run_stats *stats);
+
+/* -------------------------------------------------------------- tuning */
+
+/* Shapes of the static-region relax and the SSSP round order.  The reference
+ * has one shape: one work-item per vertex, grid 1024*108
+ * (BFS/oneGraph_BFS/globals.h:16-17). */
+typedef struct og_tuning {
+    uint32_t t1;     /* bin 0: 1 <= degree <= t1, lpv0 lanes per vertex          */
+    uint32_t lpv0;   /* 1, 2, 4, 8, 16, 32 or 64                                  */
+    uint32_t t2;     /* bin 1: t1 < degree <= t2, one wave per vertex; above: hubs */
+    uint32_t tile;   /* hub tile (edges per workgroup, staged in LDS), 64..4096   */
+    uint32_t delta;  /* SSSP near-far bucket width: a round relaxes only vertices
+                        whose distance is below the current threshold; 0 = every
+                        improved vertex every round (the reference's order)     */
+} og_tuning;
+
+og_status og_get_tuning(const og_graph *g, og_tuning *t);
+/* Installs a tuning on a prepared graph (OG_EINVAL if out of range). */
+og_status og_set_tuning(og_graph *g, const og_tuning *t);
+
+#define OG_TUNE_BFS 0u
+#define OG_TUNE_SSSP 1u
+#define OG_TUNE_CC 2u
+
+/* Autotuner: times `algo` from `source` on the prepared graph under
+ * candidate tunings (coordinate descent: delta for SSSP, then t1 with lpv0,
+ * t2, tile; the median of `runs` timed runs each), installs the fastest and
+ * returns it with its time.  Results do not depend on the tuning. */
+og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, uint32_t runs,
+                      og_tuning *best, double *best_ms);
 
 /* Multi-source BFS (no reference counterpart; the reference runs one source
  * per bfs_opt iteration, BFS/oneGraph_BFS/CalculateOpt.h:55-209).  Sources

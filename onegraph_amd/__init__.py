@@ -18,6 +18,8 @@ ASCETIC, SUBWAY, USM = 0, 1, 2
 APPROACHES = {"ascetic": ASCETIC, "subway": SUBWAY, "usm": USM}
 UNREACHED = 0xFFFFFFFF
 PLAN_NO_OVERLAP = 1
+PLAN_ZERO_COPY = 2
+PLAN_NO_PACK = 4
 
 STATUS = {
     0: "OG_OK", 1: "OG_EINVAL", 2: "OG_EIO", 3: "OG_EFORMAT", 4: "OG_EBUDGET",
@@ -61,6 +63,14 @@ class RunStats(C.Structure):
 class GraphInfo(C.Structure):
     _fields_ = [("num_vertices", C.c_uint64), ("num_edges", C.c_uint64), ("kind", C.c_uint32),
                 ("max_degree", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("t1", C.c_uint32), ("lpv0", C.c_uint32), ("t2", C.c_uint32), ("tile", C.c_uint32),
+                ("delta", C.c_uint32)]
+
+
+TUNE_BFS, TUNE_SSSP, TUNE_CC = 0, 1, 2
 
 
 class PrParams(C.Structure):
@@ -146,6 +156,9 @@ def lib():
         "og_cc": (C.c_int, [_P, _u32p, C.POINTER(RunStats)]),
         "og_msbfs": (C.c_int, [_P, _u32p, C.c_uint32, _u32p, _u64p, _u64p, C.POINTER(RunStats)]),
         "og_levels_checksum": (C.c_int, [_u32p, C.c_uint64, _u64p]),
+        "og_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+        "og_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+        "og_autotune": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Tuning), C.POINTER(C.c_double)]),
         "og_server_create": (C.c_int, [C.POINTER(_P), C.c_uint32, C.POINTER(ServerParams), C.POINTER(_P)]),
         "og_server_submit": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _u64p]),
         "og_server_poll": (C.c_int, [_P, C.POINTER(QueryResult), C.c_uint32, _u32p, C.c_int32]),
@@ -292,6 +305,22 @@ class Graph:
 
     def release(self):
         _check(lib().og_release(self._h))
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().og_get_tuning(self._h, C.byref(t)))
+        return as_dict(t)
+
+    def set_tuning(self, **kw):
+        t = Tuning(**{**self.tuning(), **kw})
+        _check(lib().og_set_tuning(self._h, C.byref(t)))
+        return as_dict(t)
+
+    def autotune(self, algo=TUNE_BFS, source=0, runs=3):
+        t = Tuning()
+        ms = C.c_double()
+        _check(lib().og_autotune(self._h, algo, source, runs, C.byref(t), C.byref(ms)))
+        return as_dict(t), ms.value
 
     def bfs(self, source, want_values=True):
         out = np.empty(self.num_vertices, np.uint32) if want_values else None

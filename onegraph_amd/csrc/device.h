@@ -48,6 +48,7 @@ struct Tuning {
     uint32_t lpv0 = 8;     // 1, 2, 4, 8, 16, 32 or 64
     uint32_t t2 = 4096;    // bin 1: t1 < deg <= t2, one wave per vertex
     uint32_t tile = 2048;  // bin 2: deg > t2, tiles of `tile` edges, one workgroup each
+    uint32_t delta = 0;    // SSSP near-far bucket width (0: every improved vertex every round)
 };
 
 // Frontier summary produced on the device each iteration and read by the host.
@@ -57,7 +58,11 @@ struct FrontierInfo {
     uint32_t nhub;          // bin2 vertices
     uint64_t static_deg;    // sum of degrees over static active vertices
     uint64_t dyn_deg;       // sum of degrees over dynamic active vertices
+    uint32_t nfar;          // active vertices held back (value >= threshold)
+    uint32_t min_far;       // smallest value among them
 };
+
+constexpr int kBcntRows = 7;            // per-block rows: bins 0-3, zero|hub, far, min far
 
 // Buffers of one frontier traversal: next-frontier flags, the compacted bin
 // lists, the dynamic degree prefix and the per-block scan scratch.  A graph's
@@ -159,6 +164,14 @@ struct DeviceState {
     void *d_staging = nullptr;       // kSlots * slot_edges edges
     uint64_t slot_edges = 0;
     bool managed = false;
+    // dynamic-region transfer format: 4-byte (to << wbits | w) when packed
+    bool packed = false;
+    uint32_t wbits = 0;
+    uint32_t deb = 4;                // bytes per dynamic-region edge on the link
+    uint64_t dyn_base = 0;           // first edge of the dynamic region (= ptr[vsplit])
+    const void *h_dyn = nullptr;     // host source of dynamic edges (packed copy or the CSR)
+    const void *hd_dyn = nullptr;    // the same, as a device address (zero-copy mode)
+    uint64_t slot_dyn = 0;           // dynamic edges per staging slot in the transfer format
     uint64_t *m_ptr = nullptr;       // USM managed copies
     void *m_edges = nullptr;
 
@@ -172,6 +185,8 @@ struct DeviceState {
     void *h_stage[kSlots] = {nullptr, nullptr};
 
     hipStream_t s_compute = nullptr, s_copy = nullptr;
+    hipStream_t s_static = nullptr;  // static-region relax, overlapping the dynamic region
+    hipEvent_t ev_static = nullptr;  // static relax of the round finished
     hipEvent_t ev_h2d[kSlots] = {nullptr, nullptr};    // copy into slot finished
     hipEvent_t ev_relax[kSlots] = {nullptr, nullptr};  // relax reading slot finished
     bool slot_h2d_pending[kSlots] = {false, false};
@@ -193,7 +208,7 @@ struct DeviceState {
     double *d_partial = nullptr;     // per-block residual / dangling partials
     double *d_scalars = nullptr;     // [0] dangling, [1] l1
     double *h_scalars = nullptr;
-    uint32_t pr_nblk = 0;
+    uint64_t pr_partial_cap = 0;     // entries of d_partial
     bool pr_fp64 = false;
     std::shared_ptr<void> pr_plan;   // PageRank bins and chunk ranges (pagerank.hip)
 
@@ -302,8 +317,10 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m);
 
 // One compaction: flags -> bin lists / hub tiles / sorted dynamic list with
 // its degree prefix, flags cleared; the summary is copied to f.h_info
-// (stream-ordered, not synchronised).
-void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s);
+// (stream-ordered, not synchronised).  With `val` non-null, vertices whose
+// value is >= thresh stay flagged and are only counted (near-far order).
+void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val = nullptr,
+                      uint32_t thresh = 0xFFFFFFFFu);
 
 // Launches the static-region relax of the bins in `fi` for op ALGO.
 template <int ALGO, bool W>

@@ -70,20 +70,37 @@ __device__ __forceinline__ bool any_flag(const uint4 (&w)[kFlagVecs]) {
     return o != 0;
 }
 
-// Bin of an active vertex: 0..2 static bins, 3 dynamic, 4 no out-edges.
-__device__ __forceinline__ int bin_of(uint64_t v, uint64_t vsplit, uint64_t deg, const Tuning &tn) {
+// Bin of an active vertex: 0..2 static bins, 3 dynamic, 4 no out-edges,
+// 5 held back for a later round (near-far order: value >= threshold).
+__device__ __forceinline__ int bin_of(uint64_t v, uint64_t vsplit, uint64_t deg, const Tuning &tn,
+                                      const uint32_t *__restrict__ val, uint32_t thresh) {
     if (deg == 0) return 4;
+    if (val && val[v] >= thresh) return 5;
     if (v >= vsplit) return 3;
     if (deg <= tn.t1) return 0;
     if (deg <= tn.t2) return 1;
     return 2;
 }
 
+__device__ __forceinline__ uint32_t block_min_u32(uint32_t x, uint64_t *lds) {
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) x = min(x, (uint32_t)__shfl_xor(x, m, kWave));
+    __syncthreads();
+    if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = x;
+    __syncthreads();
+    uint32_t r = 0xFFFFFFFFu;
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) r = min(r, (uint32_t)lds[w]);
+    __syncthreads();
+    return r;
+}
+
 // Pass 1 (WRITE = false): per-block counts.  Pass 3 (WRITE = true): emits
-// lists in vertex order, the dynamic degree prefix, and clears the flags.
+// lists in vertex order, the dynamic degree prefix, and clears the flags of
+// the emitted vertices (held-back vertices keep theirs).
 template <bool WRITE>
 __global__ __launch_bounds__(kBlock) void k_frontier(uint8_t *__restrict__ flag, uint64_t V, uint64_t Vpad,
                                                      uint64_t vsplit, const uint64_t *__restrict__ ptr, Tuning tn,
+                                                     const uint32_t *__restrict__ val, uint32_t thresh,
                                                      uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
                                                      uint32_t nblk, const FrontierInfo *__restrict__ info,
                                                      uint32_t *__restrict__ list, uint64_t *__restrict__ hub,
@@ -94,18 +111,20 @@ __global__ __launch_bounds__(kBlock) void k_frontier(uint8_t *__restrict__ flag,
     load_flags(flag, base, Vpad, w);
     const bool any = any_flag(w);
 
-    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, cz = 0, ch = 0;
+    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, cz = 0, ch = 0, cf = 0, mf = 0xFFFFFFFFu;
+    uint32_t farmask = 0;  // bit j: vertex base + j is held back
     uint64_t sdeg = 0, ddeg = 0;
     if (any) {
         for (int j = 0; j < kFlagsPerThread; ++j) {
             if (!flag_at(w, j)) continue;
             const uint64_t v = base + j;
             const uint64_t deg = ptr[v + 1] - ptr[v];
-            switch (bin_of(v, vsplit, deg, tn)) {
+            switch (bin_of(v, vsplit, deg, tn, val, thresh)) {
                 case 0: ++c0; sdeg += deg; break;
                 case 1: ++c1; sdeg += deg; break;
                 case 2: c2 += (uint32_t)((deg + tn.tile - 1) / tn.tile); ++ch; sdeg += deg; break;
                 case 3: ++c3; ddeg += deg; break;
+                case 5: ++cf; mf = min(mf, val[v]); farmask |= 1u << j; break;
                 default: ++cz; break;
             }
         }
@@ -116,12 +135,16 @@ __global__ __launch_bounds__(kBlock) void k_frontier(uint8_t *__restrict__ flag,
         uint64_t pzh = block_sum((uint64_t)cz | ((uint64_t)ch << 32), lds);
         uint64_t ts = block_sum(sdeg, lds);
         uint64_t td = block_sum(ddeg, lds);
+        uint64_t tf = block_sum(cf, lds);
+        uint32_t bm = block_min_u32(mf, lds);
         if (threadIdx.x == 0) {
             bcnt[0 * nblk + blockIdx.x] = (uint32_t)p01;
             bcnt[1 * nblk + blockIdx.x] = (uint32_t)(p01 >> 32);
             bcnt[2 * nblk + blockIdx.x] = (uint32_t)p23;
             bcnt[3 * nblk + blockIdx.x] = (uint32_t)(p23 >> 32);
             bcnt[4 * nblk + blockIdx.x] = (uint32_t)pzh | ((uint32_t)(pzh >> 32) << 16);
+            bcnt[5 * nblk + blockIdx.x] = (uint32_t)tf;
+            bcnt[6 * nblk + blockIdx.x] = bm;
             bdeg[0 * nblk + blockIdx.x] = ts;
             bdeg[1 * nblk + blockIdx.x] = td;
         }
@@ -140,10 +163,10 @@ __global__ __launch_bounds__(kBlock) void k_frontier(uint8_t *__restrict__ flag,
     uint64_t pd = bdeg[1 * nblk + blockIdx.x] + edd;
     const uint64_t dyn_base = (uint64_t)n0 + n1;
     for (int j = 0; j < kFlagsPerThread; ++j) {
-        if (!flag_at(w, j)) continue;
+        if (!flag_at(w, j) || ((farmask >> j) & 1)) continue;
         const uint64_t v = base + j;
         const uint64_t deg = ptr[v + 1] - ptr[v];
-        switch (bin_of(v, vsplit, deg, tn)) {
+        switch (bin_of(v, vsplit, deg, tn, nullptr, 0)) {
             case 0: list[p0++] = (uint32_t)v; break;
             case 1: list[p1++] = (uint32_t)v; break;
             case 2: {
@@ -160,10 +183,20 @@ __global__ __launch_bounds__(kBlock) void k_frontier(uint8_t *__restrict__ flag,
             default: break;
         }
     }
+    // rewrite the window: 1 for held-back vertices, 0 elsewhere
 #pragma unroll
     for (int k = 0; k < kFlagVecs; ++k) {
         uint64_t at = base + 16ull * k;
-        if (at < Vpad) *(uint4 *)(flag + at) = make_uint4(0, 0, 0, 0);
+        if (at >= Vpad) continue;
+        uint32_t word[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) x |= ((farmask >> (16 * k + 4 * q + b)) & 1u) << (8 * b);
+            word[q] = x;
+        }
+        *(uint4 *)(flag + at) = make_uint4(word[0], word[1], word[2], word[3]);
     }
 }
 
@@ -185,12 +218,15 @@ __global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *bcnt, uint64_t
         }
         tot4[a] = carry;
     }
-    uint64_t zsum = 0, hsum = 0, ssum = 0, dcarry = 0;
+    uint64_t zsum = 0, hsum = 0, ssum = 0, dcarry = 0, fsum = 0;
+    uint32_t fmin = 0xFFFFFFFFu;
     for (uint32_t b = 0; b < nblk; b += blockDim.x) {
         const uint32_t i = b + threadIdx.x;
         const uint32_t zh = i < nblk ? bcnt[4 * nblk + i] : 0;
         zsum += block_sum(zh & 0xFFFFu, lds);
         hsum += block_sum(zh >> 16, lds);
+        fsum += block_sum(i < nblk ? bcnt[5 * nblk + i] : 0, lds);
+        fmin = min(fmin, block_min_u32(i < nblk ? bcnt[6 * nblk + i] : 0xFFFFFFFFu, lds));
         ssum += block_sum(i < nblk ? bdeg[i] : 0, lds);
         const uint64_t x = i < nblk ? bdeg[nblk + i] : 0;
         uint64_t t;
@@ -204,6 +240,8 @@ __global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *bcnt, uint64_t
         info->nhub = (uint32_t)hsum;
         info->static_deg = ssum;
         info->dyn_deg = dcarry;
+        info->nfar = (uint32_t)fsum;
+        info->min_far = fmin;
         doff[tot4[3]] = dcarry;
     }
 }
@@ -342,16 +380,34 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
     wave_add(upd, c);
 }
 
-// Dynamic chunk: edges [c0, c1) of the concatenated active dynamic edge
-// space sit in `stage`.  Each block takes kDynEdgesPerBlock consecutive edges,
-// loads the covering slice of the degree prefix into LDS and maps each edge
-// to its source with a binary search there (every listed vertex has >= 1
-// edge, so a block covers at most kDynEdgesPerBlock vertices).
-template <int ALGO, bool W>
+// One dynamic-region edge: 8-byte AoS {to, w}, 4-byte column, or the packed
+// 4-byte transfer format (to << wbits | w).
+template <bool W, bool PACKED>
+__device__ __forceinline__ void load_dyn_edge(const void *__restrict__ src, uint64_t i, uint32_t wbits, uint32_t &to,
+                                              uint32_t &wt) {
+    if (PACKED) {
+        const uint32_t x = ((const uint32_t *)src)[i];
+        to = x >> wbits;
+        wt = x & ((1u << wbits) - 1u);
+    } else {
+        load_edge<W>(src, i, to, wt);
+    }
+}
+
+// Dynamic region, edges [c0, c1) of the concatenated active dynamic edge
+// space.  STAGED (DIRECT = false): they sit in a staging slot, edge q at
+// q - c0.  DIRECT: they are read in place from the pinned host array through
+// its device mapping, edge q of vertex u at ptr[u] - dbase + (q - doff[i]).
+// Each block takes kDynEdgesPerBlock consecutive edges, loads the covering
+// slice of the degree prefix into LDS and maps each edge to its source with a
+// binary search there (every listed vertex has >= 1 edge, so a block covers
+// at most kDynEdgesPerBlock vertices).
+template <int ALGO, bool W, bool PACKED, bool DIRECT>
 __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict__ dlist,
                                                       const uint64_t *__restrict__ doff, uint32_t ndyn,
                                                       const void *__restrict__ stage, uint64_t c0, uint64_t c1,
-                                                      RelaxArgs a, unsigned long long *upd) {
+                                                      const uint64_t *__restrict__ ptr, uint64_t dbase,
+                                                      uint32_t wbits, RelaxArgs a, unsigned long long *upd) {
     __shared__ uint64_t s_off[kDynEdgesPerBlock + 2];
     __shared__ uint32_t s_i[2];
     const uint64_t q0 = c0 + (uint64_t)blockIdx.x * kDynEdgesPerBlock;
@@ -383,7 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
             const uint32_t u = dlist[i0 + lo];
             const auto s = load_src<ALGO>(a, u);
             uint32_t to, wt;
-            load_edge<W>(stage, q - c0, to, wt);
+            const uint64_t idx = DIRECT ? ptr[u] - dbase + (q - s_off[lo]) : q - c0;
+            load_dyn_edge<W, PACKED>(stage, idx, wbits, to, wt);
             c += relax_edge<ALGO>(a, s, to, wt);
         }
     }
@@ -507,6 +564,7 @@ void device_state_free(DeviceState *d) {
     if (!d) return;
     if (d->s_compute) (void)hipStreamSynchronize(d->s_compute);
     if (d->s_copy) (void)hipStreamSynchronize(d->s_copy);
+    if (d->s_static) (void)hipStreamSynchronize(d->s_static);
     for (int s = 0; s < kSlots; ++s) {
         if (d->ev_h2d[s]) (void)hipEventDestroy(d->ev_h2d[s]);
         if (d->ev_relax[s]) (void)hipEventDestroy(d->ev_relax[s]);
@@ -514,6 +572,8 @@ void device_state_free(DeviceState *d) {
     d->mem.release();
     if (d->s_compute) (void)hipStreamDestroy(d->s_compute);
     if (d->s_copy) (void)hipStreamDestroy(d->s_copy);
+    if (d->s_static) (void)hipStreamDestroy(d->s_static);
+    if (d->ev_static) (void)hipEventDestroy(d->ev_static);
     delete d;
 }
 
@@ -533,7 +593,7 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) 
     f.list = (uint32_t *)m.dalloc(V * sizeof(uint32_t));
     f.doff = (uint64_t *)m.dalloc((V + 1) * sizeof(uint64_t));
     f.nblk = (uint32_t)((d->Vpad + kVertsPerBlock - 1) / kVertsPerBlock);
-    f.bcnt = (uint32_t *)m.dalloc(5ull * f.nblk * sizeof(uint32_t));
+    f.bcnt = (uint32_t *)m.dalloc((uint64_t)kBcntRows * f.nblk * sizeof(uint32_t));
     f.bdeg = (uint64_t *)m.dalloc(2ull * f.nblk * sizeof(uint64_t));
     f.info = (FrontierInfo *)m.dalloc(sizeof(FrontierInfo));
     f.counters = (unsigned long long *)m.dalloc(16 * sizeof(unsigned long long));
@@ -545,13 +605,14 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) 
     OG_HIP(hipMemset(f.flag, 0, d->Vpad));
 }
 
-void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s) {
+void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val, uint32_t thresh) {
     const uint64_t V = d->V;
+    if (thresh == 0xFFFFFFFFu) val = nullptr;
     hipLaunchKernelGGL(k_frontier<false>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, V, d->Vpad, d->plan.vsplit,
-                       d->d_ptr, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.doff);
+                       d->d_ptr, d->tune, val, thresh, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.doff);
     hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff);
     hipLaunchKernelGGL(k_frontier<true>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, V, d->Vpad, d->plan.vsplit,
-                       d->d_ptr, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.doff);
+                       d->d_ptr, d->tune, val, thresh, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.doff);
     OG_HIP(hipGetLastError());
     OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
 }
@@ -607,6 +668,8 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
 
     OG_HIP(hipStreamCreateWithFlags(&d->s_compute, hipStreamNonBlocking));
     OG_HIP(hipStreamCreateWithFlags(&d->s_copy, hipStreamNonBlocking));
+    OG_HIP(hipStreamCreateWithFlags(&d->s_static, hipStreamNonBlocking));
+    OG_HIP(hipEventCreateWithFlags(&d->ev_static, hipEventDisableTiming));
     for (int s = 0; s < kSlots; ++s) {
         OG_HIP(hipEventCreateWithFlags(&d->ev_h2d[s], hipEventDisableTiming));
         OG_HIP(hipEventCreateWithFlags(&d->ev_relax[s], hipEventDisableTiming));
@@ -646,6 +709,46 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
             d->h_dlist = (uint32_t *)d->palloc(V * sizeof(uint32_t));
             d->h_doff = (uint64_t *)d->palloc((V + 1) * sizeof(uint64_t));
             d->pool.reset(new WorkerPool(d->host_threads));
+            // Transfer format of the dynamic region.  The link is the bound,
+            // so weighted edges travel as one 4-byte word (to << wbits | w)
+            // whenever ids and weights fit; built once, here.
+            d->dyn_base = g->ptr[pl.vsplit];
+            d->deb = (uint32_t)eb;
+            d->h_dyn = (const uint8_t *)g->edge_data() + d->dyn_base * eb;
+            const uint64_t nd = E - d->dyn_base;
+            if (eb == 8 && nd && !(p.flags & OG_PLAN_NO_PACK)) {
+                std::atomic<uint32_t> maxw{0};
+                parallel_for(nd, 1 << 22, 0, [&](uint64_t b, uint64_t e, unsigned) {
+                    uint32_t m = 0;
+                    for (uint64_t i = b; i < e; ++i) m = std::max(m, g->wedges[2 * (d->dyn_base + i) + 1]);
+                    uint32_t cur = maxw.load();
+                    while (m > cur && !maxw.compare_exchange_weak(cur, m)) {
+                    }
+                });
+                uint32_t wb = 1;
+                while (wb < 32 && (maxw.load() >> wb)) ++wb;
+                if (wb < 32 && V - 1 < (1ull << (32 - wb))) {
+                    uint32_t *pk = (uint32_t *)d->palloc(nd * 4);
+                    const uint32_t *src = g->wedges + 2 * d->dyn_base;
+                    parallel_for(nd, 1 << 22, 0, [&](uint64_t b, uint64_t e, unsigned) {
+                        for (uint64_t i = b; i < e; ++i) pk[i] = (src[2 * i] << wb) | src[2 * i + 1];
+                    });
+                    d->packed = true;
+                    d->wbits = wb;
+                    d->deb = 4;
+                    d->h_dyn = pk;
+                }
+            }
+            d->slot_dyn = d->slot_edges * eb / d->deb;
+            if (p.flags & OG_PLAN_ZERO_COPY) {
+                if (!d->packed && !(eb == 8 ? g->pinned_wedges : g->pinned_cols))
+                    raise(OG_EUNSUPPORTED, "zero-copy loading needs the graph in pinned host memory");
+                // map the allocation's base, then offset to the dynamic region
+                const void *hbase = d->packed ? d->h_dyn : g->edge_data();
+                void *hd = nullptr;
+                OG_HIP(hipHostGetDevicePointer(&hd, (void *)hbase, 0));
+                d->hd_dyn = (const uint8_t *)hd + ((const uint8_t *)d->h_dyn - (const uint8_t *)hbase);
+            }
         }
     }
 
@@ -666,10 +769,13 @@ struct RunTimers {
 
 // Host gather of dynamic edges [c0, c1) into a pinned slot (reference
 // fillEdgeArrByMultiThread, GraphMeta.h:470-506, with the copy offsets taken
-// from the device-built degree prefix instead of a serial host scan).
+// from the device-built degree prefix instead of a serial host scan).  The
+// source is the dynamic region in its transfer format (d->h_dyn, whose first
+// element is edge d->dyn_base of the CSR).
 static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0, uint64_t c1, uint8_t *dst) {
-    const uint64_t eb = d->eb;
-    const uint8_t *src = (const uint8_t *)g->edge_data();
+    const uint64_t eb = d->deb;
+    const uint8_t *src = (const uint8_t *)d->h_dyn;
+    const uint64_t base = d->dyn_base;
     const uint64_t n = c1 - c0;
     const unsigned T = d->pool->size();
     const unsigned use = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(T, n / 16384));
@@ -685,7 +791,7 @@ static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0
             const uint32_t v = d->h_dlist[i];
             const uint64_t vb = d->h_doff[i], ve = d->h_doff[i + 1];
             const uint64_t stop = std::min(ve, p1);
-            std::memcpy(dst + (p - c0) * eb, src + (g->ptr[v] + (p - vb)) * eb, (stop - p) * eb);
+            std::memcpy(dst + (p - c0) * eb, src + (g->ptr[v] - base + (p - vb)) * eb, (stop - p) * eb);
             p = stop;
             ++i;
         }
@@ -723,11 +829,41 @@ void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo &fi, co
 template void launch_static_relax<ALGO_MSBFS, false>(DeviceState *, Frontier &, const FrontierInfo &, const RelaxArgs &, hipStream_t);
 template void launch_static_relax<ALGO_MSBFS, true>(DeviceState *, Frontier &, const FrontierInfo &, const RelaxArgs &, hipStream_t);
 
+// Static-region relax on its own stream, so it overlaps the dynamic region
+// (link bound) of the same round; returns true when launched (the caller
+// joins on ev_static before the next compaction).  All relax updates are
+// monotone (atomicMin / same-level stores / atomicOr), so the two regions
+// may interleave.
 template <int ALGO, bool W>
-static void launch_static(DeviceState *d, const FrontierInfo &fi, RunTimers &tm) {
-    if (!(fi.n[0] | fi.n[1] | fi.n[2])) return;
-    EventSpan span(d, &tm.stat, d->s_compute);
-    launch_static_relax<ALGO, W>(d, d->fr, fi, relax_args(d), d->s_compute);
+static bool launch_static(DeviceState *d, const FrontierInfo &fi, RunTimers &tm) {
+    if (!(fi.n[0] | fi.n[1] | fi.n[2])) return false;
+    {
+        EventSpan span(d, &tm.stat, d->s_static);
+        launch_static_relax<ALGO, W>(d, d->fr, fi, relax_args(d), d->s_static);
+    }
+    OG_HIP(hipEventRecord(d->ev_static, d->s_static));
+    return true;
+}
+
+template <int ALGO, bool W, bool PACKED, bool DIRECT>
+static void launch_dyn(DeviceState *d, const uint32_t *dlist, uint32_t ndyn, const void *src, uint64_t c0,
+                       uint64_t c1) {
+    hipLaunchKernelGGL((k_relax_dyn<ALGO, W, PACKED, DIRECT>), dim3(grid_for(c1 - c0, kDynEdgesPerBlock)),
+                       dim3(kBlock), 0, d->s_compute, dlist, d->fr.doff, ndyn, src, c0, c1, d->d_ptr, d->dyn_base,
+                       d->wbits, relax_args(d), d->fr.counters + 1);
+}
+
+template <int ALGO, bool W>
+static void launch_dyn_any(DeviceState *d, const uint32_t *dlist, uint32_t ndyn, const void *src, uint64_t c0,
+                           uint64_t c1, bool direct) {
+    if (d->packed) {
+        if (direct) launch_dyn<ALGO, W, true, true>(d, dlist, ndyn, src, c0, c1);
+        else launch_dyn<ALGO, W, true, false>(d, dlist, ndyn, src, c0, c1);
+    } else {
+        if (direct) launch_dyn<ALGO, W, false, true>(d, dlist, ndyn, src, c0, c1);
+        else launch_dyn<ALGO, W, false, false>(d, dlist, ndyn, src, c0, c1);
+    }
+    OG_HIP(hipGetLastError());
 }
 
 template <int ALGO, bool W>
@@ -736,12 +872,21 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
     const uint32_t ndyn = fi.n[3];
     const uint64_t D = fi.dyn_deg;
     if (!ndyn || !D) return;
-    const RelaxArgs a = relax_args(d);
     const uint32_t *d_dlist = d->fr.list + fi.n[0] + fi.n[1];
+    const uint64_t eb = d->deb;
+    if (d->flags & OG_PLAN_ZERO_COPY) {
+        // one launch reads every active dynamic edge over the link
+        EventSpan span(d, &tm.dyn, d->s_compute);
+        launch_dyn_any<ALGO, W>(d, d_dlist, ndyn, d->hd_dyn, 0, D, true);
+        st.h2d_bytes += D * eb;
+        st.chunks++;
+        return;
+    }
     OG_HIP(hipMemcpyAsync(d->h_dlist, d_dlist, (size_t)ndyn * 4, hipMemcpyDeviceToHost, d->s_copy));
     OG_HIP(hipMemcpyAsync(d->h_doff, d->fr.doff, ((size_t)ndyn + 1) * 8, hipMemcpyDeviceToHost, d->s_copy));
     OG_HIP(hipStreamSynchronize(d->s_copy));
-    const uint64_t C = d->slot_edges, eb = d->eb;
+    OG_HIP(hipStreamSynchronize(d->s_static));
+    const uint64_t C = d->slot_dyn;
     const bool serial = (d->flags & OG_PLAN_NO_OVERLAP) != 0;
     for (uint64_t c0 = 0, c = 0; c0 < D; c0 += C, ++c) {
         const uint64_t c1 = std::min(D, c0 + C);
@@ -750,7 +895,7 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
         Stopwatch sw;
         gather(d, g, ndyn, c0, c1, (uint8_t *)d->h_stage[slot]);
         gather_ms += sw.ms();
-        uint8_t *dslot = (uint8_t *)d->d_staging + (uint64_t)slot * C * eb;
+        uint8_t *dslot = (uint8_t *)d->d_staging + (uint64_t)slot * d->slot_edges * d->eb;
         if (d->slot_relax_pending[slot]) OG_HIP(hipStreamWaitEvent(d->s_copy, d->ev_relax[slot], 0));
         {
             EventSpan span(d, &tm.copy, d->s_copy);
@@ -761,10 +906,7 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
         OG_HIP(hipStreamWaitEvent(d->s_compute, d->ev_h2d[slot], 0));
         {
             EventSpan span(d, &tm.dyn, d->s_compute);
-            hipLaunchKernelGGL((k_relax_dyn<ALGO, W>), dim3(grid_for(c1 - c0, kDynEdgesPerBlock)), dim3(kBlock), 0,
-                               d->s_compute, d_dlist, d->fr.doff, ndyn, (const void *)dslot, c0, c1, a,
-                               d->fr.counters + 1);
-            OG_HIP(hipGetLastError());
+            launch_dyn_any<ALGO, W>(d, d_dlist, ndyn, (const void *)dslot, c0, c1, false);
         }
         OG_HIP(hipEventRecord(d->ev_relax[slot], d->s_compute));
         d->slot_relax_pending[slot] = true;
@@ -788,15 +930,25 @@ static void frontier_loop(og_graph *g, og_run_stats &st, RunTimers &tm, double &
                           After after) {
     DeviceState *d = g->dev;
     hipStream_t s = d->s_compute;
+    // SSSP near-far order: only vertices below `thresh` relax in a round; when
+    // none is left the threshold moves to (smallest held-back value + delta).
+    const uint32_t delta = ALGO == ALGO_SSSP ? d->tune.delta : 0;
+    uint32_t thresh = delta ? (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, 1ull + delta) : 0xFFFFFFFFu;
     for (;;) {
         {
             EventSpan span(d, &tm.front, s);
-            frontier_compact(d, d->fr, s);
+            frontier_compact(d, d->fr, s, delta ? d->d_val : nullptr, thresh);
         }
         OG_HIP(hipStreamSynchronize(s));
         const FrontierInfo fi = *d->fr.h_info;
         const uint64_t active = (uint64_t)fi.n[0] + fi.n[1] + fi.nhub + fi.n[3] + fi.nzero;
-        if (active == 0) break;
+        if (active == 0) {
+            if (delta && fi.nfar) {
+                thresh = (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, (uint64_t)fi.min_far + delta);
+                continue;
+            }
+            break;
+        }
         if (fi.n[2] > d->fr.hub_cap)
             raise(OG_EHIP, "hub tile list overflow (%u > %llu)", fi.n[2], (unsigned long long)d->fr.hub_cap);
         st.iterations++;
@@ -805,14 +957,15 @@ static void frontier_loop(og_graph *g, og_run_stats &st, RunTimers &tm, double &
         st.work_edges += fi.static_deg + fi.dyn_deg;
         st.static_work_edges += fi.static_deg;
         st.relax_static_bytes += 24ull * ((uint64_t)fi.n[0] + fi.n[1] + fi.nhub) + edge_bytes_alg<ALGO, W>() * fi.static_deg;
-        st.relax_dynamic_bytes += 24ull * fi.n[3] + edge_bytes_alg<ALGO, W>() * fi.dyn_deg;
+        st.relax_dynamic_bytes += 24ull * fi.n[3] + (edge_bytes_alg<ALGO, W>() - (W ? 8ull : 4ull) + d->deb) * fi.dyn_deg;
         st.frontier_bytes += 2ull * d->Vpad + 4ull * active;
-        launch_static<ALGO, W>(d, fi, tm);
+        const bool joined = launch_static<ALGO, W>(d, fi, tm);
         if (fi.n[3]) {
             Stopwatch ow;
             run_dynamic<ALGO, W>(d, g, fi, tm, st, gather_ms);
             overload_wall += ow.ms();
         }
+        if (joined) OG_HIP(hipStreamWaitEvent(s, d->ev_static, 0));
         after(st.iterations);
     }
 }
@@ -848,6 +1001,7 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
     hipStream_t s = d->s_compute;
 
     OG_HIP(hipStreamSynchronize(d->s_copy));
+    OG_HIP(hipStreamSynchronize(d->s_static));
     OG_HIP(hipStreamSynchronize(s));
     Stopwatch total;
     OG_HIP(hipMemsetAsync(d->fr.counters, 0, 16 * sizeof(unsigned long long), s));
@@ -901,6 +1055,7 @@ static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t
     d->events.reset();
     double gather_ms = 0, overload_wall = 0;
     OG_HIP(hipStreamSynchronize(d->s_copy));
+    OG_HIP(hipStreamSynchronize(d->s_static));
     OG_HIP(hipStreamSynchronize(s));
     Stopwatch total;
     OG_HIP(hipMemsetAsync(d->fr.counters, 0, 16 * sizeof(unsigned long long), s));
@@ -1051,4 +1206,144 @@ extern "C" og_status og_levels_checksum(const uint32_t *values, uint64_t num_ver
         if (values[v] != OG_UNREACHED) h += mix64((v << 32) | values[v]);
     *checksum = h;
     return OG_OK;
+}
+
+// ------------------------------------------------------------------ tuning
+
+namespace og {
+
+static og_status check_tuning(const og_tuning *t) {
+    const uint32_t l = t->lpv0;
+    if (!(l == 1 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64))
+        return fail(OG_EINVAL, "lpv0 must be a power of two in [1, 64]");
+    if (t->t1 < 1 || t->t2 < t->t1) return fail(OG_EINVAL, "need 1 <= t1 <= t2");
+    if (t->tile < 64 || t->tile > 4096) return fail(OG_EINVAL, "tile must be in [64, 4096]");
+    return OG_OK;
+}
+
+static void apply_tuning(og_graph *g, const og_tuning *t) {
+    DeviceState *d = g->dev;
+    d->tune.t1 = t->t1;
+    d->tune.lpv0 = t->lpv0;
+    d->tune.t2 = t->t2;
+    d->tune.tile = t->tile;
+    d->tune.delta = t->delta;
+    const uint64_t cap = hub_capacity(g, d->plan.vsplit, d->tune);
+    if (cap > d->fr.hub_cap) {
+        d->fr.hub = (uint64_t *)d->dalloc(cap * sizeof(uint64_t));
+        d->fr.hub_cap = cap;
+    }
+    d->pr_plan.reset();  // PageRank bins follow t1 / t2
+}
+
+static og_tuning tuning_of(const DeviceState *d) {
+    return og_tuning{d->tune.t1, d->tune.lpv0, d->tune.t2, d->tune.tile, d->tune.delta};
+}
+
+static double time_algo(og_graph *g, uint32_t algo, uint32_t src, uint32_t runs) {
+    std::vector<double> ts;
+    const bool w = g->kind == OG_KIND_WEIGHTED;
+    for (uint32_t r = 0; r < runs; ++r) {
+        og_run_stats st;
+        if (algo == OG_TUNE_BFS) {
+            if (w) run_frontier<ALGO_BFS, true>(g, src, nullptr, &st);
+            else run_frontier<ALGO_BFS, false>(g, src, nullptr, &st);
+        } else if (algo == OG_TUNE_SSSP) {
+            run_frontier<ALGO_SSSP, true>(g, src, nullptr, &st);
+        } else {
+            if (w) run_frontier<ALGO_CC, true>(g, 0, nullptr, &st);
+            else run_frontier<ALGO_CC, false>(g, 0, nullptr, &st);
+        }
+        ts.push_back(st.total_ms);
+    }
+    std::sort(ts.begin(), ts.end());
+    return ts[ts.size() / 2];
+}
+
+}  // namespace og
+
+extern "C" og_status og_get_tuning(const og_graph *g, og_tuning *t) {
+    clear_error();
+    if (!g || !t) return fail(OG_EINVAL, "null argument");
+    if (!g->dev) return fail(OG_ESTATE, "og_prepare has not been called for this graph");
+    *t = tuning_of(g->dev);
+    return OG_OK;
+}
+
+extern "C" og_status og_set_tuning(og_graph *g, const og_tuning *t) {
+    clear_error();
+    if (!g || !t) return fail(OG_EINVAL, "null argument");
+    if (!g->dev) return fail(OG_ESTATE, "og_prepare has not been called for this graph");
+    og_status st = check_tuning(t);
+    if (st != OG_OK) return st;
+    int expect = 0;
+    if (!g->busy.compare_exchange_strong(expect, 1)) return fail(OG_EBUSY, "graph is running another call");
+    st = guard([&]() -> og_status {
+        apply_tuning(g, t);
+        return OG_OK;
+    });
+    g->busy.store(0);
+    return st;
+}
+
+extern "C" og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, uint32_t runs, og_tuning *best,
+                                 double *best_ms) {
+    clear_error();
+    if (!g) return fail(OG_EINVAL, "graph is NULL");
+    if (!g->dev) return fail(OG_ESTATE, "og_prepare has not been called for this graph");
+    if (algo > OG_TUNE_CC) return fail(OG_EINVAL, "algo must be OG_TUNE_BFS, OG_TUNE_SSSP or OG_TUNE_CC");
+    if (g->kind == OG_KIND_PULL) return fail(OG_EUNSUPPORTED, "pull-layout graphs serve PageRank only");
+    if (algo == OG_TUNE_SSSP && g->kind != OG_KIND_WEIGHTED) return fail(OG_EUNSUPPORTED, "SSSP needs weights");
+    if (algo != OG_TUNE_CC && source >= g->V) return fail(OG_EINVAL, "source out of range");
+    if (runs == 0) runs = 1;
+    int expect = 0;
+    if (!g->busy.compare_exchange_strong(expect, 1)) return fail(OG_EBUSY, "graph is running another call");
+    og_status st = guard([&]() -> og_status {
+        OG_HIP(hipSetDevice(g->dev->device));
+        og_tuning cur = tuning_of(g->dev);
+        (void)time_algo(g, algo, source, 1);  // warm-up
+        double cur_ms = time_algo(g, algo, source, runs);
+        auto trial = [&](og_tuning cand) {
+            if (check_tuning(&cand) != OG_OK) return;
+            apply_tuning(g, &cand);
+            const double ms = time_algo(g, algo, source, runs);
+            if (ms < cur_ms) {
+                cur_ms = ms;
+                cur = cand;
+            }
+            apply_tuning(g, &cur);
+        };
+        for (int pass = 0; pass < 2; ++pass) {
+            if (algo == OG_TUNE_SSSP)
+                for (uint32_t dl : {0u, 8u, 16u, 32u, 64u, 128u, 256u, 512u}) {
+                    og_tuning c = cur;
+                    c.delta = dl;
+                    trial(c);
+                }
+            for (uint32_t t1 : {4u, 8u, 16u, 32u, 64u})
+                for (uint32_t l : {2u, 4u, 8u, 16u}) {
+                    og_tuning c = cur;
+                    c.t1 = t1;
+                    c.lpv0 = l;
+                    if (c.t2 < t1) c.t2 = t1;
+                    trial(c);
+                }
+            for (uint32_t t2 : {512u, 1024u, 2048u, 4096u, 16384u, 65536u}) {
+                og_tuning c = cur;
+                c.t2 = std::max(t2, c.t1);
+                trial(c);
+            }
+            for (uint32_t tile : {512u, 1024u, 2048u, 4096u}) {
+                og_tuning c = cur;
+                c.tile = tile;
+                trial(c);
+            }
+        }
+        apply_tuning(g, &cur);
+        if (best) *best = cur;
+        if (best_ms) *best_ms = cur_ms;
+        return OG_OK;
+    });
+    g->busy.store(0);
+    return st;
 }

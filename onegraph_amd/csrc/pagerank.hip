@@ -277,7 +277,10 @@ static void run_pr(og_graph *g, const og_pr_params &prm, T *out, og_pr_stats *st
         OG_HIP(hipMemcpy(d->d_outdeg, g->out_degree, V * 4, hipMemcpyHostToDevice));
         d->d_scalars = (double *)d->dalloc(4 * sizeof(double));
         d->h_scalars = (double *)d->palloc(4 * sizeof(double));
+    }
+    if (pp.partial_cap > d->pr_partial_cap) {
         d->d_partial = (double *)d->dalloc(pp.partial_cap * sizeof(double));
+        d->pr_partial_cap = pp.partial_cap;
     }
     if (!d->d_rank[0] || d->pr_fp64 != f64) {
         // sized for double so either precision fits

@@ -16,6 +16,9 @@ PLANS = [
     ("ascetic", 0.25, 0.8),    # the out-of-memory configuration of the benchmarks
     ("ascetic", 1 / 64, 0.5),  # tiny staging: many chunks through both slots
     ("ascetic", 0.25, 0.0),    # reference auto-advisek rule
+    ("ascetic+zerocopy", 0.25, 0.8),  # dynamic edges read in place over the link
+    ("ascetic+nopack", 1 / 64, 0.5),  # 8-byte weighted edges on the link
+    ("subway+zerocopy", 0.25, 0.0),
     ("subway", 0.25, 0.0),
     ("subway", 1 / 64, 0.0),
     ("usm", 0.0, 0.0),
@@ -30,7 +33,9 @@ def _say(*a):
 def _prepare(g, approach, frac, k, edge_bytes):
     E = g.num_edges
     budget = int(max(1, E * edge_bytes * frac)) if frac else 0
-    return g.prepare(approach, edge_budget_bytes=budget, advisek=k)
+    name, _, opt = approach.partition("+")
+    flags = {"": 0, "zerocopy": og.PLAN_ZERO_COPY, "nopack": og.PLAN_NO_PACK}[opt]
+    return g.prepare(name, edge_budget_bytes=budget, advisek=k, flags=flags)
 
 
 def _sources(ptr, E, n, seed):
@@ -149,7 +154,8 @@ def test_errors(gpu):
 
 
 @pytest.mark.parametrize("approach,frac,k", [("ascetic", 1.0, 1.0), ("ascetic", 0.25, 0.8), ("ascetic", 1 / 64, 0.5),
-                                              ("subway", 0.25, 0.0), ("usm", 0.0, 0.0)])
+                                              ("ascetic+zerocopy", 0.25, 0.8), ("subway", 0.25, 0.0),
+                                              ("usm", 0.0, 0.0)])
 def test_msbfs(gpu, rmat_w, approach, frac, k):
     ptr, to, w, _ = rmat_w.export()
     _say("msbfs rmat13", approach, frac, k)
