@@ -99,6 +99,17 @@ def roofline_of(stats_list):
     return roof, link, kt
 
 
+def plan_flags(args, approach, loader=None):
+    """og_plan_params.flags for a workload: the dynamic-region loader and
+    the link format (USM reads the CSR in place and takes neither)."""
+    import onegraph_amd as og
+
+    if approach == "usm":
+        return 0
+    loader = loader or args.loader
+    return (og.PLAN_ZERO_COPY if loader == "zerocopy" else 0) | (og.PLAN_NO_PACK if args.no_pack else 0)
+
+
 def timed(fn, steps, warmup, label):
     for i in range(warmup):
         r = fn()
@@ -161,14 +172,15 @@ def wl_cc_ooc(args):
     g = er_graph(args)
     ptr, cols, _, _ = g.export()
     V, E = len(ptr), len(cols)
-    plan = g.prepare("ascetic", edge_budget_bytes=E, advisek=0.8)
+    plan = g.prepare("ascetic", edge_budget_bytes=E, advisek=0.8, flags=plan_flags(args, "ascetic"))
     stats, wall = timed(lambda: g.cc(want_values=False)[1], args.steps, args.warmup, "cc-ooc")
     lab, _ = g.cc()
     ms = 1e3 * wall / args.steps
     cmp = {}
     if args.compare:
         for ap_name in ["subway", "usm"]:
-            g.prepare(ap_name, edge_budget_bytes=E if ap_name == "subway" else 0)
+            g.prepare(ap_name, edge_budget_bytes=E if ap_name == "subway" else 0,
+                      flags=plan_flags(args, ap_name, "copy"))
             s2, w2 = timed(lambda: g.cc(want_values=False)[1], 1, 0, f"cc-{ap_name}")
             cmp[ap_name] = {"ms": round(1e3 * w2, 2), "gteps": round(E / w2 / 1e9, 4), "h2d_bytes": s2[0]["h2d_bytes"]}
     t = time.perf_counter()
@@ -177,7 +189,7 @@ def wl_cc_ooc(args):
     roof, link, kt = roofline_of(stats)
     out = base_record(round(E / (ms * 1e-3) / 1e9, 4), "GTEPS", args.steps, args.warmup, ms, "u32",
                       {"workload": f"cc out-of-memory er n=2^{args.scale} avg 32", "V": V, "E": E,
-                       "edge_budget_bytes": E, "advisek": 0.8, "approach": "ascetic",
+                       "edge_budget_bytes": E, "advisek": 0.8, "approach": "ascetic", "loader": args.loader,
                        "static_edges": plan["static_edges"], "staging_edges": plan["staging_edges"]})
     out.update({"roofline": roof, "host_link": link, "kernels": kt, "iterations": stats[-1]["iterations"],
                 "h2d_bytes_per_step": stats[-1]["h2d_bytes"], "compare": cmp,
@@ -249,7 +261,7 @@ def wl_msbfs(args):
     ptr, cols, _, _ = g.export()
     deg = np.diff(np.append(ptr.astype(np.int64), E))
     srcs = np.random.default_rng(3004).choice(np.flatnonzero(deg > 0), args.sources, replace=False).astype(np.uint32)
-    plan = g.prepare("ascetic", edge_budget_bytes=E * 4 // 2, advisek=0.8)
+    plan = g.prepare("ascetic", edge_budget_bytes=E * 4 // 2, advisek=0.8, flags=plan_flags(args, "ascetic"))
     log(f"plan {plan}")
     if args.warmup:
         g.msbfs(srcs[:64], want_levels=False)
@@ -276,7 +288,7 @@ def wl_msbfs(args):
     out = base_record(round(trav / ms_wall / 1e9, 4), "GTEPS", 1, args.warmup, 1e3 * ms_wall, "u64-mask",
                       {"workload": f"multi-source bfs rmat{args.scale} ef{args.edge_factor} {len(srcs)} sources",
                        "V": V, "E": E, "edge_budget_bytes": E * 4 // 2, "advisek": 0.8, "approach": "ascetic",
-                       "batch": 64})
+                       "loader": args.loader, "batch": 64})
     roof, link, kt = roofline_of([st])
     out.update({"queries_per_s": round(qps, 2), "plain_queries_per_s": round(1 / per_plain, 3),
                 "speedup_vs_plain": round(qps * per_plain, 2),
@@ -493,11 +505,10 @@ def main():
     src, deg = pick_source(ptr, E, 3002)
     budget = int(E * 8 * args.budget)
 
-    def run_approach(approach, steps, warmup):
+    def run_approach(approach, steps, warmup, loader=None):
         k = args.advisek if approach == "ascetic" else 0.0
-        flags = (og.PLAN_ZERO_COPY if args.loader == "zerocopy" else 0) | (og.PLAN_NO_PACK if args.no_pack else 0)
-        plan = g.prepare(approach, edge_budget_bytes=budget if approach != "usm" else 0, advisek=k,
-                         flags=flags if approach != "usm" else 0)
+        flags = plan_flags(args, approach, loader)
+        plan = g.prepare(approach, edge_budget_bytes=budget if approach != "usm" else 0, advisek=k, flags=flags)
         log(f"{approach}: plan {plan}")
         if args.delta is not None:
             g.set_tuning(delta=args.delta)
@@ -533,6 +544,8 @@ def main():
     trav = stats[-1]["traversed_edges"]
     gteps = trav / (ms_per_step * 1e-3) / 1e9
 
+    # weighted edges in HBM and on the link: packed (to << 6 | w) or the CSR's {to, w}
+    edge_format = "8-byte {to,w}" if args.no_pack else "4-byte to<<6|w"
     agg = {}
     for key in ["relax_static_kernel_ms", "relax_static_launches", "relax_static_bytes", "relax_dynamic_kernel_ms",
                 "relax_dynamic_launches", "relax_dynamic_bytes", "frontier_kernel_ms", "frontier_launches",
@@ -553,7 +566,7 @@ def main():
             with open(tpath) as f:
                 tr = json.load(f)
             want = {"scale": args.scale, "approach": args.approach, "loader": args.loader, "delta": args.delta,
-                    "kernel_class": dom}
+                    "kernel_class": dom, "edge_format": edge_format}
             if tr.get("config") == want:
                 roof["traffic"] = tr["traffic_bytes_per_launch"]
                 roof["traffic_source"] = tr["source"]
@@ -589,7 +602,7 @@ def main():
             "edge_budget_bytes": budget, "budget_fraction": args.budget, "advisek": args.advisek,
             "approach": args.approach, "static_edges": plan["static_edges"], "staging_edges": plan["staging_edges"],
             "vsplit": plan["vsplit"], "loader": args.loader, "delta": args.delta,
-            "link_format": "8-byte {to,w}" if args.no_pack else "4-byte to<<6|w",
+            "edge_format": edge_format,
         },
         "roofline": roof,
         "host_link": link,
@@ -605,10 +618,12 @@ def main():
         for ap_name in ["ascetic", "subway", "usm"]:
             if ap_name == args.approach:
                 continue
-            p2, s2, w2 = run_approach(ap_name, max(1, args.steps // 2), 1)
+            # Subway as designed: the active subgraph is gathered and copied
+            p2, s2, w2 = run_approach(ap_name, max(1, args.steps // 2), 1, loader="copy")
             ms2 = 1e3 * w2 / len(s2)
             cmp[ap_name] = {"ms_per_step": round(ms2, 3), "gteps": round(trav / (ms2 * 1e-3) / 1e9, 4),
-                            "h2d_bytes": s2[-1]["h2d_bytes"], "iterations": s2[-1]["iterations"]}
+                            "h2d_bytes": s2[-1]["h2d_bytes"], "iterations": s2[-1]["iterations"],
+                            "loader": "copy" if ap_name == "subway" else "mapped host memory"}
         out["compare"] = cmp
 
     # correctness of the measured path + CPU baseline (oracle, single thread)

@@ -40,6 +40,7 @@ struct RelaxArgs {
     uint64_t *visit;      // multi-source: sources that reached v
     const uint64_t *front;// multi-source: sources whose frontier holds v
     uint64_t *next;       // multi-source: sources offered to v this round
+    uint32_t wbits;       // packed weighted edges: bits of the weight field
 };
 
 // Degree-bin thresholds and shapes for the static relax (autotunable).
@@ -151,7 +152,7 @@ struct DeviceState {
     og_approach approach = OG_ASCETIC;
     int device = 0;
     uint64_t V = 0, E = 0, Vpad = 0;
-    uint32_t eb = 4;                 // edge bytes
+    uint32_t eb = 4;                 // bytes per edge on the device (static region, staging) and the link
     og_plan_info plan{};
     Tuning tune;
     unsigned host_threads = 0;
@@ -164,10 +165,11 @@ struct DeviceState {
     void *d_staging = nullptr;       // kSlots * slot_edges edges
     uint64_t slot_edges = 0;
     bool managed = false;
-    // dynamic-region transfer format: 4-byte (to << wbits | w) when packed
+    // Weighted edges on the device and the link: one 4-byte word
+    // (to << wbits | w) when packed, else the CSR's 8-byte {to, w}.
     bool packed = false;
     uint32_t wbits = 0;
-    uint32_t deb = 4;                // bytes per dynamic-region edge on the link
+    uint32_t deb = 4;                // bytes per dynamic-region edge on the link (= eb)
     uint64_t dyn_base = 0;           // first edge of the dynamic region (= ptr[vsplit])
     const void *h_dyn = nullptr;     // host source of dynamic edges (packed copy or the CSR)
     const void *hd_dyn = nullptr;    // the same, as a device address (zero-copy mode)

@@ -248,9 +248,16 @@ __global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *bcnt, uint64_t
 
 // ---------------------------------------------------------------- relax
 
-template <bool W>
-__device__ __forceinline__ void load_edge(const void *__restrict__ edges, uint64_t i, uint32_t &to, uint32_t &wt) {
-    if (W) {
+// Edge i of an edge array: 4-byte column (unweighted), the packed 4-byte
+// weighted word (to << wbits | w), or the CSR's 8-byte AoS {to, w}.
+template <bool W, bool PK>
+__device__ __forceinline__ void load_edge(const void *__restrict__ edges, uint64_t i, uint32_t wbits, uint32_t &to,
+                                          uint32_t &wt) {
+    if (W && PK) {
+        const uint32_t x = ((const uint32_t *)edges)[i];
+        to = x >> wbits;
+        wt = x & ((1u << wbits) - 1u);
+    } else if (W) {
         const uint2 e = ((const uint2 *)edges)[i];
         to = e.x;
         wt = e.y;
@@ -321,7 +328,7 @@ __device__ __forceinline__ uint32_t relax_edge(const RelaxArgs &a, typename SrcO
 }
 
 // Bins 0 and 1: LPV lanes per frontier vertex (LPV = 64: one wave).
-template <int ALGO, bool W, int LPV>
+template <int ALGO, bool W, bool PK, int LPV>
 __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restrict__ list, uint32_t n,
                                                         const uint64_t *__restrict__ ptr,
                                                         const void *__restrict__ edges, RelaxArgs a,
@@ -336,7 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restri
         const auto s = load_src<ALGO>(a, u);
         for (uint64_t i = b + sub; i < e; i += LPV) {
             uint32_t to, wt;
-            load_edge<W>(edges, i, to, wt);
+            load_edge<W, PK>(edges, i, a.wbits, to, wt);
             cnt += relax_edge<ALGO>(a, s, to, wt);
         }
     }
@@ -345,12 +352,13 @@ __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restri
 
 // Bin 2: one workgroup per tile of a hub's neighbour list; the tile is
 // staged in LDS with coalesced loads before the relax pass.
-template <int ALGO, bool W>
+template <int ALGO, bool W, bool PK>
 __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict__ hub, uint32_t tile,
                                                       const uint64_t *__restrict__ ptr,
                                                       const void *__restrict__ edges, RelaxArgs a,
                                                       unsigned long long *upd) {
     extern __shared__ uint32_t s_edges[];
+    constexpr bool WIDE = W && !PK;  // 8-byte edges
     const uint64_t it = hub[blockIdx.x];
     const uint32_t u = (uint32_t)it, t = (uint32_t)(it >> 32);
     const uint64_t b = ptr[u] + (uint64_t)t * tile;
@@ -358,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
     const uint32_t cnt = (uint32_t)(e - b);
     const auto s = load_src<ALGO>(a, u);
     for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
-        if (W)
+        if (WIDE)
             ((uint2 *)s_edges)[k] = ((const uint2 *)edges)[b + k];
         else
             s_edges[k] = ((const uint32_t *)edges)[b + k];
@@ -367,31 +375,10 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
     uint32_t c = 0;
     for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
         uint32_t to, wt;
-        if (W) {
-            const uint2 x = ((const uint2 *)s_edges)[k];
-            to = x.x;
-            wt = x.y;
-        } else {
-            to = s_edges[k];
-            wt = 1;
-        }
+        load_edge<W, PK>(s_edges, k, a.wbits, to, wt);
         c += relax_edge<ALGO>(a, s, to, wt);
     }
     wave_add(upd, c);
-}
-
-// One dynamic-region edge: 8-byte AoS {to, w}, 4-byte column, or the packed
-// 4-byte transfer format (to << wbits | w).
-template <bool W, bool PACKED>
-__device__ __forceinline__ void load_dyn_edge(const void *__restrict__ src, uint64_t i, uint32_t wbits, uint32_t &to,
-                                              uint32_t &wt) {
-    if (PACKED) {
-        const uint32_t x = ((const uint32_t *)src)[i];
-        to = x >> wbits;
-        wt = x & ((1u << wbits) - 1u);
-    } else {
-        load_edge<W>(src, i, to, wt);
-    }
 }
 
 // Dynamic region, edges [c0, c1) of the concatenated active dynamic edge
@@ -440,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
             const auto s = load_src<ALGO>(a, u);
             uint32_t to, wt;
             const uint64_t idx = DIRECT ? ptr[u] - dbase + (q - s_off[lo]) : q - c0;
-            load_dyn_edge<W, PACKED>(stage, idx, wbits, to, wt);
+            load_edge<W, PACKED>(stage, idx, wbits, to, wt);
             c += relax_edge<ALGO>(a, s, to, wt);
         }
     }
@@ -617,6 +604,56 @@ void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t
     OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
 }
 
+// Bits needed by the largest edge weight (at least 1).
+static uint32_t weight_bits(const og_graph *g) {
+    std::atomic<uint32_t> maxw{0};
+    parallel_for(g->E, 1 << 22, 0, [&](uint64_t b, uint64_t e, unsigned) {
+        uint32_t m = 0;
+        for (uint64_t i = b; i < e; ++i) m = std::max(m, g->wedges[2 * i + 1]);
+        uint32_t cur = maxw.load();
+        while (m > cur && !maxw.compare_exchange_weak(cur, m)) {
+        }
+    });
+    uint32_t wb = 1;
+    while (wb < 32 && (maxw.load() >> wb)) ++wb;
+    return wb;
+}
+
+// Weighted edges [e0, e0 + n) as packed words (to << wb | w).
+static void pack_edges(const og_graph *g, uint64_t e0, uint64_t n, uint32_t wb, uint32_t *dst) {
+    const uint32_t *src = g->wedges + 2 * e0;
+    parallel_for(n, 1 << 22, 0, [&](uint64_t b, uint64_t e, unsigned) {
+        for (uint64_t i = b; i < e; ++i) dst[i] = (src[2 * i] << wb) | src[2 * i + 1];
+    });
+}
+
+// Uploads the first n weighted edges packed into dst (device): host threads
+// pack 64 MiB chunks into two pinned bounce buffers while the previous chunk
+// is in flight on s_compute.
+static void upload_packed(DeviceState *d, const og_graph *g, uint64_t n, uint32_t *dst) {
+    if (!n) return;
+    struct Bounce {
+        hipStream_t s;
+        uint32_t *b[2] = {nullptr, nullptr};
+        ~Bounce() {
+            (void)hipStreamSynchronize(s);
+            for (uint32_t *p : b)
+                if (p) (void)hipHostFree(p);
+        }
+    } bb{d->s_compute};
+    const uint64_t C = std::min<uint64_t>(n, 16ull << 20);
+    for (auto &p : bb.b) OG_HIP(hipHostMalloc((void **)&p, C * 4, 0));
+    for (uint64_t c0 = 0, k = 0; c0 < n; c0 += C, ++k) {
+        const int s = (int)(k & 1);
+        if (k >= 2) OG_HIP(hipEventSynchronize(d->ev_h2d[s]));
+        const uint64_t m = std::min(C, n - c0);
+        pack_edges(g, c0, m, d->wbits, bb.b[s]);
+        OG_HIP(hipMemcpyAsync(dst + c0, bb.b[s], m * 4, hipMemcpyHostToDevice, d->s_compute));
+        OG_HIP(hipEventRecord(d->ev_h2d[s], d->s_compute));
+    }
+    OG_HIP(hipStreamSynchronize(d->s_compute));
+}
+
 static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params *params, og_plan_info *info) {
     if (!hip_available()) raise(OG_EHIP, "no HIP device is visible");
     if (approach != OG_ASCETIC && approach != OG_SUBWAY && approach != OG_USM)
@@ -636,6 +673,20 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
     d->eb = g->edge_bytes();
     d->flags = p.flags;
     d->host_threads = p.host_threads ? p.host_threads : default_threads();
+    // Compact weighted format: when ids and weights fit one 32-bit word, the
+    // static region, the staging slots and the link all carry 4 bytes per
+    // edge instead of 8, so the same byte budget holds twice the edges.  The
+    // planner sizes its regions in these device bytes.  USM reads the CSR as
+    // it lies in host memory and keeps the 8-byte form.
+    if (g->kind == OG_KIND_WEIGHTED && approach != OG_USM && !(p.flags & OG_PLAN_NO_PACK) && g->E) {
+        const uint32_t wb = weight_bits(g);
+        if (wb < 32 && g->V - 1 < (1ull << (32 - wb))) {
+            d->packed = true;
+            d->wbits = wb;
+            d->eb = 4;
+        }
+    }
+    d->deb = d->eb;
     const uint64_t V = g->V, E = g->E, eb = d->eb;
 
     // ---- plan
@@ -695,7 +746,9 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
         if (pl.static_edges) {
             d->d_static = d->dalloc(pl.static_edges * eb);
             d->static_used = g->ptr[pl.vsplit];
-            if (d->static_used)
+            if (d->packed)
+                upload_packed(d.get(), g, d->static_used, (uint32_t *)d->d_static);
+            else if (d->static_used)
                 OG_HIP(hipMemcpyAsync(d->d_static, g->edge_data(), d->static_used * eb, hipMemcpyHostToDevice,
                                       d->s_compute));
         }
@@ -709,37 +762,18 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
             d->h_dlist = (uint32_t *)d->palloc(V * sizeof(uint32_t));
             d->h_doff = (uint64_t *)d->palloc((V + 1) * sizeof(uint64_t));
             d->pool.reset(new WorkerPool(d->host_threads));
-            // Transfer format of the dynamic region.  The link is the bound,
-            // so weighted edges travel as one 4-byte word (to << wbits | w)
-            // whenever ids and weights fit; built once, here.
+            // Host source of the dynamic region in the device format: the
+            // CSR itself, or (packed) a pinned packed copy built once here.
             d->dyn_base = g->ptr[pl.vsplit];
-            d->deb = (uint32_t)eb;
-            d->h_dyn = (const uint8_t *)g->edge_data() + d->dyn_base * eb;
             const uint64_t nd = E - d->dyn_base;
-            if (eb == 8 && nd && !(p.flags & OG_PLAN_NO_PACK)) {
-                std::atomic<uint32_t> maxw{0};
-                parallel_for(nd, 1 << 22, 0, [&](uint64_t b, uint64_t e, unsigned) {
-                    uint32_t m = 0;
-                    for (uint64_t i = b; i < e; ++i) m = std::max(m, g->wedges[2 * (d->dyn_base + i) + 1]);
-                    uint32_t cur = maxw.load();
-                    while (m > cur && !maxw.compare_exchange_weak(cur, m)) {
-                    }
-                });
-                uint32_t wb = 1;
-                while (wb < 32 && (maxw.load() >> wb)) ++wb;
-                if (wb < 32 && V - 1 < (1ull << (32 - wb))) {
-                    uint32_t *pk = (uint32_t *)d->palloc(nd * 4);
-                    const uint32_t *src = g->wedges + 2 * d->dyn_base;
-                    parallel_for(nd, 1 << 22, 0, [&](uint64_t b, uint64_t e, unsigned) {
-                        for (uint64_t i = b; i < e; ++i) pk[i] = (src[2 * i] << wb) | src[2 * i + 1];
-                    });
-                    d->packed = true;
-                    d->wbits = wb;
-                    d->deb = 4;
-                    d->h_dyn = pk;
-                }
+            if (d->packed) {
+                uint32_t *pk = (uint32_t *)d->palloc(std::max<uint64_t>(nd, 1) * 4);
+                pack_edges(g, d->dyn_base, nd, d->wbits, pk);
+                d->h_dyn = pk;
+            } else {
+                d->h_dyn = (const uint8_t *)g->edge_data() + d->dyn_base * eb;
             }
-            d->slot_dyn = d->slot_edges * eb / d->deb;
+            d->slot_dyn = d->slot_edges;
             if (p.flags & OG_PLAN_ZERO_COPY) {
                 if (!d->packed && !(eb == 8 ? g->pinned_wedges : g->pinned_cols))
                     raise(OG_EUNSUPPORTED, "zero-copy loading needs the graph in pinned host memory");
@@ -799,11 +833,12 @@ static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0
 }
 
 static RelaxArgs relax_args(DeviceState *d) {
-    return RelaxArgs{d->d_val, d->fr.flag, d->d_visit, d->d_front, d->d_next};
+    return RelaxArgs{d->d_val, d->fr.flag, d->d_visit, d->d_front, d->d_next, d->wbits};
 }
 
-template <int ALGO, bool W>
-void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo &fi, const RelaxArgs &a, hipStream_t s) {
+template <int ALGO, bool W, bool PK>
+static void launch_static_relax_fmt(DeviceState *d, Frontier &f, const FrontierInfo &fi, const RelaxArgs &a,
+                                    hipStream_t s) {
     const void *edges = d->d_static;
     unsigned long long *upd = f.counters + 0;
     const uint32_t n0 = fi.n[0], n1 = fi.n[1], n2 = fi.n[2];
@@ -812,19 +847,25 @@ void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo &fi, co
         const dim3 gr(grid_for(thr)), bl(kBlock);
         switch (d->tune.lpv0) {
 #define OG_LPV(L) \
-    case L: hipLaunchKernelGGL((k_relax_group<ALGO, W, L>), gr, bl, 0, s, f.list, n0, d->d_ptr, edges, a, upd); break;
+    case L: hipLaunchKernelGGL((k_relax_group<ALGO, W, PK, L>), gr, bl, 0, s, f.list, n0, d->d_ptr, edges, a, upd); break;
             OG_LPV(1) OG_LPV(2) OG_LPV(4) OG_LPV(8) OG_LPV(16) OG_LPV(32)
             default: OG_LPV(64)
 #undef OG_LPV
         }
     }
     if (n1)
-        hipLaunchKernelGGL((k_relax_group<ALGO, W, 64>), dim3(grid_for((uint64_t)n1 * 64)), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL((k_relax_group<ALGO, W, PK, 64>), dim3(grid_for((uint64_t)n1 * 64)), dim3(kBlock), 0, s,
                            f.list + n0, n1, d->d_ptr, edges, a, upd);
     if (n2)
-        hipLaunchKernelGGL((k_relax_hub<ALGO, W>), dim3(n2), dim3(kBlock), d->tune.tile * d->eb, s, f.hub,
+        hipLaunchKernelGGL((k_relax_hub<ALGO, W, PK>), dim3(n2), dim3(kBlock), d->tune.tile * d->eb, s, f.hub,
                            d->tune.tile, d->d_ptr, edges, a, upd);
     OG_HIP(hipGetLastError());
+}
+
+template <int ALGO, bool W>
+void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo &fi, const RelaxArgs &a, hipStream_t s) {
+    if (W && d->packed) launch_static_relax_fmt<ALGO, W, true>(d, f, fi, a, s);
+    else launch_static_relax_fmt<ALGO, W, false>(d, f, fi, a, s);
 }
 template void launch_static_relax<ALGO_MSBFS, false>(DeviceState *, Frontier &, const FrontierInfo &, const RelaxArgs &, hipStream_t);
 template void launch_static_relax<ALGO_MSBFS, true>(DeviceState *, Frontier &, const FrontierInfo &, const RelaxArgs &, hipStream_t);
@@ -956,8 +997,10 @@ static void frontier_loop(og_graph *g, og_run_stats &st, RunTimers &tm, double &
         st.static_node_sum += (uint64_t)fi.n[0] + fi.n[1] + fi.nhub;
         st.work_edges += fi.static_deg + fi.dyn_deg;
         st.static_work_edges += fi.static_deg;
-        st.relax_static_bytes += 24ull * ((uint64_t)fi.n[0] + fi.n[1] + fi.nhub) + edge_bytes_alg<ALGO, W>() * fi.static_deg;
-        st.relax_dynamic_bytes += 24ull * fi.n[3] + (edge_bytes_alg<ALGO, W>() - (W ? 8ull : 4ull) + d->deb) * fi.dyn_deg;
+        // per edge: the edge in the device format (d->eb bytes) + the target state
+        const uint64_t per_edge = edge_bytes_alg<ALGO, W>() - (W ? 8ull : 4ull) + d->eb;
+        st.relax_static_bytes += 24ull * ((uint64_t)fi.n[0] + fi.n[1] + fi.nhub) + per_edge * fi.static_deg;
+        st.relax_dynamic_bytes += 24ull * fi.n[3] + per_edge * fi.dyn_deg;
         st.frontier_bytes += 2ull * d->Vpad + 4ull * active;
         const bool joined = launch_static<ALGO, W>(d, fi, tm);
         if (fi.n[3]) {

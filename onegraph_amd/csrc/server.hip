@@ -127,11 +127,11 @@ __global__ __launch_bounds__(kBlock) void k_srv_sssp_commit(uint8_t *flag, uint6
 // quarter of a hub tile.  Lane = (edge group g, slot j) with S lanes per
 // group and 64/S groups per wave, so each round of the loop relaxes 64/S
 // edges for all S slots with coalesced rows of dist.
-template <bool W>
+template <bool PK>
 __global__ __launch_bounds__(kBlock) void k_srv_sssp_relax(const uint32_t *__restrict__ list, uint32_t n01,
                                                            const uint64_t *__restrict__ hub, uint32_t nhub_items,
                                                            uint32_t tile, const uint64_t *__restrict__ ptr,
-                                                           const void *__restrict__ edges, uint32_t S,
+                                                           const void *__restrict__ edges, uint32_t wbits, uint32_t S,
                                                            const uint64_t *__restrict__ front, uint32_t *dist,
                                                            uint64_t *next, uint8_t *flag) {
     const uint64_t wave = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
@@ -162,13 +162,14 @@ __global__ __launch_bounds__(kBlock) void k_srv_sssp_relax(const uint32_t *__res
         uint32_t to = 0;
         if (i < re) {
             uint32_t wt;
-            if (W) {
+            if (PK) {  // packed word to << wbits | w
+                const uint32_t x = ((const uint32_t *)edges)[i];
+                to = x >> wbits;
+                wt = x & ((1u << wbits) - 1u);
+            } else {   // 8-byte {to, w}
                 const uint2 e = ((const uint2 *)edges)[i];
                 to = e.x;
                 wt = e.y;
-            } else {
-                to = ((const uint32_t *)edges)[i];
-                wt = 1;
             }
             if (on && du != 0xFFFFFFFFu) {
                 uint32_t nd = du + wt;
@@ -384,15 +385,21 @@ void Engine::step() {
     const bool w = g->kind == OG_KIND_WEIGHTED;
     if (fi.n[0] | fi.n[1] | fi.n[2]) {
         if (kind == 0) {
-            RelaxArgs a{nullptr, f.flag, visit, front, next};
+            RelaxArgs a{nullptr, f.flag, visit, front, next, d->wbits};
             if (w) launch_static_relax<ALGO_MSBFS, true>(d, f, fi, a, s);
             else launch_static_relax<ALGO_MSBFS, false>(d, f, fi, a, s);
         } else {
             const uint32_t n01 = fi.n[0] + fi.n[1];
             const uint64_t waves = (uint64_t)n01 + 4ull * fi.n[2];
             const unsigned grid = (unsigned)((waves + (kBlock / kWave) - 1) / (kBlock / kWave));
-            hipLaunchKernelGGL((k_srv_sssp_relax<true>), dim3(grid), dim3(kBlock), 0, s, f.list, n01, f.hub,
-                               fi.n[2], d->tune.tile, d->d_ptr, d->d_static, S, front, dist, next, f.flag);
+            if (d->packed)
+                hipLaunchKernelGGL((k_srv_sssp_relax<true>), dim3(grid), dim3(kBlock), 0, s, f.list, n01, f.hub,
+                                   fi.n[2], d->tune.tile, d->d_ptr, d->d_static, d->wbits, S, front, dist, next,
+                                   f.flag);
+            else
+                hipLaunchKernelGGL((k_srv_sssp_relax<false>), dim3(grid), dim3(kBlock), 0, s, f.list, n01, f.hub,
+                                   fi.n[2], d->tune.tile, d->d_ptr, d->d_static, d->wbits, S, front, dist, next,
+                                   f.flag);
             OG_HIP(hipGetLastError());
         }
     }
