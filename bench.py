@@ -107,7 +107,8 @@ def plan_flags(args, approach, loader=None):
     if approach == "usm":
         return 0
     loader = loader or args.loader
-    return (og.PLAN_ZERO_COPY if loader == "zerocopy" else 0) | (og.PLAN_NO_PACK if args.no_pack else 0)
+    return ((og.PLAN_ZERO_COPY if loader == "zerocopy" else 0) | (og.PLAN_NO_PACK if args.no_pack else 0) |
+            (og.PLAN_NO_OVERLAP if args.no_overlap else 0))
 
 
 def timed(fn, steps, warmup, label):
@@ -450,10 +451,13 @@ def main():
     ap.add_argument("--compare", action="store_true", help="also time subway and usm")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--delta", type=int, default=32, help="SSSP near-far bucket width (0: plain rounds)")
+    ap.add_argument("--defer", type=int, default=None, choices=[0, 1],
+                    help="hold dynamic-region vertices back while static ones are active")
     ap.add_argument("--autotune", action="store_true", help="run the autotuner before timing")
     ap.add_argument("--loader", default="zerocopy", choices=["zerocopy", "copy"],
                     help="dynamic region: relax reads pinned host memory in place, or host gather + copy stream")
-    ap.add_argument("--no-pack", action="store_true", help="8-byte weighted edges on the link")
+    ap.add_argument("--no-pack", action="store_true", help="8-byte weighted edges in HBM and on the link")
+    ap.add_argument("--no-overlap", action="store_true", help="serialise copies, static and dynamic relax")
     ap.add_argument("--sweep-delta", default="", help="comma list: time SSSP at each delta and report")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
@@ -512,6 +516,8 @@ def main():
         log(f"{approach}: plan {plan}")
         if args.delta is not None:
             g.set_tuning(delta=args.delta)
+        if args.defer is not None:
+            g.set_tuning(defer=args.defer)
         if args.autotune:
             best, best_ms = g.autotune(og.TUNE_SSSP, src, runs=1)
             log(f"{approach}: autotuned {best} ({best_ms:.1f} ms)")

@@ -145,12 +145,16 @@ typedef struct og_plan_params {
     uint32_t flags;             /* OG_PLAN_* bits                             */
 } og_plan_params;
 
-#define OG_PLAN_NO_OVERLAP 1u   /* serialise copies and relax (for A/B timing) */
+#define OG_PLAN_NO_OVERLAP 1u   /* serialise copies, static and dynamic relax
+                                   (for A/B timing)                              */
 #define OG_PLAN_ZERO_COPY 2u    /* dynamic region: the relax reads active edges
                                    straight from pinned host memory over the link
                                    (no staging copies, no host gather)          */
-#define OG_PLAN_NO_PACK 4u      /* keep 8-byte weighted edges on the link (no
-                                   4-byte (to << wbits | w) transfer format)     */
+#define OG_PLAN_NO_PACK 4u      /* keep 8-byte weighted edges in HBM and on the
+                                   link (no 4-byte (to << wbits | w) format)     */
+#define OG_PLAN_NO_GRAPH 8u     /* enqueue device-driven rounds one API call at
+                                   a time instead of replaying a captured
+                                   hipGraph of them (for A/B timing)             */
 
 typedef struct og_plan_info {
     uint64_t total_budget_edges;  /* reference total_gpu_size               */
@@ -239,6 +243,10 @@ typedef struct og_tuning {
     uint32_t delta;  /* SSSP near-far bucket width: a round relaxes only vertices
                         whose distance is below the current threshold; 0 = every
                         improved vertex every round (the reference's order)     */
+    uint32_t defer;  /* SSSP / CC: 1 = a round with static-region work holds the
+                        dynamic-region vertices back, so the link rows are read
+                        once the in-HBM part has settled, in denser rounds; 0 =
+                        both regions every round (the reference's order)        */
 } og_tuning;
 
 og_status og_get_tuning(const og_graph *g, og_tuning *t);
@@ -250,8 +258,9 @@ og_status og_set_tuning(og_graph *g, const og_tuning *t);
 #define OG_TUNE_CC 2u
 
 /* Autotuner: times `algo` from `source` on the prepared graph under
- * candidate tunings (coordinate descent: delta for SSSP, then t1 with lpv0,
- * t2, tile; the median of `runs` timed runs each), installs the fastest and
+ * candidate tunings (coordinate descent: defer for SSSP / CC with a dynamic
+ * region, delta for SSSP, then t1 with lpv0, t2, tile; the median of `runs`
+ * timed runs each), installs the fastest and
  * returns it with its time.  Results do not depend on the tuning. */
 og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, uint32_t runs,
                       og_tuning *best, double *best_ms);

@@ -20,6 +20,7 @@ UNREACHED = 0xFFFFFFFF
 PLAN_NO_OVERLAP = 1
 PLAN_ZERO_COPY = 2
 PLAN_NO_PACK = 4
+PLAN_NO_GRAPH = 8
 
 STATUS = {
     0: "OG_OK", 1: "OG_EINVAL", 2: "OG_EIO", 3: "OG_EFORMAT", 4: "OG_EBUDGET",
@@ -67,7 +68,7 @@ class GraphInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("t1", C.c_uint32), ("lpv0", C.c_uint32), ("t2", C.c_uint32), ("tile", C.c_uint32),
-                ("delta", C.c_uint32)]
+                ("delta", C.c_uint32), ("defer", C.c_uint32)]
 
 
 TUNE_BFS, TUNE_SSSP, TUNE_CC = 0, 1, 2

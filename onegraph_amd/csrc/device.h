@@ -50,6 +50,7 @@ struct Tuning {
     uint32_t t2 = 4096;    // bin 1: t1 < deg <= t2, one wave per vertex
     uint32_t tile = 2048;  // bin 2: deg > t2, tiles of `tile` edges, one workgroup each
     uint32_t delta = 0;    // SSSP near-far bucket width (0: every improved vertex every round)
+    uint32_t defer = 0;    // SSSP / CC: hold dynamic vertices back while static ones are active
 };
 
 // Frontier summary produced on the device each iteration and read by the host.
@@ -61,9 +62,33 @@ struct FrontierInfo {
     uint64_t dyn_deg;       // sum of degrees over dynamic active vertices
     uint32_t nfar;          // active vertices held back (value >= threshold)
     uint32_t min_far;       // smallest value among them
+    uint32_t live;          // 1 while active or held-back vertices remain
+    uint32_t dyn_held;      // 1: the dynamic list was held back this round (n[3] = 0)
+};
+
+// Device-resident driver state of one traversal.  The scan of each
+// compaction advances it, so rounds can be enqueued without a host round
+// trip: the near-far threshold moves on the device and the statistics the
+// host used to add up per round accumulate here.
+struct RoundCtl {
+    uint32_t thresh;             // near-far threshold of the next compaction (0xFFFFFFFF: off)
+    uint32_t delta;              // near-far bucket width (0: off)
+    uint32_t defer;              // hold the dynamic list back while static vertices are active
+    uint32_t rounds;             // compactions that found active vertices
+    uint32_t compactions;        // all compactions
+    uint32_t live;               // last compaction's FrontierInfo::live
+    uint32_t overflow;           // multi-source: a level past 255 was needed
+    uint32_t pad;
+    uint64_t node_sum;           // active vertices, summed over rounds
+    uint64_t static_node_sum;    // ... of them in the static bins
+    uint64_t dyn_node_sum;       // ... of them in the dynamic list
+    uint64_t work_edges;         // degree sums over active vertices
+    uint64_t static_work_edges;  // ... over the static bins
+    uint64_t dyn_rounds;         // rounds with a dynamic list
 };
 
 constexpr int kBcntRows = 7;            // per-block rows: bins 0-3, zero|hub, far, min far
+constexpr uint8_t kCode0 = 2;           // count pass: flag byte = kCode0 + bin of the vertex
 
 // Buffers of one frontier traversal: next-frontier flags, the compacted bin
 // lists, the dynamic degree prefix and the per-block scan scratch.  A graph's
@@ -75,11 +100,13 @@ struct Frontier {
     uint64_t *hub = nullptr;         // hub tiles (tile << 32 | vertex)
     uint64_t hub_cap = 0;
     uint64_t *doff = nullptr;        // V + 1 degree prefix of the dynamic list
-    uint32_t *bcnt = nullptr;        // 5 * nblk per-block counts
+    uint32_t *bcnt = nullptr;        // kBcntRows * nblk per-block counts
     uint64_t *bdeg = nullptr;        // 2 * nblk per-block degree sums
     uint32_t nblk = 0;
     FrontierInfo *info = nullptr;
     FrontierInfo *h_info = nullptr;  // pinned
+    RoundCtl *ctl = nullptr;
+    RoundCtl *h_ctl = nullptr;       // pinned
     unsigned long long *counters = nullptr;    // [0] updates static, [1] updates dynamic, [2..] scratch
     unsigned long long *h_counters = nullptr;  // pinned
 };
@@ -88,6 +115,7 @@ struct Frontier {
 struct TimedClass {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> marks;
     uint64_t launches = 0;
+    double ms = 0;  // already resolved (marks of captured graphs, read after each replay)
 };
 
 class EventPool {
@@ -174,6 +202,7 @@ struct DeviceState {
     const void *h_dyn = nullptr;     // host source of dynamic edges (packed copy or the CSR)
     const void *hd_dyn = nullptr;    // the same, as a device address (zero-copy mode)
     uint64_t slot_dyn = 0;           // dynamic edges per staging slot in the transfer format
+    uint64_t dyn_resident = 0;       // zero-copy: edges [dyn_base, dyn_base + this) held in d_staging
     uint64_t *m_ptr = nullptr;       // USM managed copies
     void *m_edges = nullptr;
 
@@ -188,6 +217,7 @@ struct DeviceState {
 
     hipStream_t s_compute = nullptr, s_copy = nullptr;
     hipStream_t s_static = nullptr;  // static-region relax, overlapping the dynamic region
+    hipEvent_t ev_fork = nullptr;    // compaction of the round finished (static relax may start)
     hipEvent_t ev_static = nullptr;  // static relax of the round finished
     hipEvent_t ev_h2d[kSlots] = {nullptr, nullptr};    // copy into slot finished
     hipEvent_t ev_relax[kSlots] = {nullptr, nullptr};  // relax reading slot finished
@@ -212,7 +242,8 @@ struct DeviceState {
     double *h_scalars = nullptr;
     uint64_t pr_partial_cap = 0;     // entries of d_partial
     bool pr_fp64 = false;
-    std::shared_ptr<void> pr_plan;   // PageRank bins and chunk ranges (pagerank.hip)
+    std::shared_ptr<void> pr_plan;
+    std::shared_ptr<void> round_graph;  // captured batch of device-driven rounds (engine.hip)   // PageRank bins and chunk ranges (pagerank.hip)
 
     MemPool mem;                     // device + pinned allocations of this preparation
 
@@ -261,6 +292,43 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t x, uint64_t *t
     return before + inc - x;
 }
 
+// N independent exclusive scans over the block in one pass (one LDS round):
+// ex[i] = sum of x[i] over lower threads, tot[i] = block sum.  `lds` must
+// hold N * blockDim.x / 64 entries.  Contains __syncthreads.
+template <int N>
+__device__ __forceinline__ void block_scan_n(const uint64_t (&x)[N], uint64_t (&ex)[N], uint64_t (&tot)[N],
+                                             uint64_t *lds) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    uint64_t inc[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) inc[i] = x[i];
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const uint64_t y = shfl_up_u64(inc[i], o);
+            if (lane >= o) inc[i] += y;
+        }
+    }
+    if (lane == kWave - 1)
+#pragma unroll
+        for (int i = 0; i < N; ++i) lds[i * nw + wid] = inc[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        uint64_t before = 0, t = 0;
+        for (int w = 0; w < nw; ++w) {
+            const uint64_t s = lds[i * nw + w];
+            before += (w < wid) ? s : 0;
+            t += s;
+        }
+        ex[i] = before + inc[i] - x[i];
+        tot[i] = t;
+    }
+    __syncthreads();
+}
+
 // Sum over the block; result valid in every thread.  Contains __syncthreads.
 __device__ __forceinline__ uint64_t block_sum(uint64_t x, uint64_t *lds) {
     uint64_t t;
@@ -304,7 +372,7 @@ struct EventSpan {
 };
 
 inline double sum_ms(const TimedClass &c) {
-    double t = 0;
+    double t = c.ms;
     for (auto &m : c.marks) {
         float ms = 0;
         OG_HIP(hipEventElapsedTime(&ms, m.first, m.second));
@@ -317,15 +385,24 @@ inline double sum_ms(const TimedClass &c) {
 // graph and the current tuning; freed with the DeviceState.
 void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m);
 
-// One compaction: flags -> bin lists / hub tiles / sorted dynamic list with
-// its degree prefix, flags cleared; the summary is copied to f.h_info
-// (stream-ordered, not synchronised).  With `val` non-null, vertices whose
-// value is >= thresh stay flagged and are only counted (near-far order).
-void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val = nullptr,
-                      uint32_t thresh = 0xFFFFFFFFu);
+// Resets f.ctl for a new traversal on stream s (a kernel, so it may sit in a
+// captured graph): statistics zeroed, near-far width `delta` with the first
+// threshold 1 + delta (the source's value is 1), or none for delta = 0;
+// `defer` holds the dynamic list back in rounds with static work.
+void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer = 0);
 
-// Launches the static-region relax of the bins in `fi` for op ALGO.
+// One compaction: flags -> bin lists / hub tiles / sorted dynamic list with
+// its degree prefix, flags cleared; f.info and f.ctl updated on the device.
+// With `val` non-null, vertices whose value is >= f.ctl->thresh stay flagged
+// and are only counted (near-far order).  With to_host, the summary is also
+// copied to f.h_info (stream-ordered, not synchronised).
+void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val = nullptr,
+                      bool to_host = true);
+
+// Launches the static-region relax of the bins for op ALGO.  The kernels
+// read their counts from f.info; `hfi` (the host copy of it, or null when the
+// host does not know it) only sizes the grids and skips empty bins.
 template <int ALGO, bool W>
-void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo &fi, const RelaxArgs &a, hipStream_t s);
+void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo *hfi, const RelaxArgs &a, hipStream_t s);
 
 }  // namespace og

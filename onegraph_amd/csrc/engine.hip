@@ -56,7 +56,7 @@ __device__ __forceinline__ void load_flags(const uint8_t *flag, uint64_t base, u
     }
 }
 
-__device__ __forceinline__ bool flag_at(const uint4 (&w)[kFlagVecs], int j) {
+__device__ __forceinline__ uint32_t flag_at(const uint4 (&w)[kFlagVecs], int j) {
     const uint4 &q = w[j >> 4];
     const int b = j & 15;
     uint32_t word = (b < 4) ? q.x : (b < 8) ? q.y : (b < 12) ? q.z : q.w;
@@ -70,12 +70,22 @@ __device__ __forceinline__ bool any_flag(const uint4 (&w)[kFlagVecs]) {
     return o != 0;
 }
 
+__device__ __forceinline__ void store_flags(uint8_t *flag, uint64_t base, uint64_t Vpad,
+                                            const uint32_t (&word)[4 * kFlagVecs]) {
+#pragma unroll
+    for (int k = 0; k < kFlagVecs; ++k) {
+        const uint64_t at = base + 16ull * k;
+        if (at < Vpad)
+            *(uint4 *)(flag + at) = make_uint4(word[4 * k], word[4 * k + 1], word[4 * k + 2], word[4 * k + 3]);
+    }
+}
+
 // Bin of an active vertex: 0..2 static bins, 3 dynamic, 4 no out-edges,
-// 5 held back for a later round (near-far order: value >= threshold).
-__device__ __forceinline__ int bin_of(uint64_t v, uint64_t vsplit, uint64_t deg, const Tuning &tn,
-                                      const uint32_t *__restrict__ val, uint32_t thresh) {
+// 5 held back for a later round (near-far order: value >= threshold, checked
+// first so that held-back vertices need no degree).
+__device__ __forceinline__ int bin_of(bool far, uint64_t v, uint64_t vsplit, uint64_t deg, const Tuning &tn) {
+    if (far) return 5;
     if (deg == 0) return 4;
-    if (val && val[v] >= thresh) return 5;
     if (v >= vsplit) return 3;
     if (deg <= tn.t1) return 0;
     if (deg <= tn.t2) return 1;
@@ -94,156 +104,264 @@ __device__ __forceinline__ uint32_t block_min_u32(uint32_t x, uint64_t *lds) {
     return r;
 }
 
-// Pass 1 (WRITE = false): per-block counts.  Pass 3 (WRITE = true): emits
-// lists in vertex order, the dynamic degree prefix, and clears the flags of
-// the emitted vertices (held-back vertices keep theirs).
-template <bool WRITE>
-__global__ __launch_bounds__(kBlock) void k_frontier(uint8_t *__restrict__ flag, uint64_t V, uint64_t Vpad,
-                                                     uint64_t vsplit, const uint64_t *__restrict__ ptr, Tuning tn,
-                                                     const uint32_t *__restrict__ val, uint32_t thresh,
-                                                     uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
-                                                     uint32_t nblk, const FrontierInfo *__restrict__ info,
-                                                     uint32_t *__restrict__ list, uint64_t *__restrict__ hub,
-                                                     uint64_t *__restrict__ doff) {
-    __shared__ uint64_t lds[kBlock / kWave];
+constexpr int kLoadBatch = 8;  // flagged vertices whose row bounds are read together
+
+// Compaction pass 1: classifies every flagged vertex of the block's window,
+// stores kCode0 + bin in its flag byte and publishes the per-block counts.
+// Row bounds (and values, for near-far) of kLoadBatch vertices are read
+// before any is used, so their latencies overlap.
+__global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__ flag, uint64_t Vpad, uint64_t vsplit,
+                                                           const uint64_t *__restrict__ ptr, Tuning tn,
+                                                           const uint32_t *__restrict__ val,
+                                                           const RoundCtl *__restrict__ ctl,
+                                                           uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
+                                                           uint32_t nblk) {
+    __shared__ uint64_t lds[6 * kBlock / kWave];
+    const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kFlagsPerThread;
+    uint4 w[kFlagVecs];
+    load_flags(flag, base, Vpad, w);
+    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, cz = 0, ch = 0, cf = 0, mf = 0xFFFFFFFFu;
+    uint64_t sdeg = 0, ddeg = 0;
+    if (any_flag(w)) {
+        const uint32_t thresh = val ? ctl->thresh : 0xFFFFFFFFu;
+        uint32_t code[4 * kFlagVecs];
+#pragma unroll
+        for (int k = 0; k < 4 * kFlagVecs; ++k) code[k] = 0;
+#pragma unroll
+        for (int j0 = 0; j0 < kFlagsPerThread; j0 += kLoadBatch) {
+            uint64_t pb[kLoadBatch], pe[kLoadBatch];
+            uint32_t vv[kLoadBatch];
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                pb[k] = pe[k] = 0;
+                vv[k] = 0;
+                if (flag_at(w, j0 + k)) {
+                    const uint64_t v = base + j0 + k;
+                    pb[k] = ptr[v];
+                    pe[k] = ptr[v + 1];
+                    if (val) vv[k] = val[v];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                const int j = j0 + k;
+                if (!flag_at(w, j)) continue;
+                const uint64_t deg = pe[k] - pb[k];
+                const int b = bin_of(val && vv[k] >= thresh, base + j, vsplit, deg, tn);
+                switch (b) {
+                    case 0: ++c0; sdeg += deg; break;
+                    case 1: ++c1; sdeg += deg; break;
+                    case 2: c2 += (uint32_t)((deg + tn.tile - 1) / tn.tile); ++ch; sdeg += deg; break;
+                    case 3: ++c3; ddeg += deg; break;
+                    case 5: ++cf; mf = min(mf, vv[k]); break;
+                    default: ++cz; break;
+                }
+                code[j >> 2] |= (uint32_t)(kCode0 + b) << (8 * (j & 3));
+            }
+        }
+        store_flags(flag, base, Vpad, code);
+    }
+    const uint64_t x[6] = {(uint64_t)c0 | ((uint64_t)c1 << 32), (uint64_t)c2 | ((uint64_t)c3 << 32),
+                           (uint64_t)cz | ((uint64_t)ch << 32), sdeg, ddeg, cf};
+    uint64_t ex[6], t[6];
+    block_scan_n<6>(x, ex, t, lds);
+    const uint32_t bm = block_min_u32(mf, lds);
+    if (threadIdx.x == 0) {
+        bcnt[0 * nblk + blockIdx.x] = (uint32_t)t[0];
+        bcnt[1 * nblk + blockIdx.x] = (uint32_t)(t[0] >> 32);
+        bcnt[2 * nblk + blockIdx.x] = (uint32_t)t[1];
+        bcnt[3 * nblk + blockIdx.x] = (uint32_t)(t[1] >> 32);
+        bcnt[4 * nblk + blockIdx.x] = (uint32_t)t[2] | ((uint32_t)(t[2] >> 32) << 16);  // both < 2^16 per block
+        bcnt[5 * nblk + blockIdx.x] = (uint32_t)t[5];
+        bcnt[6 * nblk + blockIdx.x] = bm;
+        bdeg[0 * nblk + blockIdx.x] = t[3];
+        bdeg[1 * nblk + blockIdx.x] = t[4];
+    }
+}
+
+// Compaction pass 2: one workgroup turns the per-block counts into exclusive
+// offsets (in place), publishes the round's summary and advances the
+// traversal state: statistics, liveness and, when the near set is empty but
+// vertices are held back, the near-far threshold (min held-back value +
+// delta) for the next compaction.  Each thread scans a run of consecutive
+// blocks, so the whole pass is one multi-value block scan.
+__global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
+                                                        uint32_t nblk, FrontierInfo *__restrict__ info,
+                                                        uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl) {
+    __shared__ uint64_t lds[7 * 1024 / kWave];
+    const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = min(nblk, threadIdx.x * per), b1 = min(nblk, b0 + per);
+    uint64_t s01 = 0, s23 = 0, sdd = 0, zs = 0, hs = 0, fs = 0, ss = 0;
+    uint32_t fm = 0xFFFFFFFFu;
+    for (uint32_t b = b0; b < b1; ++b) {
+        s01 += (uint64_t)bcnt[0 * nblk + b] | ((uint64_t)bcnt[1 * nblk + b] << 32);
+        s23 += (uint64_t)bcnt[2 * nblk + b] | ((uint64_t)bcnt[3 * nblk + b] << 32);
+        const uint32_t zh = bcnt[4 * nblk + b];
+        zs += zh & 0xFFFFu;
+        hs += zh >> 16;
+        fs += bcnt[5 * nblk + b];
+        fm = min(fm, bcnt[6 * nblk + b]);
+        ss += bdeg[b];
+        sdd += bdeg[nblk + b];
+    }
+    const uint64_t x[7] = {s01, s23, sdd, zs, hs, fs, ss};
+    uint64_t ex[7], t[7];
+    block_scan_n<7>(x, ex, t, lds);
+    const uint32_t fmin = block_min_u32(fm, lds);
+    uint64_t r01 = ex[0], r23 = ex[1], rdd = ex[2];
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint64_t a01 = (uint64_t)bcnt[0 * nblk + b] | ((uint64_t)bcnt[1 * nblk + b] << 32);
+        const uint64_t a23 = (uint64_t)bcnt[2 * nblk + b] | ((uint64_t)bcnt[3 * nblk + b] << 32);
+        const uint64_t dd = bdeg[nblk + b];
+        bcnt[0 * nblk + b] = (uint32_t)r01;
+        bcnt[1 * nblk + b] = (uint32_t)(r01 >> 32);
+        bcnt[2 * nblk + b] = (uint32_t)r23;
+        bcnt[3 * nblk + b] = (uint32_t)(r23 >> 32);
+        bdeg[nblk + b] = rdd;
+        r01 += a01;
+        r23 += a23;
+        rdd += dd;
+    }
+    if (threadIdx.x == 0) {
+        FrontierInfo fi;
+        fi.n[0] = (uint32_t)t[0];
+        fi.n[1] = (uint32_t)(t[0] >> 32);
+        fi.n[2] = (uint32_t)t[1];
+        fi.n[3] = (uint32_t)(t[1] >> 32);
+        fi.nzero = (uint32_t)t[3];
+        fi.nhub = (uint32_t)t[4];
+        fi.static_deg = t[6];
+        fi.dyn_deg = t[2];
+        fi.nfar = (uint32_t)t[5];
+        fi.min_far = fmin;
+        RoundCtl c = *ctl;
+        const uint64_t snodes = (uint64_t)fi.n[0] + fi.n[1] + fi.nhub;
+        // deferred order: while static vertices are active, the dynamic list
+        // stays flagged for a later round (the write pass keeps its flags)
+        fi.dyn_held = (c.defer && snodes && fi.n[3]) ? 1u : 0u;
+        const uint32_t held = fi.dyn_held ? fi.n[3] : 0u;
+        if (held) {
+            fi.n[3] = 0;
+            fi.dyn_deg = 0;
+        }
+        const uint64_t active = snodes + fi.n[3] + fi.nzero;
+        fi.live = (active || fi.nfar || held) ? 1u : 0u;
+        *info = fi;
+        doff[fi.n[3]] = fi.dyn_deg;
+        c.compactions++;
+        c.live = fi.live;
+        if (active) {
+            c.rounds++;
+            c.node_sum += active;
+            c.static_node_sum += snodes;
+            c.dyn_node_sum += fi.n[3];
+            c.work_edges += fi.static_deg + fi.dyn_deg;
+            c.static_work_edges += fi.static_deg;
+            c.dyn_rounds += fi.n[3] ? 1 : 0;
+        } else if (fi.nfar && c.delta) {
+            c.thresh = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)fi.min_far + c.delta);
+        }
+        *ctl = c;
+    }
+}
+
+// Compaction pass 3: emits the lists in vertex order from the codes of pass
+// 1 (bins 0/1 need no reads; hub tiles and the dynamic degree prefix read
+// the row bounds again, kLoadBatch at a time), and rewrites the window:
+// 1 for held-back vertices, 0 elsewhere.
+__global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__ flag, uint64_t Vpad,
+                                                           const uint64_t *__restrict__ ptr, Tuning tn,
+                                                           const uint32_t *__restrict__ bcnt,
+                                                           const uint64_t *__restrict__ bdeg, uint32_t nblk,
+                                                           const FrontierInfo *__restrict__ info,
+                                                           uint32_t *__restrict__ list, uint64_t *__restrict__ hub,
+                                                           uint64_t hub_cap, uint64_t *__restrict__ doff) {
+    __shared__ uint64_t lds[3 * kBlock / kWave];
     const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kFlagsPerThread;
     uint4 w[kFlagVecs];
     load_flags(flag, base, Vpad, w);
     const bool any = any_flag(w);
-
-    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, cz = 0, ch = 0, cf = 0, mf = 0xFFFFFFFFu;
-    uint32_t farmask = 0;  // bit j: vertex base + j is held back
-    uint64_t sdeg = 0, ddeg = 0;
+    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    uint64_t ddeg = 0;
+    uint64_t deg[kFlagsPerThread];  // of bin 2 / 3 vertices
     if (any) {
-        for (int j = 0; j < kFlagsPerThread; ++j) {
-            if (!flag_at(w, j)) continue;
-            const uint64_t v = base + j;
-            const uint64_t deg = ptr[v + 1] - ptr[v];
-            switch (bin_of(v, vsplit, deg, tn, val, thresh)) {
-                case 0: ++c0; sdeg += deg; break;
-                case 1: ++c1; sdeg += deg; break;
-                case 2: c2 += (uint32_t)((deg + tn.tile - 1) / tn.tile); ++ch; sdeg += deg; break;
-                case 3: ++c3; ddeg += deg; break;
-                case 5: ++cf; mf = min(mf, val[v]); farmask |= 1u << j; break;
-                default: ++cz; break;
+#pragma unroll
+        for (int j0 = 0; j0 < kFlagsPerThread; j0 += kLoadBatch) {
+            uint64_t pb[kLoadBatch], pe[kLoadBatch];
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                pb[k] = pe[k] = 0;
+                const uint32_t cd = flag_at(w, j0 + k);
+                if (cd == kCode0 + 2 || cd == kCode0 + 3) {
+                    pb[k] = ptr[base + j0 + k];
+                    pe[k] = ptr[base + j0 + k + 1];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                const int j = j0 + k;
+                deg[j] = pe[k] - pb[k];
+                switch ((int)flag_at(w, j) - kCode0) {
+                    case 0: ++c0; break;
+                    case 1: ++c1; break;
+                    case 2: c2 += (uint32_t)((deg[j] + tn.tile - 1) / tn.tile); break;
+                    case 3: ++c3; ddeg += deg[j]; break;
+                    default: break;
+                }
             }
         }
     }
-    if (!WRITE) {
-        uint64_t p01 = block_sum((uint64_t)c0 | ((uint64_t)c1 << 32), lds);
-        uint64_t p23 = block_sum((uint64_t)c2 | ((uint64_t)c3 << 32), lds);
-        uint64_t pzh = block_sum((uint64_t)cz | ((uint64_t)ch << 32), lds);
-        uint64_t ts = block_sum(sdeg, lds);
-        uint64_t td = block_sum(ddeg, lds);
-        uint64_t tf = block_sum(cf, lds);
-        uint32_t bm = block_min_u32(mf, lds);
-        if (threadIdx.x == 0) {
-            bcnt[0 * nblk + blockIdx.x] = (uint32_t)p01;
-            bcnt[1 * nblk + blockIdx.x] = (uint32_t)(p01 >> 32);
-            bcnt[2 * nblk + blockIdx.x] = (uint32_t)p23;
-            bcnt[3 * nblk + blockIdx.x] = (uint32_t)(p23 >> 32);
-            bcnt[4 * nblk + blockIdx.x] = (uint32_t)pzh | ((uint32_t)(pzh >> 32) << 16);
-            bcnt[5 * nblk + blockIdx.x] = (uint32_t)tf;
-            bcnt[6 * nblk + blockIdx.x] = bm;
-            bdeg[0 * nblk + blockIdx.x] = ts;
-            bdeg[1 * nblk + blockIdx.x] = td;
-        }
-        return;
-    }
-    uint64_t t01, t23, tdd;
-    const uint64_t e01 = block_exclusive_scan((uint64_t)c0 | ((uint64_t)c1 << 32), &t01, lds);
-    const uint64_t e23 = block_exclusive_scan((uint64_t)c2 | ((uint64_t)c3 << 32), &t23, lds);
-    const uint64_t edd = block_exclusive_scan(ddeg, &tdd, lds);
+    const uint64_t x[3] = {(uint64_t)c0 | ((uint64_t)c1 << 32), (uint64_t)c2 | ((uint64_t)c3 << 32), ddeg};
+    uint64_t ex[3], t[3];
+    block_scan_n<3>(x, ex, t, lds);
     if (!any) return;
     const uint32_t n0 = info->n[0], n1 = info->n[1];
-    uint64_t p0 = (uint64_t)bcnt[0 * nblk + blockIdx.x] + (uint32_t)e01;
-    uint64_t p1 = (uint64_t)n0 + bcnt[1 * nblk + blockIdx.x] + (uint32_t)(e01 >> 32);
-    uint64_t p2 = (uint64_t)bcnt[2 * nblk + blockIdx.x] + (uint32_t)e23;
-    uint64_t p3 = (uint64_t)bcnt[3 * nblk + blockIdx.x] + (uint32_t)(e23 >> 32);
-    uint64_t pd = bdeg[1 * nblk + blockIdx.x] + edd;
+    uint64_t p0 = (uint64_t)bcnt[0 * nblk + blockIdx.x] + (uint32_t)ex[0];
+    uint64_t p1 = (uint64_t)n0 + bcnt[1 * nblk + blockIdx.x] + (uint32_t)(ex[0] >> 32);
+    uint64_t p2 = (uint64_t)bcnt[2 * nblk + blockIdx.x] + (uint32_t)ex[1];
+    uint64_t p3 = (uint64_t)bcnt[3 * nblk + blockIdx.x] + (uint32_t)(ex[1] >> 32);
+    uint64_t pd = bdeg[1 * nblk + blockIdx.x] + ex[2];
     const uint64_t dyn_base = (uint64_t)n0 + n1;
+    const bool held = info->dyn_held != 0;
+    uint32_t word[4 * kFlagVecs];
+#pragma unroll
+    for (int k = 0; k < 4 * kFlagVecs; ++k) word[k] = 0;
+#pragma unroll
     for (int j = 0; j < kFlagsPerThread; ++j) {
-        if (!flag_at(w, j) || ((farmask >> j) & 1)) continue;
         const uint64_t v = base + j;
-        const uint64_t deg = ptr[v + 1] - ptr[v];
-        switch (bin_of(v, vsplit, deg, tn, nullptr, 0)) {
+        switch ((int)flag_at(w, j) - kCode0) {
             case 0: list[p0++] = (uint32_t)v; break;
             case 1: list[p1++] = (uint32_t)v; break;
             case 2: {
-                const uint32_t nt = (uint32_t)((deg + tn.tile - 1) / tn.tile);
-                for (uint32_t t = 0; t < nt; ++t) hub[p2++] = ((uint64_t)t << 32) | v;
+                const uint32_t nt = (uint32_t)((deg[j] + tn.tile - 1) / tn.tile);
+                for (uint32_t q = 0; q < nt; ++q, ++p2)
+                    if (p2 < hub_cap) hub[p2] = ((uint64_t)q << 32) | v;
                 break;
             }
             case 3:
+                if (held) {
+                    word[j >> 2] |= 1u << (8 * (j & 3));
+                    break;
+                }
                 list[dyn_base + p3] = (uint32_t)v;
                 doff[p3] = pd;
                 ++p3;
-                pd += deg;
+                pd += deg[j];
                 break;
+            case 5: word[j >> 2] |= 1u << (8 * (j & 3)); break;
             default: break;
         }
     }
-    // rewrite the window: 1 for held-back vertices, 0 elsewhere
-#pragma unroll
-    for (int k = 0; k < kFlagVecs; ++k) {
-        uint64_t at = base + 16ull * k;
-        if (at >= Vpad) continue;
-        uint32_t word[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint32_t x = 0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) x |= ((farmask >> (16 * k + 4 * q + b)) & 1u) << (8 * b);
-            word[q] = x;
-        }
-        *(uint4 *)(flag + at) = make_uint4(word[0], word[1], word[2], word[3]);
-    }
+    store_flags(flag, base, Vpad, word);
 }
 
-// Pass 2: one workgroup scans the per-block counts (exclusive, in place) and
-// publishes the totals.
-__global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *bcnt, uint64_t *bdeg, uint32_t nblk,
-                                                        FrontierInfo *info, uint64_t *doff) {
-    __shared__ uint64_t lds[1024 / kWave];
-    uint64_t tot4[4];
-    for (int a = 0; a < 4; ++a) {
-        uint64_t carry = 0;
-        for (uint32_t b = 0; b < nblk; b += blockDim.x) {
-            const uint32_t i = b + threadIdx.x;
-            const uint64_t x = i < nblk ? bcnt[a * nblk + i] : 0;
-            uint64_t t;
-            const uint64_t ex = block_exclusive_scan(x, &t, lds);
-            if (i < nblk) bcnt[a * nblk + i] = (uint32_t)(carry + ex);
-            carry += t;
-        }
-        tot4[a] = carry;
-    }
-    uint64_t zsum = 0, hsum = 0, ssum = 0, dcarry = 0, fsum = 0;
-    uint32_t fmin = 0xFFFFFFFFu;
-    for (uint32_t b = 0; b < nblk; b += blockDim.x) {
-        const uint32_t i = b + threadIdx.x;
-        const uint32_t zh = i < nblk ? bcnt[4 * nblk + i] : 0;
-        zsum += block_sum(zh & 0xFFFFu, lds);
-        hsum += block_sum(zh >> 16, lds);
-        fsum += block_sum(i < nblk ? bcnt[5 * nblk + i] : 0, lds);
-        fmin = min(fmin, block_min_u32(i < nblk ? bcnt[6 * nblk + i] : 0xFFFFFFFFu, lds));
-        ssum += block_sum(i < nblk ? bdeg[i] : 0, lds);
-        const uint64_t x = i < nblk ? bdeg[nblk + i] : 0;
-        uint64_t t;
-        const uint64_t ex = block_exclusive_scan(x, &t, lds);
-        if (i < nblk) bdeg[nblk + i] = dcarry + ex;
-        dcarry += t;
-    }
-    if (threadIdx.x == 0) {
-        for (int a = 0; a < 4; ++a) info->n[a] = (uint32_t)tot4[a];
-        info->nzero = (uint32_t)zsum;
-        info->nhub = (uint32_t)hsum;
-        info->static_deg = ssum;
-        info->dyn_deg = dcarry;
-        info->nfar = (uint32_t)fsum;
-        info->min_far = fmin;
-        doff[tot4[3]] = dcarry;
-    }
+__global__ void k_ctl_reset(RoundCtl *ctl, uint32_t delta, uint32_t defer) {
+    RoundCtl c{};
+    c.delta = delta;
+    c.defer = defer;
+    c.thresh = delta ? (uint32_t)min((uint64_t)0xFFFFFFFFull, 1ull + delta) : 0xFFFFFFFFu;
+    c.live = 1;
+    *ctl = c;
 }
 
 // ---------------------------------------------------------------- relax
@@ -277,28 +395,6 @@ __device__ __forceinline__ uint32_t offer(uint32_t du, uint32_t wt) {
     return du;
 }
 
-// Lowers val[v] to nd; marks v for the next frontier when it improved.
-// BFS writes the same level from every writer in an iteration, so a plain
-// store is race-free; SSSP/CC use atomicMin so no smaller offer is lost
-// (the reference's check-then-write in K7/K11 can lose one).
-template <int ALGO>
-__device__ __forceinline__ uint32_t try_relax(uint32_t *__restrict__ val, uint8_t *__restrict__ flag, uint32_t v,
-                                              uint32_t nd) {
-    const uint32_t cur = val[v];
-    if (nd >= cur) return 0;
-    if (ALGO == ALGO_BFS) {
-        val[v] = nd;
-        flag[v] = 1;
-        return 1;
-    }
-    const uint32_t old = atomicMin(val + v, nd);
-    if (nd < old) {
-        flag[v] = 1;
-        return 1;
-    }
-    return 0;
-}
-
 // What a frontier vertex carries to its out-edges: its value, or for the
 // multi-source BFS the mask of sources whose frontier holds it.
 template <int ALGO> struct SrcOf { using T = uint32_t; };
@@ -310,128 +406,221 @@ __device__ __forceinline__ typename SrcOf<ALGO>::T load_src(const RelaxArgs &a, 
     else return a.val[u];
 }
 
-// Relaxes edge (u -> to, wt) given u's carried state; returns 1 on update.
-// Multi-source: the sources of u's frontier that have not reached `to` are
-// OR-ed into next[to] (one 64-bit atomic for all 64 searches).
+// Relaxing an edge (u -> to, wt) is split in two so that a lane can keep the
+// reads of several edges in flight before its first atomic: peek() reads the
+// target's state, commit() compares u's offer with what peek saw and applies
+// it.  Values only decrease (masks only grow), so a stale peek can cost an
+// atomic that finds nothing to do but never loses an update.
 template <int ALGO>
-__device__ __forceinline__ uint32_t relax_edge(const RelaxArgs &a, typename SrcOf<ALGO>::T s, uint32_t to,
-                                               uint32_t wt) {
+__device__ __forceinline__ typename SrcOf<ALGO>::T peek(const RelaxArgs &a, uint32_t to) {
+    if constexpr (ALGO == ALGO_MSBFS) return a.visit[to];
+    else return a.val[to];
+}
+
+// Returns 1 when `to` improved (and is flagged for the next frontier).
+// BFS writes the same level from every writer in an iteration, so a plain
+// store is race-free; SSSP/CC use atomicMin so no smaller offer is lost
+// (the reference's check-then-write in K7/K11 can lose one).  Multi-source:
+// the sources of u's frontier that have not reached `to` are OR-ed into
+// next[to] (one 64-bit atomic for all 64 searches).
+template <int ALGO>
+__device__ __forceinline__ uint32_t commit(const RelaxArgs &a, typename SrcOf<ALGO>::T s, uint32_t to, uint32_t wt,
+                                           typename SrcOf<ALGO>::T seen) {
     if constexpr (ALGO == ALGO_MSBFS) {
-        const uint64_t m = s & ~a.visit[to];
+        const uint64_t m = s & ~seen;
         if (!m) return 0;
         atomicOr((unsigned long long *)(a.next + to), (unsigned long long)m);
         a.flag[to] = 1;
         return 1;
     } else {
-        return try_relax<ALGO>(a.val, a.flag, to, offer<ALGO>(s, wt));
+        const uint32_t nd = offer<ALGO>(s, wt);
+        if (nd >= seen) return 0;
+        if (ALGO == ALGO_BFS) {
+            a.val[to] = nd;
+            a.flag[to] = 1;
+            return 1;
+        }
+        const uint32_t old = atomicMin(a.val + to, nd);
+        if (nd < old) {
+            a.flag[to] = 1;
+            return 1;
+        }
+        return 0;
     }
 }
 
-// Bins 0 and 1: LPV lanes per frontier vertex (LPV = 64: one wave).
+constexpr int kRelaxBatch = 4;  // edges per lane whose reads are issued together
+
+// The relax kernels read their work counts from the round's FrontierInfo on
+// the device and stride over them, so a launch needs no host-side count: a
+// round can be enqueued (or captured in a graph) before its compaction ran.
+
+// Bins 0 and 1: LPV lanes per frontier vertex (LPV = 64: one wave).  BIN
+// selects the list segment: bin 0 at list[0, n0), bin 1 at list[n0, n0 + n1).
 template <int ALGO, bool W, bool PK, int LPV>
-__global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restrict__ list, uint32_t n,
+__global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restrict__ list, int bin,
+                                                        const FrontierInfo *__restrict__ info,
                                                         const uint64_t *__restrict__ ptr,
                                                         const void *__restrict__ edges, RelaxArgs a,
                                                         unsigned long long *upd) {
-    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const uint64_t item = gid / LPV;
-    const uint32_t sub = (uint32_t)(gid % LPV);
+    const uint32_t n = info->n[bin];
+    const uint32_t *lst = list + (bin ? info->n[0] : 0u);
+    const uint32_t sub = threadIdx.x % LPV;
+    const uint64_t stride = (uint64_t)gridDim.x * (kBlock / LPV);
     uint32_t cnt = 0;
-    if (item < n) {
-        const uint32_t u = list[item];
+    for (uint64_t item = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / LPV; item < n; item += stride) {
+        const uint32_t u = lst[item];
         const uint64_t b = ptr[u], e = ptr[u + 1];
         const auto s = load_src<ALGO>(a, u);
-        for (uint64_t i = b + sub; i < e; i += LPV) {
-            uint32_t to, wt;
-            load_edge<W, PK>(edges, i, a.wbits, to, wt);
-            cnt += relax_edge<ALGO>(a, s, to, wt);
+        for (uint64_t i = b + sub; i < e; i += (uint64_t)kRelaxBatch * LPV) {
+            uint32_t to[kRelaxBatch], wt[kRelaxBatch];
+            typename SrcOf<ALGO>::T seen[kRelaxBatch];
+#pragma unroll
+            for (int k = 0; k < kRelaxBatch; ++k) {
+                to[k] = 0;
+                wt[k] = 0;
+                if (i + (uint64_t)k * LPV < e) load_edge<W, PK>(edges, i + (uint64_t)k * LPV, a.wbits, to[k], wt[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < kRelaxBatch; ++k)
+                if (i + (uint64_t)k * LPV < e) seen[k] = peek<ALGO>(a, to[k]);
+#pragma unroll
+            for (int k = 0; k < kRelaxBatch; ++k)
+                if (i + (uint64_t)k * LPV < e) cnt += commit<ALGO>(a, s, to[k], wt[k], seen[k]);
         }
     }
     wave_add(upd, cnt);
 }
 
-// Bin 2: one workgroup per tile of a hub's neighbour list; the tile is
-// staged in LDS with coalesced loads before the relax pass.
+// Bin 2: one workgroup per tile of a hub's neighbour list (workgroups stride
+// over the tiles); the tile is staged in LDS with coalesced loads before the
+// relax pass.
 template <int ALGO, bool W, bool PK>
-__global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict__ hub, uint32_t tile,
+__global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict__ hub, uint64_t hub_cap,
+                                                      const FrontierInfo *__restrict__ info, uint32_t tile,
                                                       const uint64_t *__restrict__ ptr,
                                                       const void *__restrict__ edges, RelaxArgs a,
                                                       unsigned long long *upd) {
     extern __shared__ uint32_t s_edges[];
     constexpr bool WIDE = W && !PK;  // 8-byte edges
-    const uint64_t it = hub[blockIdx.x];
-    const uint32_t u = (uint32_t)it, t = (uint32_t)(it >> 32);
-    const uint64_t b = ptr[u] + (uint64_t)t * tile;
-    const uint64_t e = min(ptr[u + 1], b + tile);
-    const uint32_t cnt = (uint32_t)(e - b);
-    const auto s = load_src<ALGO>(a, u);
-    for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
-        if (WIDE)
-            ((uint2 *)s_edges)[k] = ((const uint2 *)edges)[b + k];
-        else
-            s_edges[k] = ((const uint32_t *)edges)[b + k];
-    }
-    __syncthreads();
+    const uint64_t ntiles = min((uint64_t)info->n[2], hub_cap);
     uint32_t c = 0;
-    for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
-        uint32_t to, wt;
-        load_edge<W, PK>(s_edges, k, a.wbits, to, wt);
-        c += relax_edge<ALGO>(a, s, to, wt);
+    for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const uint64_t it = hub[ti];
+        const uint32_t u = (uint32_t)it, t = (uint32_t)(it >> 32);
+        const uint64_t b = ptr[u] + (uint64_t)t * tile;
+        const uint64_t e = min(ptr[u + 1], b + tile);
+        const uint32_t cnt = (uint32_t)(e - b);
+        const auto s = load_src<ALGO>(a, u);
+        for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+            if (WIDE)
+                ((uint2 *)s_edges)[k] = ((const uint2 *)edges)[b + k];
+            else
+                s_edges[k] = ((const uint32_t *)edges)[b + k];
+        }
+        __syncthreads();
+        for (uint32_t k0 = threadIdx.x; k0 < cnt; k0 += kRelaxBatch * kBlock) {
+            uint32_t to[kRelaxBatch], wt[kRelaxBatch];
+            typename SrcOf<ALGO>::T seen[kRelaxBatch];
+#pragma unroll
+            for (int j = 0; j < kRelaxBatch; ++j) {
+                to[j] = 0;
+                wt[j] = 0;
+                if (k0 + j * kBlock < cnt) load_edge<W, PK>(s_edges, k0 + j * kBlock, a.wbits, to[j], wt[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < kRelaxBatch; ++j)
+                if (k0 + j * kBlock < cnt) seen[j] = peek<ALGO>(a, to[j]);
+#pragma unroll
+            for (int j = 0; j < kRelaxBatch; ++j)
+                if (k0 + j * kBlock < cnt) c += commit<ALGO>(a, s, to[j], wt[j], seen[j]);
+        }
+        __syncthreads();  // the tile buffer is reused by the next tile
     }
     wave_add(upd, c);
 }
 
-// Dynamic region, edges [c0, c1) of the concatenated active dynamic edge
-// space.  STAGED (DIRECT = false): they sit in a staging slot, edge q at
-// q - c0.  DIRECT: they are read in place from the pinned host array through
-// its device mapping, edge q of vertex u at ptr[u] - dbase + (q - doff[i]).
-// Each block takes kDynEdgesPerBlock consecutive edges, loads the covering
-// slice of the degree prefix into LDS and maps each edge to its source with a
-// binary search there (every listed vertex has >= 1 edge, so a block covers
-// at most kDynEdgesPerBlock vertices).
+// Dynamic region, edges [c0, min(c1, D)) of the concatenated active dynamic
+// edge space (D = sum of the dynamic list's degrees; the list sits at
+// list[n0 + n1, n0 + n1 + n3)).  STAGED (DIRECT = false): they sit in a
+// staging slot, edge q at q - c0.  DIRECT: edge q of vertex u is
+// dynamic-region edge idx = ptr[u] - dbase + (q - doff[i]); the first nres of
+// those are resident in HBM (`resident`), the rest are read in place from the
+// pinned host array through its device mapping (`stage`); reads served from
+// HBM are counted in *hits.
+// Each workgroup takes chunks of kDynEdgesPerBlock consecutive edges, loads
+// the covering slice of the degree prefix into LDS and maps each edge to its
+// source with a binary search there (every listed vertex has >= 1 edge, so a
+// chunk covers at most kDynEdgesPerBlock vertices).
 template <int ALGO, bool W, bool PACKED, bool DIRECT>
-__global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict__ dlist,
-                                                      const uint64_t *__restrict__ doff, uint32_t ndyn,
-                                                      const void *__restrict__ stage, uint64_t c0, uint64_t c1,
+__global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict__ list,
+                                                      const FrontierInfo *__restrict__ info,
+                                                      const uint64_t *__restrict__ doff,
+                                                      const void *__restrict__ stage, uint64_t c0, uint64_t c1max,
                                                       const uint64_t *__restrict__ ptr, uint64_t dbase,
-                                                      uint32_t wbits, RelaxArgs a, unsigned long long *upd) {
+                                                      const void *__restrict__ resident, uint64_t nres,
+                                                      uint32_t wbits, RelaxArgs a, unsigned long long *upd,
+                                                      unsigned long long *hits) {
     __shared__ uint64_t s_off[kDynEdgesPerBlock + 2];
     __shared__ uint32_t s_i[2];
-    const uint64_t q0 = c0 + (uint64_t)blockIdx.x * kDynEdgesPerBlock;
-    const uint64_t q1 = min(q0 + kDynEdgesPerBlock, c1);
-    if (threadIdx.x < 2) {
-        const uint64_t q = threadIdx.x == 0 ? q0 : q1 - 1;
-        uint32_t lo = 0, hi = ndyn;  // largest i with doff[i] <= q; doff[0] = 0
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (doff[mid] <= q) lo = mid; else hi = mid;
-        }
-        s_i[threadIdx.x] = lo;
-    }
-    __syncthreads();
-    const uint32_t i0 = s_i[0];
-    const uint32_t nv = s_i[1] - i0 + 1;
-    for (uint32_t k = threadIdx.x; k <= nv; k += kBlock) s_off[k] = doff[i0 + k];
-    __syncthreads();
-    uint32_t c = 0;
-#pragma unroll
-    for (int j = 0; j < kDynEdgesPerThread; ++j) {
-        const uint64_t q = q0 + threadIdx.x + (uint64_t)j * kBlock;
-        if (q < q1) {
-            uint32_t lo = 0, hi = nv;
+    const uint32_t ndyn = info->n[3];
+    const uint32_t *dlist = list + info->n[0] + info->n[1];
+    const uint64_t c1 = min(c1max, info->dyn_deg);
+    uint32_t c = 0, h = 0;
+    for (uint64_t q0 = c0 + (uint64_t)blockIdx.x * kDynEdgesPerBlock; q0 < c1;
+         q0 += (uint64_t)gridDim.x * kDynEdgesPerBlock) {
+        const uint64_t q1 = min(q0 + kDynEdgesPerBlock, c1);
+        if (threadIdx.x < 2) {
+            const uint64_t q = threadIdx.x == 0 ? q0 : q1 - 1;
+            uint32_t lo = 0, hi = ndyn;  // largest i with doff[i] <= q; doff[0] = 0
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (s_off[mid] <= q) lo = mid; else hi = mid;
+                if (doff[mid] <= q) lo = mid; else hi = mid;
             }
-            const uint32_t u = dlist[i0 + lo];
-            const auto s = load_src<ALGO>(a, u);
-            uint32_t to, wt;
-            const uint64_t idx = DIRECT ? ptr[u] - dbase + (q - s_off[lo]) : q - c0;
-            load_edge<W, PACKED>(stage, idx, wbits, to, wt);
-            c += relax_edge<ALGO>(a, s, to, wt);
+            s_i[threadIdx.x] = lo;
         }
+        __syncthreads();
+        const uint32_t i0 = s_i[0];
+        const uint32_t nv = s_i[1] - i0 + 1;
+        for (uint32_t k = threadIdx.x; k <= nv; k += kBlock) s_off[k] = doff[i0 + k];
+        __syncthreads();
+        // all edge (and source-state) reads of the lane first, then the
+        // target peeks, then the commits: the link / HBM latencies overlap
+        uint32_t to[kDynEdgesPerThread], wt[kDynEdgesPerThread];
+        typename SrcOf<ALGO>::T sv[kDynEdgesPerThread], seen[kDynEdgesPerThread];
+#pragma unroll
+        for (int j = 0; j < kDynEdgesPerThread; ++j) {
+            const uint64_t q = q0 + threadIdx.x + (uint64_t)j * kBlock;
+            to[j] = 0;
+            wt[j] = 0;
+            sv[j] = 0;
+            if (q < q1) {
+                uint32_t lo = 0, hi = nv;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= q) lo = mid; else hi = mid;
+                }
+                const uint32_t u = dlist[i0 + lo];
+                sv[j] = load_src<ALGO>(a, u);
+                const uint64_t idx = DIRECT ? ptr[u] - dbase + (q - s_off[lo]) : q - c0;
+                if (DIRECT && idx < nres) {
+                    load_edge<W, PACKED>(resident, idx, wbits, to[j], wt[j]);
+                    ++h;
+                } else {
+                    load_edge<W, PACKED>(stage, idx, wbits, to[j], wt[j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kDynEdgesPerThread; ++j)
+            if (q0 + threadIdx.x + (uint64_t)j * kBlock < q1) seen[j] = peek<ALGO>(a, to[j]);
+#pragma unroll
+        for (int j = 0; j < kDynEdgesPerThread; ++j)
+            if (q0 + threadIdx.x + (uint64_t)j * kBlock < q1) c += commit<ALGO>(a, sv[j], to[j], wt[j], seen[j]);
+        __syncthreads();  // s_off / s_i are reloaded for the next chunk
     }
     wave_add(upd, c);
+    if (DIRECT) wave_add(hits, h);
 }
 
 // Order-independent digest of a result array: the sum (mod 2^64) of
@@ -483,10 +672,13 @@ __global__ void k_ms_init(const uint32_t *__restrict__ src, uint32_t n, uint64_t
 }
 
 // After a round: for every flagged vertex, the offered sources it had not
-// seen become its new frontier mask and get `level`; vertices with nothing
-// new leave the frontier.
+// seen become its new frontier mask and get the round's level (rounds with
+// active vertices so far + 1, read from the device state; a level past 255
+// sets ctl->overflow); vertices with nothing new leave the frontier.
 __global__ __launch_bounds__(kBlock) void k_ms_commit(uint8_t *__restrict__ flag, uint64_t V, uint64_t *visit,
-                                                      uint64_t *front, uint64_t *next, uint8_t *lvl, uint8_t level) {
+                                                      uint64_t *front, uint64_t *next, uint8_t *lvl, RoundCtl *ctl) {
+    const uint32_t lv32 = ctl->rounds + 1;
+    const uint8_t level = (uint8_t)min(lv32, 255u);
     for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
         if (!flag[v]) continue;
         const uint64_t vis = visit[v];
@@ -494,6 +686,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_commit(uint8_t *__restrict__ flag
         next[v] = 0;
         front[v] = nw;
         if (nw) {
+            if (lv32 > 255) ctl->overflow = 1;
             visit[v] = vis | nw;
             uint64_t m = nw;
             while (m) {
@@ -540,11 +733,6 @@ __global__ void k_ms_extract(const uint8_t *__restrict__ lvl, uint64_t V, uint32
 
 // ============================================================ host driver
 
-static unsigned grid_for(uint64_t threads, unsigned block = kBlock) {
-    uint64_t g = (threads + block - 1) / block;
-    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, 0x7FFFFFFFull));
-}
-
 static unsigned fill_grid(uint64_t n) { return (unsigned)std::min<uint64_t>(8192, std::max<uint64_t>(1, (n + 255) / 256)); }
 
 void device_state_free(DeviceState *d) {
@@ -561,6 +749,7 @@ void device_state_free(DeviceState *d) {
     if (d->s_copy) (void)hipStreamDestroy(d->s_copy);
     if (d->s_static) (void)hipStreamDestroy(d->s_static);
     if (d->ev_static) (void)hipEventDestroy(d->ev_static);
+    if (d->ev_fork) (void)hipEventDestroy(d->ev_fork);
     delete d;
 }
 
@@ -583,25 +772,33 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) 
     f.bcnt = (uint32_t *)m.dalloc((uint64_t)kBcntRows * f.nblk * sizeof(uint32_t));
     f.bdeg = (uint64_t *)m.dalloc(2ull * f.nblk * sizeof(uint64_t));
     f.info = (FrontierInfo *)m.dalloc(sizeof(FrontierInfo));
+    f.ctl = (RoundCtl *)m.dalloc(sizeof(RoundCtl));
     f.counters = (unsigned long long *)m.dalloc(16 * sizeof(unsigned long long));
     f.h_info = (FrontierInfo *)m.palloc(sizeof(FrontierInfo));
+    f.h_ctl = (RoundCtl *)m.palloc(sizeof(RoundCtl));
     f.h_counters = (unsigned long long *)m.palloc(16 * sizeof(unsigned long long));
     const uint64_t cap = std::max<uint64_t>(hub_capacity(g, d->plan.vsplit, d->tune), 1);
     f.hub = (uint64_t *)m.dalloc(cap * sizeof(uint64_t));
     f.hub_cap = cap;
     OG_HIP(hipMemset(f.flag, 0, d->Vpad));
+    RoundCtl c{};
+    c.thresh = 0xFFFFFFFFu;
+    OG_HIP(hipMemcpy(f.ctl, &c, sizeof c, hipMemcpyHostToDevice));
 }
 
-void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val, uint32_t thresh) {
-    const uint64_t V = d->V;
-    if (thresh == 0xFFFFFFFFu) val = nullptr;
-    hipLaunchKernelGGL(k_frontier<false>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, V, d->Vpad, d->plan.vsplit,
-                       d->d_ptr, d->tune, val, thresh, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.doff);
-    hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff);
-    hipLaunchKernelGGL(k_frontier<true>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, V, d->Vpad, d->plan.vsplit,
-                       d->d_ptr, d->tune, val, thresh, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.doff);
+void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, f.ctl, delta, defer);
     OG_HIP(hipGetLastError());
-    OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
+}
+
+void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val, bool to_host) {
+    hipLaunchKernelGGL(k_frontier_count, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, d->Vpad, d->plan.vsplit, d->d_ptr,
+                       d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
+    hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff, f.ctl);
+    hipLaunchKernelGGL(k_frontier_write, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, d->Vpad, d->d_ptr, d->tune, f.bcnt,
+                       f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff);
+    OG_HIP(hipGetLastError());
+    if (to_host) OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
 }
 
 // Bits needed by the largest edge weight (at least 1).
@@ -721,6 +918,7 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
     OG_HIP(hipStreamCreateWithFlags(&d->s_copy, hipStreamNonBlocking));
     OG_HIP(hipStreamCreateWithFlags(&d->s_static, hipStreamNonBlocking));
     OG_HIP(hipEventCreateWithFlags(&d->ev_static, hipEventDisableTiming));
+    OG_HIP(hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming));
     for (int s = 0; s < kSlots; ++s) {
         OG_HIP(hipEventCreateWithFlags(&d->ev_h2d[s], hipEventDisableTiming));
         OG_HIP(hipEventCreateWithFlags(&d->ev_relax[s], hipEventDisableTiming));
@@ -753,15 +951,18 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
                                       d->s_compute));
         }
         if (pl.vsplit < V) {
+            const bool zc = (p.flags & OG_PLAN_ZERO_COPY) != 0;
             d->slot_edges = pl.staging_edges / kSlots;
-            if (d->slot_edges == 0)
-                raise(OG_EBUDGET, "edge budget leaves %llu staging edges for %llu dynamic-region edges",
-                      (unsigned long long)pl.staging_edges, (unsigned long long)(E - g->ptr[pl.vsplit]));
-            d->d_staging = d->dalloc(kSlots * d->slot_edges * eb);
-            for (int s = 0; s < kSlots; ++s) d->h_stage[s] = d->palloc(d->slot_edges * eb);
-            d->h_dlist = (uint32_t *)d->palloc(V * sizeof(uint32_t));
-            d->h_doff = (uint64_t *)d->palloc((V + 1) * sizeof(uint64_t));
-            d->pool.reset(new WorkerPool(d->host_threads));
+            if (!zc) {
+                if (d->slot_edges == 0)
+                    raise(OG_EBUDGET, "edge budget leaves %llu staging edges for %llu dynamic-region edges",
+                          (unsigned long long)pl.staging_edges, (unsigned long long)(E - g->ptr[pl.vsplit]));
+                d->d_staging = d->dalloc(kSlots * d->slot_edges * eb);
+                for (int s = 0; s < kSlots; ++s) d->h_stage[s] = d->palloc(d->slot_edges * eb);
+                d->h_dlist = (uint32_t *)d->palloc(V * sizeof(uint32_t));
+                d->h_doff = (uint64_t *)d->palloc((V + 1) * sizeof(uint64_t));
+                d->pool.reset(new WorkerPool(d->host_threads));
+            }
             // Host source of the dynamic region in the device format: the
             // CSR itself, or (packed) a pinned packed copy built once here.
             d->dyn_base = g->ptr[pl.vsplit];
@@ -774,7 +975,7 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
                 d->h_dyn = (const uint8_t *)g->edge_data() + d->dyn_base * eb;
             }
             d->slot_dyn = d->slot_edges;
-            if (p.flags & OG_PLAN_ZERO_COPY) {
+            if (zc) {
                 if (!d->packed && !(eb == 8 ? g->pinned_wedges : g->pinned_cols))
                     raise(OG_EUNSUPPORTED, "zero-copy loading needs the graph in pinned host memory");
                 // map the allocation's base, then offset to the dynamic region
@@ -782,6 +983,15 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
                 void *hd = nullptr;
                 OG_HIP(hipHostGetDevicePointer(&hd, (void *)hbase, 0));
                 d->hd_dyn = (const uint8_t *)hd + ((const uint8_t *)d->h_dyn - (const uint8_t *)hbase);
+                // Reading in place needs no transit slots, so the staging
+                // share of the budget keeps the head of the dynamic region
+                // (the rows right after the static prefix) resident in HBM.
+                d->dyn_resident = std::min<uint64_t>(pl.staging_edges, nd);
+                if (d->dyn_resident) {
+                    d->d_staging = d->dalloc(d->dyn_resident * eb);
+                    OG_HIP(hipMemcpyAsync(d->d_staging, d->h_dyn, d->dyn_resident * eb, hipMemcpyHostToDevice,
+                                          d->s_compute));
+                }
             }
         }
     }
@@ -797,9 +1007,89 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
 
 // ------------------------------------------------------- frontier driver
 
+enum { kClsFront = 0, kClsStat = 1, kClsDyn = 2, kClsCopy = 3 };
+
 struct RunTimers {
     TimedClass front, stat, dyn, copy;
+    TimedClass &at(int c) { return c == kClsFront ? front : c == kClsStat ? stat : c == kClsDyn ? dyn : copy; }
 };
+
+// A captured batch of device-driven rounds, replayed with one hipGraphLaunch.
+// Its timing marks are external event-record nodes, read after each replay.
+struct RoundGraph {
+    struct Mark {
+        int cls;
+        hipEvent_t a, b;
+    };
+    uint64_t key = ~0ull;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> events;
+    std::vector<Mark> marks;
+    ~RoundGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+};
+
+// Timing marks of enqueued work.  EagerSpan: an EventSpan of the run per
+// launch sequence.  GraphSpan: a pair of event-record nodes in the graph
+// being captured.
+struct EagerSpan {
+    DeviceState *d;
+    RunTimers &tm;
+    template <class F>
+    void operator()(int cls, hipStream_t s, F &&fn) const {
+        EventSpan span(d, &tm.at(cls), s);
+        fn();
+    }
+};
+
+struct GraphSpan {
+    RoundGraph *rg;
+    template <class F>
+    void operator()(int cls, hipStream_t s, F &&fn) const {
+        hipEvent_t a = nullptr, b = nullptr;
+        OG_HIP(hipEventCreate(&a));
+        rg->events.push_back(a);
+        OG_HIP(hipEventCreate(&b));
+        rg->events.push_back(b);
+        OG_HIP(hipEventRecordWithFlags(a, s, hipEventRecordExternal));
+        fn();
+        OG_HIP(hipEventRecordWithFlags(b, s, hipEventRecordExternal));
+        rg->marks.push_back({cls, a, b});
+    }
+};
+
+// The cached graph for `key`, captured on the compute stream by body(rg)
+// when the cache holds another one (or none).
+template <class Body>
+static RoundGraph *round_graph_for(DeviceState *d, uint64_t key, Body body) {
+    auto *cur = static_cast<RoundGraph *>(d->round_graph.get());
+    if (cur && cur->key == key) return cur;
+    d->round_graph.reset();
+    auto *rg = new RoundGraph;
+    d->round_graph = std::shared_ptr<void>(rg, [](void *p) { delete static_cast<RoundGraph *>(p); });
+    hipStream_t s = d->s_compute;
+    OG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+        body(rg);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        d->round_graph.reset();
+        throw;
+    }
+    OG_HIP(hipStreamEndCapture(s, &graph));
+    const hipError_t e = hipGraphInstantiate(&rg->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        d->round_graph.reset();
+        OG_HIP(e);
+    }
+    rg->key = key;
+    return rg;
+}
 
 // Host gather of dynamic edges [c0, c1) into a pinned slot (reference
 // fillEdgeArrByMultiThread, GraphMeta.h:470-506, with the copy offsets taken
@@ -836,77 +1126,105 @@ static RelaxArgs relax_args(DeviceState *d) {
     return RelaxArgs{d->d_val, d->fr.flag, d->d_visit, d->d_front, d->d_next, d->wbits};
 }
 
+// Grids of device-count launches: the kernels stride over whatever count
+// the round's compaction produced, so without a host-side count a fixed
+// grid that fills the chip (256 CUs) is launched.
+constexpr unsigned kGroupGridCap = 2048;  // workgroups of k_relax_group
+constexpr unsigned kHubGridCap = 1024;    // workgroups of k_relax_hub
+constexpr unsigned kDynGridCap = 4096;    // workgroups of k_relax_dyn
+
+static unsigned capped(uint64_t blocks, unsigned cap) {
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, cap));
+}
+
 template <int ALGO, bool W, bool PK>
-static void launch_static_relax_fmt(DeviceState *d, Frontier &f, const FrontierInfo &fi, const RelaxArgs &a,
+static void launch_static_relax_fmt(DeviceState *d, Frontier &f, const FrontierInfo *hfi, const RelaxArgs &a,
                                     hipStream_t s) {
     const void *edges = d->d_static;
     unsigned long long *upd = f.counters + 0;
-    const uint32_t n0 = fi.n[0], n1 = fi.n[1], n2 = fi.n[2];
-    if (n0) {
-        const uint64_t thr = (uint64_t)n0 * d->tune.lpv0;
-        const dim3 gr(grid_for(thr)), bl(kBlock);
-        switch (d->tune.lpv0) {
-#define OG_LPV(L) \
-    case L: hipLaunchKernelGGL((k_relax_group<ALGO, W, PK, L>), gr, bl, 0, s, f.list, n0, d->d_ptr, edges, a, upd); break;
+    const uint32_t lpv0 = d->tune.lpv0;
+    if (!hfi || hfi->n[0]) {
+        const dim3 gr(hfi ? capped(((uint64_t)hfi->n[0] * lpv0 + kBlock - 1) / kBlock, kGroupGridCap) : kGroupGridCap);
+        switch (lpv0) {
+#define OG_LPV(L)                                                                                                 \
+    case L:                                                                                                       \
+        hipLaunchKernelGGL((k_relax_group<ALGO, W, PK, L>), gr, dim3(kBlock), 0, s, f.list, 0, f.info, d->d_ptr, \
+                           edges, a, upd);                                                                        \
+        break;
             OG_LPV(1) OG_LPV(2) OG_LPV(4) OG_LPV(8) OG_LPV(16) OG_LPV(32)
             default: OG_LPV(64)
 #undef OG_LPV
         }
     }
-    if (n1)
-        hipLaunchKernelGGL((k_relax_group<ALGO, W, PK, 64>), dim3(grid_for((uint64_t)n1 * 64)), dim3(kBlock), 0, s,
-                           f.list + n0, n1, d->d_ptr, edges, a, upd);
-    if (n2)
-        hipLaunchKernelGGL((k_relax_hub<ALGO, W, PK>), dim3(n2), dim3(kBlock), d->tune.tile * d->eb, s, f.hub,
-                           d->tune.tile, d->d_ptr, edges, a, upd);
+    if (!hfi || hfi->n[1])
+        hipLaunchKernelGGL((k_relax_group<ALGO, W, PK, 64>),
+                           dim3(hfi ? capped(((uint64_t)hfi->n[1] * 64 + kBlock - 1) / kBlock, kGroupGridCap)
+                                    : kGroupGridCap),
+                           dim3(kBlock), 0, s, f.list, 1, f.info, d->d_ptr, edges, a, upd);
+    if (!hfi || hfi->n[2])
+        hipLaunchKernelGGL((k_relax_hub<ALGO, W, PK>), dim3(hfi ? capped(hfi->n[2], kHubGridCap) : kHubGridCap),
+                           dim3(kBlock), d->tune.tile * d->eb, s, f.hub, f.hub_cap, f.info, d->tune.tile, d->d_ptr,
+                           edges, a, upd);
     OG_HIP(hipGetLastError());
 }
 
 template <int ALGO, bool W>
-void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo &fi, const RelaxArgs &a, hipStream_t s) {
-    if (W && d->packed) launch_static_relax_fmt<ALGO, W, true>(d, f, fi, a, s);
-    else launch_static_relax_fmt<ALGO, W, false>(d, f, fi, a, s);
+void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo *hfi, const RelaxArgs &a, hipStream_t s) {
+    if (W && d->packed) launch_static_relax_fmt<ALGO, W, true>(d, f, hfi, a, s);
+    else launch_static_relax_fmt<ALGO, W, false>(d, f, hfi, a, s);
 }
-template void launch_static_relax<ALGO_MSBFS, false>(DeviceState *, Frontier &, const FrontierInfo &, const RelaxArgs &, hipStream_t);
-template void launch_static_relax<ALGO_MSBFS, true>(DeviceState *, Frontier &, const FrontierInfo &, const RelaxArgs &, hipStream_t);
+template void launch_static_relax<ALGO_MSBFS, false>(DeviceState *, Frontier &, const FrontierInfo *, const RelaxArgs &, hipStream_t);
+template void launch_static_relax<ALGO_MSBFS, true>(DeviceState *, Frontier &, const FrontierInfo *, const RelaxArgs &, hipStream_t);
 
-// Static-region relax on its own stream, so it overlaps the dynamic region
-// (link bound) of the same round; returns true when launched (the caller
-// joins on ev_static before the next compaction).  All relax updates are
-// monotone (atomicMin / same-level stores / atomicOr), so the two regions
-// may interleave.
-template <int ALGO, bool W>
-static bool launch_static(DeviceState *d, const FrontierInfo &fi, RunTimers &tm) {
-    if (!(fi.n[0] | fi.n[1] | fi.n[2])) return false;
-    {
-        EventSpan span(d, &tm.stat, d->s_static);
-        launch_static_relax<ALGO, W>(d, d->fr, fi, relax_args(d), d->s_static);
+// Static-region relax of a round on s_static, so that it overlaps the
+// dynamic region (link bound): forked from the compute stream after the
+// compaction; returns true when the caller must join ev_static before the
+// next compaction.  All relax updates are monotone (atomicMin / same-level
+// stores / atomicOr), so the two regions may interleave.  NO_OVERLAP runs it
+// on the compute stream instead.  hfi: the round's summary when the host has
+// it (sizes the grids, skips empty bins), else null.
+template <int ALGO, bool W, class Span>
+static bool launch_static(DeviceState *d, const FrontierInfo *hfi, const Span &span) {
+    if (hfi && !(hfi->n[0] | hfi->n[1] | hfi->n[2])) return false;
+    if (d->flags & OG_PLAN_NO_OVERLAP) {
+        span(kClsStat, d->s_compute,
+             [&] { launch_static_relax<ALGO, W>(d, d->fr, hfi, relax_args(d), d->s_compute); });
+        return false;
     }
+    OG_HIP(hipEventRecord(d->ev_fork, d->s_compute));
+    OG_HIP(hipStreamWaitEvent(d->s_static, d->ev_fork, 0));
+    span(kClsStat, d->s_static, [&] { launch_static_relax<ALGO, W>(d, d->fr, hfi, relax_args(d), d->s_static); });
     OG_HIP(hipEventRecord(d->ev_static, d->s_static));
     return true;
 }
 
 template <int ALGO, bool W, bool PACKED, bool DIRECT>
-static void launch_dyn(DeviceState *d, const uint32_t *dlist, uint32_t ndyn, const void *src, uint64_t c0,
-                       uint64_t c1) {
-    hipLaunchKernelGGL((k_relax_dyn<ALGO, W, PACKED, DIRECT>), dim3(grid_for(c1 - c0, kDynEdgesPerBlock)),
-                       dim3(kBlock), 0, d->s_compute, dlist, d->fr.doff, ndyn, src, c0, c1, d->d_ptr, d->dyn_base,
-                       d->wbits, relax_args(d), d->fr.counters + 1);
+static void launch_dyn(DeviceState *d, unsigned grid, const void *src, uint64_t c0, uint64_t c1) {
+    hipLaunchKernelGGL((k_relax_dyn<ALGO, W, PACKED, DIRECT>), dim3(grid), dim3(kBlock), 0, d->s_compute, d->fr.list,
+                       d->fr.info, d->fr.doff, src, c0, c1, d->d_ptr, d->dyn_base, (const void *)d->d_staging,
+                       DIRECT ? d->dyn_resident : 0, d->wbits, relax_args(d), d->fr.counters + 1,
+                       d->fr.counters + 5);
 }
 
+// Dynamic relax of edges [c0, min(c1, D)); `known` = c1 - c0 when the host
+// knows the edge count (exact grid), else 0 (fixed grid).
 template <int ALGO, bool W>
-static void launch_dyn_any(DeviceState *d, const uint32_t *dlist, uint32_t ndyn, const void *src, uint64_t c0,
-                           uint64_t c1, bool direct) {
+static void launch_dyn_any(DeviceState *d, const void *src, uint64_t c0, uint64_t c1, uint64_t known, bool direct) {
+    const unsigned grid = known ? capped((known + kDynEdgesPerBlock - 1) / kDynEdgesPerBlock, kDynGridCap)
+                                : kDynGridCap;
     if (d->packed) {
-        if (direct) launch_dyn<ALGO, W, true, true>(d, dlist, ndyn, src, c0, c1);
-        else launch_dyn<ALGO, W, true, false>(d, dlist, ndyn, src, c0, c1);
+        if (direct) launch_dyn<ALGO, W, true, true>(d, grid, src, c0, c1);
+        else launch_dyn<ALGO, W, true, false>(d, grid, src, c0, c1);
     } else {
-        if (direct) launch_dyn<ALGO, W, false, true>(d, dlist, ndyn, src, c0, c1);
-        else launch_dyn<ALGO, W, false, false>(d, dlist, ndyn, src, c0, c1);
+        if (direct) launch_dyn<ALGO, W, false, true>(d, grid, src, c0, c1);
+        else launch_dyn<ALGO, W, false, false>(d, grid, src, c0, c1);
     }
     OG_HIP(hipGetLastError());
 }
 
+// Staged dynamic region of a round whose summary the host holds: the host
+// gathers the active edges into pinned slots and the copy stream moves them
+// while the other slot is relaxed.
 template <int ALGO, bool W>
 static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &fi, RunTimers &tm, og_run_stats &st,
                         double &gather_ms) {
@@ -915,18 +1233,9 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
     if (!ndyn || !D) return;
     const uint32_t *d_dlist = d->fr.list + fi.n[0] + fi.n[1];
     const uint64_t eb = d->deb;
-    if (d->flags & OG_PLAN_ZERO_COPY) {
-        // one launch reads every active dynamic edge over the link
-        EventSpan span(d, &tm.dyn, d->s_compute);
-        launch_dyn_any<ALGO, W>(d, d_dlist, ndyn, d->hd_dyn, 0, D, true);
-        st.h2d_bytes += D * eb;
-        st.chunks++;
-        return;
-    }
     OG_HIP(hipMemcpyAsync(d->h_dlist, d_dlist, (size_t)ndyn * 4, hipMemcpyDeviceToHost, d->s_copy));
     OG_HIP(hipMemcpyAsync(d->h_doff, d->fr.doff, ((size_t)ndyn + 1) * 8, hipMemcpyDeviceToHost, d->s_copy));
     OG_HIP(hipStreamSynchronize(d->s_copy));
-    OG_HIP(hipStreamSynchronize(d->s_static));
     const uint64_t C = d->slot_dyn;
     const bool serial = (d->flags & OG_PLAN_NO_OVERLAP) != 0;
     for (uint64_t c0 = 0, c = 0; c0 < D; c0 += C, ++c) {
@@ -947,7 +1256,7 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
         OG_HIP(hipStreamWaitEvent(d->s_compute, d->ev_h2d[slot], 0));
         {
             EventSpan span(d, &tm.dyn, d->s_compute);
-            launch_dyn_any<ALGO, W>(d, d_dlist, ndyn, (const void *)dslot, c0, c1, false);
+            launch_dyn_any<ALGO, W>(d, (const void *)dslot, c0, c1, c1 - c0, false);
         }
         OG_HIP(hipEventRecord(d->ev_relax[slot], d->s_compute));
         d->slot_relax_pending[slot] = true;
@@ -963,59 +1272,129 @@ constexpr uint64_t edge_bytes_alg() {
     return ALGO == ALGO_MSBFS ? 12ull : (ALGO == ALGO_SSSP ? 12ull : 8ull);
 }
 
-// Iterates compaction -> static relax -> dynamic relax -> after(round) until
-// the frontier is empty.  One host synchronisation per round (the frontier
-// summary).  Accumulates into st / tm.
+constexpr int kRoundBatch = 4;  // device-driven rounds enqueued per host check
+
+// Iterates compaction -> static relax || dynamic relax -> after() until no
+// vertex is active or held back.  Accumulates into st / tm.
+//
+// Without a host-side loader (whole graph in HBM, zero-copy dynamic region,
+// USM) the rounds are device-driven: every kernel reads its counts from the
+// round's FrontierInfo and the near-far threshold advances on the device, so
+// kRoundBatch rounds are enqueued at a time and the host reads the traversal
+// state once per batch (rounds enqueued past the end find no flags and do no
+// work).  The staged loader needs the host in every round (gather, copies),
+// so it synchronises on each round's summary.
 template <int ALGO, bool W, class After>
 static void frontier_loop(og_graph *g, og_run_stats &st, RunTimers &tm, double &gather_ms, double &overload_wall,
                           After after) {
     DeviceState *d = g->dev;
     hipStream_t s = d->s_compute;
-    // SSSP near-far order: only vertices below `thresh` relax in a round; when
-    // none is left the threshold moves to (smallest held-back value + delta).
+    // SSSP near-far order: only vertices below the threshold relax in a
+    // round; when none is left it moves to (smallest held-back value + delta).
     const uint32_t delta = ALGO == ALGO_SSSP ? d->tune.delta : 0;
-    uint32_t thresh = delta ? (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, 1ull + delta) : 0xFFFFFFFFu;
-    for (;;) {
+    const uint32_t *nf_val = delta ? d->d_val : nullptr;
+    const bool zero_copy = (d->flags & OG_PLAN_ZERO_COPY) != 0;
+    const bool has_dyn = d->plan.vsplit < d->V;
+    const bool staged = has_dyn && !zero_copy;
+    // deferring dynamic vertices reorders relaxations across levels, which
+    // only the atomicMin relaxes (SSSP, CC) tolerate
+    const uint32_t defer = (ALGO == ALGO_SSSP || ALGO == ALGO_CC) && has_dyn ? d->tune.defer : 0;
+    frontier_begin(d->fr, s, delta, defer);
+    if (!staged) {
+        const EagerSpan eager{d, tm};
+        eager(kClsFront, s, [&] { frontier_compact(d, d->fr, s, nf_val, false); });
+        // one batch: kRoundBatch x (static || dynamic relax, after, compaction),
+        // then the traversal state to the host
+        auto batch = [&](const auto &span) {
+            for (int r = 0; r < kRoundBatch; ++r) {
+                const bool joined = launch_static<ALGO, W>(d, nullptr, span);
+                if (has_dyn) span(kClsDyn, s, [&] { launch_dyn_any<ALGO, W>(d, d->hd_dyn, 0, ~0ull, 0, true); });
+                if (joined) OG_HIP(hipStreamWaitEvent(s, d->ev_static, 0));
+                after();
+                span(kClsFront, s, [&] { frontier_compact(d, d->fr, s, nf_val, false); });
+            }
+            OG_HIP(hipMemcpyAsync(d->fr.h_ctl, d->fr.ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
+        };
+        if (!(d->flags & OG_PLAN_NO_GRAPH)) {
+            const uint64_t key = (uint64_t)ALGO | (uint64_t)W << 4 | (uint64_t)(delta != 0) << 5 |
+                                 (uint64_t)has_dyn << 6 | (uint64_t)kRoundBatch << 8;
+            RoundGraph *rg = round_graph_for(d, key, [&](RoundGraph *n) { batch(GraphSpan{n}); });
+            for (;;) {
+                OG_HIP(hipGraphLaunch(rg->exec, s));
+                OG_HIP(hipStreamSynchronize(s));
+                for (const auto &m : rg->marks) {
+                    float ms = 0;
+                    if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+                        tm.at(m.cls).ms += ms;
+                        tm.at(m.cls).launches++;
+                    } else {
+                        (void)hipGetLastError();
+                    }
+                }
+                if (!d->fr.h_ctl->live) break;
+            }
+        } else {
+            for (;;) {
+                batch(eager);
+                OG_HIP(hipStreamSynchronize(s));
+                if (!d->fr.h_ctl->live) break;
+            }
+        }
+    } else {
         {
             EventSpan span(d, &tm.front, s);
-            frontier_compact(d, d->fr, s, delta ? d->d_val : nullptr, thresh);
+            frontier_compact(d, d->fr, s, nf_val, true);
         }
         OG_HIP(hipStreamSynchronize(s));
-        const FrontierInfo fi = *d->fr.h_info;
-        const uint64_t active = (uint64_t)fi.n[0] + fi.n[1] + fi.nhub + fi.n[3] + fi.nzero;
-        if (active == 0) {
-            if (delta && fi.nfar) {
-                thresh = (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, (uint64_t)fi.min_far + delta);
-                continue;
+        for (;;) {
+            const FrontierInfo fi = *d->fr.h_info;
+            if (!fi.live) break;
+            const uint64_t active = (uint64_t)fi.n[0] + fi.n[1] + fi.nhub + fi.n[3] + fi.nzero;
+            if (active) {  // else only the near-far threshold moved
+                if (fi.n[2] > d->fr.hub_cap)
+                    raise(OG_EHIP, "hub tile list overflow (%u > %llu)", fi.n[2], (unsigned long long)d->fr.hub_cap);
+                const bool joined = launch_static<ALGO, W>(d, &fi, EagerSpan{d, tm});
+                if (fi.n[3]) {
+                    Stopwatch ow;
+                    run_dynamic<ALGO, W>(d, g, fi, tm, st, gather_ms);
+                    overload_wall += ow.ms();
+                }
+                if (joined) OG_HIP(hipStreamWaitEvent(s, d->ev_static, 0));
+                after();
             }
-            break;
+            {
+                EventSpan span(d, &tm.front, s);
+                frontier_compact(d, d->fr, s, nf_val, true);
+            }
+            OG_HIP(hipStreamSynchronize(s));
         }
-        if (fi.n[2] > d->fr.hub_cap)
-            raise(OG_EHIP, "hub tile list overflow (%u > %llu)", fi.n[2], (unsigned long long)d->fr.hub_cap);
-        st.iterations++;
-        st.node_sum += active;
-        st.static_node_sum += (uint64_t)fi.n[0] + fi.n[1] + fi.nhub;
-        st.work_edges += fi.static_deg + fi.dyn_deg;
-        st.static_work_edges += fi.static_deg;
-        // per edge: the edge in the device format (d->eb bytes) + the target state
-        const uint64_t per_edge = edge_bytes_alg<ALGO, W>() - (W ? 8ull : 4ull) + d->eb;
-        st.relax_static_bytes += 24ull * ((uint64_t)fi.n[0] + fi.n[1] + fi.nhub) + per_edge * fi.static_deg;
-        st.relax_dynamic_bytes += 24ull * fi.n[3] + per_edge * fi.dyn_deg;
-        st.frontier_bytes += 2ull * d->Vpad + 4ull * active;
-        const bool joined = launch_static<ALGO, W>(d, fi, tm);
-        if (fi.n[3]) {
-            Stopwatch ow;
-            run_dynamic<ALGO, W>(d, g, fi, tm, st, gather_ms);
-            overload_wall += ow.ms();
-        }
-        if (joined) OG_HIP(hipStreamWaitEvent(s, d->ev_static, 0));
-        after(st.iterations);
+        OG_HIP(hipMemcpyAsync(d->fr.h_ctl, d->fr.ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
+        OG_HIP(hipStreamSynchronize(s));
+    }
+    const RoundCtl c = *d->fr.h_ctl;
+    if (c.overflow) raise(OG_EUNSUPPORTED, "multi-source BFS deeper than 254 levels");
+    st.iterations += c.rounds;
+    st.node_sum += c.node_sum;
+    st.static_node_sum += c.static_node_sum;
+    st.work_edges += c.work_edges;
+    st.static_work_edges += c.static_work_edges;
+    // per edge: the edge in the device format (d->eb bytes) + the target state
+    const uint64_t per_edge = edge_bytes_alg<ALGO, W>() - (W ? 8ull : 4ull) + d->eb;
+    const uint64_t dyn_edges = c.work_edges - c.static_work_edges;
+    st.relax_static_bytes += 24ull * c.static_node_sum + per_edge * c.static_work_edges;
+    st.relax_dynamic_bytes += 24ull * c.dyn_node_sum + per_edge * dyn_edges;
+    st.frontier_bytes += 2ull * d->Vpad * c.compactions + 4ull * c.node_sum;
+    if (has_dyn && zero_copy) {  // every active dynamic edge read in place (HBM-resident ones subtracted later)
+        st.h2d_bytes += dyn_edges * d->deb;
+        st.chunks += c.dyn_rounds;
     }
 }
 
 static void finish_stats(og_run_stats &st, RunTimers &tm, double gather_ms, double overload_wall,
-                         const unsigned long long *ctr, uint64_t upd_bytes) {
+                         const unsigned long long *ctr, uint64_t upd_bytes, uint64_t link_eb) {
     st.updates = ctr[0] + ctr[1];
+    // dynamic edges served from the HBM-resident head did not cross the link
+    st.h2d_bytes -= std::min<uint64_t>(st.h2d_bytes, ctr[5] * link_eb);
     st.relax_static_bytes += upd_bytes * ctr[0];
     st.relax_dynamic_bytes += upd_bytes * ctr[1];
     st.frontier_kernel_ms = sum_ms(tm.front);
@@ -1057,7 +1436,7 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
         hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d->d_val, d->fr.flag, src);
     }
     OG_HIP(hipGetLastError());
-    frontier_loop<ALGO, W>(g, st, tm, gather_ms, overload_wall, [](uint32_t) {});
+    frontier_loop<ALGO, W>(g, st, tm, gather_ms, overload_wall, []() {});
     OG_HIP(hipStreamSynchronize(s));
     st.total_ms = total.ms();
 
@@ -1070,7 +1449,7 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
     st.reached = d->fr.h_counters[2];
     st.traversed_edges = d->fr.h_counters[3];
     st.checksum = d->fr.h_counters[4];
-    finish_stats(st, tm, gather_ms, overload_wall, d->fr.h_counters, 8);
+    finish_stats(st, tm, gather_ms, overload_wall, d->fr.h_counters, 8, d->deb);
     if (stats) *stats = st;
 }
 
@@ -1114,12 +1493,10 @@ static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t
         hipLaunchKernelGGL(k_ms_init, dim3(1), dim3(kMsWidth), 0, s, d->d_msrc, nb, d->d_visit, d->d_front,
                            d->fr.flag, d->d_mlvl);
         OG_HIP(hipGetLastError());
-        uint32_t level = 1;
         const unsigned cgrid = (unsigned)std::min<uint64_t>(4096, (V + kBlock - 1) / kBlock);
-        frontier_loop<ALGO_MSBFS, W>(g, st, tm, gather_ms, overload_wall, [&](uint32_t) {
-            if (++level > 255) raise(OG_EUNSUPPORTED, "multi-source BFS deeper than 254 levels");
+        frontier_loop<ALGO_MSBFS, W>(g, st, tm, gather_ms, overload_wall, [&]() {
             hipLaunchKernelGGL(k_ms_commit, dim3(cgrid), dim3(kBlock), 0, s, d->fr.flag, V, d->d_visit, d->d_front,
-                               d->d_next, d->d_mlvl, (uint8_t)level);
+                               d->d_next, d->d_mlvl, d->fr.ctl);
             OG_HIP(hipGetLastError());
         });
         OG_HIP(hipMemsetAsync(d->d_msum, 0, 3 * kMsWidth * 8, s));
@@ -1149,7 +1526,7 @@ static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t
     st.total_ms = device_ms;
     OG_HIP(hipMemcpyAsync(d->fr.h_counters, d->fr.counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     OG_HIP(hipStreamSynchronize(s));
-    finish_stats(st, tm, gather_ms, overload_wall, d->fr.h_counters, 9);
+    finish_stats(st, tm, gather_ms, overload_wall, d->fr.h_counters, 9, d->deb);
     if (stats) *stats = st;
 }
 
@@ -1261,6 +1638,7 @@ static og_status check_tuning(const og_tuning *t) {
         return fail(OG_EINVAL, "lpv0 must be a power of two in [1, 64]");
     if (t->t1 < 1 || t->t2 < t->t1) return fail(OG_EINVAL, "need 1 <= t1 <= t2");
     if (t->tile < 64 || t->tile > 4096) return fail(OG_EINVAL, "tile must be in [64, 4096]");
+    if (t->defer > 1) return fail(OG_EINVAL, "defer must be 0 or 1");
     return OG_OK;
 }
 
@@ -1271,16 +1649,18 @@ static void apply_tuning(og_graph *g, const og_tuning *t) {
     d->tune.t2 = t->t2;
     d->tune.tile = t->tile;
     d->tune.delta = t->delta;
+    d->tune.defer = t->defer;
     const uint64_t cap = hub_capacity(g, d->plan.vsplit, d->tune);
     if (cap > d->fr.hub_cap) {
         d->fr.hub = (uint64_t *)d->dalloc(cap * sizeof(uint64_t));
         d->fr.hub_cap = cap;
     }
-    d->pr_plan.reset();  // PageRank bins follow t1 / t2
+    d->pr_plan.reset();      // PageRank bins follow t1 / t2
+    d->round_graph.reset();  // captured rounds hold the tuning and the hub list
 }
 
 static og_tuning tuning_of(const DeviceState *d) {
-    return og_tuning{d->tune.t1, d->tune.lpv0, d->tune.t2, d->tune.tile, d->tune.delta};
+    return og_tuning{d->tune.t1, d->tune.lpv0, d->tune.t2, d->tune.tile, d->tune.delta, d->tune.defer};
 }
 
 static double time_algo(og_graph *g, uint32_t algo, uint32_t src, uint32_t runs) {
@@ -1356,7 +1736,14 @@ extern "C" og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, ui
             }
             apply_tuning(g, &cur);
         };
+        const bool has_dyn = g->dev->plan.vsplit < g->V;
         for (int pass = 0; pass < 2; ++pass) {
+            if (algo != OG_TUNE_BFS && has_dyn)
+                for (uint32_t df : {0u, 1u}) {
+                    og_tuning c = cur;
+                    c.defer = df;
+                    trial(c);
+                }
             if (algo == OG_TUNE_SSSP)
                 for (uint32_t dl : {0u, 8u, 16u, 32u, 64u, 128u, 256u, 512u}) {
                     og_tuning c = cur;

@@ -386,8 +386,8 @@ void Engine::step() {
     if (fi.n[0] | fi.n[1] | fi.n[2]) {
         if (kind == 0) {
             RelaxArgs a{nullptr, f.flag, visit, front, next, d->wbits};
-            if (w) launch_static_relax<ALGO_MSBFS, true>(d, f, fi, a, s);
-            else launch_static_relax<ALGO_MSBFS, false>(d, f, fi, a, s);
+            if (w) launch_static_relax<ALGO_MSBFS, true>(d, f, &fi, a, s);
+            else launch_static_relax<ALGO_MSBFS, false>(d, f, &fi, a, s);
         } else {
             const uint32_t n01 = fi.n[0] + fi.n[1];
             const uint64_t waves = (uint64_t)n01 + 4ull * fi.n[2];

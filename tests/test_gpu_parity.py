@@ -17,7 +17,10 @@ PLANS = [
     ("ascetic", 1 / 64, 0.5),  # tiny staging: many chunks through both slots
     ("ascetic", 0.25, 0.0),    # reference auto-advisek rule
     ("ascetic+zerocopy", 0.25, 0.8),  # dynamic edges read in place over the link
-    ("ascetic+nopack", 1 / 64, 0.5),  # 8-byte weighted edges on the link
+    ("ascetic+zerocopy", 1 / 64, 0.5),  # small HBM-resident head, most dynamic reads over the link
+    ("ascetic+zerocopy+nograph", 0.25, 0.8),  # device-driven rounds enqueued eagerly
+    ("ascetic+nograph", 1.0, 1.0),
+    ("ascetic+nopack", 1 / 64, 0.5),  # 8-byte weighted edges in HBM and on the link
     ("subway+zerocopy", 0.25, 0.0),
     ("subway", 0.25, 0.0),
     ("subway", 1 / 64, 0.0),
@@ -33,8 +36,11 @@ def _say(*a):
 def _prepare(g, approach, frac, k, edge_bytes):
     E = g.num_edges
     budget = int(max(1, E * edge_bytes * frac)) if frac else 0
-    name, _, opt = approach.partition("+")
-    flags = {"": 0, "zerocopy": og.PLAN_ZERO_COPY, "nopack": og.PLAN_NO_PACK}[opt]
+    name, *opts = approach.split("+")
+    flags = 0
+    for o in opts:
+        flags |= {"zerocopy": og.PLAN_ZERO_COPY, "nopack": og.PLAN_NO_PACK, "nograph": og.PLAN_NO_GRAPH,
+                  "nooverlap": og.PLAN_NO_OVERLAP}[o]
     return g.prepare(name, edge_budget_bytes=budget, advisek=k, flags=flags)
 
 
@@ -177,3 +183,51 @@ def test_msbfs(gpu, rmat_w, approach, frac, k):
         assert st1["checksum"] == cs[list(srcs).index(s)]
     _, cs2, _, _ = rmat_w.msbfs(srcs[:64], want_levels=False)
     assert np.array_equal(cs2, cs[:64])
+
+
+@pytest.mark.parametrize("approach,frac,k", [("ascetic", 1.0, 1.0), ("ascetic+zerocopy", 0.25, 0.8),
+                                              ("ascetic+zerocopy+nooverlap", 0.25, 0.8), ("usm", 0.0, 0.0)])
+def test_graph_replays(gpu, rmat_w, approach, frac, k):
+    """Device-driven rounds replayed from a captured hipGraph: every replay of
+    every call equals the eagerly enqueued rounds and the oracle, with other
+    runs (another algorithm, another source) in between."""
+    ptr, to, w, _ = rmat_w.export()
+    srcs = _sources(ptr, len(to), 2, 9)
+    _say("graph replays rmat13", approach, frac, k)
+    _prepare(rmat_w, approach + "+nograph", frac, k, 8)
+    eager = {s: (rmat_w.bfs(s)[0], rmat_w.sssp(s)[0]) for s in srcs}
+    eager_cc = rmat_w.cc()[0]
+    _prepare(rmat_w, approach, frac, k, 8)
+    for rep in range(3):
+        for s in srcs:
+            lv, _ = rmat_w.bfs(s)
+            d, _ = rmat_w.sssp(s)
+            assert np.array_equal(lv, eager[s][0]), (rep, s)
+            assert np.array_equal(d, eager[s][1]), (rep, s)
+        assert np.array_equal(rmat_w.cc()[0], eager_cc), rep
+    for s in srcs:
+        assert np.array_equal(eager[s][0], oracle.bfs(ptr, to, s)[0])
+        assert np.array_equal(eager[s][1], oracle.sssp(ptr, to, w, s))
+    assert np.array_equal(eager_cc, oracle.cc(ptr, to)[0])
+    # a retune drops the captured rounds; the next capture must agree too
+    rmat_w.set_tuning(t1=16, lpv0=4, delta=8)
+    for s in srcs:
+        assert np.array_equal(rmat_w.sssp(s)[0], eager[s][1])
+
+
+@pytest.mark.parametrize("approach,frac,k", [("ascetic+zerocopy", 0.25, 0.8), ("ascetic", 1 / 64, 0.5),
+                                              ("ascetic+zerocopy", 1 / 64, 0.5), ("subway+zerocopy", 0.25, 0.0)])
+@pytest.mark.parametrize("delta", [0, 8])
+def test_deferred_dynamic_order(gpu, rmat_w, approach, frac, k, delta):
+    """Holding the dynamic list back while static vertices are active changes
+    the relax order, not the fixed point: SSSP and CC stay bit-exact."""
+    ptr, to, w, _ = rmat_w.export()
+    _say("deferred dynamic rmat13", approach, frac, k, delta)
+    _prepare(rmat_w, approach, frac, k, 8)
+    rmat_w.set_tuning(defer=1, delta=delta)
+    for s in _sources(ptr, len(to), 2, 13):
+        d, st = rmat_w.sssp(s)
+        assert np.array_equal(d, oracle.sssp(ptr, to, w, s)), s
+    assert np.array_equal(rmat_w.cc()[0], oracle.cc(ptr, to)[0])
+    lv, _ = rmat_w.bfs(3)  # BFS ignores defer (plain level stores)
+    assert np.array_equal(lv, oracle.bfs(ptr, to, 3)[0])
