@@ -99,6 +99,29 @@ og_status og_graph_from_csr(uint64_t num_vertices, uint64_t num_edges,
                             const uint32_t *weights, const uint32_t *out_degree,
                             og_graph_kind kind, og_graph **out);
 
+/* Edge-list conversion (no reference counterpart: the reference relies on
+ * the external Ascetic converter, README.md:7, to turn a text graph into its
+ * CSR).  From arcs src[i] -> dst[i] (weights[i] when weights != NULL) the
+ * graph gets deduplicated, row-sorted adjacency: a repeated arc keeps its
+ * smallest weight, self-loops are dropped unless OG_EDGES_KEEP_SELF_LOOPS,
+ * OG_EDGES_SYMMETRIZE adds v -> u for every u -> v (same weight).
+ * num_vertices = 0 takes max id + 1; ids >= num_vertices are OG_EINVAL. */
+#define OG_EDGES_SYMMETRIZE 1u
+#define OG_EDGES_KEEP_SELF_LOOPS 2u
+#define OG_EDGES_ONE_BASED 4u   /* text ids start at 1 (MatrixMarket sets it) */
+#define OG_EDGES_WEIGHTED 8u    /* text: the third column is a u32 weight     */
+og_status og_graph_from_edges(uint64_t num_vertices, uint64_t num_edges,
+                              const uint32_t *src, const uint32_t *dst,
+                              const uint32_t *weights, uint32_t flags, og_graph **out);
+
+/* Reads a text edge list: one arc "u v [w]" per line, separated by spaces,
+ * tabs or commas; lines starting with '#' or '%' are comments (SNAP style).
+ * A MatrixMarket coordinate file ("%%MatrixMarket" first line) is read with
+ * its size line, 1-based ids and, when "symmetric", both directions.  Builds
+ * the graph as og_graph_from_edges does (plain, or weighted with
+ * OG_EDGES_WEIGHTED).  OG_EFORMAT names the first bad line. */
+og_status og_graph_load_edgelist(const char *path, uint32_t flags, og_graph **out);
+
 /* Synthetic inputs (no reference counterpart: the reference relies on the
  * external Ascetic converter, README.md:7).  Directed R-MAT with quadrant
  * probabilities a,b,c (d = 1-a-b-c), self-loops and duplicates removed,

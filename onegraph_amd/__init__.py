@@ -21,6 +21,7 @@ PLAN_NO_OVERLAP = 1
 PLAN_ZERO_COPY = 2
 PLAN_NO_PACK = 4
 PLAN_NO_GRAPH = 8
+EDGES_SYMMETRIZE, EDGES_KEEP_SELF_LOOPS, EDGES_ONE_BASED, EDGES_WEIGHTED = 1, 2, 4, 8
 
 STATUS = {
     0: "OG_OK", 1: "OG_EINVAL", 2: "OG_EIO", 3: "OG_EFORMAT", 4: "OG_EBUDGET",
@@ -139,6 +140,8 @@ def lib():
         "og_graph_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_P)]),
         "og_graph_save": (C.c_int, [_P, C.c_char_p]),
         "og_graph_from_csr": (C.c_int, [C.c_uint64, C.c_uint64, _u64p, _u32p, _u32p, _u32p, C.c_int, C.POINTER(_P)]),
+        "og_graph_from_edges": (C.c_int, [C.c_uint64, C.c_uint64, _u32p, _u32p, _u32p, C.c_uint32, C.POINTER(_P)]),
+        "og_graph_load_edgelist": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_P)]),
         "og_graph_generate_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                              C.c_uint64, C.c_uint64, C.POINTER(_P)]),
         "og_graph_generate_er": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(_P)]),
@@ -247,6 +250,29 @@ class Graph:
         V = len(ptr)
         return cls._make(lib().og_graph_from_csr, V, E, _u64(ptr), _u32(cols) if E else _u32(np.zeros(1, np.uint32)),
                          _u32(w), _u32(od), kind)
+
+    @classmethod
+    def from_edges(cls, src, dst, weights=None, num_vertices=0, symmetrize=False, keep_self_loops=False):
+        """Arcs src[i] -> dst[i]: deduplicated (a repeated arc keeps its
+        smallest weight), self-loops dropped unless kept, rows sorted."""
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        if len(src) != len(dst):
+            raise ValueError("src and dst differ in length")
+        w = np.ascontiguousarray(weights, dtype=np.uint32) if weights is not None else None
+        if w is not None and len(w) != len(src):
+            raise ValueError("weights differ in length")
+        flags = (EDGES_SYMMETRIZE if symmetrize else 0) | (EDGES_KEEP_SELF_LOOPS if keep_self_loops else 0)
+        z = np.zeros(1, np.uint32)
+        return cls._make(lib().og_graph_from_edges, num_vertices, len(src), _u32(src if len(src) else z),
+                         _u32(dst if len(dst) else z), _u32(w) if w is None or len(w) else _u32(z), flags)
+
+    @classmethod
+    def load_edgelist(cls, path, weighted=False, symmetrize=False, keep_self_loops=False, one_based=False):
+        """Text edge list ("u v [w]" lines, '#'/'%' comments) or MatrixMarket."""
+        flags = ((EDGES_WEIGHTED if weighted else 0) | (EDGES_SYMMETRIZE if symmetrize else 0) |
+                 (EDGES_KEEP_SELF_LOOPS if keep_self_loops else 0) | (EDGES_ONE_BASED if one_based else 0))
+        return cls._make(lib().og_graph_load_edgelist, str(path).encode(), flags)
 
     @classmethod
     def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, weight_seed=0):

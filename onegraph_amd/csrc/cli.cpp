@@ -4,6 +4,7 @@
 //            [--approach ascetic|subway|usm] [--edge-budget BYTES | --budget-fraction F]
 //            [--bignode-runs N] [--repeat N] [--json] [--out FILE]
 //            [--fp64] [--pr-tol T] [--pr-iters N] [--generate SPEC [--save FILE]]
+//            [--text [--text-weights] [--symmetrize] [--one-based] ...] [--check FILE]
 //
 // Accepts every spelling the reference programs use: --type / -type
 // (BFS/oneGraph_BFS/ArgumentParser.h:48, SSSP/oneGraph/ArgumentParser.h),
@@ -68,6 +69,28 @@ og_status open_graph(const std::string &path, og_graph_kind kind, const Options 
         }
         if (st != OG_OK) return report_error(st, "generate");
         *out = g;
+    } else if (opt.text_input) {
+        // text edge list: weights from the file's third column when asked
+        // for, else seeded [1, 63] weights for the weighted algorithm
+        const bool text_w = (opt.edge_flags & OG_EDGES_WEIGHTED) != 0;
+        og_graph *g = nullptr;
+        st = og_graph_load_edgelist(path.c_str(), kind == OG_KIND_WEIGHTED ? opt.edge_flags
+                                                                              : opt.edge_flags & ~OG_EDGES_WEIGHTED,
+                                    &g);
+        if (st == OG_OK && kind == OG_KIND_WEIGHTED && !text_w) {
+            og_graph *w = nullptr;
+            st = og_graph_add_weights(g, opt.weight_seed, &w);
+            og_graph_free(g);
+            g = w;
+        }
+        if (st == OG_OK && kind == OG_KIND_PULL) {
+            og_graph *p = nullptr;
+            st = og_graph_to_pull(g, &p);
+            og_graph_free(g);
+            g = p;
+        }
+        if (st != OG_OK) return report_error(st, "convert");
+        *out = g;
     } else {
         st = og_graph_load(path.c_str(), kind, out);
         if (st != OG_OK) return report_error(st, "load");
@@ -114,6 +137,15 @@ og_status dump(const Options &opt, const void *data, size_t bytes) {
     return OG_OK;
 }
 
+// --out and --check of a result array.
+og_status finish(const Options &opt, const void *data, uint64_t n, int bytes, bool is_float) {
+    og_status st = dump(opt, data, (size_t)n * bytes);
+    if (st != OG_OK || opt.check_path.empty()) return st;
+    st = check_result(opt.check_path, data, n, bytes, is_float, opt.check_tol);
+    if (st == OG_OK) printf("check: %" PRIu64 " values match %s\n", n, opt.check_path.c_str());
+    return st;
+}
+
 using RunFn = og_status (*)(og_graph *, uint32_t, uint32_t *, og_run_stats *);
 
 og_status run_sources(const char *algo, og_graph *g, uint32_t source, const Options &opt, RunFn fn,
@@ -140,13 +172,63 @@ og_status run_sources(const char *algo, og_graph *g, uint32_t source, const Opti
             print_report(algo, st, plan, opt.json);
         }
     }
-    return dump(opt, out.data(), out.size() * 4);
+    return finish(opt, out.data(), out.size(), 4, false);
 }
 
 og_status bfs_fn(og_graph *g, uint32_t s, uint32_t *o, og_run_stats *st) { return og_bfs(g, s, o, st); }
 og_status sssp_fn(og_graph *g, uint32_t s, uint32_t *o, og_run_stats *st) { return og_sssp(g, s, o, st); }
 
 }  // namespace
+
+og_status check_result(const std::string &path, const void *got, uint64_t n, int bytes_per_value, bool is_float,
+                       double tol) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) {
+        fprintf(stderr, "onegraph: cannot read %s\n", path.c_str());
+        return OG_EIO;
+    }
+    std::vector<unsigned char> want((size_t)n * bytes_per_value);
+    const size_t rd = want.empty() ? 0 : fread(want.data(), 1, want.size(), f);
+    const bool longer = fgetc(f) != EOF;
+    fclose(f);
+    if (rd != want.size() || longer) {
+        fprintf(stderr, "check: %s holds %s than %" PRIu64 " values of %d bytes\n", path.c_str(),
+                longer ? "more" : "fewer", n, bytes_per_value);
+        return OG_EFORMAT;
+    }
+    uint64_t bad = 0, first = 0;
+    double worst = 0;
+    for (uint64_t v = 0; v < n; ++v) {
+        bool diff;
+        if (is_float) {
+            double a, b;
+            if (bytes_per_value == 8) {
+                a = ((const double *)got)[v];
+                b = ((const double *)want.data())[v];
+            } else {
+                a = ((const float *)got)[v];
+                b = ((const float *)want.data())[v];
+            }
+            const double e = a > b ? a - b : b - a;
+            worst = e > worst ? e : worst;
+            diff = !(e <= tol);
+        } else {
+            diff = ((const uint32_t *)got)[v] != ((const uint32_t *)want.data())[v];
+        }
+        if (diff && bad++ == 0) first = v;
+    }
+    if (!bad) {
+        if (is_float) printf("check: max-abs %.3e <= %.1e\n", worst, tol);
+        return OG_OK;
+    }
+    if (is_float)
+        fprintf(stderr, "check: %" PRIu64 " of %" PRIu64 " values differ by more than %.1e (max-abs %.3e), first at vertex %" PRIu64 "\n",
+                bad, n, tol, worst, first);
+    else
+        fprintf(stderr, "check: %" PRIu64 " of %" PRIu64 " values differ, first at vertex %" PRIu64 ": got %u, file %u\n",
+                bad, n, first, ((const uint32_t *)got)[first], ((const uint32_t *)want.data())[first]);
+    return OG_EFORMAT;
+}
 
 void print_report(const char *algo, const og_run_stats &st, const og_plan_info &plan, bool json, FILE *f) {
     fprintf(f, "totalProcess time is %.3f ms\n", st.total_ms);
@@ -211,7 +293,7 @@ og_status cc_opt(const std::string &path, float adviseRate, const Options &opt) 
         if (st != OG_OK) return report_error(st, "cc");
         print_report("cc", rs, plan, opt.json);
     }
-    return dump(opt, labels.data(), labels.size() * 4);
+    return finish(opt, labels.data(), labels.size(), 4, false);
 }
 
 og_status pr_opt(const std::string &path, double adviseRate, const Options &opt) {
@@ -233,7 +315,7 @@ og_status pr_opt(const std::string &path, double adviseRate, const Options &opt)
         print_report("pr", ps.run, plan, opt.json);
         printf("pagerank iterations %u final L1 %.3e (%s)\n", ps.iterations, ps.final_l1, opt.pr_fp64 ? "fp64" : "fp32");
     }
-    return opt.pr_fp64 ? dump(opt, r64.data(), r64.size() * 8) : dump(opt, r32.data(), r32.size() * 4);
+    return opt.pr_fp64 ? finish(opt, r64.data(), r64.size(), 8, true) : finish(opt, r32.data(), r32.size(), 4, true);
 }
 
 }  // namespace onegraph
@@ -244,11 +326,47 @@ static void usage(FILE *f) {
             "                [--approach ascetic|subway|usm] [--edge-budget BYTES | --budget-fraction F]\n"
             "                [--bignode-runs N] [--repeat N] [--json] [--out FILE]\n"
             "                [--fp64] [--pr-tol T] [--pr-iters N]\n"
-            "                [--generate rmat:SCALE:EF:SEED[:WSEED] | er:N:AVG:SEED] [--save FILE]\n");
+            "                [--generate rmat:SCALE:EF:SEED[:WSEED] | er:N:AVG:SEED] [--save FILE]\n"
+            "                [--text [--text-weights] [--symmetrize] [--keep-self-loops] [--one-based]\n"
+            "                 [--weight-seed S]] [--check FILE [--check-tol T]]\n"
+            "  --text: -f is a text edge list (\"u v [w]\" lines, #/%% comments) or a\n"
+            "  MatrixMarket file; with --save FILE it is converted to the reference CSR\n"
+            "  format of --type (bfs/cc plain, sssp weighted, pr pull) and written.\n"
+            "  --check FILE compares the result array with FILE (the --out layout);\n"
+            "  a mismatch exits with status 3.\n"
+            "       onegraph --compare A B [--dtype u32|f32|f64] [--check-tol T]\n"
+            "  compares two result files without a GPU (status 3 on a mismatch).\n");
+}
+
+// --compare A B: two result files of the --out layout, on the host.
+static int compare_files(const std::string &a, const std::string &b, const std::string &dtype, double tol) {
+    const int bytes = dtype == "f64" ? 8 : 4;
+    const bool fl = dtype != "u32";
+    if (dtype != "u32" && dtype != "f32" && dtype != "f64") {
+        fprintf(stderr, "onegraph: --dtype must be u32, f32 or f64\n");
+        return 2;
+    }
+    FILE *f = fopen(a.c_str(), "rb");
+    if (!f) {
+        fprintf(stderr, "onegraph: cannot read %s\n", a.c_str());
+        return 1;
+    }
+    std::vector<unsigned char> buf;
+    unsigned char chunk[1 << 16];
+    size_t n;
+    while ((n = fread(chunk, 1, sizeof chunk, f)) > 0) buf.insert(buf.end(), chunk, chunk + n);
+    fclose(f);
+    if (buf.size() % bytes) {
+        fprintf(stderr, "onegraph: %s is not a whole number of %d-byte values\n", a.c_str(), bytes);
+        return 1;
+    }
+    const og_status st = onegraph::check_result(b, buf.data(), buf.size() / bytes, bytes, fl, tol);
+    if (st == OG_OK) printf("check: %zu values match\n", buf.size() / bytes);
+    return st == OG_OK ? 0 : st == OG_EFORMAT ? 3 : 1;
 }
 
 int main(int argc, char **argv) {
-    std::string algo, input, save;
+    std::string algo, input, save, cmp_a, cmp_b, dtype = "u32";
     long long source = 0;
     double advisek = 0.0;
     onegraph::Options opt;
@@ -306,12 +424,34 @@ int main(int argc, char **argv) {
             opt.generate = need("--generate");
         } else if (a == "--save") {
             save = need("--save");
+        } else if (a == "--text" || a == "--convert") {
+            opt.text_input = true;
+        } else if (a == "--text-weights") {
+            opt.edge_flags |= OG_EDGES_WEIGHTED;
+        } else if (a == "--symmetrize") {
+            opt.edge_flags |= OG_EDGES_SYMMETRIZE;
+        } else if (a == "--keep-self-loops") {
+            opt.edge_flags |= OG_EDGES_KEEP_SELF_LOOPS;
+        } else if (a == "--one-based") {
+            opt.edge_flags |= OG_EDGES_ONE_BASED;
+        } else if (a == "--weight-seed") {
+            opt.weight_seed = strtoull(need("--weight-seed"), nullptr, 10);
+        } else if (a == "--check") {
+            opt.check_path = need("--check");
+        } else if (a == "--check-tol") {
+            opt.check_tol = atof(need("--check-tol"));
+        } else if (a == "--compare") {
+            cmp_a = need("--compare");
+            cmp_b = need("--compare");
+        } else if (a == "--dtype") {
+            dtype = need("--dtype");
         } else {
             fprintf(stderr, "onegraph: unknown argument %s\n", a.c_str());
             usage(stderr);
             return 2;
         }
     }
+    if (!cmp_a.empty()) return compare_files(cmp_a, cmp_b, dtype, opt.check_tol);
     if (algo.empty()) algo = "bfs";  // reference default (BFS/oneGraph_BFS/main.cpp:5-7)
     if (input.empty() && opt.generate.empty()) {
         fprintf(stderr, "onegraph: -f FILE (or --generate SPEC) is required\n");
@@ -347,5 +487,6 @@ int main(int argc, char **argv) {
         fprintf(stderr, "onegraph: unknown --type %s\n", algo.c_str());
         return 2;
     }
+    if (st == OG_EFORMAT && !opt.check_path.empty()) return 3;  // result check failed
     return st == OG_OK ? 0 : 1;
 }

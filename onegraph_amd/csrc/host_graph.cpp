@@ -4,6 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cctype>
+#include <cstring>
+#include <string>
+#include <vector>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -606,6 +611,266 @@ extern "C" og_status og_graph_generate_er(uint64_t n, uint32_t avg_degree, uint6
         std::unique_ptr<og_graph> g(new og_graph);
         build_csr(g.get(), n, M, true, gen);
         *out = g.release();
+        return OG_OK;
+    });
+}
+
+// ===================================================== edge-list conversion
+
+namespace {
+
+// Weighted counterpart of build_csr: arcs carry (v << 32 | w) so that a row
+// sort orders by target, then weight, and a unique-by-target keeps the
+// smallest weight of repeated arcs.
+template <class Gen>
+void build_wcsr(og_graph *g, uint64_t V, uint64_t M, bool symmetric, Gen gen) {
+    const unsigned T = default_threads();
+    std::vector<uint32_t> cnt(V + 1, 0);
+    uint32_t *c = cnt.data();
+    const uint64_t grain = 1 << 20;
+    parallel_for(M, grain, T, [&](uint64_t b, uint64_t e, unsigned) {
+        for (uint64_t i = b; i < e; ++i) {
+            uint32_t u, v, w;
+            if (!gen(i, u, v, w)) continue;
+            __atomic_fetch_add(&c[u], 1u, __ATOMIC_RELAXED);
+            if (symmetric && u != v) __atomic_fetch_add(&c[v], 1u, __ATOMIC_RELAXED);
+        }
+    });
+    std::vector<uint64_t> off(V + 1);
+    uint64_t s = 0;
+    for (uint64_t v = 0; v < V; ++v) {
+        off[v] = s;
+        s += cnt[v];
+    }
+    off[V] = s;
+    std::vector<uint64_t> cur(off.begin(), off.end() - 1);
+    uint64_t *cu = cur.data();
+    std::vector<uint64_t> tmp(s);
+    uint64_t *t = tmp.data();
+    parallel_for(M, grain, T, [&](uint64_t b, uint64_t e, unsigned) {
+        for (uint64_t i = b; i < e; ++i) {
+            uint32_t u, v, w;
+            if (!gen(i, u, v, w)) continue;
+            t[__atomic_fetch_add(&cu[u], 1ull, __ATOMIC_RELAXED)] = ((uint64_t)v << 32) | w;
+            if (symmetric && u != v) t[__atomic_fetch_add(&cu[v], 1ull, __ATOMIC_RELAXED)] = ((uint64_t)u << 32) | w;
+        }
+    });
+    cur.clear();
+    cur.shrink_to_fit();
+    parallel_for(V, 4096, T, [&](uint64_t b, uint64_t e, unsigned) {
+        for (uint64_t v = b; v < e; ++v) {
+            uint64_t *r0 = t + off[v], *r1 = t + off[v + 1];
+            std::sort(r0, r1);
+            c[v] = (uint32_t)(std::unique(r0, r1, [](uint64_t x, uint64_t y) { return (x >> 32) == (y >> 32); }) - r0);
+        }
+    });
+    uint64_t E = 0;
+    for (uint64_t v = 0; v < V; ++v) E += cnt[v];
+    graph_alloc(g, V, E, OG_KIND_WEIGHTED);
+    uint64_t acc = 0;
+    for (uint64_t v = 0; v < V; ++v) {
+        g->ptr[v] = acc;
+        acc += cnt[v];
+    }
+    g->ptr[V] = acc;
+    parallel_for(V, 16384, T, [&](uint64_t b, uint64_t e, unsigned) {
+        for (uint64_t v = b; v < e; ++v)
+            for (uint32_t k = 0; k < cnt[v]; ++k) {
+                const uint64_t x = t[off[v] + k];
+                g->wedges[2 * (g->ptr[v] + k)] = (uint32_t)(x >> 32);
+                g->wedges[2 * (g->ptr[v] + k) + 1] = (uint32_t)x;
+            }
+    });
+    graph_finish(g);
+}
+
+og_graph *edges_to_graph(uint64_t V, uint64_t M, const uint32_t *src, const uint32_t *dst, const uint32_t *wt,
+                         uint32_t flags) {
+    if (V == 0) {
+        std::atomic<uint64_t> mx{0};
+        parallel_for(M, 1 << 20, 0, [&](uint64_t b, uint64_t e, unsigned) {
+            uint64_t m = 0;
+            for (uint64_t i = b; i < e; ++i) m = std::max<uint64_t>(m, std::max(src[i], dst[i]));
+            uint64_t c = mx.load();
+            while (m > c && !mx.compare_exchange_weak(c, m)) {
+            }
+        });
+        if (!M) raise(OG_EINVAL, "no arcs and no vertex count");
+        V = mx.load() + 1;
+    } else {
+        std::atomic<uint64_t> bad{~0ull};
+        parallel_for(M, 1 << 20, 0, [&](uint64_t b, uint64_t e, unsigned) {
+            for (uint64_t i = b; i < e; ++i)
+                if (src[i] >= V || dst[i] >= V) {
+                    uint64_t c = bad.load();
+                    while (i < c && !bad.compare_exchange_weak(c, i)) {
+                    }
+                    break;
+                }
+        });
+        if (bad.load() != ~0ull)
+            raise(OG_EINVAL, "arc %llu (%u -> %u) names a vertex >= %llu", (unsigned long long)bad.load(),
+                  src[bad.load()], dst[bad.load()], (unsigned long long)V);
+    }
+    if (V > 0xFFFFFFFFull) raise(OG_EINVAL, "more than 2^32 - 1 vertices");
+    const bool sym = flags & OG_EDGES_SYMMETRIZE, loops = flags & OG_EDGES_KEEP_SELF_LOOPS;
+    std::unique_ptr<og_graph> g(new og_graph);
+    if (wt) {
+        build_wcsr(g.get(), V, M, sym, [&](uint64_t i, uint32_t &u, uint32_t &v, uint32_t &w) {
+            u = src[i];
+            v = dst[i];
+            w = wt[i];
+            return loops || u != v;
+        });
+    } else {
+        build_csr(g.get(), V, M, sym, [&](uint64_t i, uint32_t &u, uint32_t &v) {
+            u = src[i];
+            v = dst[i];
+            return loops || u != v;
+        });
+    }
+    return g.release();
+}
+
+// Parses an unsigned decimal at p (after optional blanks); false if none.
+inline bool parse_u64(const char *&p, const char *end, uint64_t &x) {
+    while (p < end && (*p == ' ' || *p == '\t' || *p == ',')) ++p;
+    if (p >= end || *p < '0' || *p > '9') return false;
+    uint64_t v = 0;
+    while (p < end && *p >= '0' && *p <= '9') {
+        if (v > (~0ull - 9) / 10) return false;
+        v = v * 10 + (uint64_t)(*p - '0');
+        ++p;
+    }
+    if (p < end && (*p == '.' || *p == 'e' || *p == 'E')) return false;  // not an integer
+    x = v;
+    return true;
+}
+
+}  // namespace
+
+extern "C" og_status og_graph_from_edges(uint64_t num_vertices, uint64_t num_edges, const uint32_t *src,
+                                         const uint32_t *dst, const uint32_t *weights, uint32_t flags,
+                                         og_graph **out) {
+    clear_error();
+    if (!out) return fail(OG_EINVAL, "out is NULL");
+    if (num_edges && (!src || !dst)) return fail(OG_EINVAL, "src/dst is NULL");
+    *out = nullptr;
+    return guard([&]() -> og_status {
+        *out = edges_to_graph(num_vertices, num_edges, src, dst, weights, flags);
+        return OG_OK;
+    });
+}
+
+extern "C" og_status og_graph_load_edgelist(const char *path, uint32_t flags, og_graph **out) {
+    clear_error();
+    if (!path || !out) return fail(OG_EINVAL, "path/out is NULL");
+    *out = nullptr;
+    return guard([&]() -> og_status {
+        FILE *f = fopen(path, "rb");
+        if (!f) raise(OG_EIO, "cannot open %s", path);
+        std::unique_ptr<FILE, int (*)(FILE *)> fc(f, fclose);
+        if (fseeko(f, 0, SEEK_END) != 0) raise(OG_EIO, "cannot seek %s", path);
+        const uint64_t size = (uint64_t)ftello(f);
+        fseeko(f, 0, SEEK_SET);
+        std::vector<char> buf(size + 1);
+        if (size && fread(buf.data(), 1, size, f) != size) raise(OG_EIO, "short read on %s", path);
+        buf[size] = '\n';
+        const char *base = buf.data(), *end = base + size;
+        auto line_of = [&](const char *at) { return 1 + (uint64_t)std::count(base, at, '\n'); };
+        auto next_line = [&](const char *p) {
+            const char *q = (const char *)memchr(p, '\n', (size_t)(end - p));
+            return q ? q + 1 : end;
+        };
+        // MatrixMarket: banner, comments, then "rows cols entries".
+        uint64_t V = 0;
+        const char *data = base;
+        bool weighted = flags & OG_EDGES_WEIGHTED;
+        if (size >= 14 && !memcmp(base, "%%MatrixMarket", 14)) {
+            const char *eol = next_line(base);
+            std::string banner(base, eol);
+            for (auto &ch : banner) ch = (char)tolower((unsigned char)ch);
+            if (banner.find("coordinate") == std::string::npos)
+                raise(OG_EFORMAT, "%s: only MatrixMarket coordinate files hold a graph", path);
+            if (banner.find("symmetric") != std::string::npos) flags |= OG_EDGES_SYMMETRIZE;
+            if (banner.find("pattern") != std::string::npos && weighted)
+                raise(OG_EFORMAT, "%s: a pattern MatrixMarket file has no weights", path);
+            flags |= OG_EDGES_ONE_BASED;
+            const char *p = eol;
+            while (p < end && (*p == '%' || *p == '\n' || *p == '\r')) p = next_line(p);
+            uint64_t rows = 0, cols = 0, nnz = 0;
+            if (!parse_u64(p, end, rows) || !parse_u64(p, end, cols) || !parse_u64(p, end, nnz))
+                raise(OG_EFORMAT, "%s:%llu: bad MatrixMarket size line", path, (unsigned long long)line_of(p));
+            V = std::max(rows, cols);
+            data = next_line(p);
+        }
+        const uint64_t one = (flags & OG_EDGES_ONE_BASED) ? 1 : 0;
+        // Line-aligned chunks, one per thread; each parses into its own arrays.
+        const unsigned T = default_threads();
+        std::vector<const char *> cut(T + 1);
+        cut[0] = data;
+        for (unsigned t = 1; t < T; ++t) {
+            const char *p = data + (uint64_t)(end - data) * t / T;
+            cut[t] = std::max(cut[t - 1], p > data ? next_line(p - 1) : data);
+        }
+        cut[T] = end;
+        struct Part {
+            std::vector<uint32_t> u, v, w;
+            const char *err = nullptr;
+            const char *why = "";
+        };
+        std::vector<Part> part(T);
+        parallel_for(T, 1, T, [&](uint64_t b, uint64_t e, unsigned) {
+            for (uint64_t t = b; t < e; ++t) {
+                Part &pt = part[t];
+                for (const char *p = cut[t]; p < cut[t + 1];) {
+                    const char *eol = next_line(p);
+                    const char *q = p;
+                    while (q < eol && (*q == ' ' || *q == '\t')) ++q;
+                    if (q >= eol || *q == '\n' || *q == '\r' || *q == '#' || *q == '%') {
+                        p = eol;
+                        continue;
+                    }
+                    uint64_t a = 0, c = 0, w = 1;
+                    if (!parse_u64(q, eol, a) || !parse_u64(q, eol, c)) {
+                        pt.err = p;
+                        pt.why = "expected two vertex ids";
+                        break;
+                    }
+                    if (weighted && !parse_u64(q, eol, w)) {
+                        pt.err = p;
+                        pt.why = "expected an integer weight";
+                        break;
+                    }
+                    if (a < one || c < one || a - one >= 0xFFFFFFFFull || c - one >= 0xFFFFFFFFull ||
+                        w > 0xFFFFFFFFull) {
+                        pt.err = p;
+                        pt.why = "vertex id or weight out of range";
+                        break;
+                    }
+                    pt.u.push_back((uint32_t)(a - one));
+                    pt.v.push_back((uint32_t)(c - one));
+                    if (weighted) pt.w.push_back((uint32_t)w);
+                    p = eol;
+                }
+            }
+        });
+        for (auto &pt : part)
+            if (pt.err) raise(OG_EFORMAT, "%s:%llu: %s", path, (unsigned long long)line_of(pt.err), pt.why);
+        uint64_t M = 0;
+        for (auto &pt : part) M += pt.u.size();
+        std::vector<uint32_t> su(M), sv(M), sw(weighted ? M : 0);
+        uint64_t at = 0;
+        for (auto &pt : part) {
+            std::copy(pt.u.begin(), pt.u.end(), su.begin() + at);
+            std::copy(pt.v.begin(), pt.v.end(), sv.begin() + at);
+            if (weighted) std::copy(pt.w.begin(), pt.w.end(), sw.begin() + at);
+            at += pt.u.size();
+            std::vector<uint32_t>().swap(pt.u);
+            std::vector<uint32_t>().swap(pt.v);
+            std::vector<uint32_t>().swap(pt.w);
+        }
+        *out = edges_to_graph(V, M, su.data(), sv.data(), weighted ? sw.data() : nullptr, flags);
         return OG_OK;
     });
 }

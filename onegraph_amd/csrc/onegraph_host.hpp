@@ -31,7 +31,20 @@ struct Options {
     uint32_t pr_max_iterations = 100;
     std::string out_path;            // optional binary dump of the result array
     std::string generate;            // "rmat:SCALE:EF:SEED[:WSEED]" or "er:N:AVG:SEED" instead of -f
+    bool text_input = false;         // the input path is a text edge list / MatrixMarket file
+    uint32_t edge_flags = 0;         // OG_EDGES_* for text input
+    uint64_t weight_seed = 1;        // text input without weights, weighted algorithm: seeded [1, 63]
+    std::string check_path;          // result array to compare with (same layout as out_path)
+    double check_tol = 1e-6;         // PageRank: max-abs tolerance of the check
 };
+
+// Result-array file (--out / --check): V little-endian values, u32 levels /
+// distances / labels (UINT32_MAX unreached), f32 or f64 ranks.  Compares
+// `got` with the file: exact for integers, max-abs <= tol for ranks.
+// OG_OK on a match; OG_EFORMAT (with a message on stderr naming the first
+// differing vertex) on a mismatch; OG_EIO when the file is unreadable.
+og_status check_result(const std::string &path, const void *got, uint64_t n, int bytes_per_value, bool is_float,
+                       double tol);
 
 // Prints the reference's labelled lines (TimeRecord "<name> time is X ms",
 // nodeSum, staticnodeSum, "move overload size") and optionally one JSON line.

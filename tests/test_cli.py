@@ -79,3 +79,30 @@ def test_cli_bignode_and_pagerank(gpu, tmp_path):
     pptr, pin, _, od = p.export()
     ref, _, _ = oracle.pagerank(od, pptr, pin)
     assert np.abs(np.fromfile(out, np.float64) - ref).max() < 1e-9
+
+
+@pytest.mark.gpu
+def test_cli_text_input_and_check(gpu, tmp_path):
+    """A text edge list runs directly (--text) and --check compares the result
+    with a dump: the oracle's levels pass, a planted difference exits 3."""
+    rng = np.random.default_rng(17)
+    V, M = 3000, 40000
+    src, dst = rng.integers(0, V, M), rng.integers(0, V, M)
+    t = tmp_path / "e.txt"
+    np.savetxt(t, np.stack([src, dst], 1), fmt="%d")
+    g = og.Graph.from_edges(src, dst, num_vertices=V)
+    ptr, to, _, _ = g.export()
+    s = int(src[0])
+    ref = oracle.bfs(ptr, to, s)[0]
+    good, bad = tmp_path / "good.u32", tmp_path / "bad.u32"
+    ref.tofile(good)
+    r = run("--type", "bfs", "-f", str(t), "--text", "-r", str(s), "--budget-fraction", "0.3",
+            "--advisek", "0.8", "--check", str(good))
+    assert r.returncode == 0, r.stderr
+    assert "values match" in r.stdout
+    wrong = ref.copy()
+    wrong[int(np.flatnonzero(ref != og.UNREACHED)[-1])] += 1
+    wrong.tofile(bad)
+    r = run("--type", "bfs", "-f", str(t), "--text", "-r", str(s), "--check", str(bad))
+    assert r.returncode == 3 and "values differ" in r.stderr
+    g.close()
