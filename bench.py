@@ -135,7 +135,8 @@ def wl_bfs_hbm(args):
     V, E = len(ptr), len(cols)
     deg = np.diff(np.append(ptr.astype(np.int64), E))
     src = int(np.flatnonzero(deg > 0)[0])
-    plan = g.prepare("ascetic", edge_budget_bytes=E * 4, advisek=1.0)
+    flags = og.PLAN_DIRECTION_OPT if args.diropt else 0
+    plan = g.prepare("ascetic", edge_budget_bytes=E * 4, advisek=1.0, flags=flags)
     stats, wall = timed(lambda: g.bfs(src, want_values=False)[1], args.steps, args.warmup, "bfs-hbm")
     lv, _ = g.bfs(src)
     ms = 1e3 * wall / args.steps
@@ -146,7 +147,8 @@ def wl_bfs_hbm(args):
     roof, link, kt = roofline_of(stats)
     out = base_record(round(trav / (ms * 1e-3) / 1e9, 4), "GTEPS", args.steps, args.warmup, ms, "u32",
                       {"workload": f"bfs whole graph in HBM rmat{args.scale} ef{args.edge_factor}", "V": V, "E": E,
-                       "source": src, "advisek": 1.0, "approach": "ascetic", "static_edges": plan["static_edges"]})
+                       "source": src, "advisek": 1.0, "approach": "ascetic", "static_edges": plan["static_edges"],
+                       "direction": "top-down/bottom-up" if args.diropt else "top-down"})
     out.update({"roofline": roof, "kernels": kt, "iterations": stats[-1]["iterations"],
                 "parity": {"bit_exact": bool(np.array_equal(lv, ref)), "reached": int((ref != og.UNREACHED).sum())},
                 "cpu_baseline": {"value": round(trav / cpu_s / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
@@ -457,6 +459,7 @@ def main():
     ap.add_argument("--loader", default="zerocopy", choices=["zerocopy", "copy"],
                     help="dynamic region: relax reads pinned host memory in place, or host gather + copy stream")
     ap.add_argument("--no-pack", action="store_true", help="8-byte weighted edges in HBM and on the link")
+    ap.add_argument("--diropt", action="store_true", help="bfs-hbm: direction-optimising BFS (in-edge CSR in HBM)")
     ap.add_argument("--no-overlap", action="store_true", help="serialise copies, static and dynamic relax")
     ap.add_argument("--sweep-delta", default="", help="comma list: time SSSP at each delta and report")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,

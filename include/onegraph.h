@@ -178,6 +178,12 @@ typedef struct og_plan_params {
 #define OG_PLAN_NO_GRAPH 8u     /* enqueue device-driven rounds one API call at
                                    a time instead of replaying a captured
                                    hipGraph of them (for A/B timing)             */
+#define OG_PLAN_DIRECTION_OPT 16u /* BFS on a graph wholly in HBM: rounds with a
+                                   large frontier run bottom-up (every
+                                   unvisited vertex scans its in-edges for a
+                                   parent, stopping at the first); needs the
+                                   in-edge CSR on the device (E * 4 + V * 8
+                                   bytes beyond the edge budget)               */
 
 typedef struct og_plan_info {
     uint64_t total_budget_edges;  /* reference total_gpu_size               */

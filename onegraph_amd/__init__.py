@@ -21,6 +21,7 @@ PLAN_NO_OVERLAP = 1
 PLAN_ZERO_COPY = 2
 PLAN_NO_PACK = 4
 PLAN_NO_GRAPH = 8
+PLAN_DIRECTION_OPT = 16
 EDGES_SYMMETRIZE, EDGES_KEEP_SELF_LOOPS, EDGES_ONE_BASED, EDGES_WEIGHTED = 1, 2, 4, 8
 
 STATUS = {

@@ -64,6 +64,8 @@ struct FrontierInfo {
     uint32_t min_far;       // smallest value among them
     uint32_t live;          // 1 while active or held-back vertices remain
     uint32_t dyn_held;      // 1: the dynamic list was held back this round (n[3] = 0)
+    uint32_t pull;          // 1: BFS round runs bottom-up (the lists are not built)
+    uint32_t pad;
 };
 
 // Device-resident driver state of one traversal.  The scan of each
@@ -78,7 +80,11 @@ struct RoundCtl {
     uint32_t compactions;        // all compactions
     uint32_t live;               // last compaction's FrontierInfo::live
     uint32_t overflow;           // multi-source: a level past 255 was needed
-    uint32_t pad;
+    uint32_t dir_opt;            // BFS: choose top-down / bottom-up per round
+    uint32_t pull;               // direction of the current round (1: bottom-up)
+    uint32_t pull_rounds;        // rounds run bottom-up
+    uint64_t unexplored;         // BFS: edges of vertices not yet in any frontier
+    uint64_t nverts;             // V (the bottom-up -> top-down switch compares the frontier with V / 24)
     uint64_t node_sum;           // active vertices, summed over rounds
     uint64_t static_node_sum;    // ... of them in the static bins
     uint64_t dyn_node_sum;       // ... of them in the dynamic list
@@ -103,6 +109,7 @@ struct Frontier {
     uint32_t *bcnt = nullptr;        // kBcntRows * nblk per-block counts
     uint64_t *bdeg = nullptr;        // 2 * nblk per-block degree sums
     uint32_t nblk = 0;
+    uint64_t V = 0;
     FrontierInfo *info = nullptr;
     FrontierInfo *h_info = nullptr;  // pinned
     RoundCtl *ctl = nullptr;
@@ -232,6 +239,9 @@ struct DeviceState {
     uint32_t *d_msrc = nullptr;      // 64 sources of the batch
     unsigned long long *d_msum = nullptr;  // 2 x 64: checksum, reached per source
     uint32_t *d_extract = nullptr;   // V scratch for one source's u32 levels
+    // direction-optimising BFS: in-edge CSR (OG_PLAN_DIRECTION_OPT)
+    uint64_t *d_in_ptr = nullptr;    // V + 1
+    uint32_t *d_in_cols = nullptr;   // E in-neighbours
 
     // PageRank state (allocated on first use)
     void *d_rank[2] = {nullptr, nullptr};
@@ -388,8 +398,9 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m);
 // Resets f.ctl for a new traversal on stream s (a kernel, so it may sit in a
 // captured graph): statistics zeroed, near-far width `delta` with the first
 // threshold 1 + delta (the source's value is 1), or none for delta = 0;
-// `defer` holds the dynamic list back in rounds with static work.
-void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer = 0);
+// `defer` holds the dynamic list back in rounds with static work;
+// dir_opt_edges > 0 turns on the BFS direction choice over that many edges.
+void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer = 0, uint64_t dir_opt_edges = 0);
 
 // One compaction: flags -> bin lists / hub tiles / sorted dynamic list with
 // its degree prefix, flags cleared; f.info and f.ctl updated on the device.

@@ -4,11 +4,13 @@
 // memory through a double-buffered staging area on a copy stream.
 //
 // One iteration (reference loop: BFS/oneGraph_BFS/CalculateOpt.h:75-188):
-//   1. k_frontier<count> / k_frontier_scan / k_frontier<write>: one pass over
-//      the next-frontier flags builds the static bin lists (ordered), the hub
-//      tile list, the sorted dynamic list and its degree prefix, and clears
-//      the flags.  Replaces K1-K6 (gpu_kernels.h:243-379), the host prefix H4
-//      (CalculateOpt.h:110-123) and every per-iteration .wait() but one.
+//   1. k_frontier_count / k_frontier_scan (one workgroup) / k_frontier_write:
+//      two passes over the next-frontier flags build the
+//      static bin lists (ordered), the hub tile list, the sorted dynamic list
+//      and its degree prefix, and clear the flags.  Replaces K1-K6
+//      (gpu_kernels.h:243-379), the host prefix H4 (CalculateOpt.h:110-123)
+//      and every per-iteration .wait(): counts stay on the device, and rounds
+//      without a host-side loader are replayed from captured hipGraphs.
 //   2. static relax: bin 0 (lpv0 lanes/vertex), bin 1 (wave/vertex), bin 2
 //      (workgroup per `tile` edges, neighbour list staged in LDS).  Replaces
 //      bfs_kernelStatic (gpu_kernels.h:211-241), sssp_kernel
@@ -19,6 +21,8 @@
 //      the slot on the copy stream while the other slot is being gathered, and
 //      an edge-parallel relax consumes it (K8/K10/K12).
 #include <algorithm>
+#include <atomic>
+#include <vector>
 #include <cstdio>
 
 #include "device.h"
@@ -106,6 +110,10 @@ __device__ __forceinline__ uint32_t block_min_u32(uint32_t x, uint64_t *lds) {
 
 constexpr int kLoadBatch = 8;  // flagged vertices whose row bounds are read together
 
+__device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg, uint32_t nblk,
+                            FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
+                            uint64_t *lds);
+
 // Compaction pass 1: classifies every flagged vertex of the block's window,
 // stores kCode0 + bin in its flag byte and publishes the per-block counts.
 // Row bounds (and values, for near-far) of kLoadBatch vertices are read
@@ -179,16 +187,27 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
     }
 }
 
-// Compaction pass 2: one workgroup turns the per-block counts into exclusive
-// offsets (in place), publishes the round's summary and advances the
-// traversal state: statistics, liveness and, when the near set is empty but
-// vertices are held back, the near-far threshold (min held-back value +
-// delta) for the next compaction.  Each thread scans a run of consecutive
-// blocks, so the whole pass is one multi-value block scan.
+// Compaction pass 2: one workgroup scans the counts (scan_counts).  A
+// separate launch rather than a last-workgroup-arrives tail in pass 1: on
+// gfx950 the device-scope release/acquire fences such a tail needs write
+// back and invalidate the whole L2 of each XCD, once per workgroup.
 __global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
                                                         uint32_t nblk, FrontierInfo *__restrict__ info,
                                                         uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl) {
     __shared__ uint64_t lds[7 * 1024 / kWave];
+    scan_counts(bcnt, bdeg, nblk, info, doff, ctl, lds);
+}
+
+// Compaction pass 2 body: one workgroup turns the per-block counts into
+// exclusive offsets (in place), publishes
+// the round's summary and advances the traversal state: statistics,
+// liveness, the BFS direction and, when the near set is empty but vertices
+// are held back, the near-far threshold (min held-back value + delta) for
+// the next compaction.  Each thread scans a run of consecutive blocks, so the
+// whole pass is one multi-value block scan.  `lds` holds 7 * blockDim / 64.
+__device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg, uint32_t nblk,
+                            FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
+                            uint64_t *lds) {
     const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
     const uint32_t b0 = min(nblk, threadIdx.x * per), b1 = min(nblk, b0 + per);
     uint64_t s01 = 0, s23 = 0, sdd = 0, zs = 0, hs = 0, fs = 0, ss = 0;
@@ -246,6 +265,19 @@ __global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *__restrict__ b
         }
         const uint64_t active = snodes + fi.n[3] + fi.nzero;
         fi.live = (active || fi.nfar || held) ? 1u : 0u;
+        // BFS direction (Beamer et al., SC'12): bottom-up once the frontier's
+        // edges exceed 1/14 of the unexplored ones, back to top-down when
+        // the frontier falls below V / 24.
+        fi.pull = 0;
+        fi.pad = 0;
+        if (c.dir_opt && active) {
+            const uint64_t mf = fi.static_deg + fi.dyn_deg;
+            if (!c.pull && mf > c.unexplored / 14) c.pull = 1;
+            else if (c.pull && active < c.nverts / 24) c.pull = 0;
+            c.unexplored -= min(mf, c.unexplored);
+            fi.pull = c.pull;
+            c.pull_rounds += c.pull;
+        }
         *info = fi;
         doff[fi.n[3]] = fi.dyn_deg;
         c.compactions++;
@@ -315,6 +347,11 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
     uint64_t ex[3], t[3];
     block_scan_n<3>(x, ex, t, lds);
     if (!any) return;
+    if (info->pull) {  // bottom-up round: no lists, the frontier is its level
+        const uint32_t zero[4 * kFlagVecs] = {};
+        store_flags(flag, base, Vpad, zero);
+        return;
+    }
     const uint32_t n0 = info->n[0], n1 = info->n[1];
     uint64_t p0 = (uint64_t)bcnt[0 * nblk + blockIdx.x] + (uint32_t)ex[0];
     uint64_t p1 = (uint64_t)n0 + bcnt[1 * nblk + blockIdx.x] + (uint32_t)(ex[0] >> 32);
@@ -355,10 +392,13 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
     store_flags(flag, base, Vpad, word);
 }
 
-__global__ void k_ctl_reset(RoundCtl *ctl, uint32_t delta, uint32_t defer) {
+__global__ void k_ctl_reset(RoundCtl *ctl, uint32_t delta, uint32_t defer, uint64_t dir_opt_edges, uint64_t V) {
     RoundCtl c{};
     c.delta = delta;
     c.defer = defer;
+    c.dir_opt = dir_opt_edges ? 1u : 0u;
+    c.unexplored = dir_opt_edges;
+    c.nverts = V;
     c.thresh = delta ? (uint32_t)min((uint64_t)0xFFFFFFFFull, 1ull + delta) : 0xFFFFFFFFu;
     c.live = 1;
     *ctl = c;
@@ -463,6 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restri
                                                         const uint64_t *__restrict__ ptr,
                                                         const void *__restrict__ edges, RelaxArgs a,
                                                         unsigned long long *upd) {
+    if (info->pull) return;
     const uint32_t n = info->n[bin];
     const uint32_t *lst = list + (bin ? info->n[0] : 0u);
     const uint32_t sub = threadIdx.x % LPV;
@@ -503,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
                                                       unsigned long long *upd) {
     extern __shared__ uint32_t s_edges[];
     constexpr bool WIDE = W && !PK;  // 8-byte edges
-    const uint64_t ntiles = min((uint64_t)info->n[2], hub_cap);
+    const uint64_t ntiles = info->pull ? 0 : min((uint64_t)info->n[2], hub_cap);
     uint32_t c = 0;
     for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const uint64_t it = hub[ti];
@@ -621,6 +662,40 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
     }
     wave_add(upd, c);
     if (DIRECT) wave_add(hits, h);
+}
+
+// Bottom-up BFS round (direction-optimising BFS, Beamer et al., SC'12): the
+// frontier is the current level L = rounds so far; every unvisited vertex
+// scans its in-edges, kRelaxBatch parents' levels at a time, and takes level
+// L + 1 at the first parent on level L.  Each vertex is written by its own
+// lane only, and a fresh L + 1 never reads as L, so no atomics are needed.
+__global__ __launch_bounds__(kBlock) void k_bfs_pull(const uint64_t *__restrict__ in_ptr,
+                                                     const uint32_t *__restrict__ in_cols, uint64_t V,
+                                                     const FrontierInfo *__restrict__ info,
+                                                     const RoundCtl *__restrict__ ctl, uint32_t *val,
+                                                     uint8_t *__restrict__ flag, unsigned long long *upd) {
+    if (!info->pull) return;
+    const uint32_t L = ctl->rounds;
+    uint32_t c = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
+        if (val[v] != OG_UNREACHED) continue;
+        const uint64_t b = in_ptr[v], e = in_ptr[v + 1];
+        for (uint64_t i = b; i < e; i += kRelaxBatch) {
+            uint32_t lv[kRelaxBatch];
+#pragma unroll
+            for (int k = 0; k < kRelaxBatch; ++k) lv[k] = i + k < e ? val[in_cols[i + k]] : 0u;
+            bool hit = false;
+#pragma unroll
+            for (int k = 0; k < kRelaxBatch; ++k) hit |= lv[k] == L;
+            if (hit) {
+                val[v] = L + 1;
+                flag[v] = 1;
+                ++c;
+                break;
+            }
+        }
+    }
+    wave_add(upd, c);
 }
 
 // Order-independent digest of a result array: the sum (mod 2^64) of
@@ -769,6 +844,7 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) 
     f.list = (uint32_t *)m.dalloc(V * sizeof(uint32_t));
     f.doff = (uint64_t *)m.dalloc((V + 1) * sizeof(uint64_t));
     f.nblk = (uint32_t)((d->Vpad + kVertsPerBlock - 1) / kVertsPerBlock);
+    f.V = V;
     f.bcnt = (uint32_t *)m.dalloc((uint64_t)kBcntRows * f.nblk * sizeof(uint32_t));
     f.bdeg = (uint64_t *)m.dalloc(2ull * f.nblk * sizeof(uint64_t));
     f.info = (FrontierInfo *)m.dalloc(sizeof(FrontierInfo));
@@ -786,8 +862,8 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) 
     OG_HIP(hipMemcpy(f.ctl, &c, sizeof c, hipMemcpyHostToDevice));
 }
 
-void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer) {
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, f.ctl, delta, defer);
+void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer, uint64_t dir_opt_edges) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, f.ctl, delta, defer, dir_opt_edges, f.V);
     OG_HIP(hipGetLastError());
 }
 
@@ -996,6 +1072,38 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
         }
     }
 
+    // ---- in-edge CSR for direction-optimising BFS (counting sort by target)
+    if (p.flags & OG_PLAN_DIRECTION_OPT) {
+        if (approach != OG_ASCETIC || pl.vsplit < V)
+            raise(OG_EUNSUPPORTED, "direction-optimising BFS needs the whole graph in HBM (Ascetic, vsplit = V)");
+        if (g->kind == OG_KIND_PULL) raise(OG_EUNSUPPORTED, "pull-layout graphs serve PageRank only");
+        const bool wt = g->kind == OG_KIND_WEIGHTED;
+        auto col = [&](uint64_t i) { return wt ? g->wedges[2 * i] : g->cols[i]; };
+        std::vector<uint64_t> iptr(V + 1, 0);
+        std::vector<std::atomic<uint64_t>> cnt(V);
+        parallel_for(V, 1 << 16, 0, [&](uint64_t b, uint64_t e, unsigned) {
+            for (uint64_t v = b; v < e; ++v) cnt[v].store(0, std::memory_order_relaxed);
+        });
+        parallel_for(E, 1 << 20, 0, [&](uint64_t b, uint64_t e, unsigned) {
+            for (uint64_t i = b; i < e; ++i) cnt[col(i)].fetch_add(1, std::memory_order_relaxed);
+        });
+        for (uint64_t v = 0; v < V; ++v) iptr[v + 1] = iptr[v] + cnt[v].load(std::memory_order_relaxed);
+        parallel_for(V, 1 << 16, 0, [&](uint64_t b, uint64_t e, unsigned) {
+            for (uint64_t v = b; v < e; ++v) cnt[v].store(iptr[v], std::memory_order_relaxed);
+        });
+        std::vector<uint32_t> icol(std::max<uint64_t>(E, 1));
+        parallel_for(V, 1 << 12, 0, [&](uint64_t b, uint64_t e, unsigned) {
+            for (uint64_t u = b; u < e; ++u)
+                for (uint64_t i = g->ptr[u]; i < g->ptr[u + 1]; ++i)
+                    icol[cnt[col(i)].fetch_add(1, std::memory_order_relaxed)] = (uint32_t)u;
+        });
+        d->d_in_ptr = (uint64_t *)d->dalloc((V + 1) * sizeof(uint64_t));
+        d->d_in_cols = (uint32_t *)d->dalloc(std::max<uint64_t>(E, 1) * sizeof(uint32_t));
+        OG_HIP(hipMemcpy(d->d_in_ptr, iptr.data(), (V + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(d->d_in_cols, icol.data(), std::max<uint64_t>(E, 1) * sizeof(uint32_t),
+                         hipMemcpyHostToDevice));
+    }
+
     // ---- vertex state
     d->d_val = (uint32_t *)d->dalloc(V * sizeof(uint32_t));
     OG_HIP(hipStreamSynchronize(d->s_compute));
@@ -1181,12 +1289,13 @@ template void launch_static_relax<ALGO_MSBFS, true>(DeviceState *, Frontier &, c
 // compaction; returns true when the caller must join ev_static before the
 // next compaction.  All relax updates are monotone (atomicMin / same-level
 // stores / atomicOr), so the two regions may interleave.  NO_OVERLAP runs it
-// on the compute stream instead.  hfi: the round's summary when the host has
+// (and a plan without a dynamic region) runs it on the compute stream
+// instead.  hfi: the round's summary when the host has
 // it (sizes the grids, skips empty bins), else null.
 template <int ALGO, bool W, class Span>
 static bool launch_static(DeviceState *d, const FrontierInfo *hfi, const Span &span) {
     if (hfi && !(hfi->n[0] | hfi->n[1] | hfi->n[2])) return false;
-    if (d->flags & OG_PLAN_NO_OVERLAP) {
+    if ((d->flags & OG_PLAN_NO_OVERLAP) || d->plan.vsplit >= d->V) {  // nothing to overlap with
         span(kClsStat, d->s_compute,
              [&] { launch_static_relax<ALGO, W>(d, d->fr, hfi, relax_args(d), d->s_compute); });
         return false;
@@ -1299,7 +1408,8 @@ static void frontier_loop(og_graph *g, og_run_stats &st, RunTimers &tm, double &
     // deferring dynamic vertices reorders relaxations across levels, which
     // only the atomicMin relaxes (SSSP, CC) tolerate
     const uint32_t defer = (ALGO == ALGO_SSSP || ALGO == ALGO_CC) && has_dyn ? d->tune.defer : 0;
-    frontier_begin(d->fr, s, delta, defer);
+    const bool dir_opt = ALGO == ALGO_BFS && d->d_in_ptr != nullptr;
+    frontier_begin(d->fr, s, delta, defer, dir_opt ? std::max<uint64_t>(d->E, 1) : 0);
     if (!staged) {
         const EagerSpan eager{d, tm};
         eager(kClsFront, s, [&] { frontier_compact(d, d->fr, s, nf_val, false); });
@@ -1309,6 +1419,13 @@ static void frontier_loop(og_graph *g, og_run_stats &st, RunTimers &tm, double &
             for (int r = 0; r < kRoundBatch; ++r) {
                 const bool joined = launch_static<ALGO, W>(d, nullptr, span);
                 if (has_dyn) span(kClsDyn, s, [&] { launch_dyn_any<ALGO, W>(d, d->hd_dyn, 0, ~0ull, 0, true); });
+                if (dir_opt)
+                    span(kClsStat, s, [&] {
+                        hipLaunchKernelGGL(k_bfs_pull, dim3(capped((d->V + kBlock - 1) / kBlock, kGroupGridCap)),
+                                           dim3(kBlock), 0, s, d->d_in_ptr, d->d_in_cols, d->V, d->fr.info,
+                                           d->fr.ctl, d->d_val, d->fr.flag, d->fr.counters + 0);
+                        OG_HIP(hipGetLastError());
+                    });
                 if (joined) OG_HIP(hipStreamWaitEvent(s, d->ev_static, 0));
                 after();
                 span(kClsFront, s, [&] { frontier_compact(d, d->fr, s, nf_val, false); });
@@ -1317,7 +1434,7 @@ static void frontier_loop(og_graph *g, og_run_stats &st, RunTimers &tm, double &
         };
         if (!(d->flags & OG_PLAN_NO_GRAPH)) {
             const uint64_t key = (uint64_t)ALGO | (uint64_t)W << 4 | (uint64_t)(delta != 0) << 5 |
-                                 (uint64_t)has_dyn << 6 | (uint64_t)kRoundBatch << 8;
+                                 (uint64_t)has_dyn << 6 | (uint64_t)dir_opt << 7 | (uint64_t)kRoundBatch << 8;
             RoundGraph *rg = round_graph_for(d, key, [&](RoundGraph *n) { batch(GraphSpan{n}); });
             for (;;) {
                 OG_HIP(hipGraphLaunch(rg->exec, s));

@@ -40,7 +40,7 @@ def _prepare(g, approach, frac, k, edge_bytes):
     flags = 0
     for o in opts:
         flags |= {"zerocopy": og.PLAN_ZERO_COPY, "nopack": og.PLAN_NO_PACK, "nograph": og.PLAN_NO_GRAPH,
-                  "nooverlap": og.PLAN_NO_OVERLAP}[o]
+                  "nooverlap": og.PLAN_NO_OVERLAP, "diropt": og.PLAN_DIRECTION_OPT}[o]
     return g.prepare(name, edge_budget_bytes=budget, advisek=k, flags=flags)
 
 
@@ -231,3 +231,50 @@ def test_deferred_dynamic_order(gpu, rmat_w, approach, frac, k, delta):
     assert np.array_equal(rmat_w.cc()[0], oracle.cc(ptr, to)[0])
     lv, _ = rmat_w.bfs(3)  # BFS ignores defer (plain level stores)
     assert np.array_equal(lv, oracle.bfs(ptr, to, 3)[0])
+
+
+@pytest.mark.parametrize("graph", ["rmat13", "rmat16", "er12", "chain"])
+@pytest.mark.parametrize("graphmode", ["", "+nograph"])
+def test_direction_optimising_bfs(gpu, graph, graphmode):
+    """Bottom-up rounds give the same levels and the same per-round frontier
+    sizes (node_sum) as the level-synchronous oracle; other algorithms on the
+    same prepared graph are unaffected."""
+    if graph == "rmat13":
+        g = og.Graph.rmat(13, 16, seed=71, weight_seed=72)
+    elif graph == "rmat16":
+        g = og.Graph.rmat(16, 16, seed=73)
+    elif graph == "er12":
+        g = og.Graph.er(1 << 12, 16, seed=74)
+    else:  # long directed chain with a shortcut star: many rounds, tiny frontiers
+        n = 3000
+        src = np.concatenate([np.arange(n - 1), np.zeros(100, np.int64)])
+        dst = np.concatenate([np.arange(1, n), np.arange(1000, 1100)])
+        g = og.Graph.from_edges(src, dst, num_vertices=n + 5)
+    ptr, to, w, _ = g.export()
+    _say("direction-optimising bfs", graph, graphmode)
+    _prepare(g, "ascetic+diropt" + graphmode, 1.0, 1.0, 8 if w is not None else 4)
+    for s in _sources(ptr, len(to), 3, 15):
+        lv, st = g.bfs(s)
+        ref, ns, it = oracle.bfs(ptr, to, s)
+        assert np.array_equal(lv, ref), (graph, s)
+        assert st["node_sum"] == ns
+        assert st["reached"] == int((ref != og.UNREACHED).sum())
+    if w is not None:
+        s = _sources(ptr, len(to), 1, 16)[1]
+        assert np.array_equal(g.sssp(s)[0], oracle.sssp(ptr, to, w, s))
+    assert np.array_equal(g.cc()[0], oracle.cc(ptr, to)[0])
+    ms_src = np.array(_sources(ptr, len(to), 3, 17), np.uint32)
+    if graph == "chain":  # thousands of levels: past the u8 levels of the batched search
+        with pytest.raises(og.OgError) as e:
+            g.msbfs(ms_src)
+        assert e.value.name == "OG_EUNSUPPORTED" and "254 levels" in str(e.value)
+    else:
+        lvm, _, _, _ = g.msbfs(ms_src)
+        assert np.array_equal(lvm[0], oracle.bfs(ptr, to, int(ms_src[0]))[0])
+    g.close()
+
+
+def test_direction_optimising_needs_hbm_resident_graph(gpu, rmat_w):
+    with pytest.raises(og.OgError) as e:
+        _prepare(rmat_w, "ascetic+diropt", 0.25, 0.8, 8)
+    assert e.value.name == "OG_EUNSUPPORTED"
