@@ -110,14 +110,32 @@ __device__ __forceinline__ uint32_t block_min_u32(uint32_t x, uint64_t *lds) {
 
 constexpr int kLoadBatch = 8;  // flagged vertices whose row bounds are read together
 
+// Bit j set when byte j of the thread's flag window is >= lo.  The passes
+// then walk the set bits only (lowest first, i.e. in vertex order), which
+// keeps their code small: a per-slot unrolled switch over 32 slots does not
+// fit the instruction cache.
+__device__ __forceinline__ uint32_t window_mask(const uint4 (&w)[kFlagVecs], uint32_t lo) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < kFlagVecs; ++k) {
+        const uint32_t q[4] = {w[k].x, w[k].y, w[k].z, w[k].w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                m |= (((q[i] >> (8 * b)) & 0xFFu) >= lo ? 1u : 0u) << (16 * k + 4 * i + b);
+    }
+    return m;
+}
+
 __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg, uint32_t nblk,
                             FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
                             uint64_t *lds);
 
 // Compaction pass 1: classifies every flagged vertex of the block's window,
 // stores kCode0 + bin in its flag byte and publishes the per-block counts.
-// Row bounds (and values, for near-far) of kLoadBatch vertices are read
-// before any is used, so their latencies overlap.
+// Row bounds (and values, for near-far) of kLoadBatch flagged vertices are
+// read before any is used, so their latencies overlap.
 __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__ flag, uint64_t Vpad, uint64_t vsplit,
                                                            const uint64_t *__restrict__ ptr, Tuning tn,
                                                            const uint32_t *__restrict__ val,
@@ -130,32 +148,32 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
     load_flags(flag, base, Vpad, w);
     uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, cz = 0, ch = 0, cf = 0, mf = 0xFFFFFFFFu;
     uint64_t sdeg = 0, ddeg = 0;
-    if (any_flag(w)) {
+    uint32_t m = window_mask(w, 1);
+    if (m) {
         const uint32_t thresh = val ? ctl->thresh : 0xFFFFFFFFu;
-        uint32_t code[4 * kFlagVecs];
-#pragma unroll
-        for (int k = 0; k < 4 * kFlagVecs; ++k) code[k] = 0;
-#pragma unroll
-        for (int j0 = 0; j0 < kFlagsPerThread; j0 += kLoadBatch) {
+        while (m) {
+            uint32_t jj[kLoadBatch];
             uint64_t pb[kLoadBatch], pe[kLoadBatch];
             uint32_t vv[kLoadBatch];
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
+                jj[k] = kFlagsPerThread;  // empty slot
                 pb[k] = pe[k] = 0;
                 vv[k] = 0;
-                if (flag_at(w, j0 + k)) {
-                    const uint64_t v = base + j0 + k;
-                    pb[k] = ptr[v];
-                    pe[k] = ptr[v + 1];
-                    if (val) vv[k] = val[v];
+                if (m) {
+                    const uint32_t j = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1;
+                    jj[k] = j;
+                    pb[k] = ptr[base + j];
+                    pe[k] = ptr[base + j + 1];
+                    if (val) vv[k] = val[base + j];
                 }
             }
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
-                const int j = j0 + k;
-                if (!flag_at(w, j)) continue;
+                if (jj[k] == kFlagsPerThread) continue;
                 const uint64_t deg = pe[k] - pb[k];
-                const int b = bin_of(val && vv[k] >= thresh, base + j, vsplit, deg, tn);
+                const int b = bin_of(val && vv[k] >= thresh, base + jj[k], vsplit, deg, tn);
                 switch (b) {
                     case 0: ++c0; sdeg += deg; break;
                     case 1: ++c1; sdeg += deg; break;
@@ -164,10 +182,9 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                     case 5: ++cf; mf = min(mf, vv[k]); break;
                     default: ++cz; break;
                 }
-                code[j >> 2] |= (uint32_t)(kCode0 + b) << (8 * (j & 3));
+                flag[base + jj[k]] = (uint8_t)(kCode0 + b);
             }
         }
-        store_flags(flag, base, Vpad, code);
     }
     const uint64_t x[6] = {(uint64_t)c0 | ((uint64_t)c1 << 32), (uint64_t)c2 | ((uint64_t)c3 << 32),
                            (uint64_t)cz | ((uint64_t)ch << 32), sdeg, ddeg, cf};
@@ -309,86 +326,105 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                                                            uint32_t *__restrict__ list, uint64_t *__restrict__ hub,
                                                            uint64_t hub_cap, uint64_t *__restrict__ doff) {
     __shared__ uint64_t lds[3 * kBlock / kWave];
+    __shared__ uint32_t s_win[4 * kFlagVecs * kBlock];  // word i of thread t at [i * kBlock + t]
     const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kFlagsPerThread;
     uint4 w[kFlagVecs];
     load_flags(flag, base, Vpad, w);
-    const bool any = any_flag(w);
+#pragma unroll
+    for (int k = 0; k < kFlagVecs; ++k) {
+        s_win[(4 * k + 0) * kBlock + threadIdx.x] = w[k].x;
+        s_win[(4 * k + 1) * kBlock + threadIdx.x] = w[k].y;
+        s_win[(4 * k + 2) * kBlock + threadIdx.x] = w[k].z;
+        s_win[(4 * k + 3) * kBlock + threadIdx.x] = w[k].w;
+    }
+    // bin of slot j (codes of pass 1; each thread reads its own words)
+    auto bin_at = [&](uint32_t j) -> int {
+        return (int)((s_win[(j >> 2) * kBlock + threadIdx.x] >> (8 * (j & 3))) & 0xFFu) - kCode0;
+    };
+    const uint32_t all = window_mask(w, kCode0);
     uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
     uint64_t ddeg = 0;
-    uint64_t deg[kFlagsPerThread];  // of bin 2 / 3 vertices
-    if (any) {
-#pragma unroll
-        for (int j0 = 0; j0 < kFlagsPerThread; j0 += kLoadBatch) {
+    // walks the set bits kLoadBatch at a time; row bounds read for bins 2 / 3
+    auto walk = [&](auto &&use) {
+        for (uint32_t m = all; m;) {
+            uint32_t jj[kLoadBatch];
+            int bb[kLoadBatch];
             uint64_t pb[kLoadBatch], pe[kLoadBatch];
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
+                jj[k] = kFlagsPerThread;
+                bb[k] = -1;
                 pb[k] = pe[k] = 0;
-                const uint32_t cd = flag_at(w, j0 + k);
-                if (cd == kCode0 + 2 || cd == kCode0 + 3) {
-                    pb[k] = ptr[base + j0 + k];
-                    pe[k] = ptr[base + j0 + k + 1];
+                if (m) {
+                    const uint32_t j = (uint32_t)__builtin_ctz(m);
+                    m &= m - 1;
+                    jj[k] = j;
+                    bb[k] = bin_at(j);
+                    if (bb[k] == 2 || bb[k] == 3) {
+                        pb[k] = ptr[base + j];
+                        pe[k] = ptr[base + j + 1];
+                    }
                 }
             }
 #pragma unroll
-            for (int k = 0; k < kLoadBatch; ++k) {
-                const int j = j0 + k;
-                deg[j] = pe[k] - pb[k];
-                switch ((int)flag_at(w, j) - kCode0) {
-                    case 0: ++c0; break;
-                    case 1: ++c1; break;
-                    case 2: c2 += (uint32_t)((deg[j] + tn.tile - 1) / tn.tile); break;
-                    case 3: ++c3; ddeg += deg[j]; break;
-                    default: break;
-                }
-            }
+            for (int k = 0; k < kLoadBatch; ++k)
+                if (jj[k] != kFlagsPerThread) use(jj[k], bb[k], pe[k] - pb[k]);
         }
-    }
+    };
+    walk([&](uint32_t, int b, uint64_t deg) {
+        switch (b) {
+            case 0: ++c0; break;
+            case 1: ++c1; break;
+            case 2: c2 += (uint32_t)((deg + tn.tile - 1) / tn.tile); break;
+            case 3: ++c3; ddeg += deg; break;
+            default: break;
+        }
+    });
     const uint64_t x[3] = {(uint64_t)c0 | ((uint64_t)c1 << 32), (uint64_t)c2 | ((uint64_t)c3 << 32), ddeg};
     uint64_t ex[3], t[3];
     block_scan_n<3>(x, ex, t, lds);
-    if (!any) return;
-    if (info->pull) {  // bottom-up round: no lists, the frontier is its level
-        const uint32_t zero[4 * kFlagVecs] = {};
-        store_flags(flag, base, Vpad, zero);
-        return;
+    if (!all) return;
+    uint32_t keep = 0;  // slots that stay flagged: held back (bin 5) or a held dynamic list
+    if (!info->pull) {  // bottom-up round: no lists, the frontier is its level
+        const uint32_t n0 = info->n[0], n1 = info->n[1];
+        uint64_t p0 = (uint64_t)bcnt[0 * nblk + blockIdx.x] + (uint32_t)ex[0];
+        uint64_t p1 = (uint64_t)n0 + bcnt[1 * nblk + blockIdx.x] + (uint32_t)(ex[0] >> 32);
+        uint64_t p2 = (uint64_t)bcnt[2 * nblk + blockIdx.x] + (uint32_t)ex[1];
+        uint64_t p3 = (uint64_t)bcnt[3 * nblk + blockIdx.x] + (uint32_t)(ex[1] >> 32);
+        uint64_t pd = bdeg[1 * nblk + blockIdx.x] + ex[2];
+        const uint64_t dyn_base = (uint64_t)n0 + n1;
+        const bool held = info->dyn_held != 0;
+        walk([&](uint32_t j, int b, uint64_t deg) {
+            const uint64_t v = base + j;
+            switch (b) {
+                case 0: list[p0++] = (uint32_t)v; break;
+                case 1: list[p1++] = (uint32_t)v; break;
+                case 2: {
+                    const uint32_t nt = (uint32_t)((deg + tn.tile - 1) / tn.tile);
+                    for (uint32_t q = 0; q < nt; ++q, ++p2)
+                        if (p2 < hub_cap) hub[p2] = ((uint64_t)q << 32) | v;
+                    break;
+                }
+                case 3:
+                    if (held) {
+                        keep |= 1u << j;
+                        break;
+                    }
+                    list[dyn_base + p3] = (uint32_t)v;
+                    doff[p3] = pd;
+                    ++p3;
+                    pd += deg;
+                    break;
+                case 5: keep |= 1u << j; break;
+                default: break;
+            }
+        });
     }
-    const uint32_t n0 = info->n[0], n1 = info->n[1];
-    uint64_t p0 = (uint64_t)bcnt[0 * nblk + blockIdx.x] + (uint32_t)ex[0];
-    uint64_t p1 = (uint64_t)n0 + bcnt[1 * nblk + blockIdx.x] + (uint32_t)(ex[0] >> 32);
-    uint64_t p2 = (uint64_t)bcnt[2 * nblk + blockIdx.x] + (uint32_t)ex[1];
-    uint64_t p3 = (uint64_t)bcnt[3 * nblk + blockIdx.x] + (uint32_t)(ex[1] >> 32);
-    uint64_t pd = bdeg[1 * nblk + blockIdx.x] + ex[2];
-    const uint64_t dyn_base = (uint64_t)n0 + n1;
-    const bool held = info->dyn_held != 0;
     uint32_t word[4 * kFlagVecs];
 #pragma unroll
     for (int k = 0; k < 4 * kFlagVecs; ++k) word[k] = 0;
 #pragma unroll
-    for (int j = 0; j < kFlagsPerThread; ++j) {
-        const uint64_t v = base + j;
-        switch ((int)flag_at(w, j) - kCode0) {
-            case 0: list[p0++] = (uint32_t)v; break;
-            case 1: list[p1++] = (uint32_t)v; break;
-            case 2: {
-                const uint32_t nt = (uint32_t)((deg[j] + tn.tile - 1) / tn.tile);
-                for (uint32_t q = 0; q < nt; ++q, ++p2)
-                    if (p2 < hub_cap) hub[p2] = ((uint64_t)q << 32) | v;
-                break;
-            }
-            case 3:
-                if (held) {
-                    word[j >> 2] |= 1u << (8 * (j & 3));
-                    break;
-                }
-                list[dyn_base + p3] = (uint32_t)v;
-                doff[p3] = pd;
-                ++p3;
-                pd += deg[j];
-                break;
-            case 5: word[j >> 2] |= 1u << (8 * (j & 3)); break;
-            default: break;
-        }
-    }
+    for (int j = 0; j < kFlagsPerThread; ++j) word[j >> 2] |= ((keep >> j) & 1u) << (8 * (j & 3));
     store_flags(flag, base, Vpad, word);
 }
 
