@@ -27,7 +27,8 @@ constexpr int kFlagsPerThread = 16 * kFlagVecs;
 constexpr int kVertsPerBlock = kBlock * kFlagsPerThread;   // 8192
 constexpr int kDynEdgesPerThread = 4;   // dynamic relax: edges per lane
 constexpr int kDynEdgesPerBlock = kBlock * kDynEdgesPerThread;
-constexpr int kSlots = 2;               // dynamic staging double buffer
+constexpr int kMaxSlots = 16;           // staging slots of the dynamic-region pipeline
+constexpr uint64_t kSlotTargetBytes = 256ull << 20;  // preferred slot (chunk) size
 
 enum Algo { ALGO_BFS = 0, ALGO_SSSP = 1, ALGO_CC = 2, ALGO_MSBFS = 3 };
 
@@ -197,7 +198,7 @@ struct DeviceState {
     uint64_t *d_ptr = nullptr;       // V + 1 (device copy; USM: managed)
     void *d_static = nullptr;        // static-region edges (USM: managed, whole graph)
     uint64_t static_used = 0;        // edges of the static vertices, = ptr[vsplit]
-    void *d_staging = nullptr;       // kSlots * slot_edges edges
+    void *d_staging = nullptr;       // nslots * slot_edges edges (zero-copy: the resident head)
     uint64_t slot_edges = 0;
     bool managed = false;
     // Weighted edges on the device and the link: one 4-byte word
@@ -220,16 +221,16 @@ struct DeviceState {
     // pinned host mirrors of the dynamic list
     uint32_t *h_dlist = nullptr;     // V
     uint64_t *h_doff = nullptr;      // V + 1
-    void *h_stage[kSlots] = {nullptr, nullptr};
+    std::vector<void *> h_stage;     // pinned gather buffer per staging slot
 
     hipStream_t s_compute = nullptr, s_copy = nullptr;
     hipStream_t s_static = nullptr;  // static-region relax, overlapping the dynamic region
     hipEvent_t ev_fork = nullptr;    // compaction of the round finished (static relax may start)
     hipEvent_t ev_static = nullptr;  // static relax of the round finished
-    hipEvent_t ev_h2d[kSlots] = {nullptr, nullptr};    // copy into slot finished
-    hipEvent_t ev_relax[kSlots] = {nullptr, nullptr};  // relax reading slot finished
-    bool slot_h2d_pending[kSlots] = {false, false};
-    bool slot_relax_pending[kSlots] = {false, false};
+    uint32_t nslots = 0;             // staging slots (a pipeline of small chunks)
+    std::vector<hipEvent_t> ev_h2d;    // copy into slot finished
+    std::vector<hipEvent_t> ev_relax;  // relax reading slot finished
+    std::vector<char> slot_h2d_pending, slot_relax_pending;
     EventPool events;
     std::unique_ptr<WorkerPool> pool;
 

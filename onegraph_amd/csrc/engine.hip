@@ -130,7 +130,7 @@ __device__ __forceinline__ uint32_t window_mask(const uint4 (&w)[kFlagVecs], uin
 
 __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg, uint32_t nblk,
                             FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
-                            uint64_t *lds);
+                            uint64_t *__restrict__ m_doff, uint64_t *lds);
 
 // Compaction pass 1: classifies every flagged vertex of the block's window,
 // stores kCode0 + bin in its flag byte and publishes the per-block counts.
@@ -210,9 +210,10 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
 // back and invalidate the whole L2 of each XCD, once per workgroup.
 __global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
                                                         uint32_t nblk, FrontierInfo *__restrict__ info,
-                                                        uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl) {
+                                                        uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
+                                                        uint64_t *__restrict__ m_doff) {
     __shared__ uint64_t lds[7 * 1024 / kWave];
-    scan_counts(bcnt, bdeg, nblk, info, doff, ctl, lds);
+    scan_counts(bcnt, bdeg, nblk, info, doff, ctl, m_doff, lds);
 }
 
 // Compaction pass 2 body: one workgroup turns the per-block counts into
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *__restrict__ b
 // whole pass is one multi-value block scan.  `lds` holds 7 * blockDim / 64.
 __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg, uint32_t nblk,
                             FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
-                            uint64_t *lds) {
+                            uint64_t *__restrict__ m_doff, uint64_t *lds) {
     const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
     const uint32_t b0 = min(nblk, threadIdx.x * per), b1 = min(nblk, b0 + per);
     uint64_t s01 = 0, s23 = 0, sdd = 0, zs = 0, hs = 0, fs = 0, ss = 0;
@@ -297,6 +298,7 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
         }
         *info = fi;
         doff[fi.n[3]] = fi.dyn_deg;
+        if (m_doff) m_doff[fi.n[3]] = fi.dyn_deg;
         c.compactions++;
         c.live = fi.live;
         if (active) {
@@ -317,14 +319,18 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
 // Compaction pass 3: emits the lists in vertex order from the codes of pass
 // 1 (bins 0/1 need no reads; hub tiles and the dynamic degree prefix read
 // the row bounds again, kLoadBatch at a time), and rewrites the window:
-// 1 for held-back vertices, 0 elsewhere.
+// 1 for held-back vertices, 0 elsewhere.  m_dlist / m_doff, when not null,
+// receive a copy of the dynamic list and its degree prefix: pinned host
+// memory the staged loader gathers from without a copy back.
 __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__ flag, uint64_t Vpad,
                                                            const uint64_t *__restrict__ ptr, Tuning tn,
                                                            const uint32_t *__restrict__ bcnt,
                                                            const uint64_t *__restrict__ bdeg, uint32_t nblk,
                                                            const FrontierInfo *__restrict__ info,
                                                            uint32_t *__restrict__ list, uint64_t *__restrict__ hub,
-                                                           uint64_t hub_cap, uint64_t *__restrict__ doff) {
+                                                           uint64_t hub_cap, uint64_t *__restrict__ doff,
+                                                           uint32_t *__restrict__ m_dlist,
+                                                           uint64_t *__restrict__ m_doff) {
     __shared__ uint64_t lds[3 * kBlock / kWave];
     __shared__ uint32_t s_win[4 * kFlagVecs * kBlock];  // word i of thread t at [i * kBlock + t]
     const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kFlagsPerThread;
@@ -412,6 +418,10 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                     }
                     list[dyn_base + p3] = (uint32_t)v;
                     doff[p3] = pd;
+                    if (m_dlist) {  // pinned host mirrors for the host-side gather
+                        m_dlist[p3] = (uint32_t)v;
+                        m_doff[p3] = pd;
+                    }
                     ++p3;
                     pd += deg;
                     break;
@@ -851,10 +861,8 @@ void device_state_free(DeviceState *d) {
     if (d->s_compute) (void)hipStreamSynchronize(d->s_compute);
     if (d->s_copy) (void)hipStreamSynchronize(d->s_copy);
     if (d->s_static) (void)hipStreamSynchronize(d->s_static);
-    for (int s = 0; s < kSlots; ++s) {
-        if (d->ev_h2d[s]) (void)hipEventDestroy(d->ev_h2d[s]);
-        if (d->ev_relax[s]) (void)hipEventDestroy(d->ev_relax[s]);
-    }
+    for (hipEvent_t e : d->ev_h2d) (void)hipEventDestroy(e);
+    for (hipEvent_t e : d->ev_relax) (void)hipEventDestroy(e);
     d->mem.release();
     if (d->s_compute) (void)hipStreamDestroy(d->s_compute);
     if (d->s_copy) (void)hipStreamDestroy(d->s_copy);
@@ -904,11 +912,16 @@ void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer, 
 }
 
 void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val, bool to_host) {
+    // (pinned mirrors of the dynamic list written by pass 3 cost one small
+    // PCIe write per vertex: slower than one bulk copy back, so unused)
+    uint32_t *m_dlist = nullptr;
+    uint64_t *m_doff = nullptr;
     hipLaunchKernelGGL(k_frontier_count, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, d->Vpad, d->plan.vsplit, d->d_ptr,
                        d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
-    hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff, f.ctl);
+    hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff, f.ctl,
+                       m_doff);
     hipLaunchKernelGGL(k_frontier_write, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, d->Vpad, d->d_ptr, d->tune, f.bcnt,
-                       f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff);
+                       f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist, m_doff);
     OG_HIP(hipGetLastError());
     if (to_host) OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
 }
@@ -944,21 +957,25 @@ static void upload_packed(DeviceState *d, const og_graph *g, uint64_t n, uint32_
     struct Bounce {
         hipStream_t s;
         uint32_t *b[2] = {nullptr, nullptr};
+        hipEvent_t ev[2] = {nullptr, nullptr};
         ~Bounce() {
             (void)hipStreamSynchronize(s);
             for (uint32_t *p : b)
                 if (p) (void)hipHostFree(p);
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
         }
     } bb{d->s_compute};
     const uint64_t C = std::min<uint64_t>(n, 16ull << 20);
     for (auto &p : bb.b) OG_HIP(hipHostMalloc((void **)&p, C * 4, 0));
+    for (auto &e : bb.ev) OG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (uint64_t c0 = 0, k = 0; c0 < n; c0 += C, ++k) {
         const int s = (int)(k & 1);
-        if (k >= 2) OG_HIP(hipEventSynchronize(d->ev_h2d[s]));
+        if (k >= 2) OG_HIP(hipEventSynchronize(bb.ev[s]));
         const uint64_t m = std::min(C, n - c0);
         pack_edges(g, c0, m, d->wbits, bb.b[s]);
         OG_HIP(hipMemcpyAsync(dst + c0, bb.b[s], m * 4, hipMemcpyHostToDevice, d->s_compute));
-        OG_HIP(hipEventRecord(d->ev_h2d[s], d->s_compute));
+        OG_HIP(hipEventRecord(bb.ev[s], d->s_compute));
     }
     OG_HIP(hipStreamSynchronize(d->s_compute));
 }
@@ -1031,10 +1048,6 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
     OG_HIP(hipStreamCreateWithFlags(&d->s_static, hipStreamNonBlocking));
     OG_HIP(hipEventCreateWithFlags(&d->ev_static, hipEventDisableTiming));
     OG_HIP(hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming));
-    for (int s = 0; s < kSlots; ++s) {
-        OG_HIP(hipEventCreateWithFlags(&d->ev_h2d[s], hipEventDisableTiming));
-        OG_HIP(hipEventCreateWithFlags(&d->ev_relax[s], hipEventDisableTiming));
-    }
 
     // ---- graph placement
     if (approach == OG_USM) {
@@ -1063,14 +1076,36 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
                                       d->s_compute));
         }
         if (pl.vsplit < V) {
-            const bool zc = (p.flags & OG_PLAN_ZERO_COPY) != 0;
-            d->slot_edges = pl.staging_edges / kSlots;
+            // PageRank streams its dynamic region through the slots (vertex-
+            // aligned chunks), so a pull-layout graph always gets them
+            const bool zc = (p.flags & OG_PLAN_ZERO_COPY) != 0 && g->kind != OG_KIND_PULL;
+            // Staging slots: gather, copy and relax of consecutive chunks
+            // overlap.  Chunks of ~kSlotTargetBytes: smaller ones cost more
+            // in per-chunk gather dispatch and launches than they save in
+            // pipeline fill.  Never smaller than the largest dynamic row,
+            // which the vertex-aligned PageRank chunks need whole.
+            uint64_t maxdyn = 0;
+            for (uint64_t v = pl.vsplit; v < V; ++v) maxdyn = std::max(maxdyn, g->ptr[v + 1] - g->ptr[v]);
+            uint32_t ns = (uint32_t)std::min<uint64_t>(kMaxSlots, std::max<uint64_t>(2, pl.staging_edges * eb /
+                                                                                          kSlotTargetBytes));
+            while (ns > 2 && pl.staging_edges / ns < maxdyn) --ns;
+            d->nslots = ns;
+            d->slot_edges = pl.staging_edges / ns;
             if (!zc) {
                 if (d->slot_edges == 0)
                     raise(OG_EBUDGET, "edge budget leaves %llu staging edges for %llu dynamic-region edges",
                           (unsigned long long)pl.staging_edges, (unsigned long long)(E - g->ptr[pl.vsplit]));
-                d->d_staging = d->dalloc(kSlots * d->slot_edges * eb);
-                for (int s = 0; s < kSlots; ++s) d->h_stage[s] = d->palloc(d->slot_edges * eb);
+                d->d_staging = d->dalloc((uint64_t)ns * d->slot_edges * eb);
+                d->h_stage.resize(ns);
+                d->ev_h2d.assign(ns, nullptr);
+                d->ev_relax.assign(ns, nullptr);
+                d->slot_h2d_pending.assign(ns, 0);
+                d->slot_relax_pending.assign(ns, 0);
+                for (uint32_t s = 0; s < ns; ++s) {
+                    d->h_stage[s] = d->palloc(d->slot_edges * eb);
+                    OG_HIP(hipEventCreateWithFlags(&d->ev_h2d[s], hipEventDisableTiming));
+                    OG_HIP(hipEventCreateWithFlags(&d->ev_relax[s], hipEventDisableTiming));
+                }
                 d->h_dlist = (uint32_t *)d->palloc(V * sizeof(uint32_t));
                 d->h_doff = (uint64_t *)d->palloc((V + 1) * sizeof(uint64_t));
                 d->pool.reset(new WorkerPool(d->host_threads));
@@ -1385,7 +1420,7 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
     const bool serial = (d->flags & OG_PLAN_NO_OVERLAP) != 0;
     for (uint64_t c0 = 0, c = 0; c0 < D; c0 += C, ++c) {
         const uint64_t c1 = std::min(D, c0 + C);
-        const int slot = (int)(c % kSlots);
+        const int slot = (int)(c % d->nslots);
         if (d->slot_h2d_pending[slot]) OG_HIP(hipEventSynchronize(d->ev_h2d[slot]));
         Stopwatch sw;
         gather(d, g, ndyn, c0, c1, (uint8_t *)d->h_stage[slot]);

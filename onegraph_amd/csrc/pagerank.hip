@@ -352,7 +352,7 @@ static void run_pr(og_graph *g, const og_pr_params &prm, T *out, og_pr_stats *st
         for (size_t c = 0; c < pp.chunks.size(); ++c) {
             const uint32_t va = pp.chunks[c].first, vb = pp.chunks[c].second;
             const uint64_t e0 = g->ptr[va], e1 = g->ptr[vb];
-            const int slot = (int)(c % kSlots);
+            const int slot = (int)(c % d->nslots);
             uint8_t *dslot = (uint8_t *)d->d_staging + (uint64_t)slot * C * eb;
             if (d->slot_relax_pending[slot]) OG_HIP(hipStreamWaitEvent(d->s_copy, d->ev_relax[slot], 0));
             if (e1 > e0) {
