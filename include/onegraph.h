@@ -275,7 +275,11 @@ typedef struct og_tuning {
     uint32_t defer;  /* SSSP / CC: 1 = a round with static-region work holds the
                         dynamic-region vertices back, so the link rows are read
                         once the in-HBM part has settled, in denser rounds; 0 =
-                        both regions every round (the reference's order)        */
+                        both regions every round (the reference's order);
+                        SSSP with delta > 0: 2 = lagged, a dynamic-region
+                        vertex relaxes only after the static vertices of its
+                        near-far bucket have settled, so its link row is
+                        read about once                                          */
 } og_tuning;
 
 og_status og_get_tuning(const og_graph *g, og_tuning *t);
@@ -336,8 +340,10 @@ og_status og_pagerank(og_graph *g, const og_pr_params *params, float *ranks_f32,
  * reference answers one query per process, BFS/oneGraph_BFS/main.cpp:3-14).
  * Queries are batched continuously: each graph has a BFS engine and, when
  * weighted, an SSSP engine with up to 64 query slots; queries that arrive
- * while others run join at the next round and share its kernels.  Results
- * equal og_bfs / og_sssp on the same graph and source. */
+ * while others run join at the next round and share its kernels.  SSSP slots
+ * each run near-far order with their own threshold (width: the graph's
+ * tuning delta, 32 when it is 0).  Results equal og_bfs / og_sssp on the
+ * same graph and source. */
 
 #define OG_SERVER_MAX_GRAPHS 8
 #define OG_QUERY_BFS 0u
@@ -347,9 +353,12 @@ og_status og_pagerank(og_graph *g, const og_pr_params *params, float *ranks_f32,
 typedef struct og_server og_server;
 
 typedef struct og_server_params {
-    uint64_t state_budget_bytes;  /* HBM for per-query state (one slot column per
-                                     query: V bytes for BFS, 4V for SSSP), shared
-                                     evenly by the engines; 0 = a quarter of free HBM */
+    uint64_t state_budget_bytes;  /* HBM for per-query state: the slot bits of the
+                                     frontier masks (24 V bytes per BFS engine,
+                                     16 V per SSSP engine) plus one 4 V distance
+                                     column per SSSP slot; every engine gets the
+                                     same power-of-two slot count S <= 64, the
+                                     largest that fits; 0 = a quarter of free HBM */
     uint32_t max_pending;         /* queued queries before submit returns OG_EBUSY; 0 = 2^20 */
     uint32_t flags;
 } og_server_params;

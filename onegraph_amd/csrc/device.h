@@ -51,7 +51,9 @@ struct Tuning {
     uint32_t t2 = 4096;    // bin 1: t1 < deg <= t2, one wave per vertex
     uint32_t tile = 2048;  // bin 2: deg > t2, tiles of `tile` edges, one workgroup each
     uint32_t delta = 0;    // SSSP near-far bucket width (0: every improved vertex every round)
-    uint32_t defer = 0;    // SSSP / CC: hold dynamic vertices back while static ones are active
+    uint32_t defer = 0;    // SSSP / CC: 1 = hold dynamic vertices back while static ones are active;
+                           // SSSP with delta > 0: 2 = dynamic vertices relax only after the
+                           // static part of their near-far bucket has settled
 };
 
 // Frontier summary produced on the device each iteration and read by the host.
@@ -76,7 +78,9 @@ struct FrontierInfo {
 struct RoundCtl {
     uint32_t thresh;             // near-far threshold of the next compaction (0xFFFFFFFF: off)
     uint32_t delta;              // near-far bucket width (0: off)
-    uint32_t defer;              // hold the dynamic list back while static vertices are active
+    uint32_t defer;              // 1: hold the dynamic list back while static vertices are active;
+                                 // 2: dynamic vertices are near only below fbound
+    uint32_t fbound;             // defer 2: dynamic vertices with value < fbound are near
     uint32_t rounds;             // compactions that found active vertices
     uint32_t compactions;        // all compactions
     uint32_t live;               // last compaction's FrontierInfo::live
@@ -94,7 +98,7 @@ struct RoundCtl {
     uint64_t dyn_rounds;         // rounds with a dynamic list
 };
 
-constexpr int kBcntRows = 7;            // per-block rows: bins 0-3, zero|hub, far, min far
+constexpr int kBcntRows = 7;            // per-block rows: bins 0-3, zero|hub, far|lagging, min far
 constexpr uint8_t kCode0 = 2;           // count pass: flag byte = kCode0 + bin of the vertex
 
 // Buffers of one frontier traversal: next-frontier flags, the compacted bin
@@ -355,6 +359,27 @@ __device__ __forceinline__ void wave_add(unsigned long long *ctr, uint32_t v) {
     if ((threadIdx.x & (kWave - 1)) == 0 && s) atomicAdd(ctr, (unsigned long long)s);
 }
 
+constexpr int kRelaxBatch = 4;  // edges per lane whose reads are issued together
+
+// Edge i of an edge array: 4-byte column (unweighted), the packed 4-byte
+// weighted word (to << wbits | w), or the CSR's 8-byte AoS {to, w}.
+template <bool W, bool PK>
+__device__ __forceinline__ void load_edge(const void *__restrict__ edges, uint64_t i, uint32_t wbits, uint32_t &to,
+                                          uint32_t &wt) {
+    if (W && PK) {
+        const uint32_t x = ((const uint32_t *)edges)[i];
+        to = x >> wbits;
+        wt = x & ((1u << wbits) - 1u);
+    } else if (W) {
+        const uint2 e = ((const uint2 *)edges)[i];
+        to = e.x;
+        wt = e.y;
+    } else {
+        to = ((const uint32_t *)edges)[i];
+        wt = 1;
+    }
+}
+
 // ------------------------------------------------------------- host timing
 
 struct Stopwatch {
@@ -399,7 +424,8 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m);
 // Resets f.ctl for a new traversal on stream s (a kernel, so it may sit in a
 // captured graph): statistics zeroed, near-far width `delta` with the first
 // threshold 1 + delta (the source's value is 1), or none for delta = 0;
-// `defer` holds the dynamic list back in rounds with static work;
+// `defer` 1 holds the dynamic list back in rounds with static work, 2 lets
+// dynamic vertices relax only once the static part of their bucket settled;
 // dir_opt_edges > 0 turns on the BFS direction choice over that many edges.
 void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer = 0, uint64_t dir_opt_edges = 0);
 

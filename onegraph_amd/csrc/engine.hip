@@ -146,11 +146,15 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
     const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kFlagsPerThread;
     uint4 w[kFlagVecs];
     load_flags(flag, base, Vpad, w);
-    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, cz = 0, ch = 0, cf = 0, mf = 0xFFFFFFFFu;
+    uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, cz = 0, ch = 0, cf = 0, cl = 0, mf = 0xFFFFFFFFu;
     uint64_t sdeg = 0, ddeg = 0;
     uint32_t m = window_mask(w, 1);
     if (m) {
         const uint32_t thresh = val ? ctl->thresh : 0xFFFFFFFFu;
+        // lagged order: a dynamic vertex below the threshold but not below
+        // fbound waits until the static part of its bucket has settled
+        const bool lagged = val && ctl->defer == 2;
+        const uint32_t fbound = lagged ? ctl->fbound : 0u;
         while (m) {
             uint32_t jj[kLoadBatch];
             uint64_t pb[kLoadBatch], pe[kLoadBatch];
@@ -173,7 +177,9 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
             for (int k = 0; k < kLoadBatch; ++k) {
                 if (jj[k] == kFlagsPerThread) continue;
                 const uint64_t deg = pe[k] - pb[k];
-                const int b = bin_of(val && vv[k] >= thresh, base + jj[k], vsplit, deg, tn);
+                const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + jj[k] >= vsplit;
+                cl += lag ? 1u : 0u;
+                const int b = bin_of((val && vv[k] >= thresh) || lag, base + jj[k], vsplit, deg, tn);
                 switch (b) {
                     case 0: ++c0; sdeg += deg; break;
                     case 1: ++c1; sdeg += deg; break;
@@ -187,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
         }
     }
     const uint64_t x[6] = {(uint64_t)c0 | ((uint64_t)c1 << 32), (uint64_t)c2 | ((uint64_t)c3 << 32),
-                           (uint64_t)cz | ((uint64_t)ch << 32), sdeg, ddeg, cf};
+                           (uint64_t)cz | ((uint64_t)ch << 32), sdeg, ddeg, (uint64_t)cf | ((uint64_t)cl << 16)};
     uint64_t ex[6], t[6];
     block_scan_n<6>(x, ex, t, lds);
     const uint32_t bm = block_min_u32(mf, lds);
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
         bcnt[2 * nblk + blockIdx.x] = (uint32_t)t[1];
         bcnt[3 * nblk + blockIdx.x] = (uint32_t)(t[1] >> 32);
         bcnt[4 * nblk + blockIdx.x] = (uint32_t)t[2] | ((uint32_t)(t[2] >> 32) << 16);  // both < 2^16 per block
-        bcnt[5 * nblk + blockIdx.x] = (uint32_t)t[5];
+        bcnt[5 * nblk + blockIdx.x] = (uint32_t)t[5];  // far | lagging << 16, both <= 8192 per block
         bcnt[6 * nblk + blockIdx.x] = bm;
         bdeg[0 * nblk + blockIdx.x] = t[3];
         bdeg[1 * nblk + blockIdx.x] = t[4];
@@ -212,7 +218,7 @@ __global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *__restrict__ b
                                                         uint32_t nblk, FrontierInfo *__restrict__ info,
                                                         uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
                                                         uint64_t *__restrict__ m_doff) {
-    __shared__ uint64_t lds[7 * 1024 / kWave];
+    __shared__ uint64_t lds[8 * 1024 / kWave];
     scan_counts(bcnt, bdeg, nblk, info, doff, ctl, m_doff, lds);
 }
 
@@ -222,13 +228,13 @@ __global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *__restrict__ b
 // liveness, the BFS direction and, when the near set is empty but vertices
 // are held back, the near-far threshold (min held-back value + delta) for
 // the next compaction.  Each thread scans a run of consecutive blocks, so the
-// whole pass is one multi-value block scan.  `lds` holds 7 * blockDim / 64.
+// whole pass is one multi-value block scan.  `lds` holds 8 * blockDim / 64.
 __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg, uint32_t nblk,
                             FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
                             uint64_t *__restrict__ m_doff, uint64_t *lds) {
     const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
     const uint32_t b0 = min(nblk, threadIdx.x * per), b1 = min(nblk, b0 + per);
-    uint64_t s01 = 0, s23 = 0, sdd = 0, zs = 0, hs = 0, fs = 0, ss = 0;
+    uint64_t s01 = 0, s23 = 0, sdd = 0, zs = 0, hs = 0, fs = 0, ss = 0, ls = 0;
     uint32_t fm = 0xFFFFFFFFu;
     for (uint32_t b = b0; b < b1; ++b) {
         s01 += (uint64_t)bcnt[0 * nblk + b] | ((uint64_t)bcnt[1 * nblk + b] << 32);
@@ -236,14 +242,16 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
         const uint32_t zh = bcnt[4 * nblk + b];
         zs += zh & 0xFFFFu;
         hs += zh >> 16;
-        fs += bcnt[5 * nblk + b];
+        const uint32_t fl = bcnt[5 * nblk + b];
+        fs += fl & 0xFFFFu;
+        ls += fl >> 16;
         fm = min(fm, bcnt[6 * nblk + b]);
         ss += bdeg[b];
         sdd += bdeg[nblk + b];
     }
-    const uint64_t x[7] = {s01, s23, sdd, zs, hs, fs, ss};
-    uint64_t ex[7], t[7];
-    block_scan_n<7>(x, ex, t, lds);
+    const uint64_t x[8] = {s01, s23, sdd, zs, hs, fs, ss, ls};
+    uint64_t ex[8], t[8];
+    block_scan_n<8>(x, ex, t, lds);
     const uint32_t fmin = block_min_u32(fm, lds);
     uint64_t r01 = ex[0], r23 = ex[1], rdd = ex[2];
     for (uint32_t b = b0; b < b1; ++b) {
@@ -275,7 +283,7 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
         const uint64_t snodes = (uint64_t)fi.n[0] + fi.n[1] + fi.nhub;
         // deferred order: while static vertices are active, the dynamic list
         // stays flagged for a later round (the write pass keeps its flags)
-        fi.dyn_held = (c.defer && snodes && fi.n[3]) ? 1u : 0u;
+        fi.dyn_held = (c.defer == 1 && snodes && fi.n[3]) ? 1u : 0u;
         const uint32_t held = fi.dyn_held ? fi.n[3] : 0u;
         if (held) {
             fi.n[3] = 0;
@@ -310,7 +318,13 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
             c.static_work_edges += fi.static_deg;
             c.dyn_rounds += fi.n[3] ? 1 : 0;
         } else if (fi.nfar && c.delta) {
-            c.thresh = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)fi.min_far + c.delta);
+            // Near set empty.  Lagged order: every vertex below the threshold
+            // has relaxed except the waiting dynamic ones, which go next (a
+            // static vertex they improve is below the threshold and joins
+            // them).  Otherwise every held-back value is >= the threshold and
+            // it moves to the smallest of them + delta.
+            if (t[7]) c.fbound = c.thresh;
+            else c.thresh = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)fi.min_far + c.delta);
         }
         *ctl = c;
     }
@@ -446,30 +460,12 @@ __global__ void k_ctl_reset(RoundCtl *ctl, uint32_t delta, uint32_t defer, uint6
     c.unexplored = dir_opt_edges;
     c.nverts = V;
     c.thresh = delta ? (uint32_t)min((uint64_t)0xFFFFFFFFull, 1ull + delta) : 0xFFFFFFFFu;
+    c.fbound = 0;
     c.live = 1;
     *ctl = c;
 }
 
 // ---------------------------------------------------------------- relax
-
-// Edge i of an edge array: 4-byte column (unweighted), the packed 4-byte
-// weighted word (to << wbits | w), or the CSR's 8-byte AoS {to, w}.
-template <bool W, bool PK>
-__device__ __forceinline__ void load_edge(const void *__restrict__ edges, uint64_t i, uint32_t wbits, uint32_t &to,
-                                          uint32_t &wt) {
-    if (W && PK) {
-        const uint32_t x = ((const uint32_t *)edges)[i];
-        to = x >> wbits;
-        wt = x & ((1u << wbits) - 1u);
-    } else if (W) {
-        const uint2 e = ((const uint2 *)edges)[i];
-        to = e.x;
-        wt = e.y;
-    } else {
-        to = ((const uint32_t *)edges)[i];
-        wt = 1;
-    }
-}
 
 template <int ALGO>
 __device__ __forceinline__ uint32_t offer(uint32_t du, uint32_t wt) {
@@ -534,8 +530,6 @@ __device__ __forceinline__ uint32_t commit(const RelaxArgs &a, typename SrcOf<AL
         return 0;
     }
 }
-
-constexpr int kRelaxBatch = 4;  // edges per lane whose reads are issued together
 
 // The relax kernels read their work counts from the round's FrontierInfo on
 // the device and stride over them, so a launch needs no host-side count: a
@@ -1826,7 +1820,7 @@ static og_status check_tuning(const og_tuning *t) {
         return fail(OG_EINVAL, "lpv0 must be a power of two in [1, 64]");
     if (t->t1 < 1 || t->t2 < t->t1) return fail(OG_EINVAL, "need 1 <= t1 <= t2");
     if (t->tile < 64 || t->tile > 4096) return fail(OG_EINVAL, "tile must be in [64, 4096]");
-    if (t->defer > 1) return fail(OG_EINVAL, "defer must be 0 or 1");
+    if (t->defer > 2) return fail(OG_EINVAL, "defer must be 0, 1 or 2");
     return OG_OK;
 }
 
@@ -1927,7 +1921,8 @@ extern "C" og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, ui
         const bool has_dyn = g->dev->plan.vsplit < g->V;
         for (int pass = 0; pass < 2; ++pass) {
             if (algo != OG_TUNE_BFS && has_dyn)
-                for (uint32_t df : {0u, 1u}) {
+                for (uint32_t df : {0u, 1u, 2u}) {
+                    if (df == 2 && algo != OG_TUNE_SSSP) continue;
                     og_tuning c = cur;
                     c.defer = df;
                     trial(c);

@@ -217,14 +217,16 @@ def test_graph_replays(gpu, rmat_w, approach, frac, k):
 
 @pytest.mark.parametrize("approach,frac,k", [("ascetic+zerocopy", 0.25, 0.8), ("ascetic", 1 / 64, 0.5),
                                               ("ascetic+zerocopy", 1 / 64, 0.5), ("subway+zerocopy", 0.25, 0.0)])
-@pytest.mark.parametrize("delta", [0, 8])
-def test_deferred_dynamic_order(gpu, rmat_w, approach, frac, k, delta):
-    """Holding the dynamic list back while static vertices are active changes
-    the relax order, not the fixed point: SSSP and CC stay bit-exact."""
+@pytest.mark.parametrize("delta,defer", [(0, 1), (8, 1), (1, 2), (8, 2), (300, 2)])
+def test_deferred_dynamic_order(gpu, rmat_w, approach, frac, k, delta, defer):
+    """Holding the dynamic list back while static vertices are active (defer
+    1), or until the static part of its near-far bucket has settled (defer 2,
+    SSSP), changes the relax order, not the fixed point: SSSP and CC stay
+    bit-exact."""
     ptr, to, w, _ = rmat_w.export()
-    _say("deferred dynamic rmat13", approach, frac, k, delta)
+    _say("deferred dynamic rmat13", approach, frac, k, delta, defer)
     _prepare(rmat_w, approach, frac, k, 8)
-    rmat_w.set_tuning(defer=1, delta=delta)
+    rmat_w.set_tuning(defer=defer, delta=delta)
     for s in _sources(ptr, len(to), 2, 13):
         d, st = rmat_w.sssp(s)
         assert np.array_equal(d, oracle.sssp(ptr, to, w, s)), s

@@ -30,6 +30,18 @@ def graphs():
         g.close()
 
 
+def _state_bytes(graphs, S):
+    """Admission rule of og_server_create: frontier masks (24 V per BFS
+    engine, 16 V per SSSP engine) + one 4 V distance column per SSSP slot."""
+    total = 0
+    for g in graphs:
+        V = g.num_vertices
+        total += 24 * V
+        if g.info()["kind"] == og.KIND_WEIGHTED:
+            total += 16 * V + 4 * V * S
+    return total
+
+
 def _queries(graphs, n, seed):
     rng = np.random.default_rng(seed)
     qs = []
@@ -63,8 +75,7 @@ def test_alone_and_in_the_mix(gpu, graphs):
     qs = _queries(graphs, 60, 7)
     _say("server mix of", len(qs))
     # small budget: 8 slots per engine, so queries queue and join mid-flight
-    per_slot = sum(g.num_vertices * (1 + (4 if g.info()["kind"] == og.KIND_WEIGHTED else 0)) for g in graphs)
-    with og.Server(graphs, state_budget_bytes=8 * per_slot) as srv:
+    with og.Server(graphs, state_budget_bytes=_state_bytes(graphs, 8)) as srv:
         info = srv.info()
         assert info["bfs_slots"] == [8, 8, 8] and info["sssp_slots"] == [8, 8, 0]
         ids = [srv.submit(gi, k, s, keep=True) for gi, k, s in qs]
@@ -106,8 +117,12 @@ def test_admission_and_reservation(gpu, graphs):
     with pytest.raises(og.OgError) as e:
         og.Server([g], state_budget_bytes=16)
     assert e.value.name == "OG_EBUDGET"
-    srv = og.Server([g], state_budget_bytes=g.num_vertices * 5 * 2, max_pending=4)
+    with pytest.raises(og.OgError) as e:  # the masks alone do not fit
+        og.Server([g], state_budget_bytes=_state_bytes([g], 1) - 1)
+    assert e.value.name == "OG_EBUDGET"
+    srv = og.Server([g], state_budget_bytes=_state_bytes([g], 2) + 1, max_pending=4)
     assert srv.info()["bfs_slots"] == [2] and srv.info()["sssp_slots"] == [2]
+    assert srv.info()["state_bytes"] == _state_bytes([g], 2)
     with pytest.raises(og.OgError) as e:  # reserved while served
         g.bfs(0)
     assert e.value.name == "OG_EBUSY"
