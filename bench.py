@@ -316,8 +316,11 @@ def wl_serve(args):
     server batches continuously; the plain path serves one query at a time
     with og_bfs / og_sssp, its latency obtained by replaying the same arrival
     trace through a FIFO queue with service times measured on a sample of its
-    queries.  value = served queries/s of the first mix at the highest rate
-    whose p99 latency is within --p99-target-ms (else at the lowest rate)."""
+    queries (graphs prepared with their in-edges, so plain BFS is
+    direction-optimising like the server's; the top-down plain rate is
+    reported beside it).  value = served queries/s of the first mix at the
+    highest rate whose p99 latency is within --p99-target-ms (else at the
+    lowest rate)."""
     import threading
 
     import onegraph_amd as og
@@ -327,7 +330,6 @@ def wl_serve(args):
     for i, sc in enumerate(scales):
         t0 = time.time()
         g = og.Graph.rmat(sc, args.edge_factor, seed=1005 + i, weight_seed=2005 + i)
-        g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0)
         ptr, _, _, _ = g.export()
         deg = np.diff(np.append(ptr.astype(np.int64), g.num_edges))
         cands.append(np.flatnonzero(deg > 0))
@@ -345,35 +347,49 @@ def wl_serve(args):
         src = np.array([int(cands[g][rng.integers(len(cands[g]))]) for g in gi])
         return t, gi, kind, src
 
-    # plain one-at-a-time path: service times on a sample, per (graph, kind)
-    service = {}
-    rng = np.random.default_rng(3005)
-    for gi, g in enumerate(graphs):
-        for kind in (0, 1):
-            ts = []
-            for _ in range(args.plain_sample // 2 if kind else args.plain_sample):
-                s = int(cands[gi][rng.integers(len(cands[gi]))])
-                t = time.perf_counter()
-                _, st = g.bfs(s, want_values=False) if kind == 0 else g.sssp(s, want_values=False)
-                ts.append(time.perf_counter() - t)
-            service[(gi, kind)] = np.array(ts)
-            log(f"plain g{gi} {'sssp' if kind else 'bfs'}: mean {1e3 * np.mean(ts):.2f} ms")
+    # plain one-at-a-time path: service times on a sample, per (graph, kind);
+    # first top-down BFS (the reference's order), then with the graphs'
+    # in-edges on the device (direction-optimising BFS), which the server
+    # also uses: the served/plain ratio is taken against the latter
+    def sample_plain(tag, flags):
+        svc = {}
+        rng = np.random.default_rng(3005)
+        for gi, g in enumerate(graphs):
+            g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0, flags=flags)
+            g.set_tuning(delta=args.delta)  # SSSP near-far width for the plain path and the lanes alike
+            for kind in (0, 1):
+                ts = []
+                for _ in range(args.plain_sample // 2 if kind else args.plain_sample):
+                    s = int(cands[gi][rng.integers(len(cands[gi]))])
+                    t = time.perf_counter()
+                    _, st = g.bfs(s, want_values=False) if kind == 0 else g.sssp(s, want_values=False)
+                    ts.append(time.perf_counter() - t)
+                svc[(gi, kind)] = np.array(ts)
+                log(f"plain {tag} g{gi} {'sssp' if kind else 'bfs'}: mean {1e3 * np.mean(ts):.2f} ms")
+        return svc
 
-    # per-query state for 64 slots per engine: masks (24 V BFS + 16 V SSSP) + a 4 V column per SSSP slot
-    full = sum(g.num_vertices * (24 + 16 + 4 * 64) for g in graphs)
+    service_td = sample_plain("top-down", 0)
+    service = sample_plain("diropt", og.PLAN_DIRECTION_OPT)
+
+    # per-query state: BFS masks (24 V per graph) + 4 SSSP lanes of ~17 V per weighted graph (+ scratch)
+    full = sum(g.num_vertices * (24 + 4 * 18) for g in graphs)
     budget = int(args.serve_budget_gib * (1 << 30)) if args.serve_budget_gib else full
 
-    def run_rate(rate, seed, bfs_share):
-        t_arr, gis, kinds, srcs = trace(rate, seed, bfs_share)
-        # plain: FIFO replay with sampled service times
-        srng = np.random.default_rng(seed + 4)
+    def replay(t_arr, gis, kinds, svc, seed):
+        """FIFO one-at-a-time service of the trace with sampled service times."""
+        srng = np.random.default_rng(seed)
         free_at, plat = 0.0, []
         for k in range(len(t_arr)):
-            sv = service[(int(gis[k]), int(kinds[k]))]
+            sv = svc[(int(gis[k]), int(kinds[k]))]
             start = max(free_at, t_arr[k])
             free_at = start + float(sv[srng.integers(len(sv))])
             plat.append(1e3 * (free_at - t_arr[k]))
-        plain_qps = len(t_arr) / max(free_at, t_arr[-1])
+        return len(t_arr) / max(free_at, t_arr[-1]), plat
+
+    def run_rate(rate, seed, bfs_share):
+        t_arr, gis, kinds, srcs = trace(rate, seed, bfs_share)
+        plain_qps, plat = replay(t_arr, gis, kinds, service, seed + 4)
+        td_qps, _ = replay(t_arr, gis, kinds, service_td, seed + 4)
         srv = og.Server(graphs, state_budget_bytes=budget)
         info0 = srv.info()
         lat, done, submit_at = [], {}, {}
@@ -422,7 +438,8 @@ def wl_serve(args):
         res = {"queries": len(t_arr), "served_qps": round(qps, 1), "p50_ms": round(_pct(lat, 50), 3),
                "p99_ms": round(_pct(lat, 99), 3), "plain_qps": round(float(plain_qps), 1),
                "plain_p50_ms": round(_pct(plat, 50), 3), "plain_p99_ms": round(_pct(plat, 99), 3),
-               "speedup_qps": round(float(qps / plain_qps), 2), "errors": bad, "rounds": info["rounds"],
+               "speedup_qps": round(float(qps / plain_qps), 2), "plain_topdown_qps": round(float(td_qps), 1),
+               "speedup_vs_topdown": round(float(qps / td_qps), 2), "errors": bad, "rounds": info["rounds"],
                "checked": int(len(pick)), "mismatches": int(mism),
                "slots": {"bfs": info0["bfs_slots"], "sssp": info0["sssp_slots"]}, "state_bytes": info0["state_bytes"]}
         for kk, name in ((0, "bfs"), (1, "sssp")):
@@ -452,7 +469,9 @@ def wl_serve(args):
                 "parity": [{"mix": k, "rate": r, "checked": v["checked"], "mismatches": v["mismatches"]}
                            for k, m in per_mix.items() for r, v in m["rates"].items()],
                 "plain_service_ms": {f"g{k[0]}_{'sssp' if k[1] else 'bfs'}": round(1e3 * float(np.mean(v)), 3)
-                                     for k, v in service.items()}})
+                                     for k, v in service.items()},
+                "plain_topdown_service_ms": {f"g{k[0]}_{'sssp' if k[1] else 'bfs'}": round(1e3 * float(np.mean(v)), 3)
+                                             for k, v in service_td.items()}})
     print(json.dumps(out), flush=True)
     for g in graphs:
         g.close()

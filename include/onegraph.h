@@ -338,12 +338,12 @@ og_status og_pagerank(og_graph *g, const og_pr_params *params, float *ranks_f32,
 
 /* Query server over device-resident graphs (no reference counterpart: the
  * reference answers one query per process, BFS/oneGraph_BFS/main.cpp:3-14).
- * Queries are batched continuously: each graph has a BFS engine and, when
- * weighted, an SSSP engine with up to 64 query slots; queries that arrive
- * while others run join at the next round and share its kernels.  SSSP slots
- * each run near-far order with their own threshold (width: the graph's
- * tuning delta, 32 when it is 0).  Results equal og_bfs / og_sssp on the
- * same graph and source. */
+ * BFS queries are batched continuously: each graph has a BFS engine with 64
+ * query slots (one bit of per-vertex masks each); queries that arrive while
+ * others run join at the next round and share its kernels.  SSSP queries run
+ * on K concurrent lanes per weighted graph, each a single-source engine with
+ * the graph's tuning.  Results equal og_bfs / og_sssp on the same graph and
+ * source. */
 
 #define OG_SERVER_MAX_GRAPHS 8
 #define OG_QUERY_BFS 0u
@@ -353,12 +353,12 @@ og_status og_pagerank(og_graph *g, const og_pr_params *params, float *ranks_f32,
 typedef struct og_server og_server;
 
 typedef struct og_server_params {
-    uint64_t state_budget_bytes;  /* HBM for per-query state: the slot bits of the
-                                     frontier masks (24 V bytes per BFS engine,
-                                     16 V per SSSP engine) plus one 4 V distance
-                                     column per SSSP slot; every engine gets the
-                                     same power-of-two slot count S <= 64, the
-                                     largest that fits; 0 = a quarter of free HBM */
+    uint64_t state_budget_bytes;  /* HBM for per-query state: the BFS engine's
+                                     frontier masks (24 V bytes per graph, 64
+                                     slots) plus, per weighted graph, K SSSP
+                                     lanes of about 17 V bytes each (K = the
+                                     largest of 4, 2, 1 that fits);
+                                     0 = a quarter of free HBM                 */
     uint32_t max_pending;         /* queued queries before submit returns OG_EBUSY; 0 = 2^20 */
     uint32_t flags;
 } og_server_params;
@@ -375,8 +375,8 @@ typedef struct og_query_result {
 
 typedef struct og_server_info {
     uint32_t num_graphs;
-    uint32_t bfs_slots[OG_SERVER_MAX_GRAPHS];
-    uint32_t sssp_slots[OG_SERVER_MAX_GRAPHS];
+    uint32_t bfs_slots[OG_SERVER_MAX_GRAPHS];   /* BFS queries in flight at once per graph */
+    uint32_t sssp_slots[OG_SERVER_MAX_GRAPHS];  /* SSSP lanes per graph                    */
     uint64_t state_bytes;         /* HBM held for per-query state */
     uint64_t submitted, completed, rejected, pending, in_flight, rounds;
 } og_server_info;

@@ -42,6 +42,8 @@ struct RelaxArgs {
     const uint64_t *front;// multi-source: sources whose frontier holds v
     uint64_t *next;       // multi-source: sources offered to v this round
     uint32_t wbits;       // packed weighted edges: bits of the weight field
+    const uint64_t *pull; // multi-source, optional: sources relaxed bottom-up this round
+                          // (left out of the top-down relax)
 };
 
 // Degree-bin thresholds and shapes for the static relax (autotunable).
@@ -215,6 +217,7 @@ struct DeviceState {
     const void *hd_dyn = nullptr;    // the same, as a device address (zero-copy mode)
     uint64_t slot_dyn = 0;           // dynamic edges per staging slot in the transfer format
     uint64_t dyn_resident = 0;       // zero-copy: edges [dyn_base, dyn_base + this) held in d_staging
+                                     // (the whole rows right after the static prefix)
     uint64_t *m_ptr = nullptr;       // USM managed copies
     void *m_edges = nullptr;
 
@@ -436,6 +439,13 @@ void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer =
 // copied to f.h_info (stream-ordered, not synchronised).
 void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val = nullptr,
                       bool to_host = true);
+
+// Per-run state of a concurrent single-source lane: shares the device arrays
+// of g->dev (the whole graph must be on the device) and owns values,
+// frontier, streams and captured rounds.  Freed with device_state_free.
+DeviceState *lane_state_create(og_graph *g);
+// og_bfs (ALGO_BFS) / og_sssp (ALGO_SSSP) from src on a lane; out may be null.
+void lane_run(og_graph *g, DeviceState *lane, int algo, uint32_t src, uint32_t *out, og_run_stats *st);
 
 // Launches the static-region relax of the bins for op ALGO.  The kernels
 // read their counts from f.info; `hfi` (the host copy of it, or null when the

@@ -484,7 +484,7 @@ template <> struct SrcOf<ALGO_MSBFS> { using T = uint64_t; };
 
 template <int ALGO>
 __device__ __forceinline__ typename SrcOf<ALGO>::T load_src(const RelaxArgs &a, uint32_t u) {
-    if constexpr (ALGO == ALGO_MSBFS) return a.front[u];
+    if constexpr (ALGO == ALGO_MSBFS) return a.pull ? a.front[u] & ~*a.pull : a.front[u];
     else return a.val[u];
 }
 
@@ -553,6 +553,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restri
         const uint32_t u = lst[item];
         const uint64_t b = ptr[u], e = ptr[u + 1];
         const auto s = load_src<ALGO>(a, u);
+        if (ALGO == ALGO_MSBFS && !s) continue;  // all of u's sources go bottom-up this round
         for (uint64_t i = b + sub; i < e; i += (uint64_t)kRelaxBatch * LPV) {
             uint32_t to[kRelaxBatch], wt[kRelaxBatch];
             typename SrcOf<ALGO>::T seen[kRelaxBatch];
@@ -593,6 +594,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
         const uint64_t e = min(ptr[u + 1], b + tile);
         const uint32_t cnt = (uint32_t)(e - b);
         const auto s = load_src<ALGO>(a, u);
+        if (ALGO == ALGO_MSBFS && !s) continue;  // block-uniform: u is the same for every thread
         for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
             if (WIDE)
                 ((uint2 *)s_edges)[k] = ((const uint2 *)edges)[b + k];
@@ -626,9 +628,12 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
 // list[n0 + n1, n0 + n1 + n3)).  STAGED (DIRECT = false): they sit in a
 // staging slot, edge q at q - c0.  DIRECT: edge q of vertex u is
 // dynamic-region edge idx = ptr[u] - dbase + (q - doff[i]); the first nres of
-// those are resident in HBM (`resident`), the rest are read in place from the
-// pinned host array through its device mapping (`stage`); reads served from
-// HBM are counted in *hits.
+// those (whole rows) are resident in HBM (`resident`), the rest are read in
+// place from the pinned host array through its device mapping (`stage`);
+// reads served from HBM are counted in *hits.  (Storing the host rows in
+// order of distance from a hub instead of vertex order was measured slower
+// on R-MAT: the active list stays in vertex order, so neighbouring lanes then
+// read scattered rows.)
 // Each workgroup takes chunks of kDynEdgesPerBlock consecutive edges, loads
 // the covering slice of the degree prefix into LDS and maps each edge to its
 // source with a binary search there (every listed vertex has >= 1 edge, so a
@@ -1126,8 +1131,12 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
                 d->hd_dyn = (const uint8_t *)hd + ((const uint8_t *)d->h_dyn - (const uint8_t *)hbase);
                 // Reading in place needs no transit slots, so the staging
                 // share of the budget keeps the head of the dynamic region
-                // (the rows right after the static prefix) resident in HBM.
-                d->dyn_resident = std::min<uint64_t>(pl.staging_edges, nd);
+                // (the whole rows right after the static prefix) resident in
+                // HBM.
+                const uint64_t cap = std::min<uint64_t>(pl.staging_edges, nd);
+                uint64_t vres = pl.vsplit;
+                while (vres < V && g->ptr[vres + 1] - d->dyn_base <= cap) ++vres;
+                d->dyn_resident = g->ptr[vres] - d->dyn_base;
                 if (d->dyn_resident) {
                     d->d_staging = d->dalloc(d->dyn_resident * eb);
                     OG_HIP(hipMemcpyAsync(d->d_staging, d->h_dyn, d->dyn_resident * eb, hipMemcpyHostToDevice,
@@ -1296,7 +1305,7 @@ static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0
 }
 
 static RelaxArgs relax_args(DeviceState *d) {
-    return RelaxArgs{d->d_val, d->fr.flag, d->d_visit, d->d_front, d->d_next, d->wbits};
+    return RelaxArgs{d->d_val, d->fr.flag, d->d_visit, d->d_front, d->d_next, d->wbits, nullptr};
 }
 
 // Grids of device-count launches: the kernels stride over whatever count
@@ -1459,9 +1468,8 @@ constexpr int kRoundBatch = 4;  // device-driven rounds enqueued per host check
 // work).  The staged loader needs the host in every round (gather, copies),
 // so it synchronises on each round's summary.
 template <int ALGO, bool W, class After>
-static void frontier_loop(og_graph *g, og_run_stats &st, RunTimers &tm, double &gather_ms, double &overload_wall,
-                          After after) {
-    DeviceState *d = g->dev;
+static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTimers &tm, double &gather_ms,
+                          double &overload_wall, After after) {
     hipStream_t s = d->s_compute;
     // SSSP near-far order: only vertices below the threshold relax in a
     // round; when none is left it moves to (smallest held-back value + delta).
@@ -1594,8 +1602,8 @@ static void finish_stats(og_run_stats &st, RunTimers &tm, double gather_ms, doub
 }
 
 template <int ALGO, bool W>
-static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats *stats) {
-    DeviceState *d = g->dev;
+static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats *stats, DeviceState *dv = nullptr) {
+    DeviceState *d = dv ? dv : g->dev;
     const uint64_t V = d->V;
     og_run_stats st{};
     st.approach = (uint32_t)d->approach;
@@ -1618,7 +1626,7 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
         hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d->d_val, d->fr.flag, src);
     }
     OG_HIP(hipGetLastError());
-    frontier_loop<ALGO, W>(g, st, tm, gather_ms, overload_wall, []() {});
+    frontier_loop<ALGO, W>(g, d, st, tm, gather_ms, overload_wall, []() {});
     OG_HIP(hipStreamSynchronize(s));
     st.total_ms = total.ms();
 
@@ -1676,7 +1684,7 @@ static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t
                            d->fr.flag, d->d_mlvl);
         OG_HIP(hipGetLastError());
         const unsigned cgrid = (unsigned)std::min<uint64_t>(4096, (V + kBlock - 1) / kBlock);
-        frontier_loop<ALGO_MSBFS, W>(g, st, tm, gather_ms, overload_wall, [&]() {
+        frontier_loop<ALGO_MSBFS, W>(g, d, st, tm, gather_ms, overload_wall, [&]() {
             hipLaunchKernelGGL(k_ms_commit, dim3(cgrid), dim3(kBlock), 0, s, d->fr.flag, V, d->d_visit, d->d_front,
                                d->d_next, d->d_mlvl, d->fr.ctl);
             OG_HIP(hipGetLastError());
@@ -1737,6 +1745,52 @@ static og_status run_algo(og_graph *g, int algo, uint32_t src, uint32_t *out, og
     });
     g->busy.store(0);
     return st;
+}
+
+// Per-run state of a concurrent single-source lane over the prepared graph:
+// it shares the graph's device arrays (row pointers, edges, in-edges) and
+// owns values, frontier buffers, streams, events and captured rounds, so
+// lanes iterate independently on their own streams.
+DeviceState *lane_state_create(og_graph *g) {
+    const DeviceState *d = g->dev;
+    if (d->plan.vsplit < d->V) raise(OG_EUNSUPPORTED, "lanes need the whole graph on the device");
+    std::unique_ptr<DeviceState, void (*)(DeviceState *)> l(new DeviceState, device_state_free);
+    l->approach = d->approach;
+    l->device = d->device;
+    l->V = d->V;
+    l->E = d->E;
+    l->Vpad = d->Vpad;
+    l->eb = d->eb;
+    l->deb = d->deb;
+    l->plan = d->plan;
+    l->tune = d->tune;
+    l->host_threads = d->host_threads;
+    l->flags = d->flags;
+    l->d_ptr = d->d_ptr;
+    l->d_static = d->d_static;
+    l->static_used = d->static_used;
+    l->managed = d->managed;
+    l->packed = d->packed;
+    l->wbits = d->wbits;
+    l->d_in_ptr = d->d_in_ptr;
+    l->d_in_cols = d->d_in_cols;
+    OG_HIP(hipSetDevice(d->device));
+    OG_HIP(hipStreamCreateWithFlags(&l->s_compute, hipStreamNonBlocking));
+    OG_HIP(hipStreamCreateWithFlags(&l->s_copy, hipStreamNonBlocking));
+    OG_HIP(hipStreamCreateWithFlags(&l->s_static, hipStreamNonBlocking));
+    OG_HIP(hipEventCreateWithFlags(&l->ev_static, hipEventDisableTiming));
+    OG_HIP(hipEventCreateWithFlags(&l->ev_fork, hipEventDisableTiming));
+    l->d_val = (uint32_t *)l->dalloc(l->V * sizeof(uint32_t));
+    frontier_alloc(l.get(), g, l->fr, l->mem);
+    OG_HIP(hipDeviceSynchronize());
+    return l.release();
+}
+
+void lane_run(og_graph *g, DeviceState *lane, int algo, uint32_t src, uint32_t *out, og_run_stats *st) {
+    const bool w = g->kind == OG_KIND_WEIGHTED;
+    if (algo == ALGO_SSSP) run_frontier<ALGO_SSSP, true>(g, src, out, st, lane);
+    else if (w) run_frontier<ALGO_BFS, true>(g, src, out, st, lane);
+    else run_frontier<ALGO_BFS, false>(g, src, out, st, lane);
 }
 
 }  // namespace og

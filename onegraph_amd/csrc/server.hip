@@ -1,32 +1,32 @@
-// Query server: continuous batching of BFS and SSSP queries on device-
-// resident graphs (no reference counterpart; the reference runs one query per
-// process, BFS/oneGraph_BFS/main.cpp:3-14).
+// Query server: continuous batching of BFS queries and concurrent SSSP
+// lanes on device-resident graphs (no reference counterpart; the reference
+// runs one query per process, BFS/oneGraph_BFS/main.cpp:3-14).
 //
-// Every (graph, kind) pair has one batch engine with S <= 64 query slots, its
-// own HIP stream, frontier buffers and host thread.  A slot is one bit of the
-// per-vertex u64 masks (BFS: visit / front / next; SSSP: front / next) and,
-// for SSSP, one column of the vertex-major distance rows dist[V][S].  Each
-// engine round ends with one host synchronisation:
+// BFS: every graph has one batch engine with 64 query slots, its own HIP
+// stream, frontier buffers and host thread.  A slot is one bit of the
+// per-vertex u64 masks visit / front / next.  Each engine round ends with
+// one host synchronisation:
 //   1. admit pending queries into free slots: the source's bit goes into
 //      `next`, so the commit below seeds it like any other offer;
-//   2. commit (one sweep over the frontier flags): BFS turns the offered,
-//      unseen slots of a vertex into its frontier mask and adds
-//      mix64(v << 32 | level) to each such slot's digest, so no level array
-//      is kept; SSSP takes the offered and held-back slots as the mask;
+//   2. commit (one sweep over the frontier flags): the offered, unseen slots
+//      of a vertex become its frontier mask and each of them adds
+//      mix64(v << 32 | level) to its slot's digest, so no level array is
+//      kept;
 //   3. compact the flags into degree bins (shared with og_bfs);
-//   4. relax: BFS through the multi-source bitmask kernels; SSSP per slot in
-//      near-far order (every slot has its own threshold), one lane group per
-//      (vertex, near slot) item, a hub tile staged in LDS once for all of its
-//      near slots;
-//   5. clear the masks of the relaxed vertices; SSSP moves the threshold of
-//      every slot whose round had held-back work only;
+//   4. relax through the multi-source bitmask kernels (one 64-bit atomic per
+//      edge serves every slot);
+//   5. clear the masks of the relaxed vertices;
 //   6. slots whose frontier emptied are finalised by a sweep that counts
-//      their reached vertices and traversed edges and clears their bits
-//      (BFS) or resets their distance column (SSSP, which also computes the
-//      digest there).
+//      their reached vertices and traversed edges and clears their bits.
 // Queries that arrive while a round runs join at the next round boundary, so
-// their work overlaps with the queries already in flight.  Slot counts come
-// from the HBM budget for per-query state given at creation (admission).
+// their work overlaps with the queries already in flight.
+//
+// SSSP: every weighted graph has K lanes, each a complete single-source
+// near-far engine (values, frontier, streams, captured rounds) over the
+// shared device-resident graph, taking queries from the graph's SSSP queue
+// one at a time.  Distances do not share across queries the way BFS masks
+// do (a 64-slot distance matrix is 256 bytes per vertex and defeats the
+// caches), so SSSP queries overlap as concurrent lanes instead.
 #include <algorithm>
 #include <condition_variable>
 #include <deque>
@@ -38,8 +38,6 @@ struct og_server;
 
 namespace og {
 
-constexpr uint32_t kUnr = 0xFFFFFFFFu;
-constexpr uint32_t kServeDelta = 32;  // SSSP near-far width when the graph's tuning has none
 
 struct SeedList {
     uint32_t n;
@@ -67,39 +65,30 @@ __device__ __forceinline__ uint64_t wave_or(uint64_t x) {
     return x;
 }
 
-// Seeds: slot bit into next[src] (the commit admits it); SSSP also sets the
-// source's distance and the slot's first threshold.
-__global__ void k_srv_seed(SeedList sl, int kind, uint32_t S, uint64_t *next, uint8_t *flag, uint32_t *dist,
-                           uint32_t *thresh, uint32_t delta) {
+// Seeds: the slot's bit goes into next[src]; the commit admits it.  The
+// slot's explored-edge count restarts.
+__global__ void k_srv_seed(SeedList sl, uint64_t *next, uint8_t *flag, unsigned long long *explored) {
     const uint32_t i = threadIdx.x;
     if (i < sl.n) {
-        const uint32_t j = sl.slot[i], s = sl.src[i];
-        atomicOr((unsigned long long *)(next + s), 1ull << j);
-        flag[s] = 1;
-        if (kind == 1) {
-            dist[(uint64_t)s * S + j] = 1;
-            thresh[j] = (uint32_t)min((uint64_t)kUnr, 1ull + delta);
-        }
+        atomicOr((unsigned long long *)(next + sl.src[i]), 1ull << sl.slot[i]);
+        flag[sl.src[i]] = 1;
+        explored[sl.slot[i]] = 0;
     }
 }
 
-// Commit sweep over the flags, 16 vertices per thread step (one uint4 load).
-// BFS: nw = next & ~visit becomes the vertex's frontier mask (when it has
+// Commit sweep over the flags, 16 vertices per thread step (one uint4 load):
+// nw = next & ~visit becomes the vertex's frontier mask (when it has
 // out-edges) and each slot in nw adds the vertex's level to its digest in
-// acc[0..63] (and writes it to a kept result array).  SSSP: next (offered
-// or held-back slots) becomes the mask.  Flags end up 1 exactly for vertices
-// with a non-empty mask; *alive gets the OR of all masks.
-template <int KIND>
+// acc[0..63] (and writes it to a kept result array).  Flags end up 1 exactly
+// for vertices with a non-empty mask; *alive gets the OR of all masks.
 __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ flag, uint64_t V, uint64_t nchunks,
                                                        const uint64_t *__restrict__ ptr, uint64_t *__restrict__ visit,
                                                        uint64_t *__restrict__ front, uint64_t *__restrict__ next,
                                                        SlotLevels L, KeepList keep, unsigned long long *acc,
-                                                       unsigned long long *alive) {
-    __shared__ unsigned long long sdig[64];
-    if (KIND == 0) {
-        if (threadIdx.x < 64) sdig[threadIdx.x] = 0;
-        __syncthreads();
-    }
+                                                       unsigned long long *fstat, unsigned long long *alive) {
+    __shared__ unsigned long long sdig[64], sfc[64], sfe[64];
+    if (threadIdx.x < 64) sdig[threadIdx.x] = sfc[threadIdx.x] = sfe[threadIdx.x] = 0;
+    __syncthreads();
     uint64_t al = 0;
     for (uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x; c < nchunks; c += (uint64_t)gridDim.x * kBlock) {
         const uint4 w = ((const uint4 *)flag)[c];
@@ -109,27 +98,33 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
             if (!((q[k >> 2] >> (8 * (k & 3))) & 0xFFu)) continue;
             const uint64_t v = c * 16 + k;
             if (v >= V) break;
-            const bool out = ptr[v + 1] != ptr[v];
-            uint64_t nw = next[v];
+            const uint64_t vis = visit[v];
+            const uint64_t nw = next[v] & ~vis;
             next[v] = 0;
-            if (KIND == 0) {
-                const uint64_t vis = visit[v];
-                nw &= ~vis;
-                if (nw) {
-                    visit[v] = vis | nw;
-                    uint64_t m = nw;
-                    while (m) {
-                        const int j = __builtin_ctzll(m);
-                        m &= m - 1;
-                        atomicAdd(&sdig[j], (unsigned long long)srv_mix64((v << 32) | L.lv[j]));
-                        if (keep.p[j]) keep.p[j][v] = L.lv[j];
-                    }
+            if (nw) {
+                visit[v] = vis | nw;
+                uint64_t m = nw;
+                while (m) {
+                    const int j = __builtin_ctzll(m);
+                    m &= m - 1;
+                    atomicAdd(&sdig[j], (unsigned long long)srv_mix64((v << 32) | L.lv[j]));
+                    if (keep.p[j]) keep.p[j][v] = L.lv[j];
                 }
             }
-            const uint64_t fr = out ? nw : 0;
+            const uint64_t deg = ptr[v + 1] - ptr[v];
+            const uint64_t fr = deg ? nw : 0;
             if (fr) {
                 front[v] = fr;
                 al |= fr;
+                if (fstat) {  // per-slot frontier size and edges: the direction choice
+                    uint64_t m = fr;
+                    while (m) {
+                        const int j = __builtin_ctzll(m);
+                        m &= m - 1;
+                        atomicAdd(&sfc[j], 1ull);
+                        atomicAdd(&sfe[j], (unsigned long long)deg);
+                    }
+                }
             } else {
                 flag[v] = 0;
             }
@@ -137,9 +132,63 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
     }
     al = wave_or(al);
     if ((threadIdx.x & (kWave - 1)) == 0 && al) atomicOr(alive, (unsigned long long)al);
-    if (KIND == 0) {
-        __syncthreads();
-        if (threadIdx.x < 64 && sdig[threadIdx.x]) atomicAdd(acc + threadIdx.x, sdig[threadIdx.x]);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        if (sdig[threadIdx.x]) atomicAdd(acc + threadIdx.x, sdig[threadIdx.x]);
+        if (sfc[threadIdx.x]) {
+            atomicAdd(fstat + threadIdx.x, sfc[threadIdx.x]);
+            atomicAdd(fstat + 64 + threadIdx.x, sfe[threadIdx.x]);
+        }
+    }
+}
+
+// Direction of every slot for the round (Beamer et al., SC'12, per slot):
+// top-down -> bottom-up once the slot's frontier edges exceed 1/14 of its
+// unexplored edges, back when its frontier holds fewer than V / 24
+// vertices.  Bit j of *pull = slot j relaxes bottom-up.  Resets the stats.
+__global__ void k_srv_direction(uint64_t used, uint64_t V, uint64_t E, unsigned long long *fstat,
+                                unsigned long long *explored, unsigned long long *pull) {
+    const uint32_t j = threadIdx.x;
+    const uint64_t fc = fstat[j], fe = fstat[64 + j];
+    const uint64_t ex = explored[j];
+    const uint64_t unexplored = E > ex + fe ? E - ex - fe : 0;
+    const bool was = (*pull >> j) & 1;
+    bool now = false;
+    if ((used >> j) & 1) now = was ? fc >= V / 24 : fe > unexplored / 14;
+    explored[j] = ex + fe;
+    fstat[j] = fstat[64 + j] = 0;
+    const uint64_t b = __ballot(now);
+    if (j == 0) *pull = b;
+}
+
+// Bottom-up round for the pulling slots P: every vertex that some slot of P
+// has not reached scans its in-edges, ORing the frontier masks of its
+// in-neighbours (restricted to those slots), and stops once every such slot
+// is found.  Found slots are offered through next[v] like a top-down relax.
+__global__ __launch_bounds__(kBlock) void k_srv_bfs_pull(const uint64_t *__restrict__ in_ptr,
+                                                         const uint32_t *__restrict__ in_cols, uint64_t V,
+                                                         const uint64_t *__restrict__ visit,
+                                                         const uint64_t *__restrict__ front, uint64_t *next,
+                                                         uint8_t *flag, const unsigned long long *pull) {
+    const uint64_t P = *pull;
+    if (!P) return;
+    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t need = P & ~visit[v];
+        if (!need) continue;
+        uint64_t acc = 0;
+        const uint64_t b = in_ptr[v], e = in_ptr[v + 1];
+        for (uint64_t i = b; i < e && acc != need; i += kRelaxBatch) {
+            uint32_t u[kRelaxBatch];
+#pragma unroll
+            for (int k = 0; k < kRelaxBatch; ++k) u[k] = i + k < e ? in_cols[i + k] : 0u;
+#pragma unroll
+            for (int k = 0; k < kRelaxBatch; ++k)
+                if (i + k < e) acc |= front[u[k]] & need;
+        }
+        if (acc) {
+            atomicOr((unsigned long long *)(next + v), (unsigned long long)acc);
+            flag[v] = 1;
+        }
     }
 }
 
@@ -152,191 +201,6 @@ __global__ __launch_bounds__(kBlock) void k_srv_clear_front(const uint32_t *__re
     const uint64_t nh = min((uint64_t)info->n[2], hub_cap);
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n01 + nh; i += (uint64_t)gridDim.x * kBlock)
         front[i < n01 ? list[i] : (uint32_t)hub[i - n01]] = 0;
-}
-
-// SSSP relax step shared by the kernels below: lanes of a group relax edges
-// of u for slot j with u's distance dj; an improvement ORs the slot's bit
-// into next[to] and flags it.
-template <bool PK>
-__device__ __forceinline__ void srv_relax_edge(uint32_t to, uint32_t wt, uint32_t seen, uint32_t dj, uint32_t j,
-                                               uint32_t S, uint32_t *dist, uint64_t *next, uint8_t *flag) {
-    uint32_t nd = dj + wt;
-    if (nd < dj) nd = 0xFFFFFFFEu;
-    if (nd >= seen) return;
-    if (nd < atomicMin(dist + (uint64_t)to * S + j, nd)) {
-        atomicOr((unsigned long long *)(next + to), 1ull << j);
-        flag[to] = 1;
-    }
-}
-
-// Near / held-back split of u's frontier slots (lane = slot): a slot is near
-// when u's distance is below the slot's threshold.  Held-back slots go back
-// into next[u] (u stays flagged) and offer their distance to the slot's
-// held minimum.
-__device__ __forceinline__ uint64_t srv_split(uint32_t u, uint64_t F, uint32_t lane, uint32_t T, uint32_t S,
-                                              const uint32_t *dist, uint64_t *next, uint8_t *flag, uint32_t &du,
-                                              uint32_t *s_near, uint32_t *s_heldmin) {
-    const bool act = (F >> lane) & 1;
-    du = act ? dist[(uint64_t)u * S + lane] : kUnr;
-    const bool nr = act && du < T;
-    const bool hd = act && !nr;
-    const uint64_t N = __ballot(nr), H = __ballot(hd);
-    if (nr) s_near[lane] = 1;
-    if (hd) atomicMin(&s_heldmin[lane], du);
-    if (H && lane == 0) {
-        atomicOr((unsigned long long *)(next + u), (unsigned long long)H);
-        flag[u] = 1;
-    }
-    return N;
-}
-
-__device__ __forceinline__ void srv_flush(const uint32_t *s_near, const uint32_t *s_heldmin, uint32_t *g_near,
-                                          uint32_t *g_heldmin) {
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        if (s_near[threadIdx.x]) g_near[threadIdx.x] = 1;
-        if (s_heldmin[threadIdx.x] != kUnr) atomicMin(g_heldmin + threadIdx.x, s_heldmin[threadIdx.x]);
-    }
-}
-
-// Bins 0 / 1: one wave per listed vertex u.  Its near slots are ranked
-// (ds_permute puts slot of rank k into lane k) and dealt to 64 / LPV lane
-// groups; a group relaxes u's edges for its slot, LPV lanes striding.
-template <int LPV, bool PK>
-__global__ __launch_bounds__(kBlock) void k_srv_sssp_relax(const uint32_t *__restrict__ list, int bin,
-                                                           const FrontierInfo *__restrict__ info,
-                                                           const uint64_t *__restrict__ ptr,
-                                                           const void *__restrict__ edges, uint32_t wbits, uint32_t S,
-                                                           const uint64_t *__restrict__ front, uint32_t *dist,
-                                                           uint64_t *next, uint8_t *flag,
-                                                           const uint32_t *__restrict__ thresh, uint32_t *g_near,
-                                                           uint32_t *g_heldmin) {
-    __shared__ uint32_t s_near[64], s_heldmin[64];
-    if (threadIdx.x < 64) {
-        s_near[threadIdx.x] = 0;
-        s_heldmin[threadIdx.x] = kUnr;
-    }
-    __syncthreads();
-    constexpr int G = kWave / LPV;
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t grp = lane / LPV, sub = lane % LPV;
-    const uint32_t T = lane < S ? thresh[lane] : 0u;
-    const uint32_t n = info->n[bin];
-    const uint32_t *lst = list + (bin ? info->n[0] : 0u);
-    const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
-    for (uint64_t item = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / kWave; item < n; item += nwaves) {
-        const uint32_t u = lst[item];
-        uint32_t du;
-        const uint64_t N = srv_split(u, front[u], lane, T, S, dist, next, flag, du, s_near, s_heldmin);
-        if (!N) continue;
-        const uint64_t b = ptr[u], e = ptr[u + 1];
-        const uint32_t np = (uint32_t)__popcll(N);
-        const uint64_t below = N & ((1ull << lane) - 1ull);
-        const uint32_t nb = (uint32_t)__popcll(below);
-        const bool nr = (N >> lane) & 1;
-        const uint32_t rank = nr ? nb : np + (lane - nb);  // a permutation of the lanes
-        const uint32_t slot_at = (uint32_t)__builtin_amdgcn_ds_permute((int)(rank * 4), (int)lane);
-        for (uint32_t r0 = 0; r0 < np; r0 += G) {
-            const uint32_t want = r0 + grp;
-            const bool has = want < np;
-            const uint32_t j = (uint32_t)__shfl((int)slot_at, (int)(want & (kWave - 1)), kWave);
-            const uint32_t dj = (uint32_t)__shfl((int)du, (int)j, kWave);
-            if (!has) continue;
-            for (uint64_t i = b + sub; i < e; i += (uint64_t)LPV * kRelaxBatch) {
-                uint32_t to[kRelaxBatch], wt[kRelaxBatch], seen[kRelaxBatch];
-#pragma unroll
-                for (int k = 0; k < kRelaxBatch; ++k) {
-                    to[k] = 0;
-                    wt[k] = 0;
-                    if (i + (uint64_t)k * LPV < e) load_edge<true, PK>(edges, i + (uint64_t)k * LPV, wbits, to[k], wt[k]);
-                }
-#pragma unroll
-                for (int k = 0; k < kRelaxBatch; ++k)
-                    seen[k] = (i + (uint64_t)k * LPV < e) ? dist[(uint64_t)to[k] * S + j] : 0u;
-#pragma unroll
-                for (int k = 0; k < kRelaxBatch; ++k)
-                    if (i + (uint64_t)k * LPV < e) srv_relax_edge<PK>(to[k], wt[k], seen[k], dj, j, S, dist, next, flag);
-            }
-        }
-    }
-    srv_flush(s_near, s_heldmin, g_near, g_heldmin);
-}
-
-// Hub tiles: one workgroup per (vertex, tile); wave 0 splits the slots, the
-// tile is staged in LDS once and every near slot is relaxed over it by the
-// whole workgroup.
-template <bool PK>
-__global__ __launch_bounds__(kBlock) void k_srv_sssp_hub(const uint64_t *__restrict__ hub, uint64_t hub_cap,
-                                                         const FrontierInfo *__restrict__ info, uint32_t tile,
-                                                         const uint64_t *__restrict__ ptr,
-                                                         const void *__restrict__ edges, uint32_t wbits, uint32_t S,
-                                                         const uint64_t *__restrict__ front, uint32_t *dist,
-                                                         uint64_t *next, uint8_t *flag,
-                                                         const uint32_t *__restrict__ thresh, uint32_t *g_near,
-                                                         uint32_t *g_heldmin) {
-    extern __shared__ uint32_t s_edges[];
-    __shared__ uint32_t s_near[64], s_heldmin[64], s_du[64];
-    __shared__ unsigned long long s_N;
-    if (threadIdx.x < 64) {
-        s_near[threadIdx.x] = 0;
-        s_heldmin[threadIdx.x] = kUnr;
-    }
-    __syncthreads();
-    const uint64_t ntiles = min((uint64_t)info->n[2], hub_cap);
-    const uint32_t T = threadIdx.x < S ? thresh[threadIdx.x] : 0u;
-    for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-        const uint64_t it = hub[ti];
-        const uint32_t u = (uint32_t)it;
-        const uint64_t b = ptr[u] + (it >> 32) * tile;
-        const uint64_t e = min(ptr[u + 1], b + tile);
-        const uint32_t cnt = (uint32_t)(e - b);
-        if (threadIdx.x < kWave) {
-            uint32_t du;
-            const uint64_t N = srv_split(u, front[u], threadIdx.x, T, S, dist, next, flag, du, s_near, s_heldmin);
-            s_du[threadIdx.x] = du;
-            if (threadIdx.x == 0) s_N = N;
-        }
-        for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
-            if (PK) s_edges[k] = ((const uint32_t *)edges)[b + k];
-            else ((uint2 *)s_edges)[k] = ((const uint2 *)edges)[b + k];
-        }
-        __syncthreads();
-        uint64_t N = s_N;
-        while (N) {
-            const uint32_t j = (uint32_t)__builtin_ctzll(N);
-            N &= N - 1;
-            const uint32_t dj = s_du[j];
-            for (uint32_t k0 = threadIdx.x; k0 < cnt; k0 += kRelaxBatch * kBlock) {
-                uint32_t to[kRelaxBatch], wt[kRelaxBatch], seen[kRelaxBatch];
-#pragma unroll
-                for (int q = 0; q < kRelaxBatch; ++q) {
-                    to[q] = 0;
-                    wt[q] = 0;
-                    if (k0 + q * kBlock < cnt) load_edge<true, PK>(s_edges, k0 + q * kBlock, wbits, to[q], wt[q]);
-                }
-#pragma unroll
-                for (int q = 0; q < kRelaxBatch; ++q)
-                    seen[q] = (k0 + q * kBlock < cnt) ? dist[(uint64_t)to[q] * S + j] : 0u;
-#pragma unroll
-                for (int q = 0; q < kRelaxBatch; ++q)
-                    if (k0 + q * kBlock < cnt) srv_relax_edge<PK>(to[q], wt[q], seen[q], dj, j, S, dist, next, flag);
-            }
-        }
-        __syncthreads();  // tile buffer, s_du and s_N are reused by the next tile
-    }
-    srv_flush(s_near, s_heldmin, g_near, g_heldmin);
-}
-
-// Per-slot near-far step: a used slot whose round had held-back work only
-// moves its threshold to (smallest held-back distance + delta).
-__global__ void k_srv_sssp_thresh(uint32_t S, uint64_t used, uint32_t delta, uint32_t *thresh, uint32_t *g_near,
-                                  uint32_t *g_heldmin) {
-    const uint32_t j = threadIdx.x;
-    if (j >= 64) return;
-    if (j < S && ((used >> j) & 1) && !g_near[j] && g_heldmin[j] != kUnr)
-        thresh[j] = (uint32_t)min((uint64_t)kUnr, (uint64_t)g_heldmin[j] + delta);
-    g_near[j] = 0;
-    g_heldmin[j] = kUnr;
 }
 
 // BFS slots in `done`: reached vertices and traversed edges into
@@ -364,53 +228,6 @@ __global__ __launch_bounds__(kBlock) void k_srv_bfs_finalize(uint64_t V, uint64_
     if (threadIdx.x < 128 && s[threadIdx.x]) atomicAdd(acc + 64 + threadIdx.x, s[threadIdx.x]);
 }
 
-// SSSP slots in `done`: digest / reached / traversed into acc, optional copy
-// of the column, and the column reset to unreached.  Lane = (row group,
-// slot), kRows rows per wave in flight.
-constexpr int kFinRows = 4;
-__global__ __launch_bounds__(kBlock) void k_srv_sssp_finalize(uint64_t V, uint32_t S, uint64_t done,
-                                                              const uint64_t *__restrict__ ptr, uint32_t *dist,
-                                                              KeepList keep, unsigned long long *acc) {
-    __shared__ unsigned long long s[3 * 64];
-    for (int t = threadIdx.x; t < 3 * 64; t += kBlock) s[t] = 0;
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t G = kWave / S, grp = lane / S, j = lane % S;
-    const bool mine = (done >> j) & 1;
-    const uint64_t wave = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
-    const uint64_t nwaves = (uint64_t)gridDim.x * kBlock / kWave;
-    uint64_t h = 0, r = 0, t = 0;
-    if (mine) {
-        uint32_t *kp = keep.p[j];
-        for (uint64_t v0 = wave * G + grp; v0 < V; v0 += (uint64_t)kFinRows * nwaves * G) {
-            uint32_t x[kFinRows];
-#pragma unroll
-            for (int q = 0; q < kFinRows; ++q) {
-                const uint64_t v = v0 + (uint64_t)q * nwaves * G;
-                x[q] = v < V ? dist[v * S + j] : kUnr;
-            }
-#pragma unroll
-            for (int q = 0; q < kFinRows; ++q) {
-                const uint64_t v = v0 + (uint64_t)q * nwaves * G;
-                if (v >= V) break;
-                if (kp) kp[v] = x[q];
-                if (x[q] != kUnr) {
-                    ++r;
-                    t += ptr[v + 1] - ptr[v];
-                    h += srv_mix64((v << 32) | x[q]);
-                    dist[v * S + j] = kUnr;
-                }
-            }
-        }
-        atomicAdd(&s[j], (unsigned long long)h);
-        atomicAdd(&s[64 + j], (unsigned long long)r);
-        atomicAdd(&s[128 + j], (unsigned long long)t);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < 3 * 64; k += kBlock)
-        if (s[k]) atomicAdd(acc + k, s[k]);
-}
-
 __global__ void k_srv_zero_acc(uint64_t done, unsigned long long *acc) {
     const uint32_t j = threadIdx.x;
     if (j < 64 && ((done >> j) & 1)) acc[j] = acc[64 + j] = acc[128 + j] = 0;
@@ -431,30 +248,36 @@ struct SlotState {
     double t_admit = 0;
 };
 
-struct Engine {
+constexpr uint32_t kSlots = 64;     // BFS query slots per graph (bits of a u64 mask)
+constexpr uint32_t kMaxLanes = 4;   // SSSP lanes per weighted graph
+
+// Device bytes of one SSSP lane: values 4 V, flags V, list 4 V, degree
+// prefix 8 V (plus per-block scratch and the hub list).
+static uint64_t lane_bytes(uint64_t V) { return 17 * V; }
+
+struct BfsEngine {
     og_server *srv = nullptr;
     og_graph *g = nullptr;
     DeviceState *d = nullptr;
     uint32_t gi = 0;
-    int kind = 0;  // 0 BFS, 1 SSSP
-    uint32_t S = 0;
-    uint32_t delta = kServeDelta;
-    uint64_t state_bytes = 0;
+    std::deque<Query> *pending = nullptr;
     MemPool mem;
     Frontier f;
     hipStream_t s = nullptr;
     uint64_t *visit = nullptr, *front = nullptr, *next = nullptr;
-    uint32_t *dist = nullptr;                                  // SSSP [V][S]
-    uint32_t *thresh = nullptr, *g_near = nullptr, *g_heldmin = nullptr;  // SSSP, 64 each
     unsigned long long *d_alive = nullptr, *h_alive = nullptr, *d_acc = nullptr, *h_acc = nullptr;
+    // direction-optimising rounds (graphs prepared with their in-edges)
+    unsigned long long *d_fstat = nullptr;     // [2][64] frontier vertices / edges per slot
+    unsigned long long *d_explored = nullptr;  // [64] edges of the slot's past frontiers
+    unsigned long long *d_pull = nullptr;      // slots relaxed bottom-up this round
+    uint64_t pull_rounds = 0;
     uint32_t *keep_buf[64] = {};
-    std::deque<Query> pending;
     SlotState slots[64];
     uint64_t used = 0;
     uint32_t round = 0;
     uint64_t rounds_run = 0;
     std::thread th;
-    ~Engine() {
+    ~BfsEngine() {
         if (s) {
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
@@ -463,19 +286,37 @@ struct Engine {
     void loop();
     void step();
     void finalize(uint64_t done);
+    void fail_held(og_status st);
     KeepList keep_list(uint64_t mask) const {
         KeepList kl{};
-        for (uint32_t j = 0; j < S; ++j)
+        for (uint32_t j = 0; j < kSlots; ++j)
             if (((mask >> j) & 1) && (slots[j].q.flags & OG_QUERY_KEEP_VALUES)) kl.p[j] = keep_buf[j];
         return kl;
     }
+};
+
+struct SsspLane {
+    og_server *srv = nullptr;
+    og_graph *g = nullptr;
+    DeviceState *lane = nullptr;
+    uint32_t gi = 0;
+    std::deque<Query> *pending = nullptr;
+    uint64_t runs = 0, rounds_run = 0;
+    Query cur{};
+    std::thread th;
+    ~SsspLane() {
+        if (lane) device_state_free(lane);
+    }
+    void loop();
 };
 
 }  // namespace og
 
 struct og_server {
     std::vector<og_graph *> graphs;
-    std::vector<std::unique_ptr<og::Engine>> engines;  // [graph * 2 + kind], null when absent
+    std::vector<std::deque<og::Query>> queues;          // [graph * 2 + kind]
+    std::vector<std::unique_ptr<og::BfsEngine>> bfs;    // [graph]
+    std::vector<std::vector<std::unique_ptr<og::SsspLane>>> lanes;  // [graph], empty when unweighted
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     bool stop = false;
@@ -496,20 +337,27 @@ static unsigned sweep_grid(uint64_t n, unsigned cap) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(cap, (n + kBlock - 1) / kBlock));
 }
 
-void Engine::finalize(uint64_t done) {
+static void push_failure(og_server *srv, const Query &q, uint32_t gi, og_status st, double t) {
+    og_query_result r{};
+    r.id = q.id;
+    r.graph = gi;
+    r.kind = q.kind;
+    r.source = q.source;
+    r.status = st;
+    r.latency_ms = t - q.t_submit;
+    srv->done.push_back(r);
+    srv->completed++;
+}
+
+void BfsEngine::finalize(uint64_t done) {
     const uint64_t V = d->V;
     const KeepList kl = keep_list(done);
-    if (kind == 0) {
-        hipLaunchKernelGGL(k_srv_bfs_finalize, dim3(sweep_grid(V, 16384)), dim3(kBlock), 0, s, V, done, d->d_ptr,
-                           visit, d_acc);
-    } else {
-        hipLaunchKernelGGL(k_srv_sssp_finalize, dim3(sweep_grid(V * S / kFinRows, 16384)), dim3(kBlock), 0, s, V, S,
-                           done, d->d_ptr, dist, kl, d_acc);
-    }
+    hipLaunchKernelGGL(k_srv_bfs_finalize, dim3(sweep_grid(V, 16384)), dim3(kBlock), 0, s, V, done, d->d_ptr, visit,
+                       d_acc);
     OG_HIP(hipGetLastError());
     OG_HIP(hipMemcpyAsync(h_acc, d_acc, 3 * 64 * 8, hipMemcpyDeviceToHost, s));
     std::vector<std::pair<uint64_t, std::vector<uint32_t>>> vals;
-    for (uint32_t j = 0; j < S; ++j) {
+    for (uint32_t j = 0; j < kSlots; ++j) {
         if (kl.p[j]) {
             vals.push_back({slots[j].q.id, std::vector<uint32_t>(V)});
             OG_HIP(hipMemcpyAsync(vals.back().second.data(), kl.p[j], V * 4, hipMemcpyDeviceToHost, s));
@@ -520,12 +368,12 @@ void Engine::finalize(uint64_t done) {
     OG_HIP(hipStreamSynchronize(s));
     const double t = now_ms(srv);
     std::lock_guard<std::mutex> lk(srv->mu);
-    for (uint32_t j = 0; j < S; ++j) {
+    for (uint32_t j = 0; j < kSlots; ++j) {
         if (!((done >> j) & 1)) continue;
         og_query_result r{};
         r.id = slots[j].q.id;
         r.graph = gi;
-        r.kind = (uint32_t)kind;
+        r.kind = OG_QUERY_BFS;
         r.source = slots[j].q.source;
         r.status = OG_OK;
         r.checksum = h_acc[j];
@@ -544,52 +392,21 @@ void Engine::finalize(uint64_t done) {
     srv->cv_done.notify_all();
 }
 
-template <bool PK>
-static void launch_srv_sssp(Engine &e) {
-    DeviceState *d = e.d;
-    Frontier &f = e.f;
-    const uint32_t lpv0 = std::min<uint32_t>(std::max<uint32_t>(d->tune.lpv0, 4), 16);
-    // bin 0 (short rows): lane groups of lpv0 lanes; bin 1: one group per wave
-    switch (lpv0) {
-        case 4:
-            hipLaunchKernelGGL((k_srv_sssp_relax<4, PK>), dim3(2048), dim3(kBlock), 0, e.s, f.list, 0, f.info,
-                               d->d_ptr, d->d_static, d->wbits, e.S, e.front, e.dist, e.next, f.flag, e.thresh,
-                               e.g_near, e.g_heldmin);
-            break;
-        case 8:
-            hipLaunchKernelGGL((k_srv_sssp_relax<8, PK>), dim3(2048), dim3(kBlock), 0, e.s, f.list, 0, f.info,
-                               d->d_ptr, d->d_static, d->wbits, e.S, e.front, e.dist, e.next, f.flag, e.thresh,
-                               e.g_near, e.g_heldmin);
-            break;
-        default:
-            hipLaunchKernelGGL((k_srv_sssp_relax<16, PK>), dim3(2048), dim3(kBlock), 0, e.s, f.list, 0, f.info,
-                               d->d_ptr, d->d_static, d->wbits, e.S, e.front, e.dist, e.next, f.flag, e.thresh,
-                               e.g_near, e.g_heldmin);
-            break;
-    }
-    hipLaunchKernelGGL((k_srv_sssp_relax<64, PK>), dim3(2048), dim3(kBlock), 0, e.s, f.list, 1, f.info, d->d_ptr,
-                       d->d_static, d->wbits, e.S, e.front, e.dist, e.next, f.flag, e.thresh, e.g_near, e.g_heldmin);
-    hipLaunchKernelGGL((k_srv_sssp_hub<PK>), dim3(1024), dim3(kBlock), d->tune.tile * d->eb, e.s, f.hub, f.hub_cap,
-                       f.info, d->tune.tile, d->d_ptr, d->d_static, d->wbits, e.S, e.front, e.dist, e.next, f.flag,
-                       e.thresh, e.g_near, e.g_heldmin);
-    OG_HIP(hipGetLastError());
-}
-
-void Engine::step() {
+void BfsEngine::step() {
     SeedList sl{};
     {
         std::unique_lock<std::mutex> lk(srv->mu);
-        srv->cv_work.wait(lk, [&] { return srv->stop || used || !pending.empty(); });
+        srv->cv_work.wait(lk, [&] { return srv->stop || used || !pending->empty(); });
         if (srv->stop) return;
         const double t = now_ms(srv);
-        while (!pending.empty() && (uint32_t)__builtin_popcountll(used) < S) {
+        while (!pending->empty() && (uint32_t)__builtin_popcountll(used) < kSlots) {
             uint32_t j = 0;
             while ((used >> j) & 1) ++j;
             slots[j].used = true;
-            slots[j].q = pending.front();
+            slots[j].q = pending->front();
             slots[j].start_round = round;
             slots[j].t_admit = t;
-            pending.pop_front();
+            pending->pop_front();
             used |= 1ull << j;
             sl.slot[sl.n] = j;
             sl.src[sl.n] = slots[j].q.source;
@@ -599,57 +416,68 @@ void Engine::step() {
         }
     }
     const uint64_t V = d->V;
-    for (uint32_t i = 0; i < sl.n; ++i) {  // kept BFS levels are written by the commits
+    for (uint32_t i = 0; i < sl.n; ++i) {  // kept levels are written by the commits
         const uint32_t j = sl.slot[i];
-        if (kind == 0 && (slots[j].q.flags & OG_QUERY_KEEP_VALUES)) {
+        if (slots[j].q.flags & OG_QUERY_KEEP_VALUES) {
             if (!keep_buf[j]) keep_buf[j] = (uint32_t *)mem.dalloc(V * 4);
             OG_HIP(hipMemsetAsync(keep_buf[j], 0xFF, V * 4, s));
-        } else if (kind == 1 && (slots[j].q.flags & OG_QUERY_KEEP_VALUES) && !keep_buf[j]) {
-            keep_buf[j] = (uint32_t *)mem.dalloc(V * 4);
         }
     }
     if (sl.n) {
-        hipLaunchKernelGGL(k_srv_seed, dim3(1), dim3(64), 0, s, sl, kind, S, next, f.flag, dist, thresh, delta);
+        hipLaunchKernelGGL(k_srv_seed, dim3(1), dim3(64), 0, s, sl, next, f.flag, d_explored);
         OG_HIP(hipGetLastError());
     }
+    const bool dir_opt = d->d_in_ptr != nullptr;
     OG_HIP(hipMemsetAsync(d_alive, 0, 8, s));
     const uint64_t nchunks = d->Vpad / 16;
     SlotLevels L{};
-    for (uint32_t j = 0; j < S; ++j) L.lv[j] = round - slots[j].start_round + 1;
-    const KeepList kl = kind == 0 ? keep_list(used) : KeepList{};
-    if (kind == 0)
-        hipLaunchKernelGGL(k_srv_commit<0>, dim3(sweep_grid(nchunks, 4096)), dim3(kBlock), 0, s, f.flag, V, nchunks,
-                           d->d_ptr, visit, front, next, L, kl, d_acc, d_alive);
-    else
-        hipLaunchKernelGGL(k_srv_commit<1>, dim3(sweep_grid(nchunks, 4096)), dim3(kBlock), 0, s, f.flag, V, nchunks,
-                           d->d_ptr, visit, front, next, L, kl, d_acc, d_alive);
+    for (uint32_t j = 0; j < kSlots; ++j) L.lv[j] = round - slots[j].start_round + 1;
+    hipLaunchKernelGGL(k_srv_commit, dim3(sweep_grid(nchunks, 4096)), dim3(kBlock), 0, s, f.flag, V, nchunks,
+                       d->d_ptr, visit, front, next, L, keep_list(used), d_acc, dir_opt ? d_fstat : nullptr,
+                       d_alive);
     OG_HIP(hipGetLastError());
-    OG_HIP(hipMemcpyAsync(h_alive, d_alive, 8, hipMemcpyDeviceToHost, s));
+    if (dir_opt)
+        hipLaunchKernelGGL(k_srv_direction, dim3(1), dim3(64), 0, s, used, V, d->E, d_fstat, d_explored, d_pull);
+    OG_HIP(hipMemcpyAsync(h_alive, d_alive, 16, hipMemcpyDeviceToHost, s));
     // Counts stay on the device: the compaction, relax and clear kernels read
     // them from f.info, so the round needs no host round trip before its end.
     frontier_compact(d, f, s, nullptr, false);
-    if (kind == 0) {
-        RelaxArgs a{nullptr, f.flag, visit, front, next, d->wbits};
-        if (g->kind == OG_KIND_WEIGHTED) launch_static_relax<ALGO_MSBFS, true>(d, f, nullptr, a, s);
-        else launch_static_relax<ALGO_MSBFS, false>(d, f, nullptr, a, s);
-    } else if (d->packed) {
-        launch_srv_sssp<true>(*this);
-    } else {
-        launch_srv_sssp<false>(*this);
-    }
+    RelaxArgs a{nullptr, f.flag, visit, front, next, d->wbits, dir_opt ? (const uint64_t *)d_pull : nullptr};
+    if (g->kind == OG_KIND_WEIGHTED) launch_static_relax<ALGO_MSBFS, true>(d, f, nullptr, a, s);
+    else launch_static_relax<ALGO_MSBFS, false>(d, f, nullptr, a, s);
+    if (dir_opt)
+        hipLaunchKernelGGL(k_srv_bfs_pull, dim3(sweep_grid(V, 8192)), dim3(kBlock), 0, s, d->d_in_ptr, d->d_in_cols,
+                           V, visit, front, next, f.flag, d_pull);
     hipLaunchKernelGGL(k_srv_clear_front, dim3(1024), dim3(kBlock), 0, s, f.list, f.hub, f.hub_cap, f.info, front);
-    if (kind == 1) hipLaunchKernelGGL(k_srv_sssp_thresh, dim3(1), dim3(64), 0, s, S, used, delta, thresh, g_near,
-                                      g_heldmin);
     OG_HIP(hipGetLastError());
     OG_HIP(hipStreamSynchronize(s));
     ++round;
     ++rounds_run;
+    pull_rounds += h_alive[1] ? 1 : 0;
     // a slot is done when the round's commit left it no frontier vertex
     const uint64_t doneset = used & ~(uint64_t)*h_alive;
     if (doneset) finalize(doneset);
 }
 
-void Engine::loop() {
+void BfsEngine::fail_held(og_status st) {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    const double t = now_ms(srv);
+    for (uint32_t j = 0; j < kSlots; ++j)
+        if (slots[j].used) {
+            push_failure(srv, slots[j].q, gi, st, t);
+            slots[j].used = false;
+            srv->in_flight--;
+        }
+    for (auto &q : *pending) {
+        push_failure(srv, q, gi, st, t);
+        srv->pending--;
+    }
+    pending->clear();
+    used = 0;
+    srv->cv_done.notify_all();
+}
+
+void BfsEngine::loop() {
     try {
         OG_HIP(hipSetDevice(d->device));
         for (;;) {
@@ -661,30 +489,60 @@ void Engine::loop() {
         }
     } catch (const Error &e) {
         // fail everything this engine holds; the server keeps serving others
-        std::lock_guard<std::mutex> lk(srv->mu);
-        const double t = now_ms(srv);
-        auto fail_q = [&](const Query &q) {
-            og_query_result r{};
-            r.id = q.id;
-            r.graph = gi;
-            r.kind = (uint32_t)kind;
-            r.source = q.source;
-            r.status = e.st;
-            r.latency_ms = t - q.t_submit;
-            srv->done.push_back(r);
-            srv->completed++;
-        };
-        for (uint32_t j = 0; j < 64; ++j)
-            if (slots[j].used) {
-                fail_q(slots[j].q);
-                srv->in_flight--;
-            }
-        for (auto &q : pending) {
-            fail_q(q);
+        fail_held(e.st);
+    }
+}
+
+void SsspLane::loop() {
+    try {
+        OG_HIP(hipSetDevice(lane->device));
+    } catch (const Error &) {
+        return;
+    }
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(srv->mu);
+            srv->cv_work.wait(lk, [&] { return srv->stop || !pending->empty(); });
+            if (srv->stop) return;
+            cur = pending->front();
+            pending->pop_front();
             srv->pending--;
+            srv->in_flight++;
         }
-        pending.clear();
-        used = 0;
+        const double t_admit = now_ms(srv);
+        std::vector<uint32_t> vals;
+        og_run_stats st{};
+        og_status status = OG_OK;
+        try {
+            if (cur.flags & OG_QUERY_KEEP_VALUES) vals.resize(lane->V);
+            lane_run(g, lane, ALGO_SSSP, cur.source, vals.empty() ? nullptr : vals.data(), &st);
+        } catch (const Error &e) {
+            status = e.st;
+        } catch (...) {
+            status = OG_ENOMEM_HOST;
+        }
+        const double t = now_ms(srv);
+        std::lock_guard<std::mutex> lk(srv->mu);
+        og_query_result r{};
+        r.id = cur.id;
+        r.graph = gi;
+        r.kind = OG_QUERY_SSSP;
+        r.source = cur.source;
+        r.status = status;
+        if (status == OG_OK) {
+            r.checksum = st.checksum;
+            r.reached = st.reached;
+            r.traversed_edges = st.traversed_edges;
+            r.rounds = st.iterations;
+            if (!vals.empty()) srv->kept[cur.id] = std::move(vals);
+        }
+        r.queue_ms = t_admit - cur.t_submit;
+        r.latency_ms = t - cur.t_submit;
+        srv->done.push_back(r);
+        srv->completed++;
+        srv->in_flight--;
+        ++runs;
+        rounds_run += st.iterations;
         srv->cv_done.notify_all();
     }
 }
@@ -695,9 +553,13 @@ static void server_free(og_server *srv) {
         srv->stop = true;
     }
     srv->cv_work.notify_all();
-    for (auto &e : srv->engines)
+    for (auto &e : srv->bfs)
         if (e && e->th.joinable()) e->th.join();
-    srv->engines.clear();
+    for (auto &ls : srv->lanes)
+        for (auto &l : ls)
+            if (l && l->th.joinable()) l->th.join();
+    srv->bfs.clear();
+    srv->lanes.clear();
     for (og_graph *g : srv->graphs) g->busy.store(0);
     delete srv;
 }
@@ -732,18 +594,14 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
                 raise(OG_EBUSY, "graph %u is running another call", locked);
             srv->graphs.push_back(graphs[locked]);
         }
-        // Admission: per-query state is the slot bits of the frontier masks
-        // (24 V bytes per BFS engine: visit / front / next; 16 V per SSSP
-        // engine: front / next) plus one 4 V distance column per SSSP slot.
-        // Every engine gets the same power-of-two slot count S <= 64, the
-        // largest whose state fits the budget.
-        uint64_t fixed = 0, per_slot = 0;
+        // Admission: the BFS engines' frontier masks (24 V bytes per graph:
+        // visit / front / next, 64 slots) and, per weighted graph, K SSSP
+        // lanes of about 17 V bytes each (values, flags, list, degree
+        // prefix): K is the largest of 4, 2, 1 that fits the budget.
+        uint64_t fixed = 0, per_lane = 0;
         for (og_graph *g : srv->graphs) {
             fixed += 24 * g->V;
-            if (g->kind == OG_KIND_WEIGHTED) {
-                fixed += 16 * g->V;
-                per_slot += 4 * g->V;
-            }
+            if (g->kind == OG_KIND_WEIGHTED) per_lane += lane_bytes(g->V);
         }
         uint64_t budget = srv->params.state_budget_bytes;
         if (!budget) {
@@ -751,60 +609,62 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             OG_HIP(hipMemGetInfo(&fr, &tot));
             budget = fr / 4;
         }
-        uint32_t S = 64;
-        while (S > 0 && fixed + (uint64_t)S * per_slot > budget) S >>= 1;
-        if (S == 0)
+        uint32_t K = kMaxLanes;
+        while (K > 0 && fixed + (uint64_t)K * per_lane > budget) K >>= 1;
+        if (fixed > budget || (per_lane && K == 0))
             raise(OG_EBUDGET,
-                  "state budget %llu bytes holds no query slot (%llu bytes of masks + %llu per slot across engines)",
-                  (unsigned long long)budget, (unsigned long long)fixed, (unsigned long long)per_slot);
-        srv->engines.resize(2 * num_graphs);
+                  "state budget %llu bytes holds no query state (%llu bytes of BFS masks + %llu per SSSP lane set)",
+                  (unsigned long long)budget, (unsigned long long)fixed, (unsigned long long)per_lane);
+        srv->queues.resize(2 * num_graphs);
+        srv->bfs.resize(num_graphs);
+        srv->lanes.resize(num_graphs);
         for (uint32_t gi = 0; gi < num_graphs; ++gi) {
             og_graph *g = srv->graphs[gi];
-            for (int kind = 0; kind < 2; ++kind) {
-                if (kind == 1 && g->kind != OG_KIND_WEIGHTED) continue;
-                std::unique_ptr<Engine> e(new Engine);
-                e->srv = srv;
-                e->g = g;
-                e->d = g->dev;
-                e->gi = gi;
-                e->kind = kind;
-                e->S = S;
-                e->delta = g->dev->tune.delta ? g->dev->tune.delta : kServeDelta;
-                const uint64_t V = g->V;
-                OG_HIP(hipSetDevice(g->dev->device));
-                OG_HIP(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
-                frontier_alloc(g->dev, g, e->f, e->mem);
-                e->front = (uint64_t *)e->mem.dalloc(V * 8);
-                e->next = (uint64_t *)e->mem.dalloc(V * 8);
-                OG_HIP(hipMemset(e->front, 0, V * 8));
-                OG_HIP(hipMemset(e->next, 0, V * 8));
-                if (kind == 0) {
-                    e->visit = (uint64_t *)e->mem.dalloc(V * 8);
-                    OG_HIP(hipMemset(e->visit, 0, V * 8));
-                    e->state_bytes = 24 * V;
-                } else {
-                    e->dist = (uint32_t *)e->mem.dalloc(V * 4 * S);
-                    OG_HIP(hipMemset(e->dist, 0xFF, V * 4 * S));
-                    e->thresh = (uint32_t *)e->mem.dalloc(64 * 4);
-                    e->g_near = (uint32_t *)e->mem.dalloc(64 * 4);
-                    e->g_heldmin = (uint32_t *)e->mem.dalloc(64 * 4);
-                    OG_HIP(hipMemset(e->thresh, 0xFF, 64 * 4));
-                    OG_HIP(hipMemset(e->g_near, 0, 64 * 4));
-                    OG_HIP(hipMemset(e->g_heldmin, 0xFF, 64 * 4));
-                    e->state_bytes = 16 * V + 4 * V * S;
-                }
-                e->d_alive = (unsigned long long *)e->mem.dalloc(8);
-                e->h_alive = (unsigned long long *)e->mem.palloc(8);
-                e->d_acc = (unsigned long long *)e->mem.dalloc(3 * 64 * 8);
-                e->h_acc = (unsigned long long *)e->mem.palloc(3 * 64 * 8);
-                OG_HIP(hipMemset(e->d_acc, 0, 3 * 64 * 8));
-                srv->state_bytes += e->state_bytes;
-                srv->engines[2 * gi + kind] = std::move(e);
+            const uint64_t V = g->V;
+            OG_HIP(hipSetDevice(g->dev->device));
+            std::unique_ptr<BfsEngine> e(new BfsEngine);
+            e->srv = srv;
+            e->g = g;
+            e->d = g->dev;
+            e->gi = gi;
+            e->pending = &srv->queues[2 * gi];
+            OG_HIP(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
+            frontier_alloc(g->dev, g, e->f, e->mem);
+            e->visit = (uint64_t *)e->mem.dalloc(V * 8);
+            e->front = (uint64_t *)e->mem.dalloc(V * 8);
+            e->next = (uint64_t *)e->mem.dalloc(V * 8);
+            OG_HIP(hipMemset(e->visit, 0, V * 8));
+            OG_HIP(hipMemset(e->front, 0, V * 8));
+            OG_HIP(hipMemset(e->next, 0, V * 8));
+            e->d_alive = (unsigned long long *)e->mem.dalloc(16);  // [0] alive slots, [1] pulling slots
+            e->h_alive = (unsigned long long *)e->mem.palloc(16);
+            e->d_pull = e->d_alive + 1;
+            e->d_fstat = (unsigned long long *)e->mem.dalloc(2 * 64 * 8);
+            e->d_explored = (unsigned long long *)e->mem.dalloc(64 * 8);
+            OG_HIP(hipMemset(e->d_alive, 0, 16));
+            OG_HIP(hipMemset(e->d_fstat, 0, 2 * 64 * 8));
+            OG_HIP(hipMemset(e->d_explored, 0, 64 * 8));
+            e->d_acc = (unsigned long long *)e->mem.dalloc(3 * 64 * 8);
+            e->h_acc = (unsigned long long *)e->mem.palloc(3 * 64 * 8);
+            OG_HIP(hipMemset(e->d_acc, 0, 3 * 64 * 8));
+            srv->state_bytes += 24 * V;
+            srv->bfs[gi] = std::move(e);
+            if (g->kind != OG_KIND_WEIGHTED) continue;
+            for (uint32_t k = 0; k < K; ++k) {
+                std::unique_ptr<SsspLane> l(new SsspLane);
+                l->srv = srv;
+                l->g = g;
+                l->gi = gi;
+                l->pending = &srv->queues[2 * gi + 1];
+                l->lane = lane_state_create(g);
+                srv->state_bytes += l->lane->device_bytes();
+                srv->lanes[gi].push_back(std::move(l));
             }
         }
         OG_HIP(hipDeviceSynchronize());
-        for (auto &e : srv->engines)
-            if (e) e->th = std::thread(&Engine::loop, e.get());
+        for (auto &e : srv->bfs) e->th = std::thread(&BfsEngine::loop, e.get());
+        for (auto &ls : srv->lanes)
+            for (auto &l : ls) l->th = std::thread(&SsspLane::loop, l.get());
         return OG_OK;
     });
     if (st != OG_OK) {
@@ -824,8 +684,8 @@ extern "C" og_status og_server_submit(og_server *srv, uint32_t graph, uint32_t k
     if (!srv) return fail(OG_EINVAL, "server is NULL");
     if (graph >= srv->graphs.size()) return fail(OG_EINVAL, "graph index %u out of range", graph);
     if (kind > 1) return fail(OG_EINVAL, "kind must be OG_QUERY_BFS or OG_QUERY_SSSP");
-    Engine *e = srv->engines[2 * graph + kind].get();
-    if (!e) return fail(OG_EUNSUPPORTED, "graph %u has no weights: SSSP queries are not served", graph);
+    if (kind == OG_QUERY_SSSP && srv->lanes[graph].empty())
+        return fail(OG_EUNSUPPORTED, "graph %u has no weights: SSSP queries are not served", graph);
     if (source >= srv->graphs[graph]->V) return fail(OG_EINVAL, "source %u >= V", source);
     std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->stop) return fail(OG_ESTATE, "server is stopping");
@@ -835,7 +695,7 @@ extern "C" og_status og_server_submit(og_server *srv, uint32_t graph, uint32_t k
         return fail(OG_EBUSY, "pending queue full (%u queries)", cap);
     }
     Query q{srv->next_id++, graph, kind, source, flags, now_ms(srv)};
-    e->pending.push_back(q);
+    srv->queues[2 * graph + kind].push_back(q);
     srv->pending++;
     srv->submitted++;
     if (id) *id = q.id;
@@ -880,10 +740,10 @@ extern "C" og_status og_server_get_info(og_server *srv, og_server_info *info) {
     std::memset(info, 0, sizeof *info);
     info->num_graphs = (uint32_t)srv->graphs.size();
     for (uint32_t gi = 0; gi < srv->graphs.size(); ++gi) {
-        if (srv->engines[2 * gi]) info->bfs_slots[gi] = srv->engines[2 * gi]->S;
-        if (srv->engines[2 * gi + 1]) info->sssp_slots[gi] = srv->engines[2 * gi + 1]->S;
-        for (int k = 0; k < 2; ++k)
-            if (srv->engines[2 * gi + k]) info->rounds += srv->engines[2 * gi + k]->rounds_run;
+        info->bfs_slots[gi] = kSlots;
+        info->sssp_slots[gi] = (uint32_t)srv->lanes[gi].size();
+        info->rounds += srv->bfs[gi]->rounds_run;
+        for (auto &l : srv->lanes[gi]) info->rounds += l->rounds_run;
     }
     info->state_bytes = srv->state_bytes;
     info->submitted = srv->submitted;

@@ -18,6 +18,7 @@ PLANS = [
     ("ascetic", 0.25, 0.0),    # reference auto-advisek rule
     ("ascetic+zerocopy", 0.25, 0.8),  # dynamic edges read in place over the link
     ("ascetic+zerocopy", 1 / 64, 0.5),  # small HBM-resident head, most dynamic reads over the link
+    ("ascetic+zerocopy+nopack", 1 / 8, 0.5),  # 8-byte rows read in place
     ("ascetic+zerocopy+nograph", 0.25, 0.8),  # device-driven rounds enqueued eagerly
     ("ascetic+nograph", 1.0, 1.0),
     ("ascetic+nopack", 1 / 64, 0.5),  # 8-byte weighted edges in HBM and on the link

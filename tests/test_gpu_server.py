@@ -1,6 +1,6 @@
 """Continuous-batching server: each query's result is the same alone and in
-the mix, equals the single-query path and the oracle; admission and
-reservation rules hold."""
+the mix (BFS slots batched with others, SSSP on concurrent lanes), equals the
+single-query path and the oracle; admission and reservation rules hold."""
 import sys
 import time
 
@@ -18,27 +18,30 @@ def _say(*a):
     sys.stdout.flush()
 
 
-@pytest.fixture(scope="module")
-def graphs():
+@pytest.fixture(scope="module", params=["topdown", "diropt"])
+def graphs(request):
+    """Three graphs; "diropt" prepares them with their in-edges, so BFS slots
+    with dense frontiers run bottom-up rounds."""
     gs = [og.Graph.rmat(12, 16, seed=51, weight_seed=52), og.Graph.er(3000, 8, seed=53).add_weights(54),
           og.Graph.rmat(11, 8, seed=55)]
+    flags = og.PLAN_DIRECTION_OPT if request.param == "diropt" else 0
     for g in gs:
         g.prepare("ascetic", edge_budget_bytes=g.num_edges * (8 if g.info()["kind"] == og.KIND_WEIGHTED else 4),
-                  advisek=1.0)
+                  advisek=1.0, flags=flags)
     yield gs
     for g in gs:
         g.close()
 
 
-def _state_bytes(graphs, S):
-    """Admission rule of og_server_create: frontier masks (24 V per BFS
-    engine, 16 V per SSSP engine) + one 4 V distance column per SSSP slot."""
+def _state_bytes(graphs, K):
+    """Admission rule of og_server_create: the BFS engines' masks (24 V bytes
+    per graph) + K SSSP lanes of 17 V bytes per weighted graph."""
     total = 0
     for g in graphs:
         V = g.num_vertices
         total += 24 * V
         if g.info()["kind"] == og.KIND_WEIGHTED:
-            total += 16 * V + 4 * V * S
+            total += 17 * V * K
     return total
 
 
@@ -74,10 +77,11 @@ def _oracle(graphs, gi, kind, src):
 def test_alone_and_in_the_mix(gpu, graphs):
     qs = _queries(graphs, 60, 7)
     _say("server mix of", len(qs))
-    # small budget: 8 slots per engine, so queries queue and join mid-flight
-    with og.Server(graphs, state_budget_bytes=_state_bytes(graphs, 8)) as srv:
+    # small budget: one SSSP lane per weighted graph, so SSSP queries queue;
+    # BFS queries join the 64-slot batches mid-flight
+    with og.Server(graphs, state_budget_bytes=_state_bytes(graphs, 1)) as srv:
         info = srv.info()
-        assert info["bfs_slots"] == [8, 8, 8] and info["sssp_slots"] == [8, 8, 0]
+        assert info["bfs_slots"] == [64, 64, 64] and info["sssp_slots"] == [1, 1, 0]
         ids = [srv.submit(gi, k, s, keep=True) for gi, k, s in qs]
         res = _wait_all(srv, ids)
         mix = {i: srv.take_values(i, qs[n][0]) for n, i in enumerate(ids)}
@@ -121,8 +125,8 @@ def test_admission_and_reservation(gpu, graphs):
         og.Server([g], state_budget_bytes=_state_bytes([g], 1) - 1)
     assert e.value.name == "OG_EBUDGET"
     srv = og.Server([g], state_budget_bytes=_state_bytes([g], 2) + 1, max_pending=4)
-    assert srv.info()["bfs_slots"] == [2] and srv.info()["sssp_slots"] == [2]
-    assert srv.info()["state_bytes"] == _state_bytes([g], 2)
+    assert srv.info()["bfs_slots"] == [64] and srv.info()["sssp_slots"] == [2]
+    assert srv.info()["state_bytes"] >= _state_bytes([g], 2)
     with pytest.raises(og.OgError) as e:  # reserved while served
         g.bfs(0)
     assert e.value.name == "OG_EBUSY"
