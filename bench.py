@@ -78,7 +78,21 @@ def base_record(metric_value, unit, steps, warmup, ms, dtype, config, higher=Tru
             "dtype": dtype, "data": "synthetic", "config": config}
 
 
-def roofline_of(stats_list):
+KERNEL_NAMES = {"relax_static": "k_relax_group / k_relax_hub (static region)",
+                "relax_dynamic": "k_relax_dyn (dynamic region)", "frontier": "k_frontier_count/scan/write"}
+
+
+def roofline_of(stats_list, zero_copy=False, traffic=None):
+    """Roofline of the dominant kernel class (most HIP-event time).
+
+    In HBM (static relax, frontier, a staged dynamic relax reading its slot)
+    the bound is HBM: achieved = algorithmic bytes / kernel time against
+    8 TB/s.  A zero-copy dynamic relax reads its edges from pinned host memory
+    over the link, so its bound is the host link: achieved = edge bytes read
+    over the link / kernel time against 63 GB/s; its HBM figures are kept in
+    "hbm".  `traffic`: memory-system bytes per launch of that class measured
+    by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy; host
+    reads included), when a committed profile of the same workload has them."""
     agg = {}
     for key in ["relax_static_kernel_ms", "relax_static_launches", "relax_static_bytes", "relax_dynamic_kernel_ms",
                 "relax_dynamic_launches", "relax_dynamic_bytes", "frontier_kernel_ms", "frontier_launches",
@@ -89,13 +103,28 @@ def roofline_of(stats_list):
     if kt:
         dom = max(kt, key=lambda c: kt[c]["ms"])
         r = kt[dom]
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(r["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_us": round(r["avg_us"], 3),
-                "bytes_per_launch": round(r["bytes_per_launch"], 1)}
+        hbm = {"achieved": round(r["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "bytes_per_launch": round(r["bytes_per_launch"], 1),
+               "traffic": traffic}
+        if dom == "relax_dynamic" and zero_copy and agg["h2d_bytes"]:
+            gbs = agg["h2d_bytes"] / (r["ms"] * 1e-3) / 1e9
+            roof = {"bound": "host-link", "kernel": KERNEL_NAMES[dom], "achieved": round(gbs, 2),
+                    "peak": LINK_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LINK_PEAK_GBS, 4),
+                    "traffic": traffic, "link_bytes_per_launch": round(agg["h2d_bytes"] / r["launches"], 1),
+                    "avg_launch_us": round(r["avg_us"], 3), "hbm": hbm}
+        else:
+            roof = {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": hbm["achieved"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": hbm["frac"], "traffic": traffic, "avg_launch_us": round(r["avg_us"], 3),
+                    "bytes_per_launch": hbm["bytes_per_launch"]}
     link = None
     if agg["copy_ms"] > 0:
         gbs = agg["h2d_bytes"] / (agg["copy_ms"] * 1e-3) / 1e9
-        link = {"achieved": round(gbs, 2), "peak": LINK_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LINK_PEAK_GBS, 4)}
+        link = {"achieved": round(gbs, 2), "peak": LINK_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LINK_PEAK_GBS, 4),
+                "measured_on": "copy stream (hipMemcpyAsync)"}
+    elif zero_copy and agg["h2d_bytes"] and agg["relax_dynamic_kernel_ms"] > 0:
+        gbs = agg["h2d_bytes"] / (agg["relax_dynamic_kernel_ms"] * 1e-3) / 1e9
+        link = {"achieved": round(gbs, 2), "peak": LINK_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LINK_PEAK_GBS, 4),
+                "measured_on": "k_relax_dyn (direct reads)"}
     return roof, link, kt
 
 
@@ -176,6 +205,8 @@ def wl_cc_ooc(args):
     ptr, cols, _, _ = g.export()
     V, E = len(ptr), len(cols)
     plan = g.prepare("ascetic", edge_budget_bytes=E, advisek=0.8, flags=plan_flags(args, "ascetic"))
+    if args.dyn_grid is not None:
+        g.set_tuning(dyn_grid=args.dyn_grid)
     stats, wall = timed(lambda: g.cc(want_values=False)[1], args.steps, args.warmup, "cc-ooc")
     lab, _ = g.cc()
     ms = 1e3 * wall / args.steps
@@ -189,7 +220,7 @@ def wl_cc_ooc(args):
     t = time.perf_counter()
     ref, _ = oracle.cc(ptr, cols)
     cpu_s = time.perf_counter() - t
-    roof, link, kt = roofline_of(stats)
+    roof, link, kt = roofline_of(stats, zero_copy=args.loader == "zerocopy")
     out = base_record(round(E / (ms * 1e-3) / 1e9, 4), "GTEPS", args.steps, args.warmup, ms, "u32",
                       {"workload": f"cc out-of-memory er n=2^{args.scale} avg 32", "V": V, "E": E,
                        "edge_budget_bytes": E, "advisek": 0.8, "approach": "ascetic", "loader": args.loader,
@@ -265,6 +296,8 @@ def wl_msbfs(args):
     deg = np.diff(np.append(ptr.astype(np.int64), E))
     srcs = np.random.default_rng(3004).choice(np.flatnonzero(deg > 0), args.sources, replace=False).astype(np.uint32)
     plan = g.prepare("ascetic", edge_budget_bytes=E * 4 // 2, advisek=0.8, flags=plan_flags(args, "ascetic"))
+    if args.dyn_grid is not None:
+        g.set_tuning(dyn_grid=args.dyn_grid)
     log(f"plan {plan}")
     if args.warmup:
         g.msbfs(srcs[:64], want_levels=False)
@@ -292,7 +325,7 @@ def wl_msbfs(args):
                       {"workload": f"multi-source bfs rmat{args.scale} ef{args.edge_factor} {len(srcs)} sources",
                        "V": V, "E": E, "edge_budget_bytes": E * 4 // 2, "advisek": 0.8, "approach": "ascetic",
                        "loader": args.loader, "batch": 64})
-    roof, link, kt = roofline_of([st])
+    roof, link, kt = roofline_of([st], zero_copy=args.loader == "zerocopy")
     out.update({"queries_per_s": round(qps, 2), "plain_queries_per_s": round(1 / per_plain, 3),
                 "speedup_vs_plain": round(qps * per_plain, 2),
                 "plain_sample": len(sample), "plain_gteps": round(np.mean([p["traversed_edges"] for p in plain]) /
@@ -504,6 +537,10 @@ def main():
     ap.add_argument("--diropt", action="store_true", help="bfs-hbm: direction-optimising BFS (in-edge CSR in HBM)")
     ap.add_argument("--no-overlap", action="store_true", help="serialise copies, static and dynamic relax")
     ap.add_argument("--sweep-delta", default="", help="comma list: time SSSP at each delta and report")
+    ap.add_argument("--dyn-grid", type=int, default=None, help="workgroups of the dynamic-region relax (0: 8192)")
+    ap.add_argument("--sweep-dyn-grid", default="", help="comma list: time SSSP at each dyn_grid and report")
+    ap.add_argument("--link-probe", action="store_true",
+                    help="only measure shader reads of pinned host memory and hipMemcpy (host-link bound)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--workload", default="sssp-ooc",
@@ -544,6 +581,10 @@ def main():
     if og.device_count() < 1:
         print("bench.py needs a GPU: no HIP device is visible", file=sys.stderr)
         sys.exit(1)
+    if args.link_probe:
+        # what the host link gives shader reads (the zero-copy loader's bound)
+        print(json.dumps({"link_probe": og.probe_host_link(1 << 30), "peak_gbs": LINK_PEAK_GBS}))
+        return
     if args.workload != "sssp-ooc":
         WORKLOADS[args.workload](args)
         return
@@ -565,9 +606,17 @@ def main():
             g.set_tuning(delta=args.delta)
         if args.defer is not None:
             g.set_tuning(defer=args.defer)
+        if args.dyn_grid is not None:
+            g.set_tuning(dyn_grid=args.dyn_grid)
         if args.autotune:
             best, best_ms = g.autotune(og.TUNE_SSSP, src, runs=1)
             log(f"{approach}: autotuned {best} ({best_ms:.1f} ms)")
+        if args.sweep_dyn_grid:
+            for dg in [int(x) for x in args.sweep_dyn_grid.split(",")]:
+                g.set_tuning(dyn_grid=dg)
+                ts = [g.sssp(src, want_values=False)[1] for _ in range(3)]
+                log(f"{approach}: dyn_grid {dg}: {min(t['total_ms'] for t in ts):.2f} ms")
+            g.set_tuning(dyn_grid=args.dyn_grid or 0)
         if args.sweep_delta:
             for dl in [int(x) for x in args.sweep_delta.split(",")]:
                 g.set_tuning(delta=dl)
@@ -600,39 +649,34 @@ def main():
     # weighted edges in HBM and on the link: packed (to << 6 | w) or the CSR's {to, w}
     edge_format = "8-byte {to,w}" if args.no_pack else "4-byte to<<6|w"
     agg = {}
-    for key in ["relax_static_kernel_ms", "relax_static_launches", "relax_static_bytes", "relax_dynamic_kernel_ms",
-                "relax_dynamic_launches", "relax_dynamic_bytes", "frontier_kernel_ms", "frontier_launches",
-                "frontier_bytes", "copy_ms", "h2d_bytes", "gather_ms", "chunks", "iterations", "work_edges"]:
+    for key in ["iterations", "work_edges", "h2d_bytes", "chunks", "gather_ms", "copy_ms"]:
         agg[key] = sum(s[key] for s in stats)
-    kt = kernel_table(agg)
-    dom = max(kt, key=lambda c: kt[c]["ms"]) if kt else None
-    roof = None
-    if dom:
-        r = kt[dom]
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(r["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_launch_us": round(r["avg_us"], 3), "bytes_per_launch": round(r["bytes_per_launch"], 1)}
-        # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-        # (calibrated against a plain copy) for this kernel, workload and setting.
-        tpath = os.path.join(HERE, "profiles", "k_relax_dyn_sssp_rmat24_traffic.json")
-        if os.path.exists(tpath):
-            with open(tpath) as f:
-                tr = json.load(f)
-            want = {"scale": args.scale, "approach": args.approach, "loader": args.loader, "delta": args.delta,
-                    "kernel_class": dom, "edge_format": edge_format}
-            if tr.get("config") == want:
-                roof["traffic"] = tr["traffic_bytes_per_launch"]
-                roof["traffic_source"] = tr["source"]
-    link = None
-    if agg["copy_ms"] > 0:
-        gbs = agg["h2d_bytes"] / (agg["copy_ms"] * 1e-3) / 1e9
-        link = {"achieved": round(gbs, 2), "peak": LINK_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LINK_PEAK_GBS, 4),
-                "bytes_per_step": agg["h2d_bytes"] / args.steps, "measured_on": "copy stream (hipMemcpyAsync)"}
-    elif agg["h2d_bytes"] and agg["relax_dynamic_kernel_ms"] > 0:
-        # zero-copy loader: the dynamic relax kernel itself reads the link
-        gbs = agg["h2d_bytes"] / (agg["relax_dynamic_kernel_ms"] * 1e-3) / 1e9
-        link = {"achieved": round(gbs, 2), "peak": LINK_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LINK_PEAK_GBS, 4),
-                "bytes_per_step": agg["h2d_bytes"] / args.steps, "measured_on": "k_relax_dyn (direct reads)"}
+    # HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE (calibrated against a plain copy) on this
+    # workload and setting, when a committed profile holds them
+    traffic = traffic_src = None
+    tpath = os.path.join(HERE, "profiles", "k_relax_dyn_sssp_rmat24_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        want = {"scale": args.scale, "approach": args.approach, "loader": args.loader, "delta": args.delta,
+                "kernel_class": "relax_dynamic", "edge_format": edge_format}
+        if tr.get("config") == want:
+            traffic, traffic_src = tr["traffic_bytes_per_launch"], tr["source"]
+    roof, link, kt = roofline_of(stats, zero_copy=args.loader == "zerocopy" and args.approach != "usm",
+                                 traffic=traffic)
+    if roof and traffic_src:
+        roof["traffic_source"] = traffic_src
+    if link:
+        link["bytes_per_step"] = agg["h2d_bytes"] / args.steps
+    # the in-HBM static relax beside it (its own roofline)
+    static = None
+    if "relax_static" in kt:
+        r = kt["relax_static"]
+        static = {"bound": "hbm", "achieved": round(r["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "avg_launch_us": round(r["avg_us"], 3),
+                  "note": "overlaps the dynamic relax on another stream, so its launch times include sharing "
+                          "the CUs with it"}
 
     out = {
         "metric": METRIC,
@@ -658,6 +702,7 @@ def main():
             "edge_format": edge_format,
         },
         "roofline": roof,
+        "static_region": static,
         "host_link": link,
         "kernels": kt,
         "per_step": {"iterations": agg["iterations"] / args.steps, "work_edges": agg["work_edges"] / args.steps,

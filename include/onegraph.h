@@ -280,6 +280,8 @@ typedef struct og_tuning {
                         vertex relaxes only after the static vertices of its
                         near-far bucket have settled, so its link row is
                         read about once                                          */
+    uint32_t dyn_grid; /* workgroups of the dynamic-region relax, 0 = 8192; fewer
+                        leave CUs to the static relax it overlaps with           */
 } og_tuning;
 
 og_status og_get_tuning(const og_graph *g, og_tuning *t);
@@ -292,8 +294,8 @@ og_status og_set_tuning(og_graph *g, const og_tuning *t);
 
 /* Autotuner: times `algo` from `source` on the prepared graph under
  * candidate tunings (coordinate descent: defer for SSSP / CC with a dynamic
- * region, delta for SSSP, then t1 with lpv0, t2, tile; the median of `runs`
- * timed runs each), installs the fastest and
+ * region, delta for SSSP, then t1 with lpv0, t2, tile, and dyn_grid with a
+ * dynamic region; the median of `runs` timed runs each), installs the fastest and
  * returns it with its time.  Results do not depend on the tuning. */
 og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, uint32_t runs,
                       og_tuning *best, double *best_ms);
@@ -307,6 +309,15 @@ og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, uint32_t runs
 og_status og_msbfs(og_graph *g, const uint32_t *sources, uint32_t num_sources,
                    uint32_t *levels, uint64_t *checksums, uint64_t *reached,
                    og_run_stats *stats);
+
+/* Host-link probe (no reference counterpart: a measurement helper for the
+ * roofline of the zero-copy loader).  Allocates `bytes` of pinned host
+ * memory and times, over 3 repetitions each: shader reads of it through the
+ * device mapping, contiguous (*contig_gbs); the same reads over rows of 1-19
+ * words scattered with about a fifth of the array taken (*scattered_gbs,
+ * useful bytes per second: the access pattern of a sparse dynamic region);
+ * and hipMemcpy host-to-device of the whole buffer (*memcpy_gbs). */
+og_status og_probe_host_link(uint64_t bytes, double *contig_gbs, double *scattered_gbs, double *memcpy_gbs);
 
 /* Digest of a BFS/SSSP result array: sum mod 2^64 of mix64(v << 32 | x[v])
  * over entries x[v] != OG_UNREACHED (mix64 = SplitMix64 finaliser).  The

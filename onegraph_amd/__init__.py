@@ -70,7 +70,7 @@ class GraphInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("t1", C.c_uint32), ("lpv0", C.c_uint32), ("t2", C.c_uint32), ("tile", C.c_uint32),
-                ("delta", C.c_uint32), ("defer", C.c_uint32)]
+                ("delta", C.c_uint32), ("defer", C.c_uint32), ("dyn_grid", C.c_uint32)]
 
 
 TUNE_BFS, TUNE_SSSP, TUNE_CC = 0, 1, 2
@@ -161,6 +161,8 @@ def lib():
         "og_cc": (C.c_int, [_P, _u32p, C.POINTER(RunStats)]),
         "og_msbfs": (C.c_int, [_P, _u32p, C.c_uint32, _u32p, _u64p, _u64p, C.POINTER(RunStats)]),
         "og_levels_checksum": (C.c_int, [_u32p, C.c_uint64, _u64p]),
+        "og_probe_host_link": (C.c_int, [C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]),
         "og_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
         "og_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
         "og_autotune": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Tuning), C.POINTER(C.c_double)]),
@@ -206,6 +208,14 @@ def levels_checksum(values):
     out = C.c_uint64()
     _check(lib().og_levels_checksum(_u32(v), len(v), C.byref(out)))
     return out.value
+
+
+def probe_host_link(nbytes=1 << 30):
+    """GB/s of shader reads of pinned host memory (contiguous and scattered
+    short rows) and of hipMemcpy host-to-device, on the current device."""
+    c, sc, mc = C.c_double(), C.c_double(), C.c_double()
+    _check(lib().og_probe_host_link(nbytes, C.byref(c), C.byref(sc), C.byref(mc)))
+    return {"contiguous_gbs": c.value, "scattered_rows_gbs": sc.value, "memcpy_gbs": mc.value}
 
 
 def version():

@@ -25,7 +25,7 @@ constexpr int kBlock = 256;             // threads per block of the frontier ker
 constexpr int kFlagVecs = 2;            // uint4 flag loads per thread
 constexpr int kFlagsPerThread = 16 * kFlagVecs;
 constexpr int kVertsPerBlock = kBlock * kFlagsPerThread;   // 8192
-constexpr int kDynEdgesPerThread = 4;   // dynamic relax: edges per lane
+constexpr int kDynEdgesPerThread = 8;   // dynamic relax: edges per lane
 constexpr int kDynEdgesPerBlock = kBlock * kDynEdgesPerThread;
 constexpr int kMaxSlots = 16;           // staging slots of the dynamic-region pipeline
 constexpr uint64_t kSlotTargetBytes = 256ull << 20;  // preferred slot (chunk) size
@@ -56,6 +56,7 @@ struct Tuning {
     uint32_t defer = 0;    // SSSP / CC: 1 = hold dynamic vertices back while static ones are active;
                            // SSSP with delta > 0: 2 = dynamic vertices relax only after the
                            // static part of their near-far bucket has settled
+    uint32_t dyn_grid = 0; // workgroups of the dynamic-region relax (0: kDynGridCap)
 };
 
 // Frontier summary produced on the device each iteration and read by the host.

@@ -1313,7 +1313,7 @@ static RelaxArgs relax_args(DeviceState *d) {
 // grid that fills the chip (256 CUs) is launched.
 constexpr unsigned kGroupGridCap = 2048;  // workgroups of k_relax_group
 constexpr unsigned kHubGridCap = 1024;    // workgroups of k_relax_hub
-constexpr unsigned kDynGridCap = 4096;    // workgroups of k_relax_dyn
+constexpr unsigned kDynGridCap = 8192;    // workgroups of k_relax_dyn (zero-copy reads want many in flight)
 
 static unsigned capped(uint64_t blocks, unsigned cap) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, cap));
@@ -1393,8 +1393,8 @@ static void launch_dyn(DeviceState *d, unsigned grid, const void *src, uint64_t 
 // knows the edge count (exact grid), else 0 (fixed grid).
 template <int ALGO, bool W>
 static void launch_dyn_any(DeviceState *d, const void *src, uint64_t c0, uint64_t c1, uint64_t known, bool direct) {
-    const unsigned grid = known ? capped((known + kDynEdgesPerBlock - 1) / kDynEdgesPerBlock, kDynGridCap)
-                                : kDynGridCap;
+    const unsigned cap = d->tune.dyn_grid ? d->tune.dyn_grid : kDynGridCap;
+    const unsigned grid = known ? capped((known + kDynEdgesPerBlock - 1) / kDynEdgesPerBlock, cap) : cap;
     if (d->packed) {
         if (direct) launch_dyn<ALGO, W, true, true>(d, grid, src, c0, c1);
         else launch_dyn<ALGO, W, true, false>(d, grid, src, c0, c1);
@@ -1853,6 +1853,100 @@ extern "C" og_status og_msbfs(og_graph *g, const uint32_t *sources, uint32_t num
     return st;
 }
 
+// ------------------------------------------------------------ link probe
+
+namespace og {
+
+__global__ __launch_bounds__(kBlock) void k_probe_contig(const uint32_t *__restrict__ p, uint64_t n, uint32_t *out) {
+    uint32_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) acc ^= p[i];
+    if (acc == 0x9E3779B9u) out[0] = acc;  // keeps the reads alive
+}
+
+// Rows r = [off[r], off[r+1]) of the concatenated space start at base[r] in p;
+// lane q finds its row by binary search (as k_relax_dyn does).
+__global__ __launch_bounds__(kBlock) void k_probe_rows(const uint32_t *__restrict__ p,
+                                                       const uint64_t *__restrict__ off,
+                                                       const uint64_t *__restrict__ base, uint32_t nrows, uint64_t m,
+                                                       uint32_t *out) {
+    uint32_t acc = 0;
+    for (uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x; q < m; q += (uint64_t)gridDim.x * kBlock) {
+        uint32_t lo = 0, hi = nrows;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= q) lo = mid; else hi = mid;
+        }
+        acc ^= p[base[lo] + (q - off[lo])];
+    }
+    if (acc == 0x9E3779B9u) out[0] = acc;
+}
+
+}  // namespace og
+
+extern "C" og_status og_probe_host_link(uint64_t bytes, double *contig_gbs, double *scattered_gbs,
+                                        double *memcpy_gbs) {
+    clear_error();
+    if (bytes < (1u << 20) || bytes > (64ull << 30)) return fail(OG_EINVAL, "probe size must be 1 MiB .. 64 GiB");
+    return guard([&]() -> og_status {
+        if (!hip_available()) raise(OG_EHIP, "no HIP device is visible");
+        MemPool m;
+        const uint64_t n = bytes / 4;
+        uint32_t *h = (uint32_t *)m.palloc(n * 4);
+        parallel_for(n, 1 << 20, 0, [&](uint64_t b, uint64_t e, unsigned) {
+            for (uint64_t i = b; i < e; ++i) h[i] = (uint32_t)i;
+        });
+        void *hd = nullptr;
+        OG_HIP(hipHostGetDevicePointer(&hd, h, 0));
+        // rows of 1..19 words with gaps of 0..39 words (SplitMix64 draws)
+        std::vector<uint64_t> off{0}, base;
+        uint64_t pos = 0, key = rng_key(7);
+        for (uint64_t r = 0;; ++r) {
+            const uint64_t x = rng_draw(key, r);
+            const uint64_t len = 1 + x % 19, gap = (x >> 32) % 40;
+            if (pos + gap + len > n) break;
+            pos += gap;
+            base.push_back(pos);
+            pos += len;
+            off.push_back(off.back() + len);
+        }
+        const uint32_t nrows = (uint32_t)base.size();
+        const uint64_t mwords = off.back();
+        uint64_t *doff = (uint64_t *)m.dalloc(off.size() * 8), *dbase = (uint64_t *)m.dalloc(base.size() * 8 + 8);
+        uint32_t *dout = (uint32_t *)m.dalloc(4), *ddst = (uint32_t *)m.dalloc(n * 4);
+        OG_HIP(hipMemcpy(doff, off.data(), off.size() * 8, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(dbase, base.data(), base.size() * 8, hipMemcpyHostToDevice));
+        hipEvent_t a, b;
+        OG_HIP(hipEventCreate(&a));
+        OG_HIP(hipEventCreate(&b));
+        auto timed = [&](double useful, auto fn) {
+            fn();
+            OG_HIP(hipDeviceSynchronize());
+            OG_HIP(hipEventRecord(a, 0));
+            for (int r = 0; r < 3; ++r) fn();
+            OG_HIP(hipEventRecord(b, 0));
+            OG_HIP(hipEventSynchronize(b));
+            float ms = 0;
+            OG_HIP(hipEventElapsedTime(&ms, a, b));
+            return 3.0 * useful / (ms * 1e-3) / 1e9;
+        };
+        const double c = timed((double)n * 4, [&] {
+            hipLaunchKernelGGL(k_probe_contig, dim3(4096), dim3(kBlock), 0, 0, (const uint32_t *)hd, n, dout);
+        });
+        const double sc = timed((double)mwords * 4, [&] {
+            hipLaunchKernelGGL(k_probe_rows, dim3(4096), dim3(kBlock), 0, 0, (const uint32_t *)hd, doff, dbase, nrows,
+                               mwords, dout);
+        });
+        const double mc = timed((double)n * 4, [&] { OG_HIP(hipMemcpyAsync(ddst, h, n * 4, hipMemcpyHostToDevice, 0)); });
+        OG_HIP(hipGetLastError());
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (contig_gbs) *contig_gbs = c;
+        if (scattered_gbs) *scattered_gbs = sc;
+        if (memcpy_gbs) *memcpy_gbs = mc;
+        return OG_OK;
+    });
+}
+
 // Host digest of a result array, equal to the one the device computes:
 // sum (mod 2^64) of mix64(v << 32 | value[v]) over v with value != UNREACHED.
 extern "C" og_status og_levels_checksum(const uint32_t *values, uint64_t num_vertices, uint64_t *checksum) {
@@ -1875,6 +1969,7 @@ static og_status check_tuning(const og_tuning *t) {
     if (t->t1 < 1 || t->t2 < t->t1) return fail(OG_EINVAL, "need 1 <= t1 <= t2");
     if (t->tile < 64 || t->tile > 4096) return fail(OG_EINVAL, "tile must be in [64, 4096]");
     if (t->defer > 2) return fail(OG_EINVAL, "defer must be 0, 1 or 2");
+    if (t->dyn_grid > 65535) return fail(OG_EINVAL, "dyn_grid must be at most 65535");
     return OG_OK;
 }
 
@@ -1886,6 +1981,7 @@ static void apply_tuning(og_graph *g, const og_tuning *t) {
     d->tune.tile = t->tile;
     d->tune.delta = t->delta;
     d->tune.defer = t->defer;
+    d->tune.dyn_grid = t->dyn_grid;
     const uint64_t cap = hub_capacity(g, d->plan.vsplit, d->tune);
     if (cap > d->fr.hub_cap) {
         d->fr.hub = (uint64_t *)d->dalloc(cap * sizeof(uint64_t));
@@ -1896,7 +1992,8 @@ static void apply_tuning(og_graph *g, const og_tuning *t) {
 }
 
 static og_tuning tuning_of(const DeviceState *d) {
-    return og_tuning{d->tune.t1, d->tune.lpv0, d->tune.t2, d->tune.tile, d->tune.delta, d->tune.defer};
+    return og_tuning{d->tune.t1, d->tune.lpv0, d->tune.t2, d->tune.tile, d->tune.delta, d->tune.defer,
+                     d->tune.dyn_grid};
 }
 
 static double time_algo(og_graph *g, uint32_t algo, uint32_t src, uint32_t runs) {
@@ -2005,6 +2102,12 @@ extern "C" og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, ui
                 c.tile = tile;
                 trial(c);
             }
+            if (has_dyn)
+                for (uint32_t dg : {512u, 1024u, 2048u, 4096u, 8192u}) {
+                    og_tuning c = cur;
+                    c.dyn_grid = dg;
+                    trial(c);
+                }
         }
         apply_tuning(g, &cur);
         if (best) *best = cur;

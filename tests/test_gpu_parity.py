@@ -281,3 +281,12 @@ def test_direction_optimising_needs_hbm_resident_graph(gpu, rmat_w):
     with pytest.raises(og.OgError) as e:
         _prepare(rmat_w, "ascetic+diropt", 0.25, 0.8, 8)
     assert e.value.name == "OG_EUNSUPPORTED"
+
+
+def test_probe_host_link(gpu):
+    """The link probe runs its three measurements and reports rates."""
+    r = og.probe_host_link(1 << 22)
+    assert set(r) == {"contiguous_gbs", "scattered_rows_gbs", "memcpy_gbs"}
+    assert all(v > 0 for v in r.values())
+    with pytest.raises(og.OgError):
+        og.probe_host_link(1 << 10)

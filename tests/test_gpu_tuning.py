@@ -17,10 +17,12 @@ def _say(*a):
     sys.stdout.flush()
 
 
-TUNINGS = [dict(t1=1, lpv0=1, t2=1, tile=64, delta=0, defer=0), dict(t1=4, lpv0=4, t2=64, tile=128, delta=1, defer=1),
-           dict(t1=64, lpv0=64, t2=65536, tile=4096, delta=7, defer=0),
-           dict(t1=16, lpv0=16, t2=512, tile=512, delta=300, defer=1),
-           dict(t1=8, lpv0=2, t2=1024, tile=256, delta=3, defer=2), dict(t1=32, lpv0=8, t2=4096, tile=2048, delta=0, defer=2)]
+TUNINGS = [dict(t1=1, lpv0=1, t2=1, tile=64, delta=0, defer=0, dyn_grid=1),
+           dict(t1=4, lpv0=4, t2=64, tile=128, delta=1, defer=1, dyn_grid=0),
+           dict(t1=64, lpv0=64, t2=65536, tile=4096, delta=7, defer=0, dyn_grid=3),
+           dict(t1=16, lpv0=16, t2=512, tile=512, delta=300, defer=1, dyn_grid=8192),
+           dict(t1=8, lpv0=2, t2=1024, tile=256, delta=3, defer=2, dyn_grid=512),
+           dict(t1=32, lpv0=8, t2=4096, tile=2048, delta=0, defer=2, dyn_grid=0)]
 
 
 @pytest.mark.parametrize("approach,frac,k", [("ascetic", 1.0, 1.0), ("ascetic", 0.25, 0.8), ("subway", 0.1, 0.0)])
@@ -49,6 +51,8 @@ def test_results_independent_of_tuning(gpu, approach, frac, k):
         g.set_tuning(t1=10, t2=5)
     with pytest.raises(og.OgError):
         g.set_tuning(defer=3)
+    with pytest.raises(og.OgError):
+        g.set_tuning(dyn_grid=70000)
     g.close()
 
 
