@@ -78,9 +78,11 @@ __global__ void k_srv_seed(SeedList sl, uint64_t *next, uint8_t *flag, unsigned 
 
 // Commit sweep over the flags, 16 vertices per thread step (one uint4 load):
 // nw = next & ~visit becomes the vertex's frontier mask (when it has
-// out-edges) and each slot in nw adds the vertex's level to its digest in
-// acc[0..63] (and writes it to a kept result array).  Flags end up 1 exactly
-// for vertices with a non-empty mask; *alive gets the OR of all masks.
+// out-edges); each slot in nw adds the vertex's level to its digest in
+// acc[0..63] (and writes it to a kept result array) and counts the vertex
+// and its out-degree in fstat (the round's newly reached vertices and their
+// edges per slot).  Flags end up 1 exactly for vertices with a non-empty
+// mask; *alive gets the OR of all masks.
 __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ flag, uint64_t V, uint64_t nchunks,
                                                        const uint64_t *__restrict__ ptr, uint64_t *__restrict__ visit,
                                                        uint64_t *__restrict__ front, uint64_t *__restrict__ next,
@@ -101,30 +103,25 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
             const uint64_t vis = visit[v];
             const uint64_t nw = next[v] & ~vis;
             next[v] = 0;
+            const uint64_t deg = ptr[v + 1] - ptr[v];
             if (nw) {
+                // each slot that reaches v: digest term, one more reached
+                // vertex, deg more traversed edges (= its frontier edges)
                 visit[v] = vis | nw;
                 uint64_t m = nw;
                 while (m) {
                     const int j = __builtin_ctzll(m);
                     m &= m - 1;
                     atomicAdd(&sdig[j], (unsigned long long)srv_mix64((v << 32) | L.lv[j]));
+                    atomicAdd(&sfc[j], 1ull);
+                    if (deg) atomicAdd(&sfe[j], (unsigned long long)deg);
                     if (keep.p[j]) keep.p[j][v] = L.lv[j];
                 }
             }
-            const uint64_t deg = ptr[v + 1] - ptr[v];
             const uint64_t fr = deg ? nw : 0;
             if (fr) {
                 front[v] = fr;
                 al |= fr;
-                if (fstat) {  // per-slot frontier size and edges: the direction choice
-                    uint64_t m = fr;
-                    while (m) {
-                        const int j = __builtin_ctzll(m);
-                        m &= m - 1;
-                        atomicAdd(&sfc[j], 1ull);
-                        atomicAdd(&sfe[j], (unsigned long long)deg);
-                    }
-                }
             } else {
                 flag[v] = 0;
             }
@@ -137,24 +134,28 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
         if (sdig[threadIdx.x]) atomicAdd(acc + threadIdx.x, sdig[threadIdx.x]);
         if (sfc[threadIdx.x]) {
             atomicAdd(fstat + threadIdx.x, sfc[threadIdx.x]);
-            atomicAdd(fstat + 64 + threadIdx.x, sfe[threadIdx.x]);
+            if (sfe[threadIdx.x]) atomicAdd(fstat + 64 + threadIdx.x, sfe[threadIdx.x]);
         }
     }
 }
 
-// Direction of every slot for the round (Beamer et al., SC'12, per slot):
-// top-down -> bottom-up once the slot's frontier edges exceed 1/14 of its
-// unexplored edges, back when its frontier holds fewer than V / 24
+// End of the commit: the round's newly reached vertices and edges of each
+// slot join its reached / traversed totals (acc[64 + j], acc[128 + j]), and,
+// with dir_opt, every slot picks its direction for the round (Beamer et al.,
+// SC'12, per slot): top-down -> bottom-up once its frontier edges exceed 1/14
+// of its unexplored edges, back when its frontier holds fewer than V / 24
 // vertices.  Bit j of *pull = slot j relaxes bottom-up.  Resets the stats.
-__global__ void k_srv_direction(uint64_t used, uint64_t V, uint64_t E, unsigned long long *fstat,
-                                unsigned long long *explored, unsigned long long *pull) {
+__global__ void k_srv_direction(uint64_t used, uint64_t V, uint64_t E, int dir_opt, unsigned long long *fstat,
+                                unsigned long long *explored, unsigned long long *acc, unsigned long long *pull) {
     const uint32_t j = threadIdx.x;
     const uint64_t fc = fstat[j], fe = fstat[64 + j];
+    acc[64 + j] += fc;
+    acc[128 + j] += fe;
     const uint64_t ex = explored[j];
     const uint64_t unexplored = E > ex + fe ? E - ex - fe : 0;
     const bool was = (*pull >> j) & 1;
     bool now = false;
-    if ((used >> j) & 1) now = was ? fc >= V / 24 : fe > unexplored / 14;
+    if (dir_opt && ((used >> j) & 1)) now = was ? fc >= V / 24 : fe > unexplored / 14;
     explored[j] = ex + fe;
     fstat[j] = fstat[64 + j] = 0;
     const uint64_t b = __ballot(now);
@@ -203,29 +204,12 @@ __global__ __launch_bounds__(kBlock) void k_srv_clear_front(const uint32_t *__re
         front[i < n01 ? list[i] : (uint32_t)hub[i - n01]] = 0;
 }
 
-// BFS slots in `done`: reached vertices and traversed edges into
-// acc[64 + j] / acc[128 + j], and their visit bits cleared for reuse.
-__global__ __launch_bounds__(kBlock) void k_srv_bfs_finalize(uint64_t V, uint64_t done,
-                                                             const uint64_t *__restrict__ ptr, uint64_t *visit,
-                                                             unsigned long long *acc) {
-    __shared__ unsigned long long s[128];
-    if (threadIdx.x < 128) s[threadIdx.x] = 0;
-    __syncthreads();
+// Clears the visit bits of finished slots (before the slots are reused).
+__global__ __launch_bounds__(kBlock) void k_srv_clear_visit(uint64_t V, uint64_t dirty, uint64_t *visit) {
     for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
         const uint64_t x = visit[v];
-        uint64_t dm = x & done;
-        if (!dm) continue;
-        visit[v] = x & ~dm;
-        const uint64_t deg = ptr[v + 1] - ptr[v];
-        while (dm) {
-            const int j = __builtin_ctzll(dm);
-            dm &= dm - 1;
-            atomicAdd(&s[j], 1ull);
-            if (deg) atomicAdd(&s[64 + j], (unsigned long long)deg);
-        }
+        if (x & dirty) visit[v] = x & ~dirty;
     }
-    __syncthreads();
-    if (threadIdx.x < 128 && s[threadIdx.x]) atomicAdd(acc + 64 + threadIdx.x, s[threadIdx.x]);
 }
 
 __global__ void k_srv_zero_acc(uint64_t done, unsigned long long *acc) {
@@ -274,6 +258,7 @@ struct BfsEngine {
     uint32_t *keep_buf[64] = {};
     SlotState slots[64];
     uint64_t used = 0;
+    uint64_t dirty = 0;  // finished slots whose visit bits are not cleared yet
     uint32_t round = 0;
     uint64_t rounds_run = 0;
     std::thread th;
@@ -349,13 +334,13 @@ static void push_failure(og_server *srv, const Query &q, uint32_t gi, og_status 
     srv->completed++;
 }
 
+// Results of the slots in `done` come from the accumulators copied at the
+// end of the round; their visit bits stay set (the slots are `dirty`) until
+// a clearing sweep, which runs for many slots at once when a pending query
+// needs a slot or half of them are dirty.
 void BfsEngine::finalize(uint64_t done) {
     const uint64_t V = d->V;
     const KeepList kl = keep_list(done);
-    hipLaunchKernelGGL(k_srv_bfs_finalize, dim3(sweep_grid(V, 16384)), dim3(kBlock), 0, s, V, done, d->d_ptr, visit,
-                       d_acc);
-    OG_HIP(hipGetLastError());
-    OG_HIP(hipMemcpyAsync(h_acc, d_acc, 3 * 64 * 8, hipMemcpyDeviceToHost, s));
     std::vector<std::pair<uint64_t, std::vector<uint32_t>>> vals;
     for (uint32_t j = 0; j < kSlots; ++j) {
         if (kl.p[j]) {
@@ -365,7 +350,8 @@ void BfsEngine::finalize(uint64_t done) {
     }
     hipLaunchKernelGGL(k_srv_zero_acc, dim3(1), dim3(64), 0, s, done, d_acc);
     OG_HIP(hipGetLastError());
-    OG_HIP(hipStreamSynchronize(s));
+    if (!vals.empty()) OG_HIP(hipStreamSynchronize(s));
+    dirty |= done;
     const double t = now_ms(srv);
     std::lock_guard<std::mutex> lk(srv->mu);
     for (uint32_t j = 0; j < kSlots; ++j) {
@@ -399,9 +385,16 @@ void BfsEngine::step() {
         srv->cv_work.wait(lk, [&] { return srv->stop || used || !pending->empty(); });
         if (srv->stop) return;
         const double t = now_ms(srv);
-        while (!pending->empty() && (uint32_t)__builtin_popcountll(used) < kSlots) {
+        const uint32_t clean = kSlots - (uint32_t)__builtin_popcountll(used | dirty);
+        if (dirty && (pending->size() > clean || __builtin_popcountll(dirty) >= (int)kSlots / 2)) {
+            hipLaunchKernelGGL(k_srv_clear_visit, dim3(sweep_grid(d->V, 16384)), dim3(kBlock), 0, s, d->V, dirty,
+                               visit);
+            OG_HIP(hipGetLastError());
+            dirty = 0;  // stream order: the sweep runs before this round's seeds and commit
+        }
+        while (!pending->empty() && (uint32_t)__builtin_popcountll(used | dirty) < kSlots) {
             uint32_t j = 0;
-            while ((used >> j) & 1) ++j;
+            while (((used | dirty) >> j) & 1) ++j;
             slots[j].used = true;
             slots[j].q = pending->front();
             slots[j].start_round = round;
@@ -433,12 +426,13 @@ void BfsEngine::step() {
     SlotLevels L{};
     for (uint32_t j = 0; j < kSlots; ++j) L.lv[j] = round - slots[j].start_round + 1;
     hipLaunchKernelGGL(k_srv_commit, dim3(sweep_grid(nchunks, 4096)), dim3(kBlock), 0, s, f.flag, V, nchunks,
-                       d->d_ptr, visit, front, next, L, keep_list(used), d_acc, dir_opt ? d_fstat : nullptr,
-                       d_alive);
+                       d->d_ptr, visit, front, next, L, keep_list(used), d_acc, d_fstat, d_alive);
     OG_HIP(hipGetLastError());
-    if (dir_opt)
-        hipLaunchKernelGGL(k_srv_direction, dim3(1), dim3(64), 0, s, used, V, d->E, d_fstat, d_explored, d_pull);
+    hipLaunchKernelGGL(k_srv_direction, dim3(1), dim3(64), 0, s, used, V, d->E, dir_opt ? 1 : 0, d_fstat,
+                       d_explored, d_acc, d_pull);
     OG_HIP(hipMemcpyAsync(h_alive, d_alive, 16, hipMemcpyDeviceToHost, s));
+    // results of the slots that finish this round are complete after the commit
+    OG_HIP(hipMemcpyAsync(h_acc, d_acc, 3 * 64 * 8, hipMemcpyDeviceToHost, s));
     // Counts stay on the device: the compaction, relax and clear kernels read
     // them from f.info, so the round needs no host round trip before its end.
     frontier_compact(d, f, s, nullptr, false);
