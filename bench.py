@@ -390,6 +390,9 @@ def wl_serve(args):
         for gi, g in enumerate(graphs):
             g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0, flags=flags)
             g.set_tuning(delta=args.delta)  # SSSP near-far width for the plain path and the lanes alike
+            if args.autotune:  # shapes for SSSP, shared by the plain path and the server's lanes
+                best, best_ms = g.autotune(og.TUNE_SSSP, int(cands[gi][0]), runs=1)
+                log(f"{tag} g{gi}: autotuned {best} ({best_ms:.2f} ms)")
             for kind in (0, 1):
                 ts = []
                 for _ in range(args.plain_sample // 2 if kind else args.plain_sample):
@@ -538,6 +541,8 @@ def main():
     ap.add_argument("--no-overlap", action="store_true", help="serialise copies, static and dynamic relax")
     ap.add_argument("--sweep-delta", default="", help="comma list: time SSSP at each delta and report")
     ap.add_argument("--dyn-grid", type=int, default=None, help="workgroups of the dynamic-region relax (0: 8192)")
+    ap.add_argument("--split-weight", type=int, default=0,
+                    help="SSSP light/heavy split: rows store edges of weight < this first (0: off)")
     ap.add_argument("--sweep-dyn-grid", default="", help="comma list: time SSSP at each dyn_grid and report")
     ap.add_argument("--link-probe", action="store_true",
                     help="only measure shader reads of pinned host memory and hipMemcpy (host-link bound)")
@@ -600,7 +605,9 @@ def main():
     def run_approach(approach, steps, warmup, loader=None):
         k = args.advisek if approach == "ascetic" else 0.0
         flags = plan_flags(args, approach, loader)
-        plan = g.prepare(approach, edge_budget_bytes=budget if approach != "usm" else 0, advisek=k, flags=flags)
+        sw = args.split_weight if approach != "usm" else 0
+        plan = g.prepare(approach, edge_budget_bytes=budget if approach != "usm" else 0, advisek=k, flags=flags,
+                         split_weight=sw)
         log(f"{approach}: plan {plan}")
         if args.delta is not None:
             g.set_tuning(delta=args.delta)

@@ -166,6 +166,15 @@ typedef struct og_plan_params {
                                    only used when edge_budget_bytes == 0      */
     uint32_t host_threads;      /* gather threads; 0 = hardware concurrency   */
     uint32_t flags;             /* OG_PLAN_* bits                             */
+    uint32_t split_weight;      /* SSSP light/heavy split (delta-stepping,
+                                   Meyer & Sanders): > 0 stores each row of
+                                   the device and link copies light edges
+                                   (weight < split_weight) first; near-far
+                                   SSSP rounds then relax light edges only and
+                                   a vertex's heavy edges once, after its
+                                   bucket has settled.  Set it to the tuning's
+                                   delta.  Packed weighted graphs, Ascetic and
+                                   Subway; 0 = no split                        */
 } og_plan_params;
 
 #define OG_PLAN_NO_OVERLAP 1u   /* serialise copies, static and dynamic relax

@@ -40,7 +40,7 @@ class OgError(RuntimeError):
 
 class PlanParams(C.Structure):
     _fields_ = [("edge_budget_bytes", C.c_uint64), ("advisek", C.c_float), ("param_size", C.c_uint32),
-                ("host_threads", C.c_uint32), ("flags", C.c_uint32)]
+                ("host_threads", C.c_uint32), ("flags", C.c_uint32), ("split_weight", C.c_uint32)]
 
 
 class PlanInfo(C.Structure):
@@ -333,10 +333,11 @@ class Graph:
         return out.value
 
     # ------------------------------------------------------------ device
-    def prepare(self, approach=ASCETIC, edge_budget_bytes=0, advisek=0.0, param_size=0, host_threads=0, flags=0):
+    def prepare(self, approach=ASCETIC, edge_budget_bytes=0, advisek=0.0, param_size=0, host_threads=0, flags=0,
+                split_weight=0):
         if isinstance(approach, str):
             approach = APPROACHES[approach]
-        p = PlanParams(edge_budget_bytes, advisek, param_size, host_threads, flags)
+        p = PlanParams(edge_budget_bytes, advisek, param_size, host_threads, flags, split_weight)
         info = PlanInfo()
         _check(lib().og_prepare(self._h, approach, C.byref(p), C.byref(info)))
         return as_dict(info)

@@ -44,7 +44,24 @@ struct RelaxArgs {
     uint32_t wbits;       // packed weighted edges: bits of the weight field
     const uint64_t *pull; // multi-source, optional: sources relaxed bottom-up this round
                           // (left out of the top-down relax)
+    const uint64_t *pmid; // light/heavy split, optional: end of each row's light edges
 };
+
+// Edge range of u in a round: the whole row, or with the light/heavy split
+// its light part [ptr[u], pmid[u]) or, in a heavy round, [pmid[u], ptr[u+1]).
+__device__ __forceinline__ void row_range(const uint64_t *__restrict__ ptr, const uint64_t *__restrict__ pmid,
+                                          bool heavy, uint64_t u, uint64_t &b, uint64_t &e) {
+    if (!pmid) {
+        b = ptr[u];
+        e = ptr[u + 1];
+    } else if (heavy) {
+        b = pmid[u];
+        e = ptr[u + 1];
+    } else {
+        b = ptr[u];
+        e = pmid[u];
+    }
+}
 
 // Degree-bin thresholds and shapes for the static relax (autotunable).
 struct Tuning {
@@ -71,7 +88,7 @@ struct FrontierInfo {
     uint32_t live;          // 1 while active or held-back vertices remain
     uint32_t dyn_held;      // 1: the dynamic list was held back this round (n[3] = 0)
     uint32_t pull;          // 1: BFS round runs bottom-up (the lists are not built)
-    uint32_t pad;
+    uint32_t heavy;         // 1: light/heavy split, this round relaxes heavy edges
 };
 
 // Device-resident driver state of one traversal.  The scan of each
@@ -84,6 +101,9 @@ struct RoundCtl {
     uint32_t defer;              // 1: hold the dynamic list back while static vertices are active;
                                  // 2: dynamic vertices are near only below fbound
     uint32_t fbound;             // defer 2: dynamic vertices with value < fbound are near
+    uint32_t split;              // light/heavy split in use (rows: light edges, then heavy)
+    uint32_t mode;               // split: 1 = the next compaction collects the heavy-pending vertices
+    uint64_t hpend;              // split: vertices light-relaxed since the last heavy round
     uint32_t rounds;             // compactions that found active vertices
     uint32_t compactions;        // all compactions
     uint32_t live;               // last compaction's FrontierInfo::live
@@ -110,6 +130,7 @@ constexpr uint8_t kCode0 = 2;           // count pass: flag byte = kCode0 + bin 
 // that engines can iterate concurrently on different streams.
 struct Frontier {
     uint8_t *flag = nullptr;         // Vpad next-frontier flags
+    uint8_t *hflag = nullptr;        // Vpad light/heavy split: heavy-pending flags
     uint32_t *list = nullptr;        // V: bin0 | bin1 | dynamic lists
     uint64_t *hub = nullptr;         // hub tiles (tile << 32 | vertex)
     uint64_t hub_cap = 0;
@@ -203,6 +224,10 @@ struct DeviceState {
 
     // graph on the device
     uint64_t *d_ptr = nullptr;       // V + 1 (device copy; USM: managed)
+    uint64_t *d_pmid = nullptr;      // light/heavy split: V, end of the light edges of each row
+    std::vector<uint64_t> h_pmid;    // host copy (the staged gather)
+    uint32_t split_weight = 0;       // light: weight < split_weight (0: no split)
+    const uint64_t *run_pmid = nullptr;  // d_pmid while a run uses the split (SSSP near-far), else null
     void *d_static = nullptr;        // static-region edges (USM: managed, whole graph)
     uint64_t static_used = 0;        // edges of the static vertices, = ptr[vsplit]
     void *d_staging = nullptr;       // nslots * slot_edges edges (zero-copy: the resident head)
@@ -431,15 +456,19 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m);
 // `defer` 1 holds the dynamic list back in rounds with static work, 2 lets
 // dynamic vertices relax only once the static part of their bucket settled;
 // dir_opt_edges > 0 turns on the BFS direction choice over that many edges.
-void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer = 0, uint64_t dir_opt_edges = 0);
+// split: light/heavy split rounds (the graph's pmid rows; SSSP near-far).
+void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer = 0, uint64_t dir_opt_edges = 0,
+                    bool split = false);
 
 // One compaction: flags -> bin lists / hub tiles / sorted dynamic list with
 // its degree prefix, flags cleared; f.info and f.ctl updated on the device.
 // With `val` non-null, vertices whose value is >= f.ctl->thresh stay flagged
 // and are only counted (near-far order).  With to_host, the summary is also
 // copied to f.h_info (stream-ordered, not synchronised).
+// pmid (light/heavy split, with frontier_begin's split): rows are ranged by
+// the round's mode.
 void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val = nullptr,
-                      bool to_host = true);
+                      bool to_host = true, const uint64_t *pmid = nullptr);
 
 // Per-run state of a concurrent single-source lane: shares the device arrays
 // of g->dev (the whole graph must be on the device) and owns values,

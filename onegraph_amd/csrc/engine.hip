@@ -136,13 +136,20 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
 // stores kCode0 + bin in its flag byte and publishes the per-block counts.
 // Row bounds (and values, for near-far) of kLoadBatch flagged vertices are
 // read before any is used, so their latencies overlap.
-__global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__ flag, uint64_t Vpad, uint64_t vsplit,
-                                                           const uint64_t *__restrict__ ptr, Tuning tn,
-                                                           const uint32_t *__restrict__ val,
+__global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__ flag_a, uint8_t *__restrict__ hflag,
+                                                           uint64_t Vpad, uint64_t vsplit,
+                                                           const uint64_t *__restrict__ ptr,
+                                                           const uint64_t *__restrict__ pmid, Tuning tn,
+                                                           const uint32_t *__restrict__ val_a,
                                                            const RoundCtl *__restrict__ ctl,
                                                            uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
                                                            uint32_t nblk) {
     __shared__ uint64_t lds[6 * kBlock / kWave];
+    // light/heavy split: a heavy compaction collects the heavy-pending
+    // vertices (no near-far test) and sizes them by their heavy edges
+    const bool heavy = pmid && ctl->mode;
+    uint8_t *flag = heavy ? hflag : flag_a;
+    const uint32_t *val = heavy ? nullptr : val_a;
     const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kFlagsPerThread;
     uint4 w[kFlagVecs];
     load_flags(flag, base, Vpad, w);
@@ -168,8 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                     const uint32_t j = (uint32_t)__builtin_ctz(m);
                     m &= m - 1;
                     jj[k] = j;
-                    pb[k] = ptr[base + j];
-                    pe[k] = ptr[base + j + 1];
+                    row_range(ptr, pmid, heavy, base + j, pb[k], pe[k]);
                     if (val) vv[k] = val[base + j];
                 }
             }
@@ -283,19 +289,36 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
         const uint64_t snodes = (uint64_t)fi.n[0] + fi.n[1] + fi.nhub;
         // deferred order: while static vertices are active, the dynamic list
         // stays flagged for a later round (the write pass keeps its flags)
-        fi.dyn_held = (c.defer == 1 && snodes && fi.n[3]) ? 1u : 0u;
+        const bool was_heavy = c.mode != 0;
+        fi.dyn_held = (c.defer == 1 && !was_heavy && snodes && fi.n[3]) ? 1u : 0u;
         const uint32_t held = fi.dyn_held ? fi.n[3] : 0u;
         if (held) {
             fi.n[3] = 0;
             fi.dyn_deg = 0;
         }
         const uint64_t active = snodes + fi.n[3] + fi.nzero;
-        fi.live = (active || fi.nfar || held) ? 1u : 0u;
+        // light/heavy split: light rounds count the vertices they relax (they
+        // turn heavy-pending); once a bucket's light rounds find nothing near,
+        // the next compaction collects the heavy-pending vertices (their
+        // distances are final within the bucket) before the threshold moves
+        bool heavy_next = false;
+        if (c.split) {
+            if (was_heavy) {
+                c.hpend = 0;
+            } else if (active) {
+                c.hpend += active;
+            } else if (c.hpend) {
+                heavy_next = true;
+            }
+        }
+        // (after a heavy round a light compaction always follows: the held-back
+        // vertices sit in the other flag array)
+        fi.live = (active || fi.nfar || held || heavy_next || was_heavy) ? 1u : 0u;
         // BFS direction (Beamer et al., SC'12): bottom-up once the frontier's
         // edges exceed 1/14 of the unexplored ones, back to top-down when
         // the frontier falls below V / 24.
         fi.pull = 0;
-        fi.pad = 0;
+        fi.heavy = c.mode;
         if (c.dir_opt && active) {
             const uint64_t mf = fi.static_deg + fi.dyn_deg;
             if (!c.pull && mf > c.unexplored / 14) c.pull = 1;
@@ -317,6 +340,8 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
             c.work_edges += fi.static_deg + fi.dyn_deg;
             c.static_work_edges += fi.static_deg;
             c.dyn_rounds += fi.n[3] ? 1 : 0;
+        } else if (heavy_next) {
+            // the threshold stays: the heavy round's offers are all >= it
         } else if (fi.nfar && c.delta) {
             // Near set empty.  Lagged order: every vertex below the threshold
             // has relaxed except the waiting dynamic ones, which go next (a
@@ -326,6 +351,7 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
             if (t[7]) c.fbound = c.thresh;
             else c.thresh = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)fi.min_far + c.delta);
         }
+        c.mode = heavy_next ? 1u : 0u;
         *ctl = c;
     }
 }
@@ -336,8 +362,9 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
 // 1 for held-back vertices, 0 elsewhere.  m_dlist / m_doff, when not null,
 // receive a copy of the dynamic list and its degree prefix: pinned host
 // memory the staged loader gathers from without a copy back.
-__global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__ flag, uint64_t Vpad,
-                                                           const uint64_t *__restrict__ ptr, Tuning tn,
+__global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__ flag_a, uint8_t *__restrict__ hflag,
+                                                           uint64_t Vpad, const uint64_t *__restrict__ ptr,
+                                                           const uint64_t *__restrict__ pmid, Tuning tn,
                                                            const uint32_t *__restrict__ bcnt,
                                                            const uint64_t *__restrict__ bdeg, uint32_t nblk,
                                                            const FrontierInfo *__restrict__ info,
@@ -347,6 +374,10 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                                                            uint64_t *__restrict__ m_doff) {
     __shared__ uint64_t lds[3 * kBlock / kWave];
     __shared__ uint32_t s_win[4 * kFlagVecs * kBlock];  // word i of thread t at [i * kBlock + t]
+    // the array and row ranges the count pass used (light/heavy split)
+    const bool heavy = pmid && info->heavy;
+    uint8_t *flag = heavy ? hflag : flag_a;
+    const bool mark = pmid && !heavy;  // light round: relaxed vertices turn heavy-pending
     const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kFlagsPerThread;
     uint4 w[kFlagVecs];
     load_flags(flag, base, Vpad, w);
@@ -380,10 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                     m &= m - 1;
                     jj[k] = j;
                     bb[k] = bin_at(j);
-                    if (bb[k] == 2 || bb[k] == 3) {
-                        pb[k] = ptr[base + j];
-                        pe[k] = ptr[base + j + 1];
-                    }
+                    if (bb[k] == 2 || bb[k] == 3) row_range(ptr, pmid, heavy, base + j, pb[k], pe[k]);
                 }
             }
 #pragma unroll
@@ -416,6 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
         const bool held = info->dyn_held != 0;
         walk([&](uint32_t j, int b, uint64_t deg) {
             const uint64_t v = base + j;
+            if (mark && b <= 4 && !(b == 3 && held)) hflag[v] = 1;
             switch (b) {
                 case 0: list[p0++] = (uint32_t)v; break;
                 case 1: list[p1++] = (uint32_t)v; break;
@@ -452,8 +481,10 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
     store_flags(flag, base, Vpad, word);
 }
 
-__global__ void k_ctl_reset(RoundCtl *ctl, uint32_t delta, uint32_t defer, uint64_t dir_opt_edges, uint64_t V) {
+__global__ void k_ctl_reset(RoundCtl *ctl, uint32_t delta, uint32_t defer, uint64_t dir_opt_edges, uint64_t V,
+                            uint32_t split) {
     RoundCtl c{};
+    c.split = split;
     c.delta = delta;
     c.defer = defer;
     c.dir_opt = dir_opt_edges ? 1u : 0u;
@@ -544,6 +575,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restri
                                                         const void *__restrict__ edges, RelaxArgs a,
                                                         unsigned long long *upd) {
     if (info->pull) return;
+    const bool heavy = info->heavy != 0;
     const uint32_t n = info->n[bin];
     const uint32_t *lst = list + (bin ? info->n[0] : 0u);
     const uint32_t sub = threadIdx.x % LPV;
@@ -551,7 +583,8 @@ __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restri
     uint32_t cnt = 0;
     for (uint64_t item = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / LPV; item < n; item += stride) {
         const uint32_t u = lst[item];
-        const uint64_t b = ptr[u], e = ptr[u + 1];
+        uint64_t b, e;
+        row_range(ptr, a.pmid, heavy, u, b, e);
         const auto s = load_src<ALGO>(a, u);
         if (ALGO == ALGO_MSBFS && !s) continue;  // all of u's sources go bottom-up this round
         for (uint64_t i = b + sub; i < e; i += (uint64_t)kRelaxBatch * LPV) {
@@ -586,12 +619,15 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
     extern __shared__ uint32_t s_edges[];
     constexpr bool WIDE = W && !PK;  // 8-byte edges
     const uint64_t ntiles = info->pull ? 0 : min((uint64_t)info->n[2], hub_cap);
+    const bool heavy = info->heavy != 0;
     uint32_t c = 0;
     for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const uint64_t it = hub[ti];
         const uint32_t u = (uint32_t)it, t = (uint32_t)(it >> 32);
-        const uint64_t b = ptr[u] + (uint64_t)t * tile;
-        const uint64_t e = min(ptr[u + 1], b + tile);
+        uint64_t rb, re;
+        row_range(ptr, a.pmid, heavy, u, rb, re);
+        const uint64_t b = rb + (uint64_t)t * tile;
+        const uint64_t e = min(re, b + tile);
         const uint32_t cnt = (uint32_t)(e - b);
         const auto s = load_src<ALGO>(a, u);
         if (ALGO == ALGO_MSBFS && !s) continue;  // block-uniform: u is the same for every thread
@@ -652,6 +688,7 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
     const uint32_t ndyn = info->n[3];
     const uint32_t *dlist = list + info->n[0] + info->n[1];
     const uint64_t c1 = min(c1max, info->dyn_deg);
+    const bool heavy = info->heavy != 0;
     uint32_t c = 0, h = 0;
     for (uint64_t q0 = c0 + (uint64_t)blockIdx.x * kDynEdgesPerBlock; q0 < c1;
          q0 += (uint64_t)gridDim.x * kDynEdgesPerBlock) {
@@ -688,7 +725,8 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
                 }
                 const uint32_t u = dlist[i0 + lo];
                 sv[j] = load_src<ALGO>(a, u);
-                const uint64_t idx = DIRECT ? ptr[u] - dbase + (q - s_off[lo]) : q - c0;
+                const uint64_t rb = (a.pmid && heavy) ? a.pmid[u] : ptr[u];
+                const uint64_t idx = DIRECT ? rb - dbase + (q - s_off[lo]) : q - c0;
                 if (DIRECT && idx < nres) {
                     load_edge<W, PACKED>(resident, idx, wbits, to[j], wt[j]);
                     ++h;
@@ -884,6 +922,10 @@ static uint64_t hub_capacity(const og_graph *g, uint64_t vsplit, const Tuning &t
 void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) {
     const uint64_t V = d->V;
     f.flag = (uint8_t *)m.dalloc(d->Vpad);
+    if (d->d_pmid) {
+        f.hflag = (uint8_t *)m.dalloc(d->Vpad);
+        OG_HIP(hipMemset(f.hflag, 0, d->Vpad));
+    }
     f.list = (uint32_t *)m.dalloc(V * sizeof(uint32_t));
     f.doff = (uint64_t *)m.dalloc((V + 1) * sizeof(uint64_t));
     f.nblk = (uint32_t)((d->Vpad + kVertsPerBlock - 1) / kVertsPerBlock);
@@ -905,22 +947,25 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) 
     OG_HIP(hipMemcpy(f.ctl, &c, sizeof c, hipMemcpyHostToDevice));
 }
 
-void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer, uint64_t dir_opt_edges) {
-    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, f.ctl, delta, defer, dir_opt_edges, f.V);
+void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer, uint64_t dir_opt_edges,
+                    bool split) {
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, f.ctl, delta, defer, dir_opt_edges, f.V,
+                       split ? 1u : 0u);
     OG_HIP(hipGetLastError());
 }
 
-void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val, bool to_host) {
+void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val, bool to_host,
+                      const uint64_t *pmid) {
     // (pinned mirrors of the dynamic list written by pass 3 cost one small
     // PCIe write per vertex: slower than one bulk copy back, so unused)
     uint32_t *m_dlist = nullptr;
     uint64_t *m_doff = nullptr;
-    hipLaunchKernelGGL(k_frontier_count, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, d->Vpad, d->plan.vsplit, d->d_ptr,
-                       d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
+    hipLaunchKernelGGL(k_frontier_count, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->plan.vsplit,
+                       d->d_ptr, pmid, d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
     hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff, f.ctl,
                        m_doff);
-    hipLaunchKernelGGL(k_frontier_write, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, d->Vpad, d->d_ptr, d->tune, f.bcnt,
-                       f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist, m_doff);
+    hipLaunchKernelGGL(k_frontier_write, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr, pmid,
+                       d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist, m_doff);
     OG_HIP(hipGetLastError());
     if (to_host) OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
 }
@@ -940,18 +985,35 @@ static uint32_t weight_bits(const og_graph *g) {
     return wb;
 }
 
-// Weighted edges [e0, e0 + n) as packed words (to << wb | w).
-static void pack_edges(const og_graph *g, uint64_t e0, uint64_t n, uint32_t wb, uint32_t *dst) {
-    const uint32_t *src = g->wedges + 2 * e0;
-    parallel_for(n, 1 << 22, 0, [&](uint64_t b, uint64_t e, unsigned) {
-        for (uint64_t i = b; i < e; ++i) dst[i] = (src[2 * i] << wb) | src[2 * i + 1];
+// Rows [v0, v1) of a weighted graph as packed words (to << wb | w) at
+// dst[ptr[v] - ptr[v0] ...]; with the light/heavy split (sw > 0) each row
+// lists its edges of weight < sw first, then the others, each part in CSR
+// order.
+static void pack_rows(const og_graph *g, uint64_t v0, uint64_t v1, uint32_t wb, uint32_t sw, uint32_t *dst) {
+    const uint64_t e0 = g->ptr[v0];
+    parallel_for(v1 - v0, 1 << 12, 0, [&](uint64_t b, uint64_t e, unsigned) {
+        for (uint64_t v = v0 + b; v < v0 + e; ++v) {
+            const uint32_t *src = g->wedges + 2 * g->ptr[v];
+            const uint64_t n = g->ptr[v + 1] - g->ptr[v];
+            uint32_t *out = dst + (g->ptr[v] - e0);
+            if (!sw) {
+                for (uint64_t i = 0; i < n; ++i) out[i] = (src[2 * i] << wb) | src[2 * i + 1];
+                continue;
+            }
+            uint64_t k = 0;
+            for (uint64_t i = 0; i < n; ++i)
+                if (src[2 * i + 1] < sw) out[k++] = (src[2 * i] << wb) | src[2 * i + 1];
+            for (uint64_t i = 0; i < n; ++i)
+                if (src[2 * i + 1] >= sw) out[k++] = (src[2 * i] << wb) | src[2 * i + 1];
+        }
     });
 }
 
-// Uploads the first n weighted edges packed into dst (device): host threads
-// pack 64 MiB chunks into two pinned bounce buffers while the previous chunk
-// is in flight on s_compute.
-static void upload_packed(DeviceState *d, const og_graph *g, uint64_t n, uint32_t *dst) {
+// Uploads the packed rows [0, vend) into dst (device): host threads pack
+// row-aligned chunks of about 64 MiB into two pinned bounce buffers while
+// the previous chunk is in flight on s_compute.
+static void upload_packed(DeviceState *d, const og_graph *g, uint64_t vend, uint32_t *dst) {
+    const uint64_t n = g->ptr[vend];
     if (!n) return;
     struct Bounce {
         hipStream_t s;
@@ -965,16 +1027,21 @@ static void upload_packed(DeviceState *d, const og_graph *g, uint64_t n, uint32_
                 if (e) (void)hipEventDestroy(e);
         }
     } bb{d->s_compute};
-    const uint64_t C = std::min<uint64_t>(n, 16ull << 20);
+    uint64_t maxrow = 0;
+    for (uint64_t v = 0; v < vend; ++v) maxrow = std::max(maxrow, g->ptr[v + 1] - g->ptr[v]);
+    const uint64_t C = std::max<uint64_t>(std::min<uint64_t>(n, 16ull << 20), maxrow);
     for (auto &p : bb.b) OG_HIP(hipHostMalloc((void **)&p, C * 4, 0));
     for (auto &e : bb.ev) OG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    for (uint64_t c0 = 0, k = 0; c0 < n; c0 += C, ++k) {
+    for (uint64_t v0 = 0, k = 0; v0 < vend; ++k) {
+        uint64_t v1 = v0;
+        while (v1 < vend && g->ptr[v1 + 1] - g->ptr[v0] <= C) ++v1;
         const int s = (int)(k & 1);
         if (k >= 2) OG_HIP(hipEventSynchronize(bb.ev[s]));
-        const uint64_t m = std::min(C, n - c0);
-        pack_edges(g, c0, m, d->wbits, bb.b[s]);
-        OG_HIP(hipMemcpyAsync(dst + c0, bb.b[s], m * 4, hipMemcpyHostToDevice, d->s_compute));
+        const uint64_t m = g->ptr[v1] - g->ptr[v0];
+        pack_rows(g, v0, v1, d->wbits, d->split_weight, bb.b[s]);
+        OG_HIP(hipMemcpyAsync(dst + g->ptr[v0], bb.b[s], m * 4, hipMemcpyHostToDevice, d->s_compute));
         OG_HIP(hipEventRecord(bb.ev[s], d->s_compute));
+        v0 = v1;
     }
     OG_HIP(hipStreamSynchronize(d->s_compute));
 }
@@ -1013,6 +1080,22 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
     }
     d->deb = d->eb;
     const uint64_t V = g->V, E = g->E, eb = d->eb;
+    // Light/heavy split: rows of the packed device / link copies list light
+    // edges first; pmid[v] is where row v's heavy edges start.
+    if (p.split_weight) {
+        if (!d->packed)
+            raise(OG_EUNSUPPORTED, "the light/heavy split needs packed weighted edges (Ascetic or Subway, ids and "
+                                   "weights in one 32-bit word, no OG_PLAN_NO_PACK)");
+        d->split_weight = p.split_weight;
+        d->h_pmid.resize(V);
+        parallel_for(V, 1 << 14, 0, [&](uint64_t b, uint64_t e, unsigned) {
+            for (uint64_t v = b; v < e; ++v) {
+                uint64_t k = g->ptr[v];
+                for (uint64_t i = g->ptr[v]; i < g->ptr[v + 1]; ++i) k += g->wedges[2 * i + 1] < p.split_weight;
+                d->h_pmid[v] = k;
+            }
+        });
+    }
 
     // ---- plan
     og_plan_info &pl = d->plan;
@@ -1065,11 +1148,16 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
     } else {
         d->d_ptr = (uint64_t *)d->dalloc((V + 1) * sizeof(uint64_t));
         OG_HIP(hipMemcpyAsync(d->d_ptr, g->ptr, (V + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, d->s_compute));
+        if (d->split_weight) {
+            d->d_pmid = (uint64_t *)d->dalloc(std::max<uint64_t>(V, 1) * sizeof(uint64_t));
+            OG_HIP(hipMemcpyAsync(d->d_pmid, d->h_pmid.data(), V * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                  d->s_compute));
+        }
         if (pl.static_edges) {
             d->d_static = d->dalloc(pl.static_edges * eb);
             d->static_used = g->ptr[pl.vsplit];
             if (d->packed)
-                upload_packed(d.get(), g, d->static_used, (uint32_t *)d->d_static);
+                upload_packed(d.get(), g, pl.vsplit, (uint32_t *)d->d_static);
             else if (d->static_used)
                 OG_HIP(hipMemcpyAsync(d->d_static, g->edge_data(), d->static_used * eb, hipMemcpyHostToDevice,
                                       d->s_compute));
@@ -1115,7 +1203,7 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
             const uint64_t nd = E - d->dyn_base;
             if (d->packed) {
                 uint32_t *pk = (uint32_t *)d->palloc(std::max<uint64_t>(nd, 1) * 4);
-                pack_edges(g, d->dyn_base, nd, d->wbits, pk);
+                pack_rows(g, pl.vsplit, V, d->wbits, d->split_weight, pk);
                 d->h_dyn = pk;
             } else {
                 d->h_dyn = (const uint8_t *)g->edge_data() + d->dyn_base * eb;
@@ -1278,7 +1366,8 @@ static RoundGraph *round_graph_for(DeviceState *d, uint64_t key, Body body) {
 // from the device-built degree prefix instead of a serial host scan).  The
 // source is the dynamic region in its transfer format (d->h_dyn, whose first
 // element is edge d->dyn_base of the CSR).
-static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0, uint64_t c1, uint8_t *dst) {
+static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0, uint64_t c1, uint8_t *dst,
+                   bool heavy) {
     const uint64_t eb = d->deb;
     const uint8_t *src = (const uint8_t *)d->h_dyn;
     const uint64_t base = d->dyn_base;
@@ -1297,7 +1386,8 @@ static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0
             const uint32_t v = d->h_dlist[i];
             const uint64_t vb = d->h_doff[i], ve = d->h_doff[i + 1];
             const uint64_t stop = std::min(ve, p1);
-            std::memcpy(dst + (p - c0) * eb, src + (g->ptr[v] - base + (p - vb)) * eb, (stop - p) * eb);
+            const uint64_t rb = heavy ? d->h_pmid[v] : g->ptr[v];  // row start of the round's range
+            std::memcpy(dst + (p - c0) * eb, src + (rb - base + (p - vb)) * eb, (stop - p) * eb);
             p = stop;
             ++i;
         }
@@ -1305,7 +1395,7 @@ static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0
 }
 
 static RelaxArgs relax_args(DeviceState *d) {
-    return RelaxArgs{d->d_val, d->fr.flag, d->d_visit, d->d_front, d->d_next, d->wbits, nullptr};
+    return RelaxArgs{d->d_val, d->fr.flag, d->d_visit, d->d_front, d->d_next, d->wbits, nullptr, d->run_pmid};
 }
 
 // Grids of device-count launches: the kernels stride over whatever count
@@ -1426,7 +1516,7 @@ static void run_dynamic(DeviceState *d, const og_graph *g, const FrontierInfo &f
         const int slot = (int)(c % d->nslots);
         if (d->slot_h2d_pending[slot]) OG_HIP(hipEventSynchronize(d->ev_h2d[slot]));
         Stopwatch sw;
-        gather(d, g, ndyn, c0, c1, (uint8_t *)d->h_stage[slot]);
+        gather(d, g, ndyn, c0, c1, (uint8_t *)d->h_stage[slot], d->run_pmid && fi.heavy);
         gather_ms += sw.ms();
         uint8_t *dslot = (uint8_t *)d->d_staging + (uint64_t)slot * d->slot_edges * d->eb;
         if (d->slot_relax_pending[slot]) OG_HIP(hipStreamWaitEvent(d->s_copy, d->ev_relax[slot], 0));
@@ -1482,10 +1572,14 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
     // only the atomicMin relaxes (SSSP, CC) tolerate
     const uint32_t defer = (ALGO == ALGO_SSSP || ALGO == ALGO_CC) && has_dyn ? d->tune.defer : 0;
     const bool dir_opt = ALGO == ALGO_BFS && d->d_in_ptr != nullptr;
-    frontier_begin(d->fr, s, delta, defer, dir_opt ? std::max<uint64_t>(d->E, 1) : 0);
+    // light/heavy split: near-far SSSP on a graph prepared with it
+    const bool split = ALGO == ALGO_SSSP && delta && d->d_pmid;
+    d->run_pmid = split ? d->d_pmid : nullptr;
+    const uint64_t *pm = d->run_pmid;
+    frontier_begin(d->fr, s, delta, defer, dir_opt ? std::max<uint64_t>(d->E, 1) : 0, split);
     if (!staged) {
         const EagerSpan eager{d, tm};
-        eager(kClsFront, s, [&] { frontier_compact(d, d->fr, s, nf_val, false); });
+        eager(kClsFront, s, [&] { frontier_compact(d, d->fr, s, nf_val, false, pm); });
         // one batch: kRoundBatch x (static || dynamic relax, after, compaction),
         // then the traversal state to the host
         auto batch = [&](const auto &span) {
@@ -1501,13 +1595,14 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
                     });
                 if (joined) OG_HIP(hipStreamWaitEvent(s, d->ev_static, 0));
                 after();
-                span(kClsFront, s, [&] { frontier_compact(d, d->fr, s, nf_val, false); });
+                span(kClsFront, s, [&] { frontier_compact(d, d->fr, s, nf_val, false, pm); });
             }
             OG_HIP(hipMemcpyAsync(d->fr.h_ctl, d->fr.ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
         };
         if (!(d->flags & OG_PLAN_NO_GRAPH)) {
             const uint64_t key = (uint64_t)ALGO | (uint64_t)W << 4 | (uint64_t)(delta != 0) << 5 |
-                                 (uint64_t)has_dyn << 6 | (uint64_t)dir_opt << 7 | (uint64_t)kRoundBatch << 8;
+                                 (uint64_t)has_dyn << 6 | (uint64_t)dir_opt << 7 | (uint64_t)split << 16 |
+                                 (uint64_t)kRoundBatch << 8;
             RoundGraph *rg = round_graph_for(d, key, [&](RoundGraph *n) { batch(GraphSpan{n}); });
             for (;;) {
                 OG_HIP(hipGraphLaunch(rg->exec, s));
@@ -1533,7 +1628,7 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
     } else {
         {
             EventSpan span(d, &tm.front, s);
-            frontier_compact(d, d->fr, s, nf_val, true);
+            frontier_compact(d, d->fr, s, nf_val, true, pm);
         }
         OG_HIP(hipStreamSynchronize(s));
         for (;;) {
@@ -1554,7 +1649,7 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
             }
             {
                 EventSpan span(d, &tm.front, s);
-                frontier_compact(d, d->fr, s, nf_val, true);
+                frontier_compact(d, d->fr, s, nf_val, true, pm);
             }
             OG_HIP(hipStreamSynchronize(s));
         }
@@ -1774,6 +1869,8 @@ DeviceState *lane_state_create(og_graph *g) {
     l->wbits = d->wbits;
     l->d_in_ptr = d->d_in_ptr;
     l->d_in_cols = d->d_in_cols;
+    l->d_pmid = d->d_pmid;
+    l->split_weight = d->split_weight;
     OG_HIP(hipSetDevice(d->device));
     OG_HIP(hipStreamCreateWithFlags(&l->s_compute, hipStreamNonBlocking));
     OG_HIP(hipStreamCreateWithFlags(&l->s_copy, hipStreamNonBlocking));

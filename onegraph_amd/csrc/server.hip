@@ -10,14 +10,17 @@
 //      `next`, so the commit below seeds it like any other offer;
 //   2. commit (one sweep over the frontier flags): the offered, unseen slots
 //      of a vertex become its frontier mask and each of them adds
-//      mix64(v << 32 | level) to its slot's digest, so no level array is
-//      kept;
+//      mix64(v << 32 | level) to its slot's digest and the vertex and its
+//      degree to the slot's reached / traversed counts, so no level array is
+//      kept; then every slot picks its direction for the round;
 //   3. compact the flags into degree bins (shared with og_bfs);
-//   4. relax through the multi-source bitmask kernels (one 64-bit atomic per
-//      edge serves every slot);
+//   4. relax: top-down slots through the multi-source bitmask kernels (one
+//      64-bit atomic per edge serves every slot), bottom-up slots in one
+//      pass over the in-edges of the vertices they have not reached (graphs
+//      prepared with OG_PLAN_DIRECTION_OPT);
 //   5. clear the masks of the relaxed vertices;
-//   6. slots whose frontier emptied are finalised by a sweep that counts
-//      their reached vertices and traversed edges and clears their bits.
+//   6. slots whose frontier emptied report their results; their visit bits
+//      are cleared later, by one sweep for many slots.
 // Queries that arrive while a round runs join at the next round boundary, so
 // their work overlaps with the queries already in flight.
 //
@@ -37,7 +40,6 @@
 struct og_server;
 
 namespace og {
-
 
 struct SeedList {
     uint32_t n;

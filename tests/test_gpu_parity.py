@@ -290,3 +290,47 @@ def test_probe_host_link(gpu):
     assert all(v > 0 for v in r.values())
     with pytest.raises(og.OgError):
         og.probe_host_link(1 << 10)
+
+
+@pytest.mark.parametrize("approach,frac,k", [("ascetic", 1.0, 1.0), ("ascetic", 0.25, 0.8), ("ascetic", 1 / 64, 0.5),
+                                              ("ascetic+zerocopy", 0.25, 0.8), ("ascetic+zerocopy", 1 / 64, 0.5),
+                                              ("subway+zerocopy", 0.25, 0.0), ("subway", 1 / 64, 0.0),
+                                              ("ascetic+zerocopy+nograph", 0.25, 0.8)])
+@pytest.mark.parametrize("split,delta", [(32, 32), (8, 32), (40, 16), (16, 0)])
+def test_light_heavy_split(gpu, rmat_w, approach, frac, k, split, delta):
+    """Rows stored light edges first: near-far SSSP relaxes light edges and
+    each vertex's heavy edges once its bucket settled; any split weight /
+    bucket width pair (and delta 0, which ignores the split) gives the
+    oracle's distances, and BFS / CC / multi-source BFS on the same
+    preparation use whole rows."""
+    ptr, to, w, _ = rmat_w.export()
+    E = len(to)
+    _say("light/heavy split rmat13", approach, frac, k, split, delta)
+    name, *opts = approach.split("+")
+    flags = 0
+    for o in opts:
+        flags |= {"zerocopy": og.PLAN_ZERO_COPY, "nograph": og.PLAN_NO_GRAPH}[o]
+    rmat_w.prepare(name, edge_budget_bytes=int(max(1, E * 4 * frac)), advisek=k, flags=flags, split_weight=split)
+    rmat_w.set_tuning(delta=delta)
+    for s in _sources(ptr, E, 3, 17):
+        d, st = rmat_w.sssp(s)
+        assert np.array_equal(d, oracle.sssp(ptr, to, w, s)), s
+        assert st["checksum"] == og.levels_checksum(d)
+    assert np.array_equal(rmat_w.bfs(5)[0], oracle.bfs(ptr, to, 5)[0])
+    assert np.array_equal(rmat_w.cc()[0], oracle.cc(ptr, to)[0])
+    lv, _, _, _ = rmat_w.msbfs([5, 9])
+    assert np.array_equal(lv[1], oracle.bfs(ptr, to, 9)[0])
+    rmat_w.set_tuning(delta=0)
+
+
+def test_light_heavy_split_refused(gpu):
+    """The split needs the packed weighted format."""
+    g = og.Graph.rmat(10, 8, seed=3, weight_seed=4)
+    with pytest.raises(og.OgError) as e:
+        g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0, flags=og.PLAN_NO_PACK, split_weight=16)
+    assert e.value.name == "OG_EUNSUPPORTED"
+    h = og.Graph.rmat(10, 8, seed=3)
+    with pytest.raises(og.OgError):
+        h.prepare("ascetic", edge_budget_bytes=h.num_edges * 4, advisek=1.0, split_weight=16)
+    g.close()
+    h.close()
