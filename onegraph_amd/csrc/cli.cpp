@@ -110,6 +110,8 @@ og_status prepare(og_graph *g, float advisek, uint32_t param_size, const Options
     p.advisek = advisek;
     p.param_size = param_size;
     p.edge_budget_bytes = opt.edge_budget_bytes;
+    p.flags = opt.plan_flags;
+    p.split_weight = info.kind == OG_KIND_WEIGHTED ? opt.split_weight : 0;
     if (opt.budget_fraction > 0) {
         const uint64_t eb = info.kind == OG_KIND_WEIGHTED ? 8 : 4;
         p.edge_budget_bytes = (uint64_t)(opt.budget_fraction * (double)(info.num_edges * eb));
@@ -117,6 +119,13 @@ og_status prepare(og_graph *g, float advisek, uint32_t param_size, const Options
     }
     og_status st = og_prepare(g, opt.approach, &p, plan);
     if (st != OG_OK) return report_error(st, "prepare");
+    if (opt.delta >= 0 && info.kind != OG_KIND_PULL) {
+        og_tuning t;
+        og_get_tuning(g, &t);
+        t.delta = (uint32_t)opt.delta;
+        st = og_set_tuning(g, &t);
+        if (st != OG_OK) return report_error(st, "tuning");
+    }
     printf("approach %s: total_gpu_size %" PRIu64 " max_partition_size %" PRIu64 " partOverloadSize %" PRIu64
            " overloadSize %" PRIu64 " adviseK %f\n",
            opt.approach == OG_ASCETIC ? "ascetic" : opt.approach == OG_SUBWAY ? "subway" : "usm",
@@ -148,6 +157,9 @@ og_status finish(const Options &opt, const void *data, uint64_t n, int bytes, bo
 
 using RunFn = og_status (*)(og_graph *, uint32_t, uint32_t *, og_run_stats *);
 
+og_status bfs_fn(og_graph *g, uint32_t s, uint32_t *o, og_run_stats *st) { return og_bfs(g, s, o, st); }
+og_status sssp_fn(og_graph *g, uint32_t s, uint32_t *o, og_run_stats *st) { return og_sssp(g, s, o, st); }
+
 og_status run_sources(const char *algo, og_graph *g, uint32_t source, const Options &opt, RunFn fn,
                       const og_plan_info &plan) {
     og_graph_info info;
@@ -163,6 +175,15 @@ og_status run_sources(const char *algo, og_graph *g, uint32_t source, const Opti
         sources.push_back(source);
     }
     std::vector<uint32_t> out(info.num_vertices);
+    if (opt.autotune) {
+        og_tuning best;
+        double ms = 0;
+        const uint32_t which = fn == sssp_fn ? OG_TUNE_SSSP : OG_TUNE_BFS;
+        og_status s = og_autotune(g, which, sources[0], 3, &best, &ms);
+        if (s != OG_OK) return report_error(s, "autotune");
+        printf("autotune: t1 %u lpv0 %u t2 %u tile %u delta %u defer %u dyn_grid %u (%.3f ms)\n", best.t1, best.lpv0,
+               best.t2, best.tile, best.delta, best.defer, best.dyn_grid, ms);
+    }
     for (size_t t = 0; t < sources.size(); ++t) {
         printf("----------testIndex = %zu-----------\nsourceNode %u\n", t, sources[t]);
         for (uint32_t r = 0; r < opt.repeat; ++r) {
@@ -174,9 +195,6 @@ og_status run_sources(const char *algo, og_graph *g, uint32_t source, const Opti
     }
     return finish(opt, out.data(), out.size(), 4, false);
 }
-
-og_status bfs_fn(og_graph *g, uint32_t s, uint32_t *o, og_run_stats *st) { return og_bfs(g, s, o, st); }
-og_status sssp_fn(og_graph *g, uint32_t s, uint32_t *o, og_run_stats *st) { return og_sssp(g, s, o, st); }
 
 }  // namespace
 
@@ -329,6 +347,13 @@ static void usage(FILE *f) {
             "                [--generate rmat:SCALE:EF:SEED[:WSEED] | er:N:AVG:SEED] [--save FILE]\n"
             "                [--text [--text-weights] [--symmetrize] [--keep-self-loops] [--one-based]\n"
             "                 [--weight-seed S]] [--check FILE [--check-tol T]]\n"
+            "                [--loader copy|zerocopy] [--delta W] [--split-weight W] [--direction-opt]\n"
+            "                [--autotune]\n"
+            "  --loader zerocopy: the dynamic region is read in place over the host link;\n"
+            "  --delta W: SSSP near-far bucket width; --split-weight W: SSSP rows store\n"
+            "  edges lighter than W first (delta-stepping light/heavy); --direction-opt:\n"
+            "  bottom-up BFS rounds (whole graph in HBM); --autotune: tune bins, delta and\n"
+            "  grid shapes from the first source before the runs.\n"
             "  --text: -f is a text edge list (\"u v [w]\" lines, #/%% comments) or a\n"
             "  MatrixMarket file; with --save FILE it is converted to the reference CSR\n"
             "  format of --type (bfs/cc plain, sssp weighted, pr pull) and written.\n"
@@ -443,6 +468,21 @@ int main(int argc, char **argv) {
         } else if (a == "--compare") {
             cmp_a = need("--compare");
             cmp_b = need("--compare");
+        } else if (a == "--loader") {
+            const std::string v = need("--loader");
+            if (v == "zerocopy") opt.plan_flags |= OG_PLAN_ZERO_COPY;
+            else if (v != "copy") {
+                fprintf(stderr, "onegraph: --loader takes copy or zerocopy\n");
+                return 2;
+            }
+        } else if (a == "--direction-opt") {
+            opt.plan_flags |= OG_PLAN_DIRECTION_OPT;
+        } else if (a == "--delta") {
+            opt.delta = atoll(need("--delta"));
+        } else if (a == "--split-weight") {
+            opt.split_weight = (uint32_t)atoi(need("--split-weight"));
+        } else if (a == "--autotune") {
+            opt.autotune = true;
         } else if (a == "--dtype") {
             dtype = need("--dtype");
         } else {

@@ -36,6 +36,11 @@ struct Options {
     uint64_t weight_seed = 1;        // text input without weights, weighted algorithm: seeded [1, 63]
     std::string check_path;          // result array to compare with (same layout as out_path)
     double check_tol = 1e-6;         // PageRank: max-abs tolerance of the check
+    // engine options with no reference counterpart
+    uint32_t plan_flags = 0;         // OG_PLAN_* (zero-copy loader, direction-optimising BFS, ...)
+    uint32_t split_weight = 0;       // og_plan_params::split_weight (SSSP light/heavy rows)
+    int64_t delta = -1;              // >= 0: og_tuning::delta (SSSP near-far bucket width)
+    bool autotune = false;           // og_autotune from the (first) source before the runs
 };
 
 // Result-array file (--out / --check): V little-endian values, u32 levels /

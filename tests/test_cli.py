@@ -20,6 +20,7 @@ def test_usage_and_errors(tmp_path):
     assert run("--type", "bfs").returncode == 2          # no -f
     assert run("--type", "bfs", "-f").returncode == 2    # flag without value
     assert run("--type", "xx", "-f", "x").returncode == 2
+    assert run("--type", "bfs", "-f", "x", "--loader", "fast").returncode == 2
     r = run("--type", "bfs", "-f", str(tmp_path / "missing.bcsr"))
     assert r.returncode == 1 and "OG_EIO" not in r.stdout  # message goes to stderr
     assert "cannot open" in r.stderr
@@ -40,6 +41,10 @@ def test_generate_and_save(tmp_path):
     ("sssp", ["-adviseK", "0.8", "--budget-fraction", "0.25"]),
     ("cc", ["--adviseK", "0.5", "--budget-fraction", "0.1", "--approach", "subway"]),
     ("bfs", ["--approach", "usm", "--method", "1"]),
+    ("sssp", ["--advisek", "0.8", "--budget-fraction", "0.25", "--loader", "zerocopy", "--delta", "16",
+              "--split-weight", "16", "--autotune"]),
+    ("bfs", ["--advisek", "1.0", "--budget-fraction", "1.0", "--direction-opt", "--repeat", "2"]),
+    ("cc", ["--advisek", "0.8", "--budget-fraction", "0.25", "--loader", "zerocopy"]),
 ])
 def test_cli_runs_match_oracle(gpu, tmp_path, algo, flags):
     f = str(tmp_path / "g.csr")
