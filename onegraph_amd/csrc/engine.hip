@@ -166,6 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
             uint32_t jj[kLoadBatch];
             uint64_t pb[kLoadBatch], pe[kLoadBatch];
             uint32_t vv[kLoadBatch];
+            bool far[kLoadBatch];
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
                 jj[k] = kFlagsPerThread;  // empty slot
@@ -175,9 +176,19 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                     const uint32_t j = (uint32_t)__builtin_ctz(m);
                     m &= m - 1;
                     jj[k] = j;
-                    row_range(ptr, pmid, heavy, base + j, pb[k], pe[k]);
                     if (val) vv[k] = val[base + j];
+                    else row_range(ptr, pmid, heavy, base + j, pb[k], pe[k]);
                 }
+            }
+            // near-far: a held-back vertex needs only its value, so the row
+            // bounds are read for the near ones alone
+#pragma unroll
+            for (int k = 0; k < kLoadBatch; ++k) {
+                far[k] = false;
+                if (jj[k] == kFlagsPerThread || !val) continue;
+                const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + jj[k] >= vsplit;
+                far[k] = vv[k] >= thresh || lag;
+                if (!far[k]) row_range(ptr, pmid, heavy, base + jj[k], pb[k], pe[k]);
             }
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
@@ -185,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                 const uint64_t deg = pe[k] - pb[k];
                 const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + jj[k] >= vsplit;
                 cl += lag ? 1u : 0u;
-                const int b = bin_of((val && vv[k] >= thresh) || lag, base + jj[k], vsplit, deg, tn);
+                const int b = bin_of(far[k], base + jj[k], vsplit, deg, tn);
                 switch (b) {
                     case 0: ++c0; sdeg += deg; break;
                     case 1: ++c1; sdeg += deg; break;
