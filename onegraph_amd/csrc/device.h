@@ -28,6 +28,13 @@ constexpr int kVertsPerBlock = kBlock * kFlagsPerThread;   // 8192
 constexpr int kDynEdgesPerThread = 8;   // dynamic relax: edges per lane
 constexpr int kDynEdgesPerBlock = kBlock * kDynEdgesPerThread;
 constexpr int kMaxSlots = 16;           // staging slots of the dynamic-region pipeline
+// Zero-copy loading keeps part of the dynamic region in HBM in whole chunks of
+// kResChunk consecutive vertices; the link moves pinned host memory in lines
+// of kLinkLine bytes, each fetched whole.
+constexpr uint32_t kResChunkShift = 5;
+constexpr uint32_t kResChunk = 1u << kResChunkShift;
+constexpr int64_t kNotResident = INT64_MIN;
+constexpr uint32_t kLinkLine = 128;
 constexpr uint64_t kSlotTargetBytes = 256ull << 20;  // preferred slot (chunk) size
 
 enum Algo { ALGO_BFS = 0, ALGO_SSSP = 1, ALGO_CC = 2, ALGO_MSBFS = 3 };
@@ -230,7 +237,7 @@ struct DeviceState {
     const uint64_t *run_pmid = nullptr;  // d_pmid while a run uses the split (SSSP near-far), else null
     void *d_static = nullptr;        // static-region edges (USM: managed, whole graph)
     uint64_t static_used = 0;        // edges of the static vertices, = ptr[vsplit]
-    void *d_staging = nullptr;       // nslots * slot_edges edges (zero-copy: the resident head)
+    void *d_staging = nullptr;       // nslots * slot_edges edges (zero-copy: the resident chunks)
     uint64_t slot_edges = 0;
     bool managed = false;
     // Weighted edges on the device and the link: one 4-byte word
@@ -242,8 +249,9 @@ struct DeviceState {
     const void *h_dyn = nullptr;     // host source of dynamic edges (packed copy or the CSR)
     const void *hd_dyn = nullptr;    // the same, as a device address (zero-copy mode)
     uint64_t slot_dyn = 0;           // dynamic edges per staging slot in the transfer format
-    uint64_t dyn_resident = 0;       // zero-copy: edges [dyn_base, dyn_base + this) held in d_staging
-                                     // (the whole rows right after the static prefix)
+    uint64_t dyn_resident = 0;       // zero-copy: dynamic edges held in d_staging (whole chunks)
+    int64_t *d_rdelta = nullptr;     // zero-copy: per chunk of kResChunk vertices (index v >> kResChunkShift),
+                                     // d_staging index - CSR edge index, or kNotResident
     uint64_t *m_ptr = nullptr;       // USM managed copies
     void *m_edges = nullptr;
 

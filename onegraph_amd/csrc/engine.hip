@@ -673,11 +673,12 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
 // Dynamic region, edges [c0, min(c1, D)) of the concatenated active dynamic
 // edge space (D = sum of the dynamic list's degrees; the list sits at
 // list[n0 + n1, n0 + n1 + n3)).  STAGED (DIRECT = false): they sit in a
-// staging slot, edge q at q - c0.  DIRECT: edge q of vertex u is
-// dynamic-region edge idx = ptr[u] - dbase + (q - doff[i]); the first nres of
-// those (whole rows) are resident in HBM (`resident`), the rest are read in
-// place from the pinned host array through its device mapping (`stage`);
-// reads served from HBM are counted in *hits.  (Storing the host rows in
+// staging slot, edge q at q - c0.  DIRECT: edge q of vertex u is CSR edge
+// x = ptr[u] + (q - doff[i]); when u's chunk of kResChunk vertices is
+// resident in HBM (rdelta[u >> kResChunkShift] != kNotResident) it is read
+// from `resident` at x + rdelta, else in place from the pinned host array
+// through its device mapping (`stage`, at x - dbase); reads served from HBM
+// are counted in *hits.  (Storing the host rows in
 // order of distance from a hub instead of vertex order was measured slower
 // on R-MAT: the active list stays in vertex order, so neighbouring lanes then
 // read scattered rows.)
@@ -691,7 +692,8 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
                                                       const uint64_t *__restrict__ doff,
                                                       const void *__restrict__ stage, uint64_t c0, uint64_t c1max,
                                                       const uint64_t *__restrict__ ptr, uint64_t dbase,
-                                                      const void *__restrict__ resident, uint64_t nres,
+                                                      const void *__restrict__ resident,
+                                                      const int64_t *__restrict__ rdelta,
                                                       uint32_t wbits, RelaxArgs a, unsigned long long *upd,
                                                       unsigned long long *hits) {
     __shared__ uint64_t s_off[kDynEdgesPerBlock + 2];
@@ -736,13 +738,17 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
                 }
                 const uint32_t u = dlist[i0 + lo];
                 sv[j] = load_src<ALGO>(a, u);
-                const uint64_t rb = (a.pmid && heavy) ? a.pmid[u] : ptr[u];
-                const uint64_t idx = DIRECT ? rb - dbase + (q - s_off[lo]) : q - c0;
-                if (DIRECT && idx < nres) {
-                    load_edge<W, PACKED>(resident, idx, wbits, to[j], wt[j]);
-                    ++h;
+                if (DIRECT) {
+                    const uint64_t x = ((a.pmid && heavy) ? a.pmid[u] : ptr[u]) + (q - s_off[lo]);
+                    const int64_t dl = rdelta ? rdelta[u >> kResChunkShift] : kNotResident;
+                    if (dl != kNotResident) {
+                        load_edge<W, PACKED>(resident, (uint64_t)((int64_t)x + dl), wbits, to[j], wt[j]);
+                        ++h;
+                    } else {
+                        load_edge<W, PACKED>(stage, x - dbase, wbits, to[j], wt[j]);
+                    }
                 } else {
-                    load_edge<W, PACKED>(stage, idx, wbits, to[j], wt[j]);
+                    load_edge<W, PACKED>(stage, q - c0, wbits, to[j], wt[j]);
                 }
             }
         }
@@ -1057,6 +1063,77 @@ static void upload_packed(DeviceState *d, const og_graph *g, uint64_t vend, uint
     OG_HIP(hipStreamSynchronize(d->s_compute));
 }
 
+// Zero-copy loading reads dynamic rows in place, so the staging share of the
+// edge budget (cap edges) holds part of the dynamic region in HBM: whole
+// chunks of kResChunk consecutive vertices, chosen by the link lines their
+// rows would cost per edge.  A row of b bytes at an arbitrary 4-byte offset
+// touches (b + kLinkLine - 4) / kLinkLine lines of kLinkLine bytes on average,
+// each fetched whole, so chunks of short rows are taken first (ties: lower
+// ids).  The chunk table (8 bytes per chunk) is charged to the same share.
+static void stage_resident(DeviceState *d, const og_graph *g, uint64_t vsplit, uint64_t cap) {
+    const uint64_t V = g->V, eb = d->eb;
+    const uint64_t c0 = vsplit >> kResChunkShift, c1 = (V + kResChunk - 1) >> kResChunkShift;
+    const uint64_t table_edges = (c1 * sizeof(int64_t) + eb - 1) / eb;
+    if (vsplit >= V || cap <= table_edges) return;
+    cap -= table_edges;
+    const uint64_t nch = c1 - c0;
+    auto first = [&](uint64_t i) { return std::max<uint64_t>((c0 + i) << kResChunkShift, vsplit); };
+    auto last = [&](uint64_t i) { return std::min<uint64_t>((c0 + i + 1) << kResChunkShift, V); };
+    std::vector<uint64_t> cedges(nch);
+    std::vector<double> score(nch);
+    parallel_for(nch, 1 << 12, 0, [&](uint64_t b, uint64_t e, unsigned) {
+        for (uint64_t i = b; i < e; ++i) {
+            double lines = 0;
+            for (uint64_t v = first(i); v < last(i); ++v) {
+                const uint64_t n = g->ptr[v + 1] - g->ptr[v];
+                if (n) lines += (double)(n * eb + kLinkLine - 4) / kLinkLine;
+            }
+            cedges[i] = g->ptr[last(i)] - g->ptr[first(i)];
+            score[i] = cedges[i] ? lines / (double)cedges[i] : 0.0;
+        }
+    });
+    std::vector<uint32_t> order(nch);  // nch <= 2^27: vertex ids are 32-bit
+    for (uint64_t i = 0; i < nch; ++i) order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return score[x] > score[y]; });
+    std::vector<uint8_t> take(nch, 0);
+    uint64_t used = 0;
+    for (uint32_t i : order) {
+        if (!cedges[i] || used + cedges[i] > cap) continue;
+        take[i] = 1;
+        used += cedges[i];
+    }
+    if (!used) return;
+    // the resident buffer holds the chosen chunks in vertex order; runs of
+    // adjacent chunks are gathered from the host source in one piece
+    struct Run {
+        uint64_t src, dst, n;
+    };
+    std::vector<Run> runs;
+    std::vector<int64_t> delta(c1, kNotResident);
+    uint64_t pos = 0;
+    for (uint64_t i = 0; i < nch; ++i) {
+        if (!take[i]) continue;
+        const uint64_t b = g->ptr[first(i)];
+        delta[c0 + i] = (int64_t)pos - (int64_t)b;
+        if (!runs.empty() && runs.back().src + runs.back().n == b) runs.back().n += cedges[i];
+        else runs.push_back({b, pos, cedges[i]});
+        pos += cedges[i];
+    }
+    MemPool tmp;
+    uint8_t *hb = (uint8_t *)tmp.palloc(used * eb);
+    const uint8_t *src = (const uint8_t *)d->h_dyn;  // its edge 0 is CSR edge dyn_base
+    parallel_for(runs.size(), 64, 0, [&](uint64_t b, uint64_t e, unsigned) {
+        for (uint64_t r = b; r < e; ++r)
+            memcpy(hb + runs[r].dst * eb, src + (runs[r].src - d->dyn_base) * eb, runs[r].n * eb);
+    });
+    d->dyn_resident = used;
+    d->d_staging = d->dalloc(used * eb);
+    d->d_rdelta = (int64_t *)d->dalloc(c1 * sizeof(int64_t));
+    OG_HIP(hipMemcpyAsync(d->d_staging, hb, used * eb, hipMemcpyHostToDevice, d->s_compute));
+    OG_HIP(hipMemcpyAsync(d->d_rdelta, delta.data(), c1 * sizeof(int64_t), hipMemcpyHostToDevice, d->s_compute));
+    OG_HIP(hipStreamSynchronize(d->s_compute));
+}
+
 static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params *params, og_plan_info *info) {
     if (!hip_available()) raise(OG_EHIP, "no HIP device is visible");
     if (approach != OG_ASCETIC && approach != OG_SUBWAY && approach != OG_USM)
@@ -1229,18 +1306,9 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
                 OG_HIP(hipHostGetDevicePointer(&hd, (void *)hbase, 0));
                 d->hd_dyn = (const uint8_t *)hd + ((const uint8_t *)d->h_dyn - (const uint8_t *)hbase);
                 // Reading in place needs no transit slots, so the staging
-                // share of the budget keeps the head of the dynamic region
-                // (the whole rows right after the static prefix) resident in
-                // HBM.
-                const uint64_t cap = std::min<uint64_t>(pl.staging_edges, nd);
-                uint64_t vres = pl.vsplit;
-                while (vres < V && g->ptr[vres + 1] - d->dyn_base <= cap) ++vres;
-                d->dyn_resident = g->ptr[vres] - d->dyn_base;
-                if (d->dyn_resident) {
-                    d->d_staging = d->dalloc(d->dyn_resident * eb);
-                    OG_HIP(hipMemcpyAsync(d->d_staging, d->h_dyn, d->dyn_resident * eb, hipMemcpyHostToDevice,
-                                          d->s_compute));
-                }
+                // share of the budget keeps part of the dynamic region
+                // resident in HBM instead.
+                stage_resident(d.get(), g, pl.vsplit, pl.staging_edges);
             }
         }
     }
@@ -1486,7 +1554,7 @@ template <int ALGO, bool W, bool PACKED, bool DIRECT>
 static void launch_dyn(DeviceState *d, unsigned grid, const void *src, uint64_t c0, uint64_t c1) {
     hipLaunchKernelGGL((k_relax_dyn<ALGO, W, PACKED, DIRECT>), dim3(grid), dim3(kBlock), 0, d->s_compute, d->fr.list,
                        d->fr.info, d->fr.doff, src, c0, c1, d->d_ptr, d->dyn_base, (const void *)d->d_staging,
-                       DIRECT ? d->dyn_resident : 0, d->wbits, relax_args(d), d->fr.counters + 1,
+                       DIRECT ? (const int64_t *)d->d_rdelta : nullptr, d->wbits, relax_args(d), d->fr.counters + 1,
                        d->fr.counters + 5);
 }
 
