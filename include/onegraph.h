@@ -181,7 +181,10 @@ typedef struct og_plan_params {
                                    (for A/B timing)                              */
 #define OG_PLAN_ZERO_COPY 2u    /* dynamic region: the relax reads active edges
                                    straight from pinned host memory over the link
-                                   (no staging copies, no host gather)          */
+                                   (no staging copies, no host gather); the
+                                   staging share of the budget keeps the chunks
+                                   of 32 vertices whose rows cost the most link
+                                   lines per edge resident in HBM instead      */
 #define OG_PLAN_NO_PACK 4u      /* keep 8-byte weighted edges in HBM and on the
                                    link (no 4-byte (to << wbits | w) format)     */
 #define OG_PLAN_NO_GRAPH 8u     /* enqueue device-driven rounds one API call at
