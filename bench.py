@@ -454,11 +454,14 @@ def wl_serve(args):
         srv.close()
         bad = 0
         lat_kind = {0: [], 1: []}
+        lat_graph = {}
         for qid, r in done.items():
             k, lag = submit_at[qid]
             lt = r["latency_ms"] + 1e3 * lag  # from the scheduled arrival, not from the submit call
             lat.append(lt)
             lat_kind[r["kind"]].append(lt)
+            lat_graph.setdefault(f"{'sssp' if r['kind'] else 'bfs'}_scale{scales[r['graph']]}", []).append(
+                (lt, r["queue_ms"] + 1e3 * lag, r["rounds"]))
             if r["status"] != 0:
                 bad += 1
         # spot-check against the single-query path (graphs are free again)
@@ -482,6 +485,13 @@ def wl_serve(args):
             if lat_kind[kk]:
                 res[f"{name}_p50_ms"] = round(_pct(lat_kind[kk], 50), 3)
                 res[f"{name}_p99_ms"] = round(_pct(lat_kind[kk], 99), 3)
+        res["per_graph"] = {
+            k: {"n": len(v), "p50_ms": round(_pct([x[0] for x in v], 50), 3),
+                "p99_ms": round(_pct([x[0] for x in v], 99), 3),
+                "queue_p99_ms": round(_pct([x[1] for x in v], 99), 3),
+                "mean_rounds": round(float(np.mean([x[2] for x in v])), 1),
+                "ms_per_round": round(float(np.mean([(x[0] - x[1]) / max(x[2], 1) for x in v])), 3)}
+            for k, v in sorted(lat_graph.items())}
         return res
 
     per_mix = {}

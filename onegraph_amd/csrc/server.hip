@@ -78,30 +78,49 @@ __global__ void k_srv_seed(SeedList sl, uint64_t *next, uint8_t *flag, unsigned 
     }
 }
 
-// Commit sweep over the flags, 16 vertices per thread step (one uint4 load):
+// Commit sweep over the flags, consecutive lanes on consecutive vertices (so
+// a wave's visit / next / front / ptr traffic is whole cache lines, and a
+// wave whose 64 flags are clear moves on after one coalesced byte load):
 // nw = next & ~visit becomes the vertex's frontier mask (when it has
 // out-edges); each slot in nw adds the vertex's level to its digest in
 // acc[0..63] (and writes it to a kept result array) and counts the vertex
 // and its out-degree in fstat (the round's newly reached vertices and their
 // edges per slot).  Flags end up 1 exactly for vertices with a non-empty
-// mask; *alive gets the OR of all masks.
-__global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ flag, uint64_t V, uint64_t nchunks,
+// mask; *alive gets the OR of all masks.  The per-slot sums are kept in
+// kCommitCopies copies in LDS, lane l adding into copy l % kCommitCopies:
+// lanes of a wave that reach the same slot would otherwise all add into one
+// address, which the LDS serialises; a stride of 65 words puts the copies of
+// a slot on distinct banks.  The slots' levels and kept-result pointers are
+// staged in LDS first: indexed per lane, the by-value kernel arguments would
+// be read from memory once per (vertex, slot).
+constexpr int kCommitCopies = 16;
+constexpr int kCommitStride = 65;
+__global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ flag, uint64_t V,
                                                        const uint64_t *__restrict__ ptr, uint64_t *__restrict__ visit,
                                                        uint64_t *__restrict__ front, uint64_t *__restrict__ next,
                                                        SlotLevels L, KeepList keep, unsigned long long *acc,
                                                        unsigned long long *fstat, unsigned long long *alive) {
-    __shared__ unsigned long long sdig[64], sfc[64], sfe[64];
-    if (threadIdx.x < 64) sdig[threadIdx.x] = sfc[threadIdx.x] = sfe[threadIdx.x] = 0;
+    __shared__ unsigned long long sdig[kCommitCopies * kCommitStride], sfc[kCommitCopies * kCommitStride],
+        sfe[kCommitCopies * kCommitStride];
+    __shared__ uint32_t slv[64];
+    __shared__ uint32_t *skeep[64];
+    __shared__ unsigned long long skm;
+    for (int i = threadIdx.x; i < kCommitCopies * kCommitStride; i += kBlock) sdig[i] = sfc[i] = sfe[i] = 0;
+    if (threadIdx.x < 64) {
+        slv[threadIdx.x] = L.lv[threadIdx.x];
+        skeep[threadIdx.x] = keep.p[threadIdx.x];
+        const unsigned long long km = __ballot(keep.p[threadIdx.x] != nullptr);
+        if (threadIdx.x == 0) skm = km;
+    }
     __syncthreads();
+    const int cp = (int)(threadIdx.x % kCommitCopies) * kCommitStride;
+    const uint64_t kmask = skm;
     uint64_t al = 0;
-    for (uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x; c < nchunks; c += (uint64_t)gridDim.x * kBlock) {
-        const uint4 w = ((const uint4 *)flag)[c];
-        if (!(w.x | w.y | w.z | w.w)) continue;
-        const uint32_t q[4] = {w.x, w.y, w.z, w.w};
-        for (int k = 0; k < 16; ++k) {
-            if (!((q[k >> 2] >> (8 * (k & 3))) & 0xFFu)) continue;
-            const uint64_t v = c * 16 + k;
-            if (v >= V) break;
+    for (uint64_t b = (uint64_t)blockIdx.x * kBlock; b < V; b += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t v = b + threadIdx.x;
+        const bool on = v < V && flag[v];
+        if (!__ballot(on)) continue;  // b is the same for the whole wave
+        if (on) {
             const uint64_t vis = visit[v];
             const uint64_t nw = next[v] & ~vis;
             next[v] = 0;
@@ -114,10 +133,11 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
                 while (m) {
                     const int j = __builtin_ctzll(m);
                     m &= m - 1;
-                    atomicAdd(&sdig[j], (unsigned long long)srv_mix64((v << 32) | L.lv[j]));
-                    atomicAdd(&sfc[j], 1ull);
-                    if (deg) atomicAdd(&sfe[j], (unsigned long long)deg);
-                    if (keep.p[j]) keep.p[j][v] = L.lv[j];
+                    const uint32_t lv = slv[j];
+                    atomicAdd(&sdig[cp + j], (unsigned long long)srv_mix64((v << 32) | lv));
+                    atomicAdd(&sfc[cp + j], 1ull);
+                    if (deg) atomicAdd(&sfe[cp + j], (unsigned long long)deg);
+                    if ((kmask >> j) & 1) skeep[j][v] = lv;
                 }
             }
             const uint64_t fr = deg ? nw : 0;
@@ -133,10 +153,16 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
     if ((threadIdx.x & (kWave - 1)) == 0 && al) atomicOr(alive, (unsigned long long)al);
     __syncthreads();
     if (threadIdx.x < 64) {
-        if (sdig[threadIdx.x]) atomicAdd(acc + threadIdx.x, sdig[threadIdx.x]);
-        if (sfc[threadIdx.x]) {
-            atomicAdd(fstat + threadIdx.x, sfc[threadIdx.x]);
-            if (sfe[threadIdx.x]) atomicAdd(fstat + 64 + threadIdx.x, sfe[threadIdx.x]);
+        unsigned long long dg = 0, fc = 0, fe = 0;
+        for (int c = 0; c < kCommitCopies; ++c) {
+            dg += sdig[c * kCommitStride + threadIdx.x];
+            fc += sfc[c * kCommitStride + threadIdx.x];
+            fe += sfe[c * kCommitStride + threadIdx.x];
+        }
+        if (dg) atomicAdd(acc + threadIdx.x, dg);
+        if (fc) {
+            atomicAdd(fstat + threadIdx.x, fc);
+            if (fe) atomicAdd(fstat + 64 + threadIdx.x, fe);
         }
     }
 }
@@ -424,10 +450,9 @@ void BfsEngine::step() {
     }
     const bool dir_opt = d->d_in_ptr != nullptr;
     OG_HIP(hipMemsetAsync(d_alive, 0, 8, s));
-    const uint64_t nchunks = d->Vpad / 16;
     SlotLevels L{};
     for (uint32_t j = 0; j < kSlots; ++j) L.lv[j] = round - slots[j].start_round + 1;
-    hipLaunchKernelGGL(k_srv_commit, dim3(sweep_grid(nchunks, 4096)), dim3(kBlock), 0, s, f.flag, V, nchunks,
+    hipLaunchKernelGGL(k_srv_commit, dim3(sweep_grid(V, 4096)), dim3(kBlock), 0, s, f.flag, V,
                        d->d_ptr, visit, front, next, L, keep_list(used), d_acc, d_fstat, d_alive);
     OG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_srv_direction, dim3(1), dim3(64), 0, s, used, V, d->E, dir_opt ? 1 : 0, d_fstat,
