@@ -134,8 +134,12 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
 
 // Compaction pass 1: classifies every flagged vertex of the block's window,
 // stores kCode0 + bin in its flag byte and publishes the per-block counts.
-// Row bounds (and values, for near-far) of kLoadBatch flagged vertices are
-// read before any is used, so their latencies overlap.
+// The window's flags are loaded 16 bytes per lane and transposed through
+// LDS, so that slot j of thread t is vertex j * kBlock + t of the window:
+// consecutive lanes then read consecutive row bounds and values and store
+// consecutive flag bytes (only the block totals depend on the mapping).  Row
+// bounds (and values, for near-far) of kLoadBatch flagged vertices are read
+// before any is used, so their latencies overlap.
 __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__ flag_a, uint8_t *__restrict__ hflag,
                                                            uint64_t Vpad, uint64_t vsplit,
                                                            const uint64_t *__restrict__ ptr,
@@ -145,17 +149,25 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                                                            uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
                                                            uint32_t nblk) {
     __shared__ uint64_t lds[6 * kBlock / kWave];
+    __shared__ uint4 s_win[kVertsPerBlock / 16];  // the window's flag bytes
     // light/heavy split: a heavy compaction collects the heavy-pending
     // vertices (no near-far test) and sizes them by their heavy edges
     const bool heavy = pmid && ctl->mode;
     uint8_t *flag = heavy ? hflag : flag_a;
     const uint32_t *val = heavy ? nullptr : val_a;
-    const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kFlagsPerThread;
+    const uint64_t wbase = (uint64_t)blockIdx.x * kVertsPerBlock;
     uint4 w[kFlagVecs];
-    load_flags(flag, base, Vpad, w);
+    load_flags(flag, wbase + (uint64_t)threadIdx.x * kFlagsPerThread, Vpad, w);
+#pragma unroll
+    for (int k = 0; k < kFlagVecs; ++k) s_win[threadIdx.x * kFlagVecs + k] = w[k];
+    __syncthreads();
+    const uint8_t *sb = (const uint8_t *)s_win;
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < kFlagsPerThread; ++j) m |= (sb[j * kBlock + threadIdx.x] != 0 ? 1u : 0u) << j;
+    const uint64_t base = wbase + threadIdx.x;  // slot j: vertex base + j * kBlock
     uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, cz = 0, ch = 0, cf = 0, cl = 0, mf = 0xFFFFFFFFu;
     uint64_t sdeg = 0, ddeg = 0;
-    uint32_t m = window_mask(w, 1);
     if (m) {
         const uint32_t thresh = val ? ctl->thresh : 0xFFFFFFFFu;
         // lagged order: a dynamic vertex below the threshold but not below
@@ -176,8 +188,8 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                     const uint32_t j = (uint32_t)__builtin_ctz(m);
                     m &= m - 1;
                     jj[k] = j;
-                    if (val) vv[k] = val[base + j];
-                    else row_range(ptr, pmid, heavy, base + j, pb[k], pe[k]);
+                    if (val) vv[k] = val[base + (uint64_t)j * kBlock];
+                    else row_range(ptr, pmid, heavy, base + (uint64_t)j * kBlock, pb[k], pe[k]);
                 }
             }
             // near-far: a held-back vertex needs only its value, so the row
@@ -186,17 +198,17 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
             for (int k = 0; k < kLoadBatch; ++k) {
                 far[k] = false;
                 if (jj[k] == kFlagsPerThread || !val) continue;
-                const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + jj[k] >= vsplit;
+                const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + (uint64_t)jj[k] * kBlock >= vsplit;
                 far[k] = vv[k] >= thresh || lag;
-                if (!far[k]) row_range(ptr, pmid, heavy, base + jj[k], pb[k], pe[k]);
+                if (!far[k]) row_range(ptr, pmid, heavy, base + (uint64_t)jj[k] * kBlock, pb[k], pe[k]);
             }
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
                 if (jj[k] == kFlagsPerThread) continue;
                 const uint64_t deg = pe[k] - pb[k];
-                const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + jj[k] >= vsplit;
+                const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + (uint64_t)jj[k] * kBlock >= vsplit;
                 cl += lag ? 1u : 0u;
-                const int b = bin_of(far[k], base + jj[k], vsplit, deg, tn);
+                const int b = bin_of(far[k], base + (uint64_t)jj[k] * kBlock, vsplit, deg, tn);
                 switch (b) {
                     case 0: ++c0; sdeg += deg; break;
                     case 1: ++c1; sdeg += deg; break;
@@ -205,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                     case 5: ++cf; mf = min(mf, vv[k]); break;
                     default: ++cz; break;
                 }
-                flag[base + jj[k]] = (uint8_t)(kCode0 + b);
+                flag[base + (uint64_t)jj[k] * kBlock] = (uint8_t)(kCode0 + b);
             }
         }
     }

@@ -21,16 +21,22 @@ def _say(*a):
 @pytest.fixture(scope="module", params=["topdown", "diropt"])
 def graphs(request):
     """Three graphs; "diropt" prepares them with their in-edges, so BFS slots
-    with dense frontiers run bottom-up rounds."""
+    with dense frontiers run bottom-up rounds (graphs.in_edges tells which)."""
     gs = [og.Graph.rmat(12, 16, seed=51, weight_seed=52), og.Graph.er(3000, 8, seed=53).add_weights(54),
           og.Graph.rmat(11, 8, seed=55)]
     flags = og.PLAN_DIRECTION_OPT if request.param == "diropt" else 0
     for g in gs:
         g.prepare("ascetic", edge_budget_bytes=g.num_edges * (8 if g.info()["kind"] == og.KIND_WEIGHTED else 4),
                   advisek=1.0, flags=flags)
-    yield gs
+    yield _Graphs(gs, request.param == "diropt")
     for g in gs:
         g.close()
+
+
+class _Graphs(list):
+    def __init__(self, gs, in_edges):
+        super().__init__(gs)
+        self.in_edges = in_edges
 
 
 def _state_bytes(graphs, K):
@@ -150,3 +156,25 @@ def test_admission_and_reservation(gpu, graphs):
         og.Server([h])
     assert e.value.name == "OG_EUNSUPPORTED"
     h.close()
+
+
+def test_burst_of_full_batches(gpu, graphs):
+    """64 BFS queries arriving at once fill every slot of an engine (their
+    dense rounds pull or push all together); queries arriving one by one
+    afterwards run alone.  Every checksum equals the plain path's."""
+    g = graphs[0]
+    ptr, to, _, _ = g.export()
+    deg = np.diff(np.append(ptr.astype(np.int64), len(to)))
+    srcs = np.random.default_rng(17).choice(np.flatnonzero(deg > 0), 96, replace=False)
+    _say("burst of 64 BFS, then 32 one by one, in_edges", graphs.in_edges)
+    with og.Server([g]) as srv:
+        ids = [srv.submit(0, og.QUERY_BFS, int(s)) for s in srcs[:64]]
+        res = _wait_all(srv, ids)
+        for s in srcs[64:]:
+            i = srv.submit(0, og.QUERY_BFS, int(s))
+            ids.append(i)
+            res.update(_wait_all(srv, [i]))
+            time.sleep(0.001)
+    for n, s in enumerate(srcs):
+        _, st = g.bfs(int(s), want_values=False)
+        assert st["checksum"] == res[ids[n]]["checksum"], (n, int(s))
