@@ -550,6 +550,7 @@ def main():
     ap.add_argument("--diropt", action="store_true", help="bfs-hbm: direction-optimising BFS (in-edge CSR in HBM)")
     ap.add_argument("--no-overlap", action="store_true", help="serialise copies, static and dynamic relax")
     ap.add_argument("--sweep-delta", default="", help="comma list: time SSSP at each delta and report")
+    ap.add_argument("--source-seed", type=int, default=3002, help="sssp-ooc: seed of the source pick")
     ap.add_argument("--dyn-grid", type=int, default=None, help="workgroups of the dynamic-region relax (0: 8192)")
     ap.add_argument("--split-weight", type=int, default=0,
                     help="SSSP light/heavy split: rows store edges of weight < this first (0: off)")
@@ -609,7 +610,7 @@ def main():
     V, E = g.num_vertices, g.num_edges
     log(f"graph: rmat scale {args.scale} V={V} E={E} generated in {time.time() - t0:.1f}s")
     ptr, to, w, _ = g.export()
-    src, deg = pick_source(ptr, E, 3002)
+    src, deg = pick_source(ptr, E, args.source_seed)
     budget = int(E * 8 * args.budget)
 
     def run_approach(approach, steps, warmup, loader=None):

@@ -1673,10 +1673,25 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
         eager(kClsFront, s, [&] { frontier_compact(d, d->fr, s, nf_val, false, pm); });
         // one batch: kRoundBatch x (static || dynamic relax, after, compaction),
         // then the traversal state to the host
+        // With both regions the dynamic relax is enqueued before the static
+        // fork: a replay submits the graph's kernels in capture order, and
+        // the link-bound relax should not wait for the static launches.
+        const bool overlap = has_dyn && !(d->flags & OG_PLAN_NO_OVERLAP);
         auto batch = [&](const auto &span) {
             for (int r = 0; r < kRoundBatch; ++r) {
-                const bool joined = launch_static<ALGO, W>(d, nullptr, span);
-                if (has_dyn) span(kClsDyn, s, [&] { launch_dyn_any<ALGO, W>(d, d->hd_dyn, 0, ~0ull, 0, true); });
+                bool joined = false;
+                if (overlap) {
+                    OG_HIP(hipEventRecord(d->ev_fork, s));
+                    span(kClsDyn, s, [&] { launch_dyn_any<ALGO, W>(d, d->hd_dyn, 0, ~0ull, 0, true); });
+                    OG_HIP(hipStreamWaitEvent(d->s_static, d->ev_fork, 0));
+                    span(kClsStat, d->s_static,
+                         [&] { launch_static_relax<ALGO, W>(d, d->fr, nullptr, relax_args(d), d->s_static); });
+                    OG_HIP(hipEventRecord(d->ev_static, d->s_static));
+                    joined = true;
+                } else {
+                    joined = launch_static<ALGO, W>(d, nullptr, span);
+                    if (has_dyn) span(kClsDyn, s, [&] { launch_dyn_any<ALGO, W>(d, d->hd_dyn, 0, ~0ull, 0, true); });
+                }
                 if (dir_opt)
                     span(kClsStat, s, [&] {
                         hipLaunchKernelGGL(k_bfs_pull, dim3(capped((d->V + kBlock - 1) / kBlock, kGroupGridCap)),
