@@ -190,6 +190,9 @@ typedef struct og_plan_params {
 #define OG_PLAN_NO_GRAPH 8u     /* enqueue device-driven rounds one API call at
                                    a time instead of replaying a captured
                                    hipGraph of them (for A/B timing)             */
+#define OG_PLAN_NO_FUSE 32u     /* zero-copy rounds: relax the static and the
+                                   dynamic region in two launches on two
+                                   streams instead of one (for A/B timing)     */
 #define OG_PLAN_DIRECTION_OPT 16u /* BFS on a graph wholly in HBM: rounds with a
                                    large frontier run bottom-up (every
                                    unvisited vertex scans its in-edges for a

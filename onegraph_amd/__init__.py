@@ -22,6 +22,7 @@ PLAN_ZERO_COPY = 2
 PLAN_NO_PACK = 4
 PLAN_NO_GRAPH = 8
 PLAN_DIRECTION_OPT = 16
+PLAN_NO_FUSE = 32
 EDGES_SYMMETRIZE, EDGES_KEEP_SELF_LOOPS, EDGES_ONE_BASED, EDGES_WEIGHTED = 1, 2, 4, 8
 
 STATUS = {

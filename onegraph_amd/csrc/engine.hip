@@ -591,20 +591,21 @@ __device__ __forceinline__ uint32_t commit(const RelaxArgs &a, typename SrcOf<AL
 
 // Bins 0 and 1: LPV lanes per frontier vertex (LPV = 64: one wave).  BIN
 // selects the list segment: bin 0 at list[0, n0), bin 1 at list[n0, n0 + n1).
+// The bodies take the workgroup's index and the number of workgroups of its
+// kind (bid, nb), so that one launch can run several kinds side by side.
 template <int ALGO, bool W, bool PK, int LPV>
-__global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restrict__ list, int bin,
-                                                        const FrontierInfo *__restrict__ info,
-                                                        const uint64_t *__restrict__ ptr,
-                                                        const void *__restrict__ edges, RelaxArgs a,
-                                                        unsigned long long *upd) {
-    if (info->pull) return;
+__device__ __forceinline__ uint32_t relax_group_body(const uint32_t *__restrict__ list, int bin,
+                                                     const FrontierInfo *__restrict__ info,
+                                                     const uint64_t *__restrict__ ptr, const void *__restrict__ edges,
+                                                     const RelaxArgs &a, uint32_t bid, uint32_t nb) {
+    if (info->pull) return 0;
     const bool heavy = info->heavy != 0;
     const uint32_t n = info->n[bin];
     const uint32_t *lst = list + (bin ? info->n[0] : 0u);
     const uint32_t sub = threadIdx.x % LPV;
-    const uint64_t stride = (uint64_t)gridDim.x * (kBlock / LPV);
+    const uint64_t stride = (uint64_t)nb * (kBlock / LPV);
     uint32_t cnt = 0;
-    for (uint64_t item = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) / LPV; item < n; item += stride) {
+    for (uint64_t item = ((uint64_t)bid * kBlock + threadIdx.x) / LPV; item < n; item += stride) {
         const uint32_t u = lst[item];
         uint64_t b, e;
         row_range(ptr, a.pmid, heavy, u, b, e);
@@ -627,24 +628,28 @@ __global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restri
                 if (i + (uint64_t)k * LPV < e) cnt += commit<ALGO>(a, s, to[k], wt[k], seen[k]);
         }
     }
-    wave_add(upd, cnt);
+    return cnt;
 }
 
-// Bin 2: one workgroup per tile of a hub's neighbour list (workgroups stride
-// over the tiles); the tile is staged in LDS with coalesced loads before the
-// relax pass.
+template <int ALGO, bool W, bool PK, int LPV>
+__global__ __launch_bounds__(kBlock) void k_relax_group(const uint32_t *__restrict__ list, int bin,
+                                                        const FrontierInfo *__restrict__ info,
+                                                        const uint64_t *__restrict__ ptr,
+                                                        const void *__restrict__ edges, RelaxArgs a,
+                                                        unsigned long long *upd) {
+    wave_add(upd, relax_group_body<ALGO, W, PK, LPV>(list, bin, info, ptr, edges, a, blockIdx.x, gridDim.x));
+}
+
 template <int ALGO, bool W, bool PK>
-__global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict__ hub, uint64_t hub_cap,
-                                                      const FrontierInfo *__restrict__ info, uint32_t tile,
-                                                      const uint64_t *__restrict__ ptr,
-                                                      const void *__restrict__ edges, RelaxArgs a,
-                                                      unsigned long long *upd) {
-    extern __shared__ uint32_t s_edges[];
+__device__ __forceinline__ uint32_t relax_hub_body(const uint64_t *__restrict__ hub, uint64_t hub_cap,
+                                                   const FrontierInfo *__restrict__ info, uint32_t tile,
+                                                   const uint64_t *__restrict__ ptr, const void *__restrict__ edges,
+                                                   const RelaxArgs &a, uint32_t *s_edges, uint32_t bid, uint32_t nb) {
     constexpr bool WIDE = W && !PK;  // 8-byte edges
     const uint64_t ntiles = info->pull ? 0 : min((uint64_t)info->n[2], hub_cap);
     const bool heavy = info->heavy != 0;
     uint32_t c = 0;
-    for (uint64_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+    for (uint64_t ti = bid; ti < ntiles; ti += nb) {
         const uint64_t it = hub[ti];
         const uint32_t u = (uint32_t)it, t = (uint32_t)(it >> 32);
         uint64_t rb, re;
@@ -679,7 +684,20 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
         }
         __syncthreads();  // the tile buffer is reused by the next tile
     }
-    wave_add(upd, c);
+    return c;
+}
+
+// Bin 2: one workgroup per tile of a hub's neighbour list (workgroups stride
+// over the tiles); the tile is staged in LDS with coalesced loads before the
+// relax pass.
+template <int ALGO, bool W, bool PK>
+__global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict__ hub, uint64_t hub_cap,
+                                                      const FrontierInfo *__restrict__ info, uint32_t tile,
+                                                      const uint64_t *__restrict__ ptr,
+                                                      const void *__restrict__ edges, RelaxArgs a,
+                                                      unsigned long long *upd) {
+    extern __shared__ uint32_t s_edges[];
+    wave_add(upd, relax_hub_body<ALGO, W, PK>(hub, hub_cap, info, tile, ptr, edges, a, s_edges, blockIdx.x, gridDim.x));
 }
 
 // Dynamic region, edges [c0, min(c1, D)) of the concatenated active dynamic
@@ -698,25 +716,29 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
 // the covering slice of the degree prefix into LDS and maps each edge to its
 // source with a binary search there (every listed vertex has >= 1 edge, so a
 // chunk covers at most kDynEdgesPerBlock vertices).
+// LDS of the dynamic relax body: the degree-prefix slice and its bounds
+struct DynLds {
+    uint64_t off[kDynEdgesPerBlock + 2];
+    uint32_t i[2];
+};
+
 template <int ALGO, bool W, bool PACKED, bool DIRECT>
-__global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict__ list,
-                                                      const FrontierInfo *__restrict__ info,
-                                                      const uint64_t *__restrict__ doff,
-                                                      const void *__restrict__ stage, uint64_t c0, uint64_t c1max,
-                                                      const uint64_t *__restrict__ ptr, uint64_t dbase,
-                                                      const void *__restrict__ resident,
-                                                      const int64_t *__restrict__ rdelta,
-                                                      uint32_t wbits, RelaxArgs a, unsigned long long *upd,
-                                                      unsigned long long *hits) {
-    __shared__ uint64_t s_off[kDynEdgesPerBlock + 2];
-    __shared__ uint32_t s_i[2];
+__device__ __forceinline__ uint32_t relax_dyn_body(const uint32_t *__restrict__ list,
+                                                   const FrontierInfo *__restrict__ info,
+                                                   const uint64_t *__restrict__ doff, const void *__restrict__ stage,
+                                                   uint64_t c0, uint64_t c1max, const uint64_t *__restrict__ ptr,
+                                                   uint64_t dbase, const void *__restrict__ resident,
+                                                   const int64_t *__restrict__ rdelta, uint32_t wbits,
+                                                   const RelaxArgs &a, DynLds &L, uint32_t bid, uint32_t nb,
+                                                   uint32_t &hits) {
+    uint64_t *s_off = L.off;
+    uint32_t *s_i = L.i;
     const uint32_t ndyn = info->n[3];
     const uint32_t *dlist = list + info->n[0] + info->n[1];
     const uint64_t c1 = min(c1max, info->dyn_deg);
     const bool heavy = info->heavy != 0;
     uint32_t c = 0, h = 0;
-    for (uint64_t q0 = c0 + (uint64_t)blockIdx.x * kDynEdgesPerBlock; q0 < c1;
-         q0 += (uint64_t)gridDim.x * kDynEdgesPerBlock) {
+    for (uint64_t q0 = c0 + (uint64_t)bid * kDynEdgesPerBlock; q0 < c1; q0 += (uint64_t)nb * kDynEdgesPerBlock) {
         const uint64_t q1 = min(q0 + kDynEdgesPerBlock, c1);
         if (threadIdx.x < 2) {
             const uint64_t q = threadIdx.x == 0 ? q0 : q1 - 1;
@@ -772,8 +794,78 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
             if (q0 + threadIdx.x + (uint64_t)j * kBlock < q1) c += commit<ALGO>(a, sv[j], to[j], wt[j], seen[j]);
         __syncthreads();  // s_off / s_i are reloaded for the next chunk
     }
+    hits = h;
+    return c;
+}
+
+template <int ALGO, bool W, bool PACKED, bool DIRECT>
+__global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict__ list,
+                                                      const FrontierInfo *__restrict__ info,
+                                                      const uint64_t *__restrict__ doff,
+                                                      const void *__restrict__ stage, uint64_t c0, uint64_t c1max,
+                                                      const uint64_t *__restrict__ ptr, uint64_t dbase,
+                                                      const void *__restrict__ resident,
+                                                      const int64_t *__restrict__ rdelta,
+                                                      uint32_t wbits, RelaxArgs a, unsigned long long *upd,
+                                                      unsigned long long *hits) {
+    __shared__ DynLds L;
+    uint32_t h = 0;
+    const uint32_t c = relax_dyn_body<ALGO, W, PACKED, DIRECT>(list, info, doff, stage, c0, c1max, ptr, dbase, resident,
+                                                               rdelta, wbits, a, L, blockIdx.x, gridDim.x, h);
     wave_add(upd, c);
     if (DIRECT) wave_add(hits, h);
+}
+
+// One round's relax in a single launch: static bins 0 and 1, hub tiles and
+// the zero-copy dynamic region side by side, so a device-driven round needs
+// no second stream (the fork and join of which cost tens of microseconds per
+// round in a replayed graph, most of a small round).  A workgroup takes its
+// kind from blockIdx.x modulo a period of kFuseStatic + cd workgroups (2
+// bin 0, 2 bin 1, 1 hub tile, cd dynamic): the kinds interleave in dispatch
+// order, so the link-bound dynamic relax and the static bins progress
+// together as the two streams did.  With K = gridDim.x / period, the kinds
+// get 2K, 2K, K and cd * K workgroups.
+struct RoundRelax {
+    const uint32_t *list;
+    const FrontierInfo *info;
+    const uint64_t *ptr;
+    const void *sedges;         // static region
+    const uint64_t *hub;
+    uint64_t hub_cap;
+    uint32_t tile;
+    uint32_t cd;                // dynamic workgroups per period
+    const uint64_t *doff;
+    const void *stage;          // dynamic region in pinned host memory (device address)
+    uint64_t dbase;
+    const void *resident;       // resident chunks in HBM
+    const int64_t *rdelta;
+    unsigned long long *counters;  // [0] static updates, [1] dynamic updates, [5] HBM-served dynamic edges
+};
+constexpr uint32_t kFuseStatic = 5;
+
+template <int ALGO, bool W, bool PK, int LPV>
+__global__ __launch_bounds__(kBlock) void k_relax_round(RoundRelax r, RelaxArgs a) {
+    extern __shared__ uint64_t s_fuse[];
+    const uint32_t P = kFuseStatic + r.cd;
+    const uint32_t K = gridDim.x / P;
+    const uint32_t k = blockIdx.x / P, j = blockIdx.x - k * P;
+    if (j < 2) {
+        wave_add(r.counters + 0,
+                 relax_group_body<ALGO, W, PK, LPV>(r.list, 0, r.info, r.ptr, r.sedges, a, 2 * k + j, 2 * K));
+    } else if (j < 4) {
+        wave_add(r.counters + 0,
+                 relax_group_body<ALGO, W, PK, 64>(r.list, 1, r.info, r.ptr, r.sedges, a, 2 * k + j - 2, 2 * K));
+    } else if (j == 4) {
+        wave_add(r.counters + 0, relax_hub_body<ALGO, W, PK>(r.hub, r.hub_cap, r.info, r.tile, r.ptr, r.sedges, a,
+                                                              (uint32_t *)s_fuse, k, K));
+    } else {
+        uint32_t h = 0;
+        const uint32_t c = relax_dyn_body<ALGO, W, PK, true>(r.list, r.info, r.doff, r.stage, 0, ~0ull, r.ptr, r.dbase,
+                                                             r.resident, r.rdelta, a.wbits, a, *(DynLds *)s_fuse,
+                                                             r.cd * k + (j - kFuseStatic), r.cd * K, h);
+        wave_add(r.counters + 1, c);
+        wave_add(r.counters + 5, h);
+    }
 }
 
 // Bottom-up BFS round (direction-optimising BFS, Beamer et al., SC'12): the
@@ -1394,7 +1486,9 @@ struct RoundGraph {
 
 // Timing marks of enqueued work.  EagerSpan: an EventSpan of the run per
 // launch sequence.  GraphSpan: a pair of event-record nodes in the graph
-// being captured.
+// being captured; with `chain`, a span that follows another on the same
+// stream with nothing enqueued between them starts at that span's end event
+// (each event node costs the replay about ten microseconds).
 struct EagerSpan {
     DeviceState *d;
     RunTimers &tm;
@@ -1407,17 +1501,26 @@ struct EagerSpan {
 
 struct GraphSpan {
     RoundGraph *rg;
+    bool chain = false;
+    mutable hipEvent_t last = nullptr;
+    mutable hipStream_t last_s = nullptr;
     template <class F>
     void operator()(int cls, hipStream_t s, F &&fn) const {
         hipEvent_t a = nullptr, b = nullptr;
-        OG_HIP(hipEventCreate(&a));
-        rg->events.push_back(a);
+        if (chain && last && last_s == s) {
+            a = last;
+        } else {
+            OG_HIP(hipEventCreate(&a));
+            rg->events.push_back(a);
+            OG_HIP(hipEventRecordWithFlags(a, s, hipEventRecordExternal));
+        }
         OG_HIP(hipEventCreate(&b));
         rg->events.push_back(b);
-        OG_HIP(hipEventRecordWithFlags(a, s, hipEventRecordExternal));
         fn();
         OG_HIP(hipEventRecordWithFlags(b, s, hipEventRecordExternal));
         rg->marks.push_back({cls, a, b});
+        last = b;
+        last_s = s;
     }
 };
 
@@ -1570,6 +1673,54 @@ static void launch_dyn(DeviceState *d, unsigned grid, const void *src, uint64_t 
                        d->fr.counters + 5);
 }
 
+// The fused round relax (k_relax_round) of a zero-copy round on the compute
+// stream: the static grids of the separate launches (kGroupGridCap twice,
+// kHubGridCap) and the dynamic grid rounded up to a multiple of kHubGridCap.
+template <int ALGO, bool W, bool PK, int LPV>
+static void launch_round_lpv(DeviceState *d, const RoundRelax &r, unsigned grid, size_t lds) {
+    hipLaunchKernelGGL((k_relax_round<ALGO, W, PK, LPV>), dim3(grid), dim3(kBlock), lds, d->s_compute, r,
+                       relax_args(d));
+}
+
+template <int ALGO, bool W, bool PK>
+static void launch_round_fmt(DeviceState *d) {
+    static_assert(2 * kHubGridCap == kGroupGridCap, "period layout: 2 + 2 + 1 static workgroups per K");
+    const unsigned dyn = d->tune.dyn_grid ? d->tune.dyn_grid : kDynGridCap;
+    RoundRelax r{};
+    r.list = d->fr.list;
+    r.info = d->fr.info;
+    r.ptr = d->d_ptr;
+    r.sedges = d->d_static;
+    r.hub = d->fr.hub;
+    r.hub_cap = d->fr.hub_cap;
+    r.tile = d->tune.tile;
+    r.cd = (dyn + kHubGridCap - 1) / kHubGridCap;
+    r.doff = d->fr.doff;
+    r.stage = d->hd_dyn;
+    r.dbase = d->dyn_base;
+    r.resident = d->d_staging;
+    r.rdelta = d->d_rdelta;
+    r.counters = d->fr.counters;
+    const unsigned grid = (kFuseStatic + r.cd) * kHubGridCap;
+    const size_t lds = std::max<size_t>(sizeof(DynLds), (size_t)d->tune.tile * d->eb);
+    switch (d->tune.lpv0) {
+        case 1: launch_round_lpv<ALGO, W, PK, 1>(d, r, grid, lds); break;
+        case 2: launch_round_lpv<ALGO, W, PK, 2>(d, r, grid, lds); break;
+        case 4: launch_round_lpv<ALGO, W, PK, 4>(d, r, grid, lds); break;
+        case 8: launch_round_lpv<ALGO, W, PK, 8>(d, r, grid, lds); break;
+        case 16: launch_round_lpv<ALGO, W, PK, 16>(d, r, grid, lds); break;
+        case 32: launch_round_lpv<ALGO, W, PK, 32>(d, r, grid, lds); break;
+        default: launch_round_lpv<ALGO, W, PK, 64>(d, r, grid, lds); break;
+    }
+    OG_HIP(hipGetLastError());
+}
+
+template <int ALGO, bool W>
+static void launch_round_relax(DeviceState *d) {
+    if (W && d->packed) launch_round_fmt<ALGO, W, true>(d);
+    else launch_round_fmt<ALGO, W, false>(d);
+}
+
 // Dynamic relax of edges [c0, min(c1, D)); `known` = c1 - c0 when the host
 // knows the edge count (exact grid), else 0 (fixed grid).
 template <int ALGO, bool W>
@@ -1648,9 +1799,10 @@ constexpr int kRoundBatch = 4;  // device-driven rounds enqueued per host check
 // state once per batch (rounds enqueued past the end find no flags and do no
 // work).  The staged loader needs the host in every round (gather, copies),
 // so it synchronises on each round's summary.
+// quiet_after: after() enqueues nothing (its rounds' timing spans may chain).
 template <int ALGO, bool W, class After>
 static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTimers &tm, double &gather_ms,
-                          double &overload_wall, After after) {
+                          double &overload_wall, After after, bool quiet_after) {
     hipStream_t s = d->s_compute;
     // SSSP near-far order: only vertices below the threshold relax in a
     // round; when none is left it moves to (smallest held-back value + delta).
@@ -1673,14 +1825,19 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
         eager(kClsFront, s, [&] { frontier_compact(d, d->fr, s, nf_val, false, pm); });
         // one batch: kRoundBatch x (static || dynamic relax, after, compaction),
         // then the traversal state to the host
-        // With both regions the dynamic relax is enqueued before the static
-        // fork: a replay submits the graph's kernels in capture order, and
-        // the link-bound relax should not wait for the static launches.
+        // With both regions the static and dynamic relax of a round run as
+        // one launch (k_relax_round); with OG_PLAN_NO_FUSE on two streams,
+        // the dynamic relax enqueued before the static fork (a replay submits
+        // the graph's kernels in capture order, and the link-bound relax
+        // should not wait for the static launches).
         const bool overlap = has_dyn && !(d->flags & OG_PLAN_NO_OVERLAP);
+        const bool fuse = overlap && !(d->flags & OG_PLAN_NO_FUSE);
         auto batch = [&](const auto &span) {
             for (int r = 0; r < kRoundBatch; ++r) {
                 bool joined = false;
-                if (overlap) {
+                if (fuse) {
+                    span(kClsDyn, s, [&] { launch_round_relax<ALGO, W>(d); });
+                } else if (overlap) {
                     OG_HIP(hipEventRecord(d->ev_fork, s));
                     span(kClsDyn, s, [&] { launch_dyn_any<ALGO, W>(d, d->hd_dyn, 0, ~0ull, 0, true); });
                     OG_HIP(hipStreamWaitEvent(d->s_static, d->ev_fork, 0));
@@ -1709,7 +1866,11 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
             const uint64_t key = (uint64_t)ALGO | (uint64_t)W << 4 | (uint64_t)(delta != 0) << 5 |
                                  (uint64_t)has_dyn << 6 | (uint64_t)dir_opt << 7 | (uint64_t)split << 16 |
                                  (uint64_t)kRoundBatch << 8;
-            RoundGraph *rg = round_graph_for(d, key, [&](RoundGraph *n) { batch(GraphSpan{n}); });
+            RoundGraph *rg = round_graph_for(d, key, [&](RoundGraph *n) {
+                GraphSpan gs{n};
+                gs.chain = fuse && quiet_after;  // fused rounds: relax span, then compaction span
+                batch(gs);
+            });
             for (;;) {
                 OG_HIP(hipGraphLaunch(rg->exec, s));
                 OG_HIP(hipStreamSynchronize(s));
@@ -1827,7 +1988,7 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
         hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d->d_val, d->fr.flag, src);
     }
     OG_HIP(hipGetLastError());
-    frontier_loop<ALGO, W>(g, d, st, tm, gather_ms, overload_wall, []() {});
+    frontier_loop<ALGO, W>(g, d, st, tm, gather_ms, overload_wall, []() {}, true);
     OG_HIP(hipStreamSynchronize(s));
     st.total_ms = total.ms();
 
@@ -1889,7 +2050,7 @@ static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t
             hipLaunchKernelGGL(k_ms_commit, dim3(cgrid), dim3(kBlock), 0, s, d->fr.flag, V, d->d_visit, d->d_front,
                                d->d_next, d->d_mlvl, d->fr.ctl);
             OG_HIP(hipGetLastError());
-        });
+        }, false);
         OG_HIP(hipMemsetAsync(d->d_msum, 0, 3 * kMsWidth * 8, s));
         hipLaunchKernelGGL(k_ms_digest, dim3((unsigned)std::min<uint64_t>(1024, (V + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, s, d->d_mlvl, d->d_ptr, V, d->d_msum);

@@ -20,6 +20,8 @@ PLANS = [
     ("ascetic+zerocopy", 1 / 64, 0.5),  # small HBM-resident head, most dynamic reads over the link
     ("ascetic+zerocopy+nopack", 1 / 8, 0.5),  # 8-byte rows read in place
     ("ascetic+zerocopy+nograph", 0.25, 0.8),  # device-driven rounds enqueued eagerly
+    ("ascetic+zerocopy+nofuse", 0.25, 0.8),  # static and dynamic relax on two streams
+    ("ascetic+zerocopy+nofuse", 1 / 64, 0.5),
     ("ascetic+nograph", 1.0, 1.0),
     ("ascetic+nopack", 1 / 64, 0.5),  # 8-byte weighted edges in HBM and on the link
     ("subway+zerocopy", 0.25, 0.0),
@@ -41,7 +43,7 @@ def _prepare(g, approach, frac, k, edge_bytes):
     flags = 0
     for o in opts:
         flags |= {"zerocopy": og.PLAN_ZERO_COPY, "nopack": og.PLAN_NO_PACK, "nograph": og.PLAN_NO_GRAPH,
-                  "nooverlap": og.PLAN_NO_OVERLAP, "diropt": og.PLAN_DIRECTION_OPT}[o]
+                  "nooverlap": og.PLAN_NO_OVERLAP, "diropt": og.PLAN_DIRECTION_OPT, "nofuse": og.PLAN_NO_FUSE}[o]
     return g.prepare(name, edge_budget_bytes=budget, advisek=k, flags=flags)
 
 
@@ -295,7 +297,8 @@ def test_probe_host_link(gpu):
 @pytest.mark.parametrize("approach,frac,k", [("ascetic", 1.0, 1.0), ("ascetic", 0.25, 0.8), ("ascetic", 1 / 64, 0.5),
                                               ("ascetic+zerocopy", 0.25, 0.8), ("ascetic+zerocopy", 1 / 64, 0.5),
                                               ("subway+zerocopy", 0.25, 0.0), ("subway", 1 / 64, 0.0),
-                                              ("ascetic+zerocopy+nograph", 0.25, 0.8)])
+                                              ("ascetic+zerocopy+nograph", 0.25, 0.8),
+                                              ("ascetic+zerocopy+nofuse", 0.25, 0.8)])
 @pytest.mark.parametrize("split,delta", [(32, 32), (8, 32), (40, 16), (16, 0)])
 def test_light_heavy_split(gpu, rmat_w, approach, frac, k, split, delta):
     """Rows stored light edges first: near-far SSSP relaxes light edges and
@@ -309,7 +312,7 @@ def test_light_heavy_split(gpu, rmat_w, approach, frac, k, split, delta):
     name, *opts = approach.split("+")
     flags = 0
     for o in opts:
-        flags |= {"zerocopy": og.PLAN_ZERO_COPY, "nograph": og.PLAN_NO_GRAPH}[o]
+        flags |= {"zerocopy": og.PLAN_ZERO_COPY, "nograph": og.PLAN_NO_GRAPH, "nofuse": og.PLAN_NO_FUSE}[o]
     rmat_w.prepare(name, edge_budget_bytes=int(max(1, E * 4 * frac)), advisek=k, flags=flags, split_weight=split)
     rmat_w.set_tuning(delta=delta)
     for s in _sources(ptr, E, 3, 17):

@@ -25,12 +25,19 @@ TUNINGS = [dict(t1=1, lpv0=1, t2=1, tile=64, delta=0, defer=0, dyn_grid=1),
            dict(t1=32, lpv0=8, t2=4096, tile=2048, delta=0, defer=2, dyn_grid=0)]
 
 
-@pytest.mark.parametrize("approach,frac,k", [("ascetic", 1.0, 1.0), ("ascetic", 0.25, 0.8), ("subway", 0.1, 0.0)])
+@pytest.mark.parametrize("approach,frac,k", [("ascetic", 1.0, 1.0), ("ascetic", 0.25, 0.8), ("subway", 0.1, 0.0),
+                                              ("ascetic+zerocopy", 0.25, 0.8), ("ascetic+zerocopy+nopack", 0.25, 0.8)])
 def test_results_independent_of_tuning(gpu, approach, frac, k):
+    """Every tuning gives the oracle's results; the zero-copy plans run the
+    fused round relax with every bin width, tile (LDS size) and grid."""
     g = og.Graph.rmat(12, 16, seed=61, weight_seed=62)
     ptr, to, w, _ = g.export()
     E = len(to)
-    g.prepare(approach, edge_budget_bytes=int(E * 8 * frac), advisek=k)
+    name, *opts = approach.split("+")
+    flags = 0
+    for o in opts:
+        flags |= {"zerocopy": og.PLAN_ZERO_COPY, "nopack": og.PLAN_NO_PACK}[o]
+    g.prepare(name, edge_budget_bytes=int(E * 8 * frac), advisek=k, flags=flags)
     src = int(np.flatnonzero(np.diff(np.append(ptr.astype(np.int64), E)) > 0)[1])
     ref_b, _, _ = oracle.bfs(ptr, to, src)
     ref_s = oracle.sssp(ptr, to, w, src)
