@@ -684,12 +684,15 @@ def main():
     # FETCH_SIZE / WRITE_SIZE (calibrated against a plain copy) on this
     # workload and setting, when a committed profile holds them
     traffic = traffic_src = None
-    tpath = os.path.join(HERE, "profiles", "k_relax_dyn_sssp_rmat24_traffic.json")
+    fused = (sum(x["relax_static_kernel_ms"] for x in stats) == 0 and sum(x["relax_static_bytes"] for x in stats) > 0
+             and sum(x["relax_dynamic_kernel_ms"] for x in stats) > 0)
+    kclass = "relax_round" if fused else "relax_dynamic"
+    tpath = os.path.join(HERE, "profiles", f"k_{kclass.replace('relax_dynamic', 'relax_dyn')}_sssp_rmat24_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
         want = {"scale": args.scale, "approach": args.approach, "loader": args.loader, "delta": args.delta,
-                "kernel_class": "relax_dynamic", "edge_format": edge_format}
+                "kernel_class": kclass, "edge_format": edge_format}
         if tr.get("config") == want:
             traffic, traffic_src = tr["traffic_bytes_per_launch"], tr["source"]
     roof, link, kt = roofline_of(stats, zero_copy=args.loader == "zerocopy" and args.approach != "usm",
