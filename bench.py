@@ -173,7 +173,7 @@ def wl_bfs_hbm(args):
     V, E = len(ptr), len(cols)
     deg = np.diff(np.append(ptr.astype(np.int64), E))
     src = int(np.flatnonzero(deg > 0)[0])
-    flags = og.PLAN_DIRECTION_OPT if args.diropt else 0
+    flags = (og.PLAN_DIRECTION_OPT if args.diropt else 0) | (og.PLAN_NO_FUSE if args.no_fuse else 0)
     plan = g.prepare("ascetic", edge_budget_bytes=E * 4, advisek=1.0, flags=flags)
     stats, wall = timed(lambda: g.bfs(src, want_values=False)[1], args.steps, args.warmup, "bfs-hbm")
     lv, _ = g.bfs(src)

@@ -824,7 +824,8 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
 // bin 0, 2 bin 1, 1 hub tile, cd dynamic): the kinds interleave in dispatch
 // order, so the link-bound dynamic relax and the static bins progress
 // together as the two streams did.  With K = gridDim.x / period, the kinds
-// get 2K, 2K, K and cd * K workgroups.
+// get 2K, 2K, K and cd * K workgroups.  A graph wholly in HBM runs it with
+// cd = 0: its three static bins in one launch instead of three.
 struct RoundRelax {
     const uint32_t *list;
     const FrontierInfo *info;
@@ -1673,9 +1674,10 @@ static void launch_dyn(DeviceState *d, unsigned grid, const void *src, uint64_t 
                        d->fr.counters + 5);
 }
 
-// The fused round relax (k_relax_round) of a zero-copy round on the compute
-// stream: the static grids of the separate launches (kGroupGridCap twice,
-// kHubGridCap) and the dynamic grid rounded up to a multiple of kHubGridCap.
+// The fused round relax (k_relax_round) of a device-driven round on the
+// compute stream: the static grids of the separate launches (kGroupGridCap
+// twice, kHubGridCap) and, with_dyn (zero-copy), the dynamic grid rounded up
+// to a multiple of kHubGridCap.
 template <int ALGO, bool W, bool PK, int LPV>
 static void launch_round_lpv(DeviceState *d, const RoundRelax &r, unsigned grid, size_t lds) {
     hipLaunchKernelGGL((k_relax_round<ALGO, W, PK, LPV>), dim3(grid), dim3(kBlock), lds, d->s_compute, r,
@@ -1683,7 +1685,7 @@ static void launch_round_lpv(DeviceState *d, const RoundRelax &r, unsigned grid,
 }
 
 template <int ALGO, bool W, bool PK>
-static void launch_round_fmt(DeviceState *d) {
+static void launch_round_fmt(DeviceState *d, bool with_dyn) {
     static_assert(2 * kHubGridCap == kGroupGridCap, "period layout: 2 + 2 + 1 static workgroups per K");
     const unsigned dyn = d->tune.dyn_grid ? d->tune.dyn_grid : kDynGridCap;
     RoundRelax r{};
@@ -1694,7 +1696,7 @@ static void launch_round_fmt(DeviceState *d) {
     r.hub = d->fr.hub;
     r.hub_cap = d->fr.hub_cap;
     r.tile = d->tune.tile;
-    r.cd = (dyn + kHubGridCap - 1) / kHubGridCap;
+    r.cd = with_dyn ? (dyn + kHubGridCap - 1) / kHubGridCap : 0;
     r.doff = d->fr.doff;
     r.stage = d->hd_dyn;
     r.dbase = d->dyn_base;
@@ -1702,7 +1704,7 @@ static void launch_round_fmt(DeviceState *d) {
     r.rdelta = d->d_rdelta;
     r.counters = d->fr.counters;
     const unsigned grid = (kFuseStatic + r.cd) * kHubGridCap;
-    const size_t lds = std::max<size_t>(sizeof(DynLds), (size_t)d->tune.tile * d->eb);
+    const size_t lds = std::max<size_t>(with_dyn ? sizeof(DynLds) : 0, (size_t)d->tune.tile * d->eb);
     switch (d->tune.lpv0) {
         case 1: launch_round_lpv<ALGO, W, PK, 1>(d, r, grid, lds); break;
         case 2: launch_round_lpv<ALGO, W, PK, 2>(d, r, grid, lds); break;
@@ -1716,9 +1718,9 @@ static void launch_round_fmt(DeviceState *d) {
 }
 
 template <int ALGO, bool W>
-static void launch_round_relax(DeviceState *d) {
-    if (W && d->packed) launch_round_fmt<ALGO, W, true>(d);
-    else launch_round_fmt<ALGO, W, false>(d);
+static void launch_round_relax(DeviceState *d, bool with_dyn) {
+    if (W && d->packed) launch_round_fmt<ALGO, W, true>(d, with_dyn);
+    else launch_round_fmt<ALGO, W, false>(d, with_dyn);
 }
 
 // Dynamic relax of edges [c0, min(c1, D)); `known` = c1 - c0 when the host
@@ -1830,13 +1832,17 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
         // the dynamic relax enqueued before the static fork (a replay submits
         // the graph's kernels in capture order, and the link-bound relax
         // should not wait for the static launches).
+        // A graph wholly in HBM runs its three static bins as one fused launch.
         const bool overlap = has_dyn && !(d->flags & OG_PLAN_NO_OVERLAP);
         const bool fuse = overlap && !(d->flags & OG_PLAN_NO_FUSE);
+        const bool fuse_static = !has_dyn && !(d->flags & OG_PLAN_NO_FUSE);
         auto batch = [&](const auto &span) {
             for (int r = 0; r < kRoundBatch; ++r) {
                 bool joined = false;
                 if (fuse) {
-                    span(kClsDyn, s, [&] { launch_round_relax<ALGO, W>(d); });
+                    span(kClsDyn, s, [&] { launch_round_relax<ALGO, W>(d, true); });
+                } else if (fuse_static) {
+                    span(kClsStat, s, [&] { launch_round_relax<ALGO, W>(d, false); });
                 } else if (overlap) {
                     OG_HIP(hipEventRecord(d->ev_fork, s));
                     span(kClsDyn, s, [&] { launch_dyn_any<ALGO, W>(d, d->hd_dyn, 0, ~0ull, 0, true); });
@@ -1868,7 +1874,7 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
                                  (uint64_t)kRoundBatch << 8;
             RoundGraph *rg = round_graph_for(d, key, [&](RoundGraph *n) {
                 GraphSpan gs{n};
-                gs.chain = fuse && quiet_after;  // fused rounds: relax span, then compaction span
+                gs.chain = (fuse || fuse_static) && quiet_after;  // one stream: relax span(s), then compaction
                 batch(gs);
             });
             for (;;) {
