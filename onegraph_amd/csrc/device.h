@@ -490,5 +490,19 @@ void lane_run(og_graph *g, DeviceState *lane, int algo, uint32_t src, uint32_t *
 // host does not know it) only sizes the grids and skips empty bins.
 template <int ALGO, bool W>
 void launch_static_relax(DeviceState *d, Frontier &f, const FrontierInfo *hfi, const RelaxArgs &a, hipStream_t s);
+// Served BFS: the bottom-up scan's arrays (graphs with their in-edges).
+struct SrvPull {
+    const uint64_t *in_ptr;
+    const uint32_t *in_cols;
+    uint64_t V;
+    const uint64_t *visit, *front;
+    uint64_t *next;
+    uint8_t *flag;
+    const unsigned long long *pull;  // bit j: slot j scans bottom-up this round
+};
+// One launch relaxing a served round: top-down slots through the static
+// bins and, with `pull`, the bottom-up scan beside them.
+void launch_srv_round(DeviceState *d, Frontier &f, const RelaxArgs &a, hipStream_t s, const SrvPull *pull,
+                      bool weighted);
 
 }  // namespace og

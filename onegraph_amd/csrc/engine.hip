@@ -869,6 +869,59 @@ __global__ __launch_bounds__(kBlock) void k_relax_round(RoundRelax r, RelaxArgs 
     }
 }
 
+// Served BFS (the query server's 64-slot engines): bottom-up scan for the
+// pulling slots P = *pull.  Every vertex that some slot of P has not reached
+// scans its in-edges, ORing the frontier masks of its in-neighbours
+// (restricted to those slots), and stops once every such slot is found.
+// Found slots are offered through next[v] like a top-down relax.
+__device__ __forceinline__ void srv_pull_body(const SrvPull &p, uint32_t bid, uint32_t nb) {
+    const uint64_t P = *p.pull;
+    if (!P) return;
+    for (uint64_t v = (uint64_t)bid * kBlock + threadIdx.x; v < p.V; v += (uint64_t)nb * kBlock) {
+        const uint64_t need = P & ~p.visit[v];
+        if (!need) continue;
+        uint64_t acc = 0;
+        const uint64_t b = p.in_ptr[v], e = p.in_ptr[v + 1];
+        for (uint64_t i = b; i < e && acc != need; i += kRelaxBatch) {
+            uint32_t u[kRelaxBatch];
+#pragma unroll
+            for (int k = 0; k < kRelaxBatch; ++k) u[k] = i + k < e ? p.in_cols[i + k] : 0u;
+#pragma unroll
+            for (int k = 0; k < kRelaxBatch; ++k)
+                if (i + k < e) acc |= p.front[u[k]] & need;
+        }
+        if (acc) {
+            atomicOr((unsigned long long *)(p.next + v), (unsigned long long)acc);
+            p.flag[v] = 1;
+        }
+    }
+}
+
+// A served BFS round's relax in one launch: the top-down slots through the
+// static bins (as k_relax_round with cd = 0) and, beside them, the bottom-up
+// scan of the pulling slots (cp workgroups per period of kFuseStatic + cp).
+// The two only read visit / front and only OR into next and set flags, so
+// they may run side by side.
+template <bool W, bool PK, int LPV>
+__global__ __launch_bounds__(kBlock) void k_srv_round(RoundRelax r, RelaxArgs a, SrvPull p, uint32_t cp) {
+    extern __shared__ uint64_t s_fuse[];
+    const uint32_t P = kFuseStatic + cp;
+    const uint32_t K = gridDim.x / P;
+    const uint32_t k = blockIdx.x / P, j = blockIdx.x - k * P;
+    if (j < 2) {
+        wave_add(r.counters + 0, relax_group_body<ALGO_MSBFS, W, PK, LPV>(r.list, 0, r.info, r.ptr, r.sedges, a,
+                                                                         2 * k + j, 2 * K));
+    } else if (j < 4) {
+        wave_add(r.counters + 0, relax_group_body<ALGO_MSBFS, W, PK, 64>(r.list, 1, r.info, r.ptr, r.sedges, a,
+                                                                        2 * k + j - 2, 2 * K));
+    } else if (j == 4) {
+        wave_add(r.counters + 0, relax_hub_body<ALGO_MSBFS, W, PK>(r.hub, r.hub_cap, r.info, r.tile, r.ptr, r.sedges,
+                                                                    a, (uint32_t *)s_fuse, k, K));
+    } else {
+        srv_pull_body(p, cp * k + (j - kFuseStatic), cp * K);
+    }
+}
+
 // Bottom-up BFS round (direction-optimising BFS, Beamer et al., SC'12): the
 // frontier is the current level L = rounds so far; every unvisited vertex
 // scans its in-edges, kRelaxBatch parents' levels at a time, and takes level
@@ -1721,6 +1774,41 @@ template <int ALGO, bool W>
 static void launch_round_relax(DeviceState *d, bool with_dyn) {
     if (W && d->packed) launch_round_fmt<ALGO, W, true>(d, with_dyn);
     else launch_round_fmt<ALGO, W, false>(d, with_dyn);
+}
+
+template <bool W, bool PK>
+static void launch_srv_round_fmt(DeviceState *d, Frontier &f, const RelaxArgs &a, hipStream_t s, const SrvPull *pull) {
+    RoundRelax r{};
+    r.list = f.list;
+    r.info = f.info;
+    r.ptr = d->d_ptr;
+    r.sedges = d->d_static;
+    r.hub = f.hub;
+    r.hub_cap = f.hub_cap;
+    r.tile = d->tune.tile;
+    r.counters = f.counters;
+    SrvPull p{};
+    if (pull) p = *pull;
+    const uint32_t cp = pull ? kDynGridCap / kHubGridCap : 0;  // the separate pull's 8192 workgroups
+    const unsigned grid = (kFuseStatic + cp) * kHubGridCap;
+    const size_t lds = (size_t)d->tune.tile * d->eb;
+    switch (d->tune.lpv0) {
+#define OG_SRV(L)                                                                                                  \
+    case L:                                                                                                        \
+        hipLaunchKernelGGL((k_srv_round<W, PK, L>), dim3(grid), dim3(kBlock), lds, s, r, a, p, cp);               \
+        break;
+        OG_SRV(1) OG_SRV(2) OG_SRV(4) OG_SRV(8) OG_SRV(16) OG_SRV(32)
+        default: OG_SRV(64)
+#undef OG_SRV
+    }
+    OG_HIP(hipGetLastError());
+}
+
+void launch_srv_round(DeviceState *d, Frontier &f, const RelaxArgs &a, hipStream_t s, const SrvPull *pull,
+                      bool weighted) {
+    if (weighted && d->packed) launch_srv_round_fmt<true, true>(d, f, a, s, pull);
+    else if (weighted) launch_srv_round_fmt<true, false>(d, f, a, s, pull);
+    else launch_srv_round_fmt<false, false>(d, f, a, s, pull);
 }
 
 // Dynamic relax of edges [c0, min(c1, D)); `known` = c1 - c0 when the host

@@ -14,10 +14,10 @@
 //      degree to the slot's reached / traversed counts, so no level array is
 //      kept; then every slot picks its direction for the round;
 //   3. compact the flags into degree bins (shared with og_bfs);
-//   4. relax: top-down slots through the multi-source bitmask kernels (one
-//      64-bit atomic per edge serves every slot), bottom-up slots in one
-//      pass over the in-edges of the vertices they have not reached (graphs
-//      prepared with OG_PLAN_DIRECTION_OPT);
+//   4. relax, one launch: top-down slots through the multi-source bitmask
+//      bins (one 64-bit atomic per edge serves every slot) and, beside them,
+//      bottom-up slots in one pass over the in-edges of the vertices they
+//      have not reached (graphs prepared with OG_PLAN_DIRECTION_OPT);
 //   5. clear the masks of the relaxed vertices;
 //   6. slots whose frontier emptied report their results; their visit bits
 //      are cleared later, by one sweep for many slots.
@@ -188,37 +188,6 @@ __global__ void k_srv_direction(uint64_t used, uint64_t V, uint64_t E, int dir_o
     fstat[j] = fstat[64 + j] = 0;
     const uint64_t b = __ballot(now);
     if (j == 0) *pull = b;
-}
-
-// Bottom-up round for the pulling slots P: every vertex that some slot of P
-// has not reached scans its in-edges, ORing the frontier masks of its
-// in-neighbours (restricted to those slots), and stops once every such slot
-// is found.  Found slots are offered through next[v] like a top-down relax.
-__global__ __launch_bounds__(kBlock) void k_srv_bfs_pull(const uint64_t *__restrict__ in_ptr,
-                                                         const uint32_t *__restrict__ in_cols, uint64_t V,
-                                                         const uint64_t *__restrict__ visit,
-                                                         const uint64_t *__restrict__ front, uint64_t *next,
-                                                         uint8_t *flag, const unsigned long long *pull) {
-    const uint64_t P = *pull;
-    if (!P) return;
-    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
-        const uint64_t need = P & ~visit[v];
-        if (!need) continue;
-        uint64_t acc = 0;
-        const uint64_t b = in_ptr[v], e = in_ptr[v + 1];
-        for (uint64_t i = b; i < e && acc != need; i += kRelaxBatch) {
-            uint32_t u[kRelaxBatch];
-#pragma unroll
-            for (int k = 0; k < kRelaxBatch; ++k) u[k] = i + k < e ? in_cols[i + k] : 0u;
-#pragma unroll
-            for (int k = 0; k < kRelaxBatch; ++k)
-                if (i + k < e) acc |= front[u[k]] & need;
-        }
-        if (acc) {
-            atomicOr((unsigned long long *)(next + v), (unsigned long long)acc);
-            flag[v] = 1;
-        }
-    }
 }
 
 // After the relax: the masks of the round's listed vertices (bins 0/1 and
@@ -464,11 +433,8 @@ void BfsEngine::step() {
     // them from f.info, so the round needs no host round trip before its end.
     frontier_compact(d, f, s, nullptr, false);
     RelaxArgs a{nullptr, f.flag, visit, front, next, d->wbits, dir_opt ? (const uint64_t *)d_pull : nullptr};
-    if (g->kind == OG_KIND_WEIGHTED) launch_static_relax<ALGO_MSBFS, true>(d, f, nullptr, a, s);
-    else launch_static_relax<ALGO_MSBFS, false>(d, f, nullptr, a, s);
-    if (dir_opt)
-        hipLaunchKernelGGL(k_srv_bfs_pull, dim3(sweep_grid(V, 8192)), dim3(kBlock), 0, s, d->d_in_ptr, d->d_in_cols,
-                           V, visit, front, next, f.flag, d_pull);
+    const SrvPull pl{d->d_in_ptr, d->d_in_cols, V, visit, front, next, f.flag, d_pull};
+    launch_srv_round(d, f, a, s, dir_opt ? &pl : nullptr, g->kind == OG_KIND_WEIGHTED);
     hipLaunchKernelGGL(k_srv_clear_front, dim3(1024), dim3(kBlock), 0, s, f.list, f.hub, f.hub_cap, f.info, front);
     OG_HIP(hipGetLastError());
     OG_HIP(hipStreamSynchronize(s));
