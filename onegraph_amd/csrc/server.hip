@@ -4,8 +4,9 @@
 //
 // BFS: every graph has one batch engine with 64 query slots, its own HIP
 // stream, frontier buffers and host thread.  A slot is one bit of the
-// per-vertex u64 masks visit / front / next.  Each engine round ends with
-// one host synchronisation:
+// per-vertex u64 masks visit / front / next.  The host waits once per round,
+// for the commit's summary (step 2), and enqueues the next round while the
+// rest of this one runs:
 //   1. admit pending queries into free slots: the source's bit goes into
 //      `next`, so the commit below seeds it like any other offer;
 //   2. commit (one sweep over the frontier flags): the offered, unseen slots
@@ -245,6 +246,7 @@ struct BfsEngine {
     MemPool mem;
     Frontier f;
     hipStream_t s = nullptr;
+    hipEvent_t ev_summary = nullptr;  // the round's alive mask and slot sums are on the host
     uint64_t *visit = nullptr, *front = nullptr, *next = nullptr;
     unsigned long long *d_alive = nullptr, *h_alive = nullptr, *d_acc = nullptr, *h_acc = nullptr;
     // direction-optimising rounds (graphs prepared with their in-edges)
@@ -264,6 +266,7 @@ struct BfsEngine {
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
         }
+        if (ev_summary) (void)hipEventDestroy(ev_summary);
     }
     void loop();
     void step();
@@ -429,6 +432,7 @@ void BfsEngine::step() {
     OG_HIP(hipMemcpyAsync(h_alive, d_alive, 16, hipMemcpyDeviceToHost, s));
     // results of the slots that finish this round are complete after the commit
     OG_HIP(hipMemcpyAsync(h_acc, d_acc, 3 * 64 * 8, hipMemcpyDeviceToHost, s));
+    OG_HIP(hipEventRecord(ev_summary, s));
     // Counts stay on the device: the compaction, relax and clear kernels read
     // them from f.info, so the round needs no host round trip before its end.
     frontier_compact(d, f, s, nullptr, false);
@@ -437,7 +441,11 @@ void BfsEngine::step() {
     launch_srv_round(d, f, a, s, dir_opt ? &pl : nullptr, g->kind == OG_KIND_WEIGHTED);
     hipLaunchKernelGGL(k_srv_clear_front, dim3(1024), dim3(kBlock), 0, s, f.list, f.hub, f.hub_cap, f.info, front);
     OG_HIP(hipGetLastError());
-    OG_HIP(hipStreamSynchronize(s));
+    // The host needs only the commit's summary: while it finishes slots and
+    // admits queries, the round's compaction and relax go on, and the next
+    // round's launches queue behind them in stream order (its seeds and
+    // commit see everything this round offered).
+    OG_HIP(hipEventSynchronize(ev_summary));
     ++round;
     ++rounds_run;
     pull_rounds += h_alive[1] ? 1 : 0;
@@ -616,6 +624,7 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             e->gi = gi;
             e->pending = &srv->queues[2 * gi];
             OG_HIP(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
+            OG_HIP(hipEventCreateWithFlags(&e->ev_summary, hipEventDisableTiming));
             frontier_alloc(g->dev, g, e->f, e->mem);
             e->visit = (uint64_t *)e->mem.dalloc(V * 8);
             e->front = (uint64_t *)e->mem.dalloc(V * 8);
