@@ -249,9 +249,17 @@ struct DeviceState {
     const void *h_dyn = nullptr;     // host source of dynamic edges (packed copy or the CSR)
     const void *hd_dyn = nullptr;    // the same, as a device address (zero-copy mode)
     uint64_t slot_dyn = 0;           // dynamic edges per staging slot in the transfer format
+    // zero-copy with the light/heavy split: the host copy of the dynamic
+    // region holds every row's light part in vertex order, then every heavy
+    // part (light part of v at lpos[v - vsplit], heavy part at
+    // split_nl + (ptr[v] - dyn_base) - lpos[v - vsplit]); lpos has
+    // V - vsplit + 1 entries
+    uint64_t *d_lpos = nullptr;
+    uint64_t split_nl = 0;           // light edges of the dynamic region
     uint64_t dyn_resident = 0;       // zero-copy: dynamic edges held in d_staging (whole chunks)
     int64_t *d_rdelta = nullptr;     // zero-copy: per chunk of kResChunk vertices (index v >> kResChunkShift),
-                                     // d_staging index - CSR edge index, or kNotResident
+                                     // d_staging index - host-array position, or kNotResident; two per
+                                     // chunk (light block, heavy block) with d_lpos
     uint64_t *m_ptr = nullptr;       // USM managed copies
     void *m_edges = nullptr;
 

@@ -729,8 +729,9 @@ __device__ __forceinline__ uint32_t relax_dyn_body(const uint32_t *__restrict__ 
                                                    uint64_t c0, uint64_t c1max, const uint64_t *__restrict__ ptr,
                                                    uint64_t dbase, const void *__restrict__ resident,
                                                    const int64_t *__restrict__ rdelta, uint32_t wbits,
-                                                   const RelaxArgs &a, DynLds &L, uint32_t bid, uint32_t nb,
-                                                   uint32_t &hits) {
+                                                   const RelaxArgs &a, const uint64_t *__restrict__ lpos,
+                                                   uint64_t nl, uint64_t vsplit, DynLds &L, uint32_t bid,
+                                                   uint32_t nb, uint32_t &hits) {
     uint64_t *s_off = L.off;
     uint32_t *s_i = L.i;
     const uint32_t ndyn = info->n[3];
@@ -773,13 +774,32 @@ __device__ __forceinline__ uint32_t relax_dyn_body(const uint32_t *__restrict__ 
                 const uint32_t u = dlist[i0 + lo];
                 sv[j] = load_src<ALGO>(a, u);
                 if (DIRECT) {
-                    const uint64_t x = ((a.pmid && heavy) ? a.pmid[u] : ptr[u]) + (q - s_off[lo]);
-                    const int64_t dl = rdelta ? rdelta[u >> kResChunkShift] : kNotResident;
+                    // hp: position of the edge in the host copy of the region
+                    const uint64_t k = q - s_off[lo];
+                    uint64_t hp;
+                    uint64_t slot = u >> kResChunkShift;  // entry of the resident table
+                    if (lpos) {  // split layout: light parts, then heavy parts
+                        const uint64_t lp = lpos[u - vsplit];
+                        const uint64_t hb = nl + (ptr[u] - dbase) - lp;
+                        bool hv;
+                        if (a.pmid) {  // a split round reads one part
+                            hv = heavy;
+                            hp = (hv ? hb : lp) + k;
+                        } else {  // whole rows: the light part, then the heavy part
+                            const uint64_t nlu = lpos[u - vsplit + 1] - lp;
+                            hv = k >= nlu;
+                            hp = hv ? hb + (k - nlu) : lp + k;
+                        }
+                        slot = 2 * slot + (hv ? 1 : 0);
+                    } else {
+                        hp = ((a.pmid && heavy) ? a.pmid[u] : ptr[u]) - dbase + k;
+                    }
+                    const int64_t dl = rdelta ? rdelta[slot] : kNotResident;
                     if (dl != kNotResident) {
-                        load_edge<W, PACKED>(resident, (uint64_t)((int64_t)x + dl), wbits, to[j], wt[j]);
+                        load_edge<W, PACKED>(resident, (uint64_t)((int64_t)hp + dl), wbits, to[j], wt[j]);
                         ++h;
                     } else {
-                        load_edge<W, PACKED>(stage, x - dbase, wbits, to[j], wt[j]);
+                        load_edge<W, PACKED>(stage, hp, wbits, to[j], wt[j]);
                     }
                 } else {
                     load_edge<W, PACKED>(stage, q - c0, wbits, to[j], wt[j]);
@@ -806,12 +826,14 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
                                                       const uint64_t *__restrict__ ptr, uint64_t dbase,
                                                       const void *__restrict__ resident,
                                                       const int64_t *__restrict__ rdelta,
-                                                      uint32_t wbits, RelaxArgs a, unsigned long long *upd,
+                                                      uint32_t wbits, RelaxArgs a, const uint64_t *__restrict__ lpos,
+                                                      uint64_t nl, uint64_t vsplit, unsigned long long *upd,
                                                       unsigned long long *hits) {
     __shared__ DynLds L;
     uint32_t h = 0;
     const uint32_t c = relax_dyn_body<ALGO, W, PACKED, DIRECT>(list, info, doff, stage, c0, c1max, ptr, dbase, resident,
-                                                               rdelta, wbits, a, L, blockIdx.x, gridDim.x, h);
+                                                               rdelta, wbits, a, lpos, nl, vsplit, L, blockIdx.x,
+                                                               gridDim.x, h);
     wave_add(upd, c);
     if (DIRECT) wave_add(hits, h);
 }
@@ -840,6 +862,8 @@ struct RoundRelax {
     uint64_t dbase;
     const void *resident;       // resident chunks in HBM
     const int64_t *rdelta;
+    const uint64_t *lpos;       // split layout of the host copy (null: rows whole)
+    uint64_t nl, vsplit;
     unsigned long long *counters;  // [0] static updates, [1] dynamic updates, [5] HBM-served dynamic edges
 };
 constexpr uint32_t kFuseStatic = 5;
@@ -862,8 +886,9 @@ __global__ __launch_bounds__(kBlock) void k_relax_round(RoundRelax r, RelaxArgs 
     } else {
         uint32_t h = 0;
         const uint32_t c = relax_dyn_body<ALGO, W, PK, true>(r.list, r.info, r.doff, r.stage, 0, ~0ull, r.ptr, r.dbase,
-                                                             r.resident, r.rdelta, a.wbits, a, *(DynLds *)s_fuse,
-                                                             r.cd * k + (j - kFuseStatic), r.cd * K, h);
+                                                             r.resident, r.rdelta, a.wbits, a, r.lpos, r.nl, r.vsplit,
+                                                             *(DynLds *)s_fuse, r.cd * k + (j - kFuseStatic), r.cd * K,
+                                                             h);
         wave_add(r.counters + 1, c);
         wave_add(r.counters + 5, h);
     }
@@ -1184,6 +1209,26 @@ static void pack_rows(const og_graph *g, uint64_t v0, uint64_t v1, uint32_t wb, 
     });
 }
 
+// Dynamic rows [v0, V) in the split layout: the light part of v at
+// dst + lpos[v - v0], its heavy part at dst + nl + (ptr[v] - ptr[v0]) -
+// lpos[v - v0] (nl = lpos[V - v0], all light parts first).
+static void pack_split_rows(const og_graph *g, uint64_t v0, uint32_t wb, uint32_t sw, const uint64_t *lpos,
+                            uint32_t *dst) {
+    const uint64_t e0 = g->ptr[v0], nl = lpos[g->V - v0];
+    parallel_for(g->V - v0, 1 << 12, 0, [&](uint64_t b, uint64_t e, unsigned) {
+        for (uint64_t v = v0 + b; v < v0 + e; ++v) {
+            const uint32_t *src = g->wedges + 2 * g->ptr[v];
+            const uint64_t n = g->ptr[v + 1] - g->ptr[v], lp = lpos[v - v0];
+            uint32_t *lo = dst + lp, *hi = dst + nl + (g->ptr[v] - e0) - lp;
+            for (uint64_t i = 0; i < n; ++i) {
+                const uint32_t w = (src[2 * i] << wb) | src[2 * i + 1];
+                if (src[2 * i + 1] < sw) *lo++ = w;
+                else *hi++ = w;
+            }
+        }
+    });
+}
+
 // Uploads the packed rows [0, vend) into dst (device): host threads pack
 // row-aligned chunks of about 64 MiB into two pinned bounce buffers while
 // the previous chunk is in flight on s_compute.
@@ -1228,10 +1273,12 @@ static void upload_packed(DeviceState *d, const og_graph *g, uint64_t vend, uint
 // touches (b + kLinkLine - 4) / kLinkLine lines of kLinkLine bytes on average,
 // each fetched whole, so chunks of short rows are taken first (ties: lower
 // ids).  The chunk table (8 bytes per chunk) is charged to the same share.
-static void stage_resident(DeviceState *d, const og_graph *g, uint64_t vsplit, uint64_t cap) {
+static void stage_resident(DeviceState *d, const og_graph *g, uint64_t vsplit, uint64_t cap,
+                           const std::vector<uint64_t> &lpos) {
     const uint64_t V = g->V, eb = d->eb;
+    const bool split = !lpos.empty();  // two blocks per chunk: light parts, heavy parts
     const uint64_t c0 = vsplit >> kResChunkShift, c1 = (V + kResChunk - 1) >> kResChunkShift;
-    const uint64_t table_edges = (c1 * sizeof(int64_t) + eb - 1) / eb;
+    const uint64_t table_edges = ((split ? 2 : 1) * c1 * sizeof(int64_t) + eb - 1) / eb;
     if (vsplit >= V || cap <= table_edges) return;
     cap -= table_edges;
     const uint64_t nch = c1 - c0;
@@ -1261,34 +1308,53 @@ static void stage_resident(DeviceState *d, const og_graph *g, uint64_t vsplit, u
         used += cedges[i];
     }
     if (!used) return;
-    // the resident buffer holds the chosen chunks in vertex order; runs of
-    // adjacent chunks are gathered from the host source in one piece
+    // The resident buffer holds the chosen chunks in vertex order (with the
+    // split layout, each chunk's light block then its heavy block); runs of
+    // host positions that follow each other are gathered in one piece.
+    // Positions are in the host copy of the dynamic region.
     struct Run {
         uint64_t src, dst, n;
     };
     std::vector<Run> runs;
-    std::vector<int64_t> delta(c1, kNotResident);
+    auto add_run = [&](uint64_t src, uint64_t dst, uint64_t n) {
+        if (!n) return;
+        if (!runs.empty() && runs.back().src + runs.back().n == src && runs.back().dst + runs.back().n == dst)
+            runs.back().n += n;
+        else
+            runs.push_back({src, dst, n});
+    };
+    const uint64_t dbase = d->dyn_base, nl = split ? lpos[V - vsplit] : 0;
+    std::vector<int64_t> delta((split ? 2 : 1) * c1, kNotResident);
     uint64_t pos = 0;
     for (uint64_t i = 0; i < nch; ++i) {
         if (!take[i]) continue;
-        const uint64_t b = g->ptr[first(i)];
-        delta[c0 + i] = (int64_t)pos - (int64_t)b;
-        if (!runs.empty() && runs.back().src + runs.back().n == b) runs.back().n += cedges[i];
-        else runs.push_back({b, pos, cedges[i]});
+        const uint64_t v0 = first(i), v1 = last(i);
+        if (!split) {
+            const uint64_t h0 = g->ptr[v0] - dbase;
+            delta[c0 + i] = (int64_t)pos - (int64_t)h0;
+            add_run(h0, pos, cedges[i]);
+        } else {
+            const uint64_t l0 = lpos[v0 - vsplit], l1 = lpos[v1 - vsplit];
+            const uint64_t hh0 = nl + (g->ptr[v0] - dbase) - l0;
+            delta[2 * (c0 + i)] = (int64_t)pos - (int64_t)l0;
+            delta[2 * (c0 + i) + 1] = (int64_t)(pos + (l1 - l0)) - (int64_t)hh0;
+            add_run(l0, pos, l1 - l0);
+            add_run(hh0, pos + (l1 - l0), cedges[i] - (l1 - l0));
+        }
         pos += cedges[i];
     }
     MemPool tmp;
     uint8_t *hb = (uint8_t *)tmp.palloc(used * eb);
-    const uint8_t *src = (const uint8_t *)d->h_dyn;  // its edge 0 is CSR edge dyn_base
+    const uint8_t *src = (const uint8_t *)d->h_dyn;
     parallel_for(runs.size(), 64, 0, [&](uint64_t b, uint64_t e, unsigned) {
-        for (uint64_t r = b; r < e; ++r)
-            memcpy(hb + runs[r].dst * eb, src + (runs[r].src - d->dyn_base) * eb, runs[r].n * eb);
+        for (uint64_t r = b; r < e; ++r) memcpy(hb + runs[r].dst * eb, src + runs[r].src * eb, runs[r].n * eb);
     });
     d->dyn_resident = used;
     d->d_staging = d->dalloc(used * eb);
-    d->d_rdelta = (int64_t *)d->dalloc(c1 * sizeof(int64_t));
+    d->d_rdelta = (int64_t *)d->dalloc(delta.size() * sizeof(int64_t));
     OG_HIP(hipMemcpyAsync(d->d_staging, hb, used * eb, hipMemcpyHostToDevice, d->s_compute));
-    OG_HIP(hipMemcpyAsync(d->d_rdelta, delta.data(), c1 * sizeof(int64_t), hipMemcpyHostToDevice, d->s_compute));
+    OG_HIP(hipMemcpyAsync(d->d_rdelta, delta.data(), delta.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                          d->s_compute));
     OG_HIP(hipStreamSynchronize(d->s_compute));
 }
 
@@ -1447,9 +1513,25 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
             // CSR itself, or (packed) a pinned packed copy built once here.
             d->dyn_base = g->ptr[pl.vsplit];
             const uint64_t nd = E - d->dyn_base;
+            // zero-copy reads with the light/heavy split: all light parts
+            // first, so that a light round's reads of neighbouring rows share
+            // the link's 128-byte lines (staged chunks keep rows whole)
+            std::vector<uint64_t> lpos;
+            if (zc && d->split_weight) {
+                lpos.resize(V - pl.vsplit + 1);
+                lpos[0] = 0;
+                for (uint64_t v = pl.vsplit; v < V; ++v)
+                    lpos[v - pl.vsplit + 1] = lpos[v - pl.vsplit] + (d->h_pmid[v] - g->ptr[v]);
+                d->split_nl = lpos.back();
+                d->d_lpos = (uint64_t *)d->dalloc(lpos.size() * sizeof(uint64_t));
+                OG_HIP(hipMemcpyAsync(d->d_lpos, lpos.data(), lpos.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                      d->s_compute));
+                OG_HIP(hipStreamSynchronize(d->s_compute));
+            }
             if (d->packed) {
                 uint32_t *pk = (uint32_t *)d->palloc(std::max<uint64_t>(nd, 1) * 4);
-                pack_rows(g, pl.vsplit, V, d->wbits, d->split_weight, pk);
+                if (!lpos.empty()) pack_split_rows(g, pl.vsplit, d->wbits, d->split_weight, lpos.data(), pk);
+                else pack_rows(g, pl.vsplit, V, d->wbits, d->split_weight, pk);
                 d->h_dyn = pk;
             } else {
                 d->h_dyn = (const uint8_t *)g->edge_data() + d->dyn_base * eb;
@@ -1466,7 +1548,7 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
                 // Reading in place needs no transit slots, so the staging
                 // share of the budget keeps part of the dynamic region
                 // resident in HBM instead.
-                stage_resident(d.get(), g, pl.vsplit, pl.staging_edges);
+                stage_resident(d.get(), g, pl.vsplit, pl.staging_edges, lpos);
             }
         }
     }
@@ -1723,8 +1805,9 @@ template <int ALGO, bool W, bool PACKED, bool DIRECT>
 static void launch_dyn(DeviceState *d, unsigned grid, const void *src, uint64_t c0, uint64_t c1) {
     hipLaunchKernelGGL((k_relax_dyn<ALGO, W, PACKED, DIRECT>), dim3(grid), dim3(kBlock), 0, d->s_compute, d->fr.list,
                        d->fr.info, d->fr.doff, src, c0, c1, d->d_ptr, d->dyn_base, (const void *)d->d_staging,
-                       DIRECT ? (const int64_t *)d->d_rdelta : nullptr, d->wbits, relax_args(d), d->fr.counters + 1,
-                       d->fr.counters + 5);
+                       DIRECT ? (const int64_t *)d->d_rdelta : nullptr, d->wbits, relax_args(d),
+                       DIRECT ? (const uint64_t *)d->d_lpos : nullptr, d->split_nl, d->plan.vsplit,
+                       d->fr.counters + 1, d->fr.counters + 5);
 }
 
 // The fused round relax (k_relax_round) of a device-driven round on the
@@ -1755,6 +1838,9 @@ static void launch_round_fmt(DeviceState *d, bool with_dyn) {
     r.dbase = d->dyn_base;
     r.resident = d->d_staging;
     r.rdelta = d->d_rdelta;
+    r.lpos = d->d_lpos;
+    r.nl = d->split_nl;
+    r.vsplit = d->plan.vsplit;
     r.counters = d->fr.counters;
     const unsigned grid = (kFuseStatic + r.cd) * kHubGridCap;
     const size_t lds = std::max<size_t>(with_dyn ? sizeof(DynLds) : 0, (size_t)d->tune.tile * d->eb);
