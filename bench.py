@@ -6,6 +6,8 @@ edge factor 16, a/b/c/d = .57/.19/.19/.05 (graph seed 1002), self-loops and
 duplicates removed, u32 weights uniform in [1, 63] (seed 2002); device edge
 budget = 25% of the CSR edge+weight bytes; Ascetic plan with advisek = 0.8;
 source = seeded uniform pick (seed 3002) among vertices with out-degree > 0.
+Engine settings (not part of the workload): zero-copy dynamic region,
+delta-stepping with light edges of weight < 8 and buckets of width 256.
 
 A step is one full SSSP from that source (graph resident in pinned host
 memory + the static prefix in HBM, as prepared once before timing).  The
@@ -548,7 +550,8 @@ def main():
     ap.add_argument("--approach", default="ascetic", choices=["ascetic", "subway", "usm"])
     ap.add_argument("--compare", action="store_true", help="also time subway and usm")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--delta", type=int, default=32, help="SSSP near-far bucket width (0: plain rounds)")
+    ap.add_argument("--delta", type=int, default=None,
+                    help="SSSP near-far bucket width (0: plain rounds; default 256 with the light/heavy split, else 32)")
     ap.add_argument("--defer", type=int, default=None, choices=[0, 1, 2],
                     help="dynamic-region order: 1 held back while static vertices are active, "
                          "2 lagged behind the static part of each near-far bucket")
@@ -563,8 +566,9 @@ def main():
     ap.add_argument("--sweep-delta", default="", help="comma list: time SSSP at each delta and report")
     ap.add_argument("--source-seed", type=int, default=3002, help="sssp-ooc: seed of the source pick")
     ap.add_argument("--dyn-grid", type=int, default=None, help="workgroups of the dynamic-region relax (0: 8192)")
-    ap.add_argument("--split-weight", type=int, default=0,
-                    help="SSSP light/heavy split: rows store edges of weight < this first (0: off)")
+    ap.add_argument("--split-weight", type=int, default=None,
+                    help="SSSP light/heavy split: edges of weight < this are light (0: off; default 8 for "
+                         "sssp-ooc, off elsewhere)")
     ap.add_argument("--sweep-dyn-grid", default="", help="comma list: time SSSP at each dyn_grid and report")
     ap.add_argument("--link-probe", action="store_true",
                     help="only measure shader reads of pinned host memory and hipMemcpy (host-link bound)")
@@ -613,8 +617,20 @@ def main():
         print(json.dumps({"link_probe": og.probe_host_link(1 << 30), "peak_gbs": LINK_PEAK_GBS}))
         return
     if args.workload != "sssp-ooc":
+        if args.split_weight is None:
+            args.split_weight = 0
+        if args.delta is None:
+            args.delta = 32
         WORKLOADS[args.workload](args)
         return
+    # headline SSSP: delta-stepping with light edges (weight < 8) relaxed in
+    # the buckets' rounds and heavy ones once per settled vertex; with the
+    # split the work hardly depends on the bucket width, so wide buckets
+    # (fewer rounds) pay
+    if args.split_weight is None:
+        args.split_weight = 8
+    if args.delta is None:
+        args.delta = 256 if args.split_weight else 32
 
     t0 = time.time()
     g = og.Graph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, seed=1002, weight_seed=2002)
@@ -692,7 +708,7 @@ def main():
         with open(tpath) as f:
             tr = json.load(f)
         want = {"scale": args.scale, "approach": args.approach, "loader": args.loader, "delta": args.delta,
-                "kernel_class": kclass, "edge_format": edge_format}
+                "split_weight": args.split_weight, "kernel_class": kclass, "edge_format": edge_format}
         if tr.get("config") == want:
             traffic, traffic_src = tr["traffic_bytes_per_launch"], tr["source"]
     roof, link, kt = roofline_of(stats, zero_copy=args.loader == "zerocopy" and args.approach != "usm",
@@ -733,7 +749,7 @@ def main():
             "V": V, "E": E, "source": src,
             "edge_budget_bytes": budget, "budget_fraction": args.budget, "advisek": args.advisek,
             "approach": args.approach, "static_edges": plan["static_edges"], "staging_edges": plan["staging_edges"],
-            "vsplit": plan["vsplit"], "loader": args.loader, "delta": args.delta,
+            "vsplit": plan["vsplit"], "loader": args.loader, "delta": args.delta, "split_weight": args.split_weight,
             "edge_format": edge_format,
         },
         "roofline": roof,

@@ -168,13 +168,16 @@ typedef struct og_plan_params {
     uint32_t flags;             /* OG_PLAN_* bits                             */
     uint32_t split_weight;      /* SSSP light/heavy split (delta-stepping,
                                    Meyer & Sanders): > 0 stores each row of
-                                   the device and link copies light edges
-                                   (weight < split_weight) first; near-far
-                                   SSSP rounds then relax light edges only and
-                                   a vertex's heavy edges once, after its
-                                   bucket has settled.  Set it to the tuning's
-                                   delta.  Packed weighted graphs, Ascetic and
-                                   Subway; 0 = no split                        */
+                                   the device copy light edges (weight <
+                                   split_weight) first, and the zero-copy
+                                   host copy all light parts before all heavy
+                                   parts; near-far SSSP rounds then relax
+                                   light edges only and a vertex's heavy
+                                   edges once, after its bucket has settled.
+                                   With weights in [1, 63], 8 with a bucket
+                                   width (tuning delta) of 256 does the least
+                                   work per round.  Packed weighted graphs,
+                                   Ascetic and Subway; 0 = no split          */
 } og_plan_params;
 
 #define OG_PLAN_NO_OVERLAP 1u   /* serialise copies, static and dynamic relax
