@@ -399,7 +399,8 @@ def wl_serve(args):
         svc = {}
         rng = np.random.default_rng(3005)
         for gi, g in enumerate(graphs):
-            g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0, flags=flags)
+            g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0, flags=flags,
+                      split_weight=args.split_weight)
             g.set_tuning(delta=args.delta)  # SSSP near-far width for the plain path and the lanes alike
             if args.autotune:  # shapes for SSSP, shared by the plain path and the server's lanes
                 best, best_ms = g.autotune(og.TUNE_SSSP, int(cands[gi][0]), runs=1)
@@ -521,7 +522,8 @@ def wl_serve(args):
                       1e3 / max(best["served_qps"], 1e-9), "u32",
                       {"workload": "served bfs/sssp queries, rmat " + "/".join(map(str, scales)) + " ef16",
                        "rates": rates, "mixes": mixes, "duration_s": args.duration, "state_budget_bytes": budget,
-                       "p99_target_ms": args.p99_target_ms})
+                       "p99_target_ms": args.p99_target_ms, "sssp_delta": args.delta,
+                       "split_weight": args.split_weight})
     out.update({"serve": per_mix,
                 "parity": [{"mix": k, "rate": r, "checked": v["checked"], "mismatches": v["mismatches"]}
                            for k, m in per_mix.items() for r, v in m["rates"].items()],
@@ -617,10 +619,10 @@ def main():
         print(json.dumps({"link_probe": og.probe_host_link(1 << 30), "peak_gbs": LINK_PEAK_GBS}))
         return
     if args.workload != "sssp-ooc":
-        if args.split_weight is None:
-            args.split_weight = 0
+        if args.split_weight is None:  # the served SSSP lanes take the headline's split too
+            args.split_weight = 8 if args.workload == "serve" else 0
         if args.delta is None:
-            args.delta = 32
+            args.delta = 256 if args.split_weight else 32
         WORKLOADS[args.workload](args)
         return
     # headline SSSP: delta-stepping with light edges (weight < 8) relaxed in

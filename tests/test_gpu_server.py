@@ -18,17 +18,24 @@ def _say(*a):
     sys.stdout.flush()
 
 
-@pytest.fixture(scope="module", params=["topdown", "diropt"])
+@pytest.fixture(scope="module", params=["topdown", "diropt", "diropt+split"])
 def graphs(request):
     """Three graphs; "diropt" prepares them with their in-edges, so BFS slots
-    with dense frontiers run bottom-up rounds (graphs.in_edges tells which)."""
+    with dense frontiers run bottom-up rounds (graphs.in_edges tells which);
+    "+split" also stores the weighted graphs' rows light edges first, so the
+    SSSP lanes run delta-stepping with wide buckets while the BFS slots read
+    whole rows."""
     gs = [og.Graph.rmat(12, 16, seed=51, weight_seed=52), og.Graph.er(3000, 8, seed=53).add_weights(54),
           og.Graph.rmat(11, 8, seed=55)]
-    flags = og.PLAN_DIRECTION_OPT if request.param == "diropt" else 0
+    flags = og.PLAN_DIRECTION_OPT if request.param.startswith("diropt") else 0
+    split = request.param.endswith("+split")
     for g in gs:
-        g.prepare("ascetic", edge_budget_bytes=g.num_edges * (8 if g.info()["kind"] == og.KIND_WEIGHTED else 4),
-                  advisek=1.0, flags=flags)
-    yield _Graphs(gs, request.param == "diropt")
+        weighted = g.info()["kind"] == og.KIND_WEIGHTED
+        g.prepare("ascetic", edge_budget_bytes=g.num_edges * (8 if weighted else 4), advisek=1.0, flags=flags,
+                  split_weight=8 if split and weighted else 0)
+        if split and weighted:
+            g.set_tuning(delta=256)
+    yield _Graphs(gs, request.param.startswith("diropt"))
     for g in gs:
         g.close()
 
