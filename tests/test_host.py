@@ -316,3 +316,13 @@ def test_levels_checksum_matches_restatement():
     with np.errstate(over="ignore"):
         ref = int(mix64((v << np.uint64(32)) | x[keep].astype(np.uint64)).sum(dtype=np.uint64))
     assert og.levels_checksum(x) == ref
+
+
+def test_plan_flags_match_header():
+    """The Python PLAN_* constants are the header's OG_PLAN_* values."""
+    hdr = open(os.path.join(ROOT, "include", "onegraph.h")).read()
+    defs = dict(re.findall(r"#define OG_PLAN_([A-Z_]+) (\d+)u", hdr))
+    assert set(defs) >= {"NO_OVERLAP", "ZERO_COPY", "NO_PACK", "NO_GRAPH", "DIRECTION_OPT", "NO_FUSE"}
+    for name, value in defs.items():
+        assert getattr(og, "PLAN_" + name) == int(value), name
+    assert len(set(defs.values())) == len(defs)  # distinct bits
