@@ -126,6 +126,9 @@ struct RoundCtl {
     uint64_t work_edges;         // degree sums over active vertices
     uint64_t static_work_edges;  // ... over the static bins
     uint64_t dyn_rounds;         // rounds with a dynamic list
+    uint32_t halt;               // 1 (device-driven runs): once a compaction finds nothing live, the
+                                 // count passes of rounds enqueued past the end skip their sweep
+    uint32_t pad_;
 };
 
 constexpr int kBcntRows = 7;            // per-block rows: bins 0-3, zero|hub, far|lagging, min far

@@ -150,6 +150,9 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                                                            uint32_t nblk) {
     __shared__ uint64_t lds[6 * kBlock / kWave];
     __shared__ uint4 s_win[kVertsPerBlock / 16];  // the window's flag bytes
+    // a round enqueued after the traversal ended: no vertex is flagged and
+    // the block counts of the last pass (all zero) stand
+    if (ctl->halt && !ctl->live) return;
     // light/heavy split: a heavy compaction collects the heavy-pending
     // vertices (no near-far test) and sizes them by their heavy edges
     const bool heavy = pmid && ctl->mode;
@@ -397,6 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                                                            uint64_t *__restrict__ m_doff) {
     __shared__ uint64_t lds[3 * kBlock / kWave];
     __shared__ uint32_t s_win[4 * kFlagVecs * kBlock];  // word i of thread t at [i * kBlock + t]
+    if (!info->live) return;  // nothing is flagged: no list, and the window is already clear
     // the array and row ranges the count pass used (light/heavy split)
     const bool heavy = pmid && info->heavy;
     uint8_t *flag = heavy ? hflag : flag_a;
@@ -516,6 +520,7 @@ __global__ void k_ctl_reset(RoundCtl *ctl, uint32_t delta, uint32_t defer, uint6
     c.thresh = delta ? (uint32_t)min((uint64_t)0xFFFFFFFFull, 1ull + delta) : 0xFFFFFFFFu;
     c.fbound = 0;
     c.live = 1;
+    c.halt = 1;
     *ctl = c;
 }
 
