@@ -135,7 +135,8 @@ def roofline_of(stats_list, zero_copy=False, traffic=None):
     elif zero_copy and agg["h2d_bytes"] and agg["relax_dynamic_kernel_ms"] > 0:
         gbs = agg["h2d_bytes"] / (agg["relax_dynamic_kernel_ms"] * 1e-3) / 1e9
         link = {"achieved": round(gbs, 2), "peak": LINK_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / LINK_PEAK_GBS, 4),
-                "measured_on": "k_relax_dyn (direct reads)"}
+                "measured_on": ("k_relax_round" if kt.get("relax_dynamic", {}).get("fused") else "k_relax_dyn")
+                + " (direct reads)"}
     return roof, link, kt
 
 
