@@ -15,6 +15,9 @@
  *   - BFS/SSSP results use the reference CPU convention: the source holds 1,
  *     a vertex at hop h (distance d) holds h+1 (d+1), unreached holds
  *     0xFFFFFFFF (BFS/cpu_BFS/main.cpp:141-145, SSSP/oneGraph/GraphMeta.h:374-381);
+ *     an SSSP distance d with d + 1 >= 0xFFFFFFFE holds 0xFFFFFFFE (the
+ *     reference's u32 sum wraps there instead, SSSP/oneGraph/gpu_kernels.h:
+ *     186-198; weights in [1, 63] never reach it below 2^26 hops);
  *   - CC labels: min-label propagation from value[i]=i
  *     (CC/serial/serial.cpp:164-213).
  */
@@ -152,7 +155,10 @@ og_status og_graph_export(const og_graph *g, uint64_t *ptr, uint32_t *cols,
  * after `last` whose degree exceeds 3000, else `last`. */
 og_status og_graph_find_big_node(const og_graph *g, uint32_t last, uint32_t *out);
 
-void og_graph_free(og_graph *g);
+/* Frees the graph and its device state.  OG_EBUSY (and the graph left
+ * intact) while another call runs on it or an og_server holds it; NULL is
+ * OG_OK. */
+og_status og_graph_free(og_graph *g);
 
 /* ------------------------------------------------------------- planning */
 

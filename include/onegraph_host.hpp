@@ -51,6 +51,12 @@ struct Options {
 og_status check_result(const std::string &path, const void *got, uint64_t n, int bytes_per_value, bool is_float,
                        double tol);
 
+// Loads `path` as `kind` (the reference reader, GraphMeta::readDataFromFile,
+// BFS/oneGraph_BFS/GraphMeta.h:125-145), or a text edge list with
+// opt.text_input, or builds opt.generate instead; prints the reference's
+// "readDataFromFile" timing line.  The caller frees *out with og_graph_free.
+og_status open_graph(const std::string &path, og_graph_kind kind, const Options &opt, og_graph **out);
+
 // Prints the reference's labelled lines (TimeRecord "<name> time is X ms",
 // nodeSum, staticnodeSum, "move overload size") and optionally one JSON line.
 void print_report(const char *algo, const og_run_stats &st, const og_plan_info &plan, bool json, FILE *f = stdout);

@@ -152,7 +152,7 @@ def lib():
         "og_graph_get_info": (C.c_int, [_P, C.POINTER(GraphInfo)]),
         "og_graph_export": (C.c_int, [_P, _u64p, _u32p, _u32p, _u32p]),
         "og_graph_find_big_node": (C.c_int, [_P, C.c_uint32, _u32p]),
-        "og_graph_free": (None, [_P]),
+        "og_graph_free": (C.c_int, [_P]),
         "og_plan_compute": (C.c_int, [C.c_uint64, C.c_uint64, _u64p, C.c_uint32, C.POINTER(PlanParams),
                                       C.POINTER(PlanInfo)]),
         "og_prepare": (C.c_int, [_P, C.c_int, C.POINTER(PlanParams), C.POINTER(PlanInfo)]),
@@ -406,7 +406,7 @@ class Graph:
     # ------------------------------------------------------------ lifetime
     def close(self):
         if self._h and self._h.value:
-            lib().og_graph_free(self._h)
+            _check(lib().og_graph_free(self._h))
             self._h = C.c_void_p(0)
 
     def __del__(self):

@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libonegraph.so")
 CLI = os.path.join(HERE, "onegraph")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["host_graph.cpp", "engine.hip", "pagerank.hip", "server.hip"]
+SOURCES = ["host_graph.cpp", "drivers.cpp", "engine.hip", "pagerank.hip", "server.hip"]
 
 COMMON = [
     "--offload-arch=gfx950",

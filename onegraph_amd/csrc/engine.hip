@@ -530,8 +530,10 @@ template <int ALGO>
 __device__ __forceinline__ uint32_t offer(uint32_t du, uint32_t wt) {
     if (ALGO == ALGO_BFS) return du + 1;
     if (ALGO == ALGO_SSSP) {
-        const uint32_t s = du + wt;
-        return s < du ? 0xFFFFFFFEu : s;
+        // distance + 1 saturates at 0xFFFFFFFE (0xFFFFFFFF means unreached):
+        // min() is monotone, so the fixed point is min(dist + 1, 0xFFFFFFFE)
+        const uint64_t s = (uint64_t)du + wt;
+        return s > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)s;
     }
     return du;
 }

@@ -457,12 +457,20 @@ extern "C" og_status og_graph_find_big_node(const og_graph *g, uint32_t last, ui
     return OG_OK;
 }
 
-extern "C" void og_graph_free(og_graph *g) {
-    if (!g) return;
+extern "C" og_status og_graph_free(og_graph *g) {
+    clear_error();
+    if (!g) return OG_OK;
+    // a graph reserved by a running call or by an og_server keeps its
+    // device buffers: freeing them under live kernels would be a device
+    // use-after-free, so the call is refused and the graph left alone
+    int expect = 0;
+    if (!g->busy.compare_exchange_strong(expect, 1))
+        return fail(OG_EBUSY, "graph is in use (a running call or an og_server holds it); destroy the server first");
     if (g->dev) device_state_free(g->dev);
     g->dev = nullptr;
     graph_release_host(g);
     delete g;
+    return OG_OK;
 }
 
 // ======================================================= CSR construction

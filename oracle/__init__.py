@@ -40,6 +40,8 @@ def lib():
         L.og_oracle_pagerank.restype = None
         L.og_oracle_pagerank_par.argtypes = L.og_oracle_pagerank.argtypes + [C.c_int]
         L.og_oracle_pagerank_par.restype = None
+        L.og_oracle_loop_seconds.argtypes = []
+        L.og_oracle_loop_seconds.restype = C.c_double
         _lib = L
     return _lib
 
@@ -92,9 +94,17 @@ def cc(ptr, cols):
     return out, n
 
 
-def pagerank(out_degree, ptr, in_cols, damping=0.85, tol=1e-7, max_it=100, threads=1):
+def loop_seconds():
+    """Wall seconds of the last oracle call's compute loop (allocation and
+    initialisation excluded)."""
+    return lib().og_oracle_loop_seconds()
+
+
+def pagerank(out_degree, ptr, in_cols, damping=0.85, tol=1e-7, max_it=100, threads=1, parallel=None):
     """FP64 normalised PageRank over the pull layout; returns (ranks, iterations, l1).
-    threads > 1 splits the vertex loops (same per-vertex summation order)."""
+    threads > 1 (or parallel=True) runs og_oracle_pagerank_par, which spreads
+    the vertex blocks over threads with the same summation order (identical
+    results)."""
     ptr, in_cols = _prep(ptr, in_cols)
     od = np.ascontiguousarray(out_degree, dtype=np.uint32)
     V, E = len(ptr), len(in_cols)
@@ -103,7 +113,7 @@ def pagerank(out_degree, ptr, in_cols, damping=0.85, tol=1e-7, max_it=100, threa
     l1 = C.c_double()
     args = (V, E, _p32(od), _p64(ptr), _p32(in_cols if E else np.zeros(1, np.uint32)),
             damping, tol, max_it, out.ctypes.data_as(_f64p), C.byref(it), C.byref(l1))
-    if threads > 1:
+    if parallel or (parallel is None and threads > 1):
         lib().og_oracle_pagerank_par(*args, threads)
     else:
         lib().og_oracle_pagerank(*args)

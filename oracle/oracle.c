@@ -18,14 +18,33 @@
  *                         connected_components fixtures, and the update count
  *                         against the reference serial CC program compiled
  *                         from its own source (oracle/_ref/cc_serial).
+ *   og_oracle_sssp     -- also: distance + 1 saturates at 0xFFFFFFFE (a
+ *                         fixture with weights near 2^31 checks the cap).
  *   og_oracle_pagerank -- closed forms (cycle, complete graph, star), networkx
  *                         pagerank fixtures, mass conservation.
+ *   og_oracle_pagerank_par -- equal to og_oracle_pagerank bit for bit (ranks,
+ *                         iterations, residual) at 1, 2 and 16 threads.
+ *
+ * og_oracle_loop_seconds() returns the wall time of the last call's compute
+ * loop alone (allocation and initialisation excluded), the boundary the
+ * reference CPU programs time (CPUCalTime, BFS/cpu_BFS/main.cpp:156-208).
  */
 #include "oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
+
+static double g_loop_s;
+
+static double now_s(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+double og_oracle_loop_seconds(void) { return g_loop_s; }
 
 static uint64_t row_end(uint64_t V, uint64_t E, const uint64_t *ptr, uint64_t v) {
     return v + 1 < V ? ptr[v + 1] : E;
@@ -45,6 +64,7 @@ void og_oracle_bfs(uint64_t V, uint64_t E, const uint64_t *ptr, const uint32_t *
     value[src] = 1;
     uint64_t active = 1, sum = 0;
     uint32_t iter = 0;
+    const double t0 = now_s();
     while (active > 0) {
         sum += active;
         for (uint64_t u = 0; u < V; ++u) {
@@ -66,6 +86,7 @@ void og_oracle_bfs(uint64_t V, uint64_t E, const uint64_t *ptr, const uint32_t *
         }
         ++iter;
     }
+    g_loop_s = now_s() - t0;
     if (node_sum) *node_sum = sum;
     if (iterations) *iterations = iter;
     free(act);
@@ -116,6 +137,7 @@ void og_oracle_sssp(uint64_t V, uint64_t E, const uint64_t *ptr, const uint32_t 
     for (uint64_t i = 0; i < V; ++i) dist[i] = UINT64_MAX;
     dist[src] = 0;
     HeapItem s0 = {0, src};
+    const double t0 = now_s();
     heap_push(heap, &n, s0);
     while (n) {
         HeapItem it = heap_pop(heap, &n);
@@ -131,7 +153,12 @@ void og_oracle_sssp(uint64_t V, uint64_t E, const uint64_t *ptr, const uint32_t 
             }
         }
     }
-    for (uint64_t i = 0; i < V; ++i) value[i] = dist[i] == UINT64_MAX ? 0xFFFFFFFFu : (uint32_t)(dist[i] + 1);
+    g_loop_s = now_s() - t0;
+    /* distance + 1 saturates at 0xFFFFFFFE, the largest value that is not
+     * "unreached" (the engine's convention for paths longer than 2^32 - 2) */
+    for (uint64_t i = 0; i < V; ++i)
+        value[i] = dist[i] == UINT64_MAX ? 0xFFFFFFFFu
+                                         : (dist[i] >= 0xFFFFFFFDull ? 0xFFFFFFFEu : (uint32_t)(dist[i] + 1));
     free(dist);
     free(done);
     free(heap);
@@ -151,6 +178,7 @@ uint64_t og_oracle_cc(uint64_t V, uint64_t E, const uint64_t *ptr, const uint32_
         act[i] = 1;
     }
     uint64_t updates = 0, active = 1;
+    const double t0 = now_s();
     for (;;) {
         if (active == 0) break;
         for (uint64_t u = 0; u < V; ++u) {
@@ -172,10 +200,18 @@ uint64_t og_oracle_cc(uint64_t V, uint64_t E, const uint64_t *ptr, const uint32_
             next[i] = 0;
         }
     }
+    g_loop_s = now_s() - t0;
     free(act);
     free(next);
     return updates;
 }
+
+/* Dangling mass and L1 residual are reduced over fixed blocks of
+ * PR_BLOCK vertices: each block sums in vertex order and the block partials
+ * are added in block order.  Both PageRank functions below use exactly this
+ * order, so their ranks, iteration counts and residuals are identical for
+ * every thread count. */
+#define PR_BLOCK 4096u
 
 /* Normalised PageRank by FP64 power iteration over the pull layout (the loop
  * shape of pr_cal, PR/cpu/pr.cpp:136-149; the numerics are the textbook
@@ -191,80 +227,96 @@ void og_oracle_pagerank(uint64_t V, uint64_t E, const uint32_t *outdeg, const ui
     for (uint64_t i = 0; i < V; ++i) rank[i] = inv_n;
     uint32_t it = 0;
     double l1 = 0;
+    const double t0 = now_s();
     while (it < max_it) {
         double dang = 0;
-        for (uint64_t u = 0; u < V; ++u) {
-            contrib[u] = outdeg[u] ? rank[u] / (double)outdeg[u] : 0.0;
-            if (!outdeg[u]) dang += rank[u];
+        for (uint64_t b0 = 0; b0 < V; b0 += PR_BLOCK) {
+            const uint64_t b1 = b0 + PR_BLOCK < V ? b0 + PR_BLOCK : V;
+            double d = 0;
+            for (uint64_t u = b0; u < b1; ++u) {
+                contrib[u] = outdeg[u] ? rank[u] / (double)outdeg[u] : 0.0;
+                if (!outdeg[u]) d += rank[u];
+            }
+            dang += d;
         }
         const double base = (1.0 - damping) * inv_n + damping * dang * inv_n;
         l1 = 0;
-        for (uint64_t v = 0; v < V; ++v) {
-            double s = 0;
-            for (uint64_t i = ptr[v]; i < row_end(V, E, ptr, v); ++i) s += contrib[in_cols[i]];
-            next[v] = base + damping * s;
-            l1 += fabs(next[v] - rank[v]);
+        for (uint64_t b0 = 0; b0 < V; b0 += PR_BLOCK) {
+            const uint64_t b1 = b0 + PR_BLOCK < V ? b0 + PR_BLOCK : V;
+            double l = 0;
+            for (uint64_t v = b0; v < b1; ++v) {
+                double s = 0;
+                for (uint64_t i = ptr[v]; i < row_end(V, E, ptr, v); ++i) s += contrib[in_cols[i]];
+                next[v] = base + damping * s;
+                l += fabs(next[v] - rank[v]);
+            }
+            l1 += l;
         }
         memcpy(rank, next, V * sizeof(double));
         ++it;
         if (l1 < tol) break;
     }
+    g_loop_s = now_s() - t0;
     if (iterations) *iterations = it;
     if (final_l1) *final_l1 = l1;
     free(next);
     free(contrib);
 }
 
-/* og_oracle_pagerank with the vertex loops split over `threads` OpenMP
- * threads.  Each rank is summed in the same in-edge order as the serial
- * version, so ranks are identical to it; the L1 and dangling sums are
- * reduced from fixed per-thread blocks. */
+/* og_oracle_pagerank with the blocks of each vertex loop spread over
+ * `threads` OpenMP threads (the CPU baseline's all-core variant).  Every
+ * rank is summed in the same in-edge order and the block partials of the
+ * dangling mass and the residual are stored per block and added in block
+ * order afterwards, so the results equal og_oracle_pagerank's bit for bit. */
 void og_oracle_pagerank_par(uint64_t V, uint64_t E, const uint32_t *outdeg, const uint64_t *ptr,
                             const uint32_t *in_cols, double damping, double tol, uint32_t max_it, double *rank,
                             uint32_t *iterations, double *final_l1, int threads) {
+    const uint64_t nblk = (V + PR_BLOCK - 1) / PR_BLOCK;
     double *next = (double *)malloc(V * sizeof(double));
     double *contrib = (double *)malloc(V * sizeof(double));
+    double *part = (double *)malloc((nblk ? nblk : 1) * sizeof(double));
     const double inv_n = 1.0 / (double)V;
     for (uint64_t i = 0; i < V; ++i) rank[i] = inv_n;
     uint32_t it = 0;
     double l1 = 0;
     if (threads < 1) threads = 1;
-    double part[256];
-    if (threads > 256) threads = 256;
+    const double t0 = now_s();
     while (it < max_it) {
 #pragma omp parallel for num_threads(threads) schedule(static)
-        for (int t = 0; t < threads; ++t) {
-            const uint64_t b = V * (uint64_t)t / (uint64_t)threads, e = V * (uint64_t)(t + 1) / (uint64_t)threads;
+        for (int64_t k = 0; k < (int64_t)nblk; ++k) {
+            const uint64_t b0 = (uint64_t)k * PR_BLOCK, b1 = b0 + PR_BLOCK < V ? b0 + PR_BLOCK : V;
             double d = 0;
-            for (uint64_t u = b; u < e; ++u) {
+            for (uint64_t u = b0; u < b1; ++u) {
                 contrib[u] = outdeg[u] ? rank[u] / (double)outdeg[u] : 0.0;
                 if (!outdeg[u]) d += rank[u];
             }
-            part[t] = d;
+            part[k] = d;
         }
         double dang = 0;
-        for (int t = 0; t < threads; ++t) dang += part[t];
+        for (uint64_t k = 0; k < nblk; ++k) dang += part[k];
         const double base = (1.0 - damping) * inv_n + damping * dang * inv_n;
 #pragma omp parallel for num_threads(threads) schedule(static)
-        for (int t = 0; t < threads; ++t) {
-            const uint64_t b = V * (uint64_t)t / (uint64_t)threads, e = V * (uint64_t)(t + 1) / (uint64_t)threads;
+        for (int64_t k = 0; k < (int64_t)nblk; ++k) {
+            const uint64_t b0 = (uint64_t)k * PR_BLOCK, b1 = b0 + PR_BLOCK < V ? b0 + PR_BLOCK : V;
             double l = 0;
-            for (uint64_t v = b; v < e; ++v) {
+            for (uint64_t v = b0; v < b1; ++v) {
                 double s = 0;
                 for (uint64_t i = ptr[v]; i < row_end(V, E, ptr, v); ++i) s += contrib[in_cols[i]];
                 next[v] = base + damping * s;
                 l += fabs(next[v] - rank[v]);
             }
-            part[t] = l;
+            part[k] = l;
         }
         l1 = 0;
-        for (int t = 0; t < threads; ++t) l1 += part[t];
+        for (uint64_t k = 0; k < nblk; ++k) l1 += part[k];
         memcpy(rank, next, V * sizeof(double));
         ++it;
         if (l1 < tol) break;
     }
+    g_loop_s = now_s() - t0;
     if (iterations) *iterations = it;
     if (final_l1) *final_l1 = l1;
     free(next);
     free(contrib);
+    free(part);
 }

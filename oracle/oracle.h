@@ -20,6 +20,9 @@ void og_oracle_pagerank(uint64_t V, uint64_t E, const uint32_t *outdeg, const ui
 void og_oracle_pagerank_par(uint64_t V, uint64_t E, const uint32_t *outdeg, const uint64_t *ptr,
                             const uint32_t *in_cols, double damping, double tol, uint32_t max_it, double *rank,
                             uint32_t *iterations, double *final_l1, int threads);
+/* Seconds spent in the compute loop of the last oracle call (this thread's
+ * process; allocation and initialisation excluded). */
+double og_oracle_loop_seconds(void);
 
 #ifdef __cplusplus
 }

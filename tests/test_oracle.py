@@ -196,3 +196,112 @@ def test_pagerank_closed_forms():
     # all dangling: stays uniform; mass conserved
     r, it, _ = oracle.pagerank(np.zeros(4, np.uint32), np.zeros(4, np.uint64), np.zeros(0, np.uint32), d, 1e-9, 50)
     assert np.abs(r - 0.25).max() < 1e-15 and it == 1
+
+
+def _pull_of(ptr, cols):
+    """Pull layout (out-degree, in-edge CSR sorted by source) of a CSR."""
+    V, E = len(ptr), len(cols)
+    src = np.repeat(np.arange(V), np.diff(np.append(ptr.astype(np.int64), E)))
+    order = np.lexsort((src, cols))
+    in_src = src[order].astype(np.uint32)
+    in_ptr = np.concatenate([[0], np.cumsum(np.bincount(cols, minlength=V))[:-1]]).astype(np.uint64)
+    outdeg = np.bincount(src, minlength=V).astype(np.uint32)
+    return outdeg, in_ptr, in_src
+
+
+def _er_pull(n, avg, seed):
+    """Symmetrised G(n, m = n * avg / 2) in numpy, as a pull layout."""
+    rng = np.random.default_rng(seed)
+    m = n * avg // 2
+    a = rng.integers(0, n, m)
+    b = rng.integers(0, n, m)
+    keep = a != b
+    u = np.concatenate([a[keep], b[keep]])
+    v = np.concatenate([b[keep], a[keep]])
+    key = np.unique(u.astype(np.int64) * n + v)
+    u, v = key // n, (key % n).astype(np.uint32)
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(u, minlength=n))[:-1]]).astype(np.uint64)
+    return _pull_of(ptr, v)
+
+
+def _rmat_like(n, m, seed):
+    """A skewed directed graph with many dangling vertices (no out-edges)."""
+    rng = np.random.default_rng(seed)
+    a = (rng.pareto(1.2, m) * 3).astype(np.int64) % n
+    b = rng.integers(0, n, m)
+    keep = a != b
+    key = np.unique(a[keep] * n + b[keep])
+    u, v = key // n, (key % n).astype(np.uint32)
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(u, minlength=n))[:-1]]).astype(np.uint64)
+    return _pull_of(ptr, v)
+
+
+def _pull_cases():
+    cases = []
+    for path in GOLDEN:
+        z = np.load(path)
+        cases.append((os.path.basename(path), _pull_of(z["ptr"], z["cols"])))
+    cases.append(("er_2^16_avg32", _er_pull(1 << 16, 32, 11)))
+    cases.append(("skewed_dangling", _rmat_like(20000, 120000, 12)))
+    return cases
+
+
+@pytest.mark.parametrize("threads", [1, 2, 16])
+def test_pagerank_par_equals_serial(threads):
+    """og_oracle_pagerank_par (the CPU baseline's all-core variant) against
+    the pinned serial function: ranks bit for bit, same iteration count and
+    residual, including graphs with dangling mass and ones that hit the
+    iteration cap (tolerance 0)."""
+    for name, (od, ip, ic) in _pull_cases():
+        for tol, cap in ((1e-7, 100), (0.0, 7)):
+            r0, it0, l0 = oracle.pagerank(od, ip, ic, 0.85, tol, cap, parallel=False)
+            r1, it1, l1 = oracle.pagerank(od, ip, ic, 0.85, tol, cap, threads=threads, parallel=True)
+            assert it0 == it1, name
+            assert l0 == l1, name
+            assert np.array_equal(r0.view(np.uint64), r1.view(np.uint64)), name
+        assert abs(r0.sum() - 1.0) < 1e-9, name
+
+
+def test_pagerank_par_on_closed_form():
+    """The threaded function on its own: a directed cycle stays uniform and
+    the star's closed form holds at 16 threads."""
+    n = 10000
+    ptr = np.arange(n, dtype=np.uint64)
+    in_cols = np.array([(i - 1) % n for i in range(n)], np.uint32)
+    r, _, _ = oracle.pagerank(np.ones(n, np.uint32), ptr, in_cols, 0.85, 1e-14, 1000, threads=16)
+    assert np.abs(r - 1 / n).max() < 1e-18
+    d, m = 0.85, 5000
+    nn = m + 1
+    od = np.array([m] + [1] * m, np.uint32)
+    ptr = np.array([0] + [m + i for i in range(m)], np.uint64)
+    in_cols = np.array(list(range(1, m + 1)) + [0] * m, np.uint32)
+    r, _, _ = oracle.pagerank(od, ptr, in_cols, d, 1e-16, 10000, threads=16)
+    leaf = (1 + d / m) / (nn * (1 + d))
+    center = (1 - d) / nn + d * m * leaf
+    assert abs(r[0] - center) < 1e-13 and np.abs(r[1:] - leaf).max() < 1e-16
+
+
+def test_sssp_saturates_below_unreached():
+    """distance + 1 is capped at 0xFFFFFFFE (0xFFFFFFFF stays 'unreached'):
+    a chain whose sums cross 2^32, closed forms computed here."""
+    ws = [0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 5, 0xFFFFFFF0]
+    n = len(ws) + 2
+    edges = [(i, i + 1) for i in range(len(ws))]
+    ptr, dst, w = csr_from_edges(n, edges, ws)
+    got = oracle.sssp(ptr, dst, w, 0)
+    dist, acc = [0], 0
+    for x in ws:
+        acc += x
+        dist.append(acc)
+    want = [min(d + 1, 0xFFFFFFFE) for d in dist] + [U]
+    assert got.tolist() == want
+    # the largest distance that is not capped: d + 1 = 0xFFFFFFFD
+    ptr, dst, w = csr_from_edges(3, [(0, 1), (1, 2)], [0xFFFFFFFC, 1])
+    assert oracle.sssp(ptr, dst, w, 0).tolist() == [1, 0xFFFFFFFD, 0xFFFFFFFE]
+
+
+def test_loop_seconds_reported():
+    ptr, cols, _ = csr_from_edges(4, [(0, 1), (1, 2), (2, 3)])
+    oracle.bfs(ptr, cols, 0)
+    t = oracle.loop_seconds()
+    assert 0.0 <= t < 1.0
