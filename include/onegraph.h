@@ -343,6 +343,21 @@ og_status og_msbfs(og_graph *g, const uint32_t *sources, uint32_t num_sources,
  * and hipMemcpy host-to-device of the whole buffer (*memcpy_gbs). */
 og_status og_probe_host_link(uint64_t bytes, double *contig_gbs, double *scattered_gbs, double *memcpy_gbs);
 
+/* Allocation-free BFS (no counterpart in the reference): og_bfs over a CSR
+ * that already sits in device memory, for callers that cannot allocate at
+ * run time (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  row_ptr (V + 1 entries, row_ptr[V] = E), cols (E)
+ * and levels (V, written in the og_bfs convention) are device pointers;
+ * stream is a hipStream_t (NULL: the null stream).  The call allocates
+ * nothing: its flags, bin lists, hub tiles and counters live in one static
+ * device array of about 10.4 MiB per GPU, held until the process ends,
+ * which limits it to V <= 2^21 and E <= 2^26 (OG_EINVAL beyond).  Calls
+ * that overlap on one GPU share that array and race; it synchronises the
+ * stream once per 4 rounds.  stats may be NULL (kernel timings stay 0). */
+og_status og_bfs_device_noalloc(const uint64_t *row_ptr, const uint32_t *cols, uint64_t num_vertices,
+                                uint64_t num_edges, uint32_t source, uint32_t *levels, void *stream,
+                                og_run_stats *stats);
+
 /* Digest of a BFS/SSSP result array: sum mod 2^64 of mix64(v << 32 | x[v])
  * over entries x[v] != OG_UNREACHED (mix64 = SplitMix64 finaliser).  The
  * device computes the same digest into og_run_stats.checksum. */

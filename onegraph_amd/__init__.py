@@ -162,6 +162,8 @@ def lib():
         "og_cc": (C.c_int, [_P, _u32p, C.POINTER(RunStats)]),
         "og_msbfs": (C.c_int, [_P, _u32p, C.c_uint32, _u32p, _u64p, _u64p, C.POINTER(RunStats)]),
         "og_levels_checksum": (C.c_int, [_u32p, C.c_uint64, _u64p]),
+        "og_bfs_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_void_p,
+                                            C.c_void_p, C.POINTER(RunStats)]),
         "og_probe_host_link": (C.c_int, [C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]),
         "og_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
@@ -201,6 +203,16 @@ def device_count():
     n = C.c_int(0)
     _check(lib().og_device_count(C.byref(n)))
     return n.value
+
+
+def bfs_device_noalloc(row_ptr, cols, num_vertices, num_edges, source, levels, stream=None):
+    """og_bfs_device_noalloc on device addresses (ints): row_ptr holds V + 1
+    u64 offsets, cols E u32 ids, levels receives V u32 levels.  Returns the
+    run statistics."""
+    st = RunStats()
+    _check(lib().og_bfs_device_noalloc(C.c_void_p(row_ptr), C.c_void_p(cols), num_vertices, num_edges, source,
+                                       C.c_void_p(levels), C.c_void_p(stream), C.byref(st)))
+    return as_dict(st)
 
 
 def levels_checksum(values):

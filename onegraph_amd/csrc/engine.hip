@@ -2668,3 +2668,178 @@ extern "C" og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, ui
     g->busy.store(0);
     return st;
 }
+
+// ==================================================== allocation-free BFS
+//
+// og_bfs_device_noalloc: og_bfs over a CSR that already sits in device
+// memory, for callers that cannot allocate (hipMalloc / hipFree / an
+// allocating library call) at run time.  Every buffer beyond the caller's
+// inputs and output -- next-frontier flags, bin lists, hub tiles, per-block
+// counts, the round's summary and control block, counters -- lives in the
+// file-scope device array `scratch` below, sized for V <= kNoAllocMaxV and
+// E <= kNoAllocMaxE.  Rounds are enqueued directly (no captured hipGraph, no
+// events) and cleared with fill kernels, so the call allocates nothing.
+// Each GPU holds its own copy of `scratch` until the process ends; calls
+// that overlap on one GPU share it and race.
+
+namespace og {
+
+constexpr uint64_t kNoAllocMaxV = 1ull << 21;  // vertices
+constexpr uint64_t kNoAllocMaxE = 1ull << 26;  // edges
+constexpr uint64_t kNoAlign = 256;
+constexpr uint64_t na_up(uint64_t x) { return (x + kNoAlign - 1) / kNoAlign * kNoAlign; }
+// Hub tiles of any graph within the limits under the default tuning: a vertex
+// of degree d > t2 takes ceil(d / tile) <= d / tile + 1 tiles, and at most
+// E / (t2 + 1) vertices exceed t2.
+constexpr uint64_t kNoAllocHubCap = kNoAllocMaxE / 2048 + kNoAllocMaxE / 4097 + 1;
+constexpr uint64_t kNoAllocMaxBlk = (kNoAllocMaxV + kVertsPerBlock - 1) / kVertsPerBlock;
+constexpr uint64_t kNaFlag = 0;
+constexpr uint64_t kNaList = kNaFlag + na_up(kNoAllocMaxV);
+constexpr uint64_t kNaHub = kNaList + na_up(4 * kNoAllocMaxV);
+constexpr uint64_t kNaDoff = kNaHub + na_up(8 * kNoAllocHubCap);
+constexpr uint64_t kNaBcnt = kNaDoff + na_up(8 * 16);
+constexpr uint64_t kNaBdeg = kNaBcnt + na_up(4 * kBcntRows * kNoAllocMaxBlk);
+constexpr uint64_t kNaInfo = kNaBdeg + na_up(8 * 2 * kNoAllocMaxBlk);
+constexpr uint64_t kNaCtl = kNaInfo + na_up(sizeof(FrontierInfo));
+constexpr uint64_t kNaCounters = kNaCtl + na_up(sizeof(RoundCtl));
+constexpr uint64_t kNoAllocScratchBytes = kNaCounters + na_up(16 * 8);
+
+// The scratch memory of og_bfs_device_noalloc.  In host code `scratch` is a
+// stand-in for the symbol, not a device address: kernels get the address
+// that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_na_mu;
+static void *g_na_ptr[kNoAllocMaxDevices];  // device address of `scratch`, per device
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+static uint8_t *noalloc_scratch(int dev) {
+    if (dev < 0 || dev >= kNoAllocMaxDevices) raise(OG_EUNSUPPORTED, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            raise(OG_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        }
+        g_na_ptr[dev] = p;
+    }
+    return (uint8_t *)g_na_ptr[dev];
+}
+
+static void noalloc_bfs(const uint64_t *d_ptr, const uint32_t *d_cols, uint64_t V, uint64_t E, uint32_t src,
+                        uint32_t *d_levels, hipStream_t s, og_run_stats *stats) {
+    int dev = 0;
+    OG_HIP(hipGetDevice(&dev));
+    uint8_t *base = noalloc_scratch(dev);
+    uint64_t ends[2] = {1, 1};
+    OG_HIP(hipMemcpyAsync(&ends[0], d_ptr, 8, hipMemcpyDeviceToHost, s));
+    OG_HIP(hipMemcpyAsync(&ends[1], d_ptr + V, 8, hipMemcpyDeviceToHost, s));
+    OG_HIP(hipStreamSynchronize(s));
+    if (ends[0] != 0 || ends[1] != E)
+        raise(OG_EINVAL, "row offsets must run from 0 to E=%llu over V+1 entries (got %llu .. %llu)",
+              (unsigned long long)E, (unsigned long long)ends[0], (unsigned long long)ends[1]);
+    const uint64_t Vpad = (V + 15) / 16 * 16;
+    const uint32_t nblk = (uint32_t)((Vpad + kVertsPerBlock - 1) / kVertsPerBlock);
+    uint8_t *flag = base + kNaFlag;
+    uint32_t *list = (uint32_t *)(base + kNaList);
+    uint64_t *hub = (uint64_t *)(base + kNaHub);
+    uint64_t *doff = (uint64_t *)(base + kNaDoff);
+    uint32_t *bcnt = (uint32_t *)(base + kNaBcnt);
+    uint64_t *bdeg = (uint64_t *)(base + kNaBdeg);
+    FrontierInfo *info = (FrontierInfo *)(base + kNaInfo);
+    RoundCtl *ctl = (RoundCtl *)(base + kNaCtl);
+    unsigned long long *ctr = (unsigned long long *)(base + kNaCounters);
+    const Tuning tn;  // default bins: the hub bound above assumes its t2 and tile
+    static_assert(Tuning{}.t2 == 4096 && Tuning{}.tile == 2048, "hub capacity bound follows the default tuning");
+    Stopwatch total;
+    // clear what must start at zero: fill kernels (no memset nodes)
+    hipLaunchKernelGGL(k_fill_u8, dim3(fill_grid(Vpad)), dim3(256), 0, s, flag, Vpad, (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)ctr, (uint64_t)32, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid(V)), dim3(256), 0, s, d_levels, V, OG_UNREACHED);
+    hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d_levels, flag, src);
+    hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, ctl, 0u, 0u, (uint64_t)0, V, 0u);
+    auto compact = [&] {
+        hipLaunchKernelGGL(k_frontier_count, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, V,
+                           d_ptr, (const uint64_t *)nullptr, tn, (const uint32_t *)nullptr, ctl, bcnt, bdeg, nblk);
+        hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, bcnt, bdeg, nblk, info, doff, ctl,
+                           (uint64_t *)nullptr);
+        hipLaunchKernelGGL(k_frontier_write, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, d_ptr,
+                           (const uint64_t *)nullptr, tn, bcnt, bdeg, nblk, info, list, hub, kNoAllocHubCap, doff,
+                           (uint32_t *)nullptr, (uint64_t *)nullptr);
+    };
+    RoundRelax r{};
+    r.list = list;
+    r.info = info;
+    r.ptr = d_ptr;
+    r.sedges = d_cols;
+    r.hub = hub;
+    r.hub_cap = kNoAllocHubCap;
+    r.tile = tn.tile;
+    r.cd = 0;  // the whole graph is static: three bins in one launch
+    r.counters = ctr;
+    const RelaxArgs a{d_levels, flag, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
+    static_assert(Tuning{}.lpv0 == 8, "the relax below is instantiated for 8 lanes per bin-0 vertex");
+    compact();
+    OG_HIP(hipGetLastError());
+    RoundCtl h{};
+    for (;;) {
+        for (int k = 0; k < kRoundBatch; ++k) {
+            hipLaunchKernelGGL((k_relax_round<ALGO_BFS, false, false, 8>), dim3(kFuseStatic * kHubGridCap),
+                               dim3(kBlock), (size_t)tn.tile * 4, s, r, a);
+            compact();
+        }
+        OG_HIP(hipGetLastError());
+        OG_HIP(hipMemcpyAsync(&h, ctl, sizeof h, hipMemcpyDeviceToHost, s));
+        OG_HIP(hipStreamSynchronize(s));
+        if (!h.live) break;
+    }
+    const double ms = total.ms();
+    hipLaunchKernelGGL(k_reach_stats, dim3((unsigned)std::min<uint64_t>(2048, (V + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, d_levels, d_ptr, V, ctr + 2);
+    OG_HIP(hipGetLastError());
+    unsigned long long hc[16];
+    OG_HIP(hipMemcpyAsync(hc, ctr, sizeof hc, hipMemcpyDeviceToHost, s));
+    OG_HIP(hipStreamSynchronize(s));
+    if (stats) {
+        og_run_stats st{};
+        st.approach = OG_ASCETIC;
+        st.iterations = h.rounds;
+        st.node_sum = h.node_sum;
+        st.static_node_sum = h.static_node_sum;
+        st.work_edges = h.work_edges;
+        st.static_work_edges = h.static_work_edges;
+        st.updates = hc[0];
+        st.reached = hc[2];
+        st.traversed_edges = hc[3];
+        st.checksum = hc[4];
+        st.total_ms = ms;
+        *stats = st;
+    }
+}
+
+}  // namespace og
+
+extern "C" og_status og_bfs_device_noalloc(const uint64_t *row_ptr, const uint32_t *cols, uint64_t num_vertices,
+                                           uint64_t num_edges, uint32_t source, uint32_t *levels, void *stream,
+                                           og_run_stats *stats) {
+    clear_error();
+    if (!row_ptr || !levels || (num_edges && !cols)) return fail(OG_EINVAL, "null device pointer");
+    if (num_vertices == 0 || source >= num_vertices)
+        return fail(OG_EINVAL, "source %u out of range for V=%llu", source, (unsigned long long)num_vertices);
+    if (num_vertices > kNoAllocMaxV || num_edges > kNoAllocMaxE)
+        return fail(OG_EINVAL,
+                    "og_bfs_device_noalloc handles V <= %llu and E <= %llu (its scratch is fixed at %llu bytes); "
+                    "got V=%llu E=%llu",
+                    (unsigned long long)kNoAllocMaxV, (unsigned long long)kNoAllocMaxE,
+                    (unsigned long long)kNoAllocScratchBytes, (unsigned long long)num_vertices,
+                    (unsigned long long)num_edges);
+    return guard([&]() -> og_status {
+        if (!hip_available()) raise(OG_EHIP, "no HIP device is visible");
+        noalloc_bfs(row_ptr, cols, num_vertices, num_edges, source, levels, (hipStream_t)stream, stats);
+        return OG_OK;
+    });
+}
