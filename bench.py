@@ -420,8 +420,10 @@ def wl_serve(args):
     service_td = sample_plain("top-down", 0)
     service = sample_plain("diropt", og.PLAN_DIRECTION_OPT)
 
-    # per-query state: BFS masks (24 V per graph) + 4 SSSP lanes of ~17 V per weighted graph (+ scratch)
-    full = sum(g.num_vertices * (24 + 4 * 18) for g in graphs)
+    # per-query state (og_server_create's admission rule): per graph the BFS
+    # engine (40 V bytes), per weighted graph the 16-slot SSSP engine (88 V + 8)
+    full = sum(40 * g.num_vertices + (88 * g.num_vertices + 8 if g.info()["kind"] == og.KIND_WEIGHTED else 0)
+               for g in graphs)
     budget = int(args.serve_budget_gib * (1 << 30)) if args.serve_budget_gib else full
 
     def replay(t_arr, gis, kinds, svc, seed):

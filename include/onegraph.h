@@ -390,9 +390,12 @@ og_status og_pagerank(og_graph *g, const og_pr_params *params, float *ranks_f32,
  * reference answers one query per process, BFS/oneGraph_BFS/main.cpp:3-14).
  * BFS queries are batched continuously: each graph has a BFS engine with 64
  * query slots (one bit of per-vertex masks each); queries that arrive while
- * others run join at the next round and share its kernels.  SSSP queries run
- * on K concurrent lanes per weighted graph, each a single-source engine with
- * the graph's tuning.  Results equal og_bfs / og_sssp on the same graph and
+ * others run join at the next round and share its kernels.  SSSP queries are
+ * batched the same way: each weighted graph has an SSSP engine with 16 query
+ * slots whose distances lie side by side (one 64-byte line per vertex); each
+ * slot runs near-far rounds with its own threshold (bucket width = the
+ * graph's tuning delta) and a round's edge reads serve every slot that is
+ * near at the source.  Results equal og_bfs / og_sssp on the same graph and
  * source. */
 
 #define OG_SERVER_MAX_GRAPHS 8
@@ -403,15 +406,23 @@ og_status og_pagerank(og_graph *g, const og_pr_params *params, float *ranks_f32,
 typedef struct og_server og_server;
 
 typedef struct og_server_params {
-    uint64_t state_budget_bytes;  /* HBM for per-query state: the BFS engine's
-                                     frontier masks (24 V bytes per graph, 64
-                                     slots) plus, per weighted graph, K SSSP
-                                     lanes of about 17 V bytes each (K = the
-                                     largest of 4, 2, 1 that fits);
+    uint64_t state_budget_bytes;  /* HBM for per-query state, all of which
+                                     must fit (else OG_EBUDGET): per graph the
+                                     BFS engine's masks and 4 kept-levels
+                                     buffers (40 V bytes), per weighted graph
+                                     the SSSP engine's 16 distance columns,
+                                     pending masks, active list and degree
+                                     prefix (88 V + 8 bytes);
                                      0 = a quarter of free HBM                 */
     uint32_t max_pending;         /* queued queries before submit returns OG_EBUSY; 0 = 2^20 */
-    uint32_t flags;
+    uint32_t flags;               /* OG_SERVER_* bits                       */
 } og_server_params;
+
+/* Testing only: the server's SSSP engines stop at their first round with
+ * OG_EHIP, as after a device error, without touching the GPU (the failure
+ * path: held queries fail, later SSSP submits return the error, BFS engines
+ * keep serving). */
+#define OG_SERVER_TEST_FAIL_SSSP 0x80000000u
 
 typedef struct og_query_result {
     uint64_t id;

@@ -89,6 +89,7 @@ class PrStats(C.Structure):
 SERVER_MAX_GRAPHS = 8
 QUERY_BFS, QUERY_SSSP = 0, 1
 QUERY_KEEP_VALUES = 1
+SERVER_TEST_FAIL_SSSP = 0x80000000  # testing: SSSP engines stop at their first round (og.h OG_SERVER_TEST_FAIL_SSSP)
 
 
 class ServerParams(C.Structure):
@@ -437,10 +438,10 @@ class Graph:
 class Server:
     """Continuous-batching query server over device-resident graphs."""
 
-    def __init__(self, graphs, state_budget_bytes=0, max_pending=0):
+    def __init__(self, graphs, state_budget_bytes=0, max_pending=0, flags=0):
         self._graphs = list(graphs)
         arr = (_P * len(self._graphs))(*[g._h.value for g in self._graphs])
-        p = ServerParams(state_budget_bytes, max_pending, 0)
+        p = ServerParams(state_budget_bytes, max_pending, flags)
         h = _P()
         _check(lib().og_server_create(arr, len(self._graphs), C.byref(p), C.byref(h)))
         self._h = h

@@ -489,12 +489,6 @@ void frontier_begin(Frontier &f, hipStream_t s, uint32_t delta, uint32_t defer =
 void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t *val = nullptr,
                       bool to_host = true, const uint64_t *pmid = nullptr);
 
-// Per-run state of a concurrent single-source lane: shares the device arrays
-// of g->dev (the whole graph must be on the device) and owns values,
-// frontier, streams and captured rounds.  Freed with device_state_free.
-DeviceState *lane_state_create(og_graph *g);
-// og_bfs (ALGO_BFS) / og_sssp (ALGO_SSSP) from src on a lane; out may be null.
-void lane_run(og_graph *g, DeviceState *lane, int algo, uint32_t src, uint32_t *out, og_run_stats *st);
 
 // Launches the static-region relax of the bins for op ALGO.  The kernels
 // read their counts from f.info; `hfi` (the host copy of it, or null when the

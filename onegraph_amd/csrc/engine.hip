@@ -2296,54 +2296,6 @@ static og_status run_algo(og_graph *g, int algo, uint32_t src, uint32_t *out, og
     return st;
 }
 
-// Per-run state of a concurrent single-source lane over the prepared graph:
-// it shares the graph's device arrays (row pointers, edges, in-edges) and
-// owns values, frontier buffers, streams, events and captured rounds, so
-// lanes iterate independently on their own streams.
-DeviceState *lane_state_create(og_graph *g) {
-    const DeviceState *d = g->dev;
-    if (d->plan.vsplit < d->V) raise(OG_EUNSUPPORTED, "lanes need the whole graph on the device");
-    std::unique_ptr<DeviceState, void (*)(DeviceState *)> l(new DeviceState, device_state_free);
-    l->approach = d->approach;
-    l->device = d->device;
-    l->V = d->V;
-    l->E = d->E;
-    l->Vpad = d->Vpad;
-    l->eb = d->eb;
-    l->deb = d->deb;
-    l->plan = d->plan;
-    l->tune = d->tune;
-    l->host_threads = d->host_threads;
-    l->flags = d->flags;
-    l->d_ptr = d->d_ptr;
-    l->d_static = d->d_static;
-    l->static_used = d->static_used;
-    l->managed = d->managed;
-    l->packed = d->packed;
-    l->wbits = d->wbits;
-    l->d_in_ptr = d->d_in_ptr;
-    l->d_in_cols = d->d_in_cols;
-    l->d_pmid = d->d_pmid;
-    l->split_weight = d->split_weight;
-    OG_HIP(hipSetDevice(d->device));
-    OG_HIP(hipStreamCreateWithFlags(&l->s_compute, hipStreamNonBlocking));
-    OG_HIP(hipStreamCreateWithFlags(&l->s_copy, hipStreamNonBlocking));
-    OG_HIP(hipStreamCreateWithFlags(&l->s_static, hipStreamNonBlocking));
-    OG_HIP(hipEventCreateWithFlags(&l->ev_static, hipEventDisableTiming));
-    OG_HIP(hipEventCreateWithFlags(&l->ev_fork, hipEventDisableTiming));
-    l->d_val = (uint32_t *)l->dalloc(l->V * sizeof(uint32_t));
-    frontier_alloc(l.get(), g, l->fr, l->mem);
-    OG_HIP(hipDeviceSynchronize());
-    return l.release();
-}
-
-void lane_run(og_graph *g, DeviceState *lane, int algo, uint32_t src, uint32_t *out, og_run_stats *st) {
-    const bool w = g->kind == OG_KIND_WEIGHTED;
-    if (algo == ALGO_SSSP) run_frontier<ALGO_SSSP, true>(g, src, out, st, lane);
-    else if (w) run_frontier<ALGO_BFS, true>(g, src, out, st, lane);
-    else run_frontier<ALGO_BFS, false>(g, src, out, st, lane);
-}
-
 }  // namespace og
 
 using namespace og;

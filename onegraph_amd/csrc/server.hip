@@ -25,12 +25,12 @@
 // Queries that arrive while a round runs join at the next round boundary, so
 // their work overlaps with the queries already in flight.
 //
-// SSSP: every weighted graph has K lanes, each a complete single-source
-// near-far engine (values, frontier, streams, captured rounds) over the
-// shared device-resident graph, taking queries from the graph's SSSP queue
-// one at a time.  Distances do not share across queries the way BFS masks
-// do (a 64-slot distance matrix is 256 bytes per vertex and defeats the
-// caches), so SSSP queries overlap as concurrent lanes instead.
+// SSSP: every weighted graph has one batch engine with kSs = 16 query slots
+// (see "batched SSSP" below): each vertex keeps the 16 slots' distances in
+// one 64-byte line, every slot runs near-far rounds with its own threshold,
+// and the slots near at a vertex share the reads of its out-edges.  Queries
+// join at round boundaries like BFS queries.  (A 64-slot distance matrix,
+// 256 bytes per vertex, was tried first and lost to the caches.)
 #include <algorithm>
 #include <condition_variable>
 #include <deque>
@@ -215,6 +215,380 @@ __global__ void k_srv_zero_acc(uint64_t done, unsigned long long *acc) {
     if (j < 64 && ((done >> j) & 1)) acc[j] = acc[64 + j] = acc[128 + j] = 0;
 }
 
+// ---------------------------------------------------- batched SSSP (server)
+//
+// Every weighted graph has one SSSP engine with kSs query slots.  Vertex v
+// keeps the kSs slots' distances side by side in one 64-byte line
+// (dist[v * kSs + j], distance + 1 as og_sssp stores it), and a pending mask
+// pend[v] (bit j: slot j improved v and has not relaxed it since).  Each
+// slot runs near-far SSSP with its own threshold thresh[j]: a round relaxes
+// the pending (v, j) with dist < thresh[j]; when a slot has nothing near
+// left but far entries, its threshold moves to (smallest far value + delta)
+// for the next round.  The slots of a round share the edge reads: an active
+// vertex carries the mask of its near slots, and one pass over its out-edges
+// offers every near slot's distance at once.  Rounds:
+//   select  one sweep over pend: per pending vertex, the near mask from its
+//           distance line; per-block counts, masks and per-slot far minima;
+//   scan    one workgroup: list offsets, the slots' near / far / done masks
+//           and next thresholds, the summary the host reads;
+//   write   the active list (vertex, near mask) and its degree prefix;
+//   relax   edge-parallel over the active list: four lanes per edge, lane l
+//           covering slots 4l..4l+3 (one 16-byte piece of the target's
+//           line), atomicMin per improved slot, one atomicOr of the improved
+//           slots into the target's pending mask.
+// Results need no extra pass: every successful atomicMin of (v, j) from old
+// to new adds mix64(v << 32 | new) - mix64(v << 32 | old) (old = unreached
+// adds nothing and counts v as reached, with its out-degree as traversed) to
+// slot j's sums, so the sums telescope to og_levels_checksum of the final
+// distances.  A finished slot's column is reset to "unreached" by a later
+// sweep shared by many slots, before the slot takes another query.
+
+constexpr uint32_t kSs = 16;                  // SSSP slots per graph (distances per 64-byte vertex line)
+constexpr uint32_t kSsWin = 8192;             // vertices per workgroup of the select / write sweeps
+constexpr uint32_t kSsPer = kSsWin / kBlock;  // per thread: vertices base + t + kBlock * k, k < kSsPer
+constexpr uint32_t kSsGroupEdges = 4;         // edges per 4-lane group per chunk
+constexpr uint32_t kSsChunk = (kBlock / 4) * kSsGroupEdges;  // active edges per relax workgroup iteration
+constexpr unsigned kSsRelaxGrid = 4096;       // relax workgroups (they stride over the round's edges)
+
+struct SsCtl {
+    uint32_t thresh[kSs];  // near-far threshold of each slot (0xFFFFFFFF: every pending entry is near)
+    uint32_t delta;        // bucket width (0: no near-far order)
+    uint32_t nactive;      // vertices in this round's active list
+    uint64_t dsum;         // their degree sum (the relax's edge space)
+    uint32_t near, far;    // slots with near / far pending entries after the select sweep
+    uint32_t done;         // slots in use with nothing pending: finished
+    uint32_t rounds;       // select sweeps so far
+};
+
+struct SsSeeds {
+    uint32_t n;
+    uint32_t slot[kSs];
+    uint32_t src[kSs];
+};
+
+// New queries: slot j starts at its source (value 1, pending), its sums at
+// the source's own terms, its threshold at 1 + delta.
+__global__ void k_ss_seed(SsSeeds sl, uint32_t *dist, uint32_t *pend, const uint64_t *__restrict__ ptr,
+                          unsigned long long *acc, SsCtl *ctl) {
+    const uint32_t i = threadIdx.x;
+    if (i >= sl.n) return;
+    const uint32_t j = sl.slot[i], v = sl.src[i];
+    dist[(uint64_t)v * kSs + j] = 1;
+    atomicOr(pend + v, 1u << j);
+    acc[j] = srv_mix64(((uint64_t)v << 32) | 1u);
+    acc[kSs + j] = 1;
+    acc[2 * kSs + j] = ptr[v + 1] - ptr[v];
+    const uint32_t dl = ctl->delta;
+    ctl->thresh[j] = dl ? (uint32_t)min((uint64_t)0xFFFFFFFFull, 1ull + dl) : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint32_t quad_at(const uint4 &x, int b) {
+    return b == 0 ? x.x : b == 1 ? x.y : b == 2 ? x.z : x.w;
+}
+
+// Select sweep: near mask of every pending vertex, pend[v] = near << 16 |
+// far (near listed only for vertices with out-edges); per block the count and degree sum of vertices with a near slot, the
+// OR of near and far masks, and per slot the smallest far value (LDS
+// atomicMin, tried only when it would lower the block's minimum).
+__global__ __launch_bounds__(kBlock) void k_ss_select(uint32_t *__restrict__ pend, const uint32_t *__restrict__ dist,
+                                                      const uint64_t *__restrict__ ptr, uint64_t V,
+                                                      const SsCtl *__restrict__ ctl, uint32_t *__restrict__ bnv,
+                                                      uint64_t *__restrict__ bdeg, uint32_t *__restrict__ bmask,
+                                                      uint32_t *__restrict__ bmin, uint32_t nblk) {
+    __shared__ uint32_t s_thr[kSs], s_min[kSs];
+    __shared__ uint64_t lds[2 * kBlock / kWave];
+    __shared__ uint32_t s_or[2];
+    if (threadIdx.x < kSs) {
+        s_thr[threadIdx.x] = ctl->thresh[threadIdx.x];
+        s_min[threadIdx.x] = 0xFFFFFFFFu;
+    }
+    if (threadIdx.x < 2) s_or[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kSsWin + threadIdx.x;
+    uint32_t cnt = 0, nm = 0, fm = 0;
+    uint64_t deg = 0;
+    for (uint32_t k = 0; k < kSsPer; ++k) {
+        const uint64_t v = base + (uint64_t)k * kBlock;
+        if (v >= V) break;
+        const uint32_t p = pend[v] & 0xFFFFu;
+        if (!p) continue;
+        uint32_t near = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t qb = (p >> (4 * q)) & 0xFu;
+            if (!qb) continue;
+            const uint4 x = *(const uint4 *)(dist + v * kSs + 4 * q);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                if (!((qb >> b) & 1u)) continue;
+                const int j = 4 * q + b;
+                const uint32_t d = quad_at(x, b);
+                if (d < s_thr[j]) near |= 1u << j;
+                else if (d < s_min[j]) atomicMin(&s_min[j], d);
+            }
+        }
+        const uint32_t far = p & ~near;
+        // a near vertex without out-edges has nothing to relax: its near
+        // entries are consumed here and it is not listed (so every listed
+        // vertex owns at least one edge of the relax's edge space)
+        const uint64_t dv = near ? ptr[v + 1] - ptr[v] : 0;
+        pend[v] = (dv ? near << 16 : 0u) | far;
+        nm |= near;
+        fm |= far;
+        if (dv) {
+            ++cnt;
+            deg += dv;
+        }
+    }
+    const uint64_t x[2] = {cnt, deg};
+    uint64_t ex[2], t[2];
+    block_scan_n<2>(x, ex, t, lds);
+    nm = (uint32_t)wave_or(nm);
+    fm = (uint32_t)wave_or(fm);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (nm) atomicOr(&s_or[0], nm);
+        if (fm) atomicOr(&s_or[1], fm);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bnv[blockIdx.x] = (uint32_t)t[0];
+        bdeg[blockIdx.x] = t[1];
+        bmask[blockIdx.x] = s_or[0];
+        bmask[nblk + blockIdx.x] = s_or[1];
+    }
+    if (threadIdx.x < kSs) bmin[(uint64_t)threadIdx.x * nblk + blockIdx.x] = s_min[threadIdx.x];
+}
+
+// Scan: exclusive offsets of the blocks' list counts and degree sums (in
+// place), the slots' near / far masks, the threshold moves of slots with far
+// entries only, the done mask of the slots in `used`, and the list's total
+// degree at doff[nactive].
+__global__ __launch_bounds__(1024) void k_ss_scan(uint32_t *__restrict__ bnv, uint64_t *__restrict__ bdeg,
+                                                  const uint32_t *__restrict__ bmask, const uint32_t *__restrict__ bmin,
+                                                  uint32_t nblk, uint32_t used, SsCtl *__restrict__ ctl,
+                                                  uint64_t *__restrict__ doff) {
+    __shared__ uint64_t lds[2 * 1024 / kWave];
+    __shared__ uint32_t s_or[2], s_min[kSs];
+    if (threadIdx.x < 2) s_or[threadIdx.x] = 0;
+    if (threadIdx.x < kSs) s_min[threadIdx.x] = 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = min(nblk, threadIdx.x * per), b1 = min(nblk, b0 + per);
+    uint64_t c = 0, dg = 0;
+    uint32_t nm = 0, fm = 0;
+    for (uint32_t b = b0; b < b1; ++b) {
+        c += bnv[b];
+        dg += bdeg[b];
+        nm |= bmask[b];
+        fm |= bmask[nblk + b];
+    }
+    const uint64_t x[2] = {c, dg};
+    uint64_t ex[2], t[2];
+    block_scan_n<2>(x, ex, t, lds);
+    uint64_t rc = ex[0], rd = ex[1];
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t a = bnv[b];
+        const uint64_t e = bdeg[b];
+        bnv[b] = (uint32_t)rc;
+        bdeg[b] = rd;
+        rc += a;
+        rd += e;
+    }
+    nm = (uint32_t)wave_or(nm);
+    fm = (uint32_t)wave_or(fm);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (nm) atomicOr(&s_or[0], nm);
+        if (fm) atomicOr(&s_or[1], fm);
+    }
+    __syncthreads();
+    const uint32_t near = s_or[0], far = s_or[1];
+    const uint32_t moving = far & ~near;  // slots whose threshold moves for the next round
+    for (uint32_t m = moving; m;) {
+        const int j = __builtin_ctz(m);
+        m &= m - 1;
+        uint32_t mn = 0xFFFFFFFFu;
+        for (uint32_t b = b0; b < b1; ++b) mn = min(mn, bmin[(uint64_t)j * nblk + b]);
+        if (mn != 0xFFFFFFFFu) atomicMin(&s_min[j], mn);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        SsCtl cl = *ctl;
+        cl.nactive = (uint32_t)t[0];
+        cl.dsum = t[1];
+        cl.near = near;
+        cl.far = far;
+        cl.done = used & ~(near | far);
+        cl.rounds++;
+        for (uint32_t m = moving; m;) {
+            const int j = __builtin_ctz(m);
+            m &= m - 1;
+            cl.thresh[j] = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)s_min[j] + cl.delta);
+        }
+        *ctl = cl;
+        doff[t[0]] = t[1];
+    }
+}
+
+// Write sweep: the active list in block order (vertex, near mask), its
+// degree prefix, and pend[v] back to the far slots only.
+__global__ __launch_bounds__(kBlock) void k_ss_write(uint32_t *__restrict__ pend, const uint64_t *__restrict__ ptr,
+                                                     uint64_t V, const uint32_t *__restrict__ bnv,
+                                                     const uint64_t *__restrict__ bdeg, uint32_t *__restrict__ alist,
+                                                     uint32_t *__restrict__ amask, uint64_t *__restrict__ doff) {
+    __shared__ uint64_t lds[2 * kBlock / kWave];
+    const uint64_t base = (uint64_t)blockIdx.x * kSsWin + threadIdx.x;
+    uint32_t cnt = 0;
+    uint64_t deg = 0;
+    for (uint32_t k = 0; k < kSsPer; ++k) {
+        const uint64_t v = base + (uint64_t)k * kBlock;
+        if (v >= V) break;
+        if (pend[v] >> 16) {
+            ++cnt;
+            deg += ptr[v + 1] - ptr[v];
+        }
+    }
+    const uint64_t x[2] = {cnt, deg};
+    uint64_t ex[2], t[2];
+    block_scan_n<2>(x, ex, t, lds);
+    if (!t[0]) return;
+    uint64_t pos = bnv[blockIdx.x] + ex[0], dpos = bdeg[blockIdx.x] + ex[1];
+    for (uint32_t k = 0; k < kSsPer && cnt; ++k) {
+        const uint64_t v = base + (uint64_t)k * kBlock;
+        if (v >= V) break;
+        const uint32_t p = pend[v];
+        if (!(p >> 16)) continue;
+        alist[pos] = (uint32_t)v;
+        amask[pos] = p >> 16;
+        doff[pos] = dpos;
+        ++pos;
+        dpos += ptr[v + 1] - ptr[v];
+        pend[v] = p & 0xFFFFu;
+        --cnt;
+    }
+}
+
+// Relax: workgroups stride over the round's active edge space in chunks of
+// kSsChunk; a chunk's slice of the degree prefix is staged in LDS and each
+// edge finds its source by binary search there.  Four lanes share an edge;
+// lane l reads the 16-byte pieces of the source's and the target's lines
+// holding slots 4l..4l+3 (when u is near for any of them) and offers them.
+template <bool W, bool PK>
+__global__ __launch_bounds__(kBlock) void k_ss_relax(const uint32_t *__restrict__ alist,
+                                                     const uint32_t *__restrict__ amask,
+                                                     const uint64_t *__restrict__ doff, const SsCtl *__restrict__ ctl,
+                                                     const uint64_t *__restrict__ ptr, const void *__restrict__ edges,
+                                                     uint32_t wbits, uint32_t *dist, uint32_t *pend,
+                                                     unsigned long long *acc) {
+    __shared__ uint64_t s_off[kSsChunk + 2];
+    __shared__ uint32_t s_i[2];
+    __shared__ unsigned long long s_acc[3 * kSs];
+    if (threadIdx.x < 3 * kSs) s_acc[threadIdx.x] = 0;
+    const uint32_t n = ctl->nactive;
+    const uint64_t D = ctl->dsum;
+    const uint32_t g = threadIdx.x >> 2, l = threadIdx.x & 3u;
+    for (uint64_t q0 = (uint64_t)blockIdx.x * kSsChunk; q0 < D; q0 += (uint64_t)gridDim.x * kSsChunk) {
+        const uint64_t q1 = min(q0 + kSsChunk, D);
+        __syncthreads();  // s_off / s_i of the previous chunk are no longer read
+        if (threadIdx.x < 2) {
+            const uint64_t q = threadIdx.x == 0 ? q0 : q1 - 1;
+            uint32_t lo = 0, hi = n;  // largest i with doff[i] <= q; doff[0] = 0
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (doff[mid] <= q) lo = mid; else hi = mid;
+            }
+            s_i[threadIdx.x] = lo;
+        }
+        __syncthreads();
+        const uint32_t i0 = s_i[0];
+        const uint32_t nv = s_i[1] - i0 + 1;
+        for (uint32_t k = threadIdx.x; k <= nv; k += kBlock) s_off[k] = doff[i0 + k];
+        __syncthreads();
+        uint32_t to[kSsGroupEdges], wt[kSsGroupEdges], sub[kSsGroupEdges];
+        uint4 du[kSsGroupEdges], dt[kSsGroupEdges];
+#pragma unroll
+        for (uint32_t e = 0; e < kSsGroupEdges; ++e) {
+            const uint64_t q = q0 + g + (uint64_t)e * (kBlock / 4);
+            to[e] = 0;
+            wt[e] = 0;
+            sub[e] = 0;
+            if (q < q1) {
+                uint32_t lo = 0, hi = nv;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= q) lo = mid; else hi = mid;
+                }
+                const uint32_t u = alist[i0 + lo];
+                sub[e] = (amask[i0 + lo] >> (4 * l)) & 0xFu;
+                load_edge<W, PK>(edges, ptr[u] + (q - s_off[lo]), wbits, to[e], wt[e]);
+                if (sub[e]) du[e] = *(const uint4 *)(dist + (uint64_t)u * kSs + 4 * l);
+            }
+        }
+#pragma unroll
+        for (uint32_t e = 0; e < kSsGroupEdges; ++e)
+            if (sub[e]) dt[e] = *(const uint4 *)(dist + (uint64_t)to[e] * kSs + 4 * l);
+#pragma unroll
+        for (uint32_t e = 0; e < kSsGroupEdges; ++e) {
+            uint32_t imp = 0;
+            if (sub[e]) {
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    if (!((sub[e] >> b) & 1u)) continue;
+                    const uint64_t s64 = (uint64_t)quad_at(du[e], b) + wt[e];
+                    const uint32_t nd = s64 > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)s64;
+                    if (nd >= quad_at(dt[e], b)) continue;
+                    const uint32_t j = 4 * l + b;
+                    const uint32_t old = atomicMin(dist + (uint64_t)to[e] * kSs + j, nd);
+                    if (nd >= old) continue;
+                    imp |= 1u << j;
+                    const uint64_t key = (uint64_t)to[e] << 32;
+                    unsigned long long dd = srv_mix64(key | nd);
+                    if (old == OG_UNREACHED) {
+                        atomicAdd(&s_acc[kSs + j], 1ull);
+                        atomicAdd(&s_acc[2 * kSs + j], (unsigned long long)(ptr[to[e] + 1] - ptr[to[e]]));
+                    } else {
+                        dd -= srv_mix64(key | old);
+                    }
+                    atomicAdd(&s_acc[j], dd);
+                }
+            }
+            imp |= __shfl_xor(imp, 1, kWave);
+            imp |= __shfl_xor(imp, 2, kWave);
+            if (l == 0 && imp) atomicOr(pend + to[e], imp);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * kSs && s_acc[threadIdx.x]) atomicAdd(acc + threadIdx.x, s_acc[threadIdx.x]);
+}
+
+// Resets the distance columns of the slots in `dirty` to "unreached".
+__global__ __launch_bounds__(kBlock) void k_ss_clear(uint32_t *dist, uint64_t V, uint32_t dirty) {
+    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t qb = (dirty >> (4 * q)) & 0xFu;
+            if (!qb) continue;
+            uint4 *p = (uint4 *)(dist + v * kSs + 4 * q);
+            uint4 x = qb == 0xFu ? make_uint4(0, 0, 0, 0) : *p;
+            if (qb & 1u) x.x = OG_UNREACHED;
+            if (qb & 2u) x.y = OG_UNREACHED;
+            if (qb & 4u) x.z = OG_UNREACHED;
+            if (qb & 8u) x.w = OG_UNREACHED;
+            *p = x;
+        }
+    }
+}
+
+// Column j of the distance lines (a finished query's result array).
+__global__ void k_ss_extract(const uint32_t *__restrict__ dist, uint64_t V, uint32_t j, uint32_t *out) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+        out[v] = dist[v * kSs + j];
+}
+
+__global__ void k_ss_zero_acc(uint32_t done, unsigned long long *acc) {
+    const uint32_t j = threadIdx.x;
+    if (j < kSs && ((done >> j) & 1u)) acc[j] = acc[kSs + j] = acc[2 * kSs + j] = 0;
+}
+
 // ------------------------------------------------------------------ host
 
 struct Query {
@@ -228,14 +602,24 @@ struct SlotState {
     Query q{};
     uint32_t start_round = 0;
     double t_admit = 0;
+    int keep = -1;  // BFS: index of the kept-levels buffer the commits write, or -1
 };
 
 constexpr uint32_t kSlots = 64;     // BFS query slots per graph (bits of a u64 mask)
-constexpr uint32_t kMaxLanes = 4;   // SSSP lanes per weighted graph
+constexpr uint32_t kKeepBufs = 4;   // BFS: kept-levels buffers per graph (4 V bytes each)
 
-// Device bytes of one SSSP lane: values 4 V, flags V, list 4 V, degree
-// prefix 8 V (plus per-block scratch and the hub list).
-static uint64_t lane_bytes(uint64_t V) { return 17 * V; }
+// Device bytes of per-query state the admission rule charges: the BFS
+// engine's masks and kept-levels buffers, and the SSSP engine's distance
+// lines (4 kSs V), pending masks, active list with masks and degree prefix,
+// and the extraction buffer of kept results.
+static uint64_t bfs_state_bytes(uint64_t V) { return 24 * V + kKeepBufs * 4 * V; }
+static uint64_t sssp_state_bytes(uint64_t V) { return 4ull * kSs * V + 4 * V + 8 * V + 8 * (V + 1) + 4 * V; }
+
+// What an engine thread marks when it fails: its queue stops taking queries.
+struct EngineFailure {
+    og_status st = OG_OK;
+    std::string msg;
+};
 
 struct BfsEngine {
     og_server *srv = nullptr;
@@ -243,6 +627,7 @@ struct BfsEngine {
     DeviceState *d = nullptr;
     uint32_t gi = 0;
     std::deque<Query> *pending = nullptr;
+    EngineFailure *failure = nullptr;
     MemPool mem;
     Frontier f;
     hipStream_t s = nullptr;
@@ -253,13 +638,14 @@ struct BfsEngine {
     unsigned long long *d_fstat = nullptr;     // [2][64] frontier vertices / edges per slot
     unsigned long long *d_explored = nullptr;  // [64] edges of the slot's past frontiers
     unsigned long long *d_pull = nullptr;      // slots relaxed bottom-up this round
-    uint64_t pull_rounds = 0;
-    uint32_t *keep_buf[64] = {};
+    std::atomic<uint64_t> pull_rounds{0};
+    uint32_t *keep_buf[kKeepBufs] = {};        // allocated with the engine, charged to the state budget
+    uint32_t keep_used = 0;                    // bit k: keep_buf[k] belongs to a slot in flight
     SlotState slots[64];
     uint64_t used = 0;
     uint64_t dirty = 0;  // finished slots whose visit bits are not cleared yet
     uint32_t round = 0;
-    uint64_t rounds_run = 0;
+    std::atomic<uint64_t> rounds_run{0};
     std::thread th;
     ~BfsEngine() {
         if (s) {
@@ -271,28 +657,50 @@ struct BfsEngine {
     void loop();
     void step();
     void finalize(uint64_t done);
-    void fail_held(og_status st);
+    void fail_held(og_status st, const std::string &msg);
     KeepList keep_list(uint64_t mask) const {
         KeepList kl{};
         for (uint32_t j = 0; j < kSlots; ++j)
-            if (((mask >> j) & 1) && (slots[j].q.flags & OG_QUERY_KEEP_VALUES)) kl.p[j] = keep_buf[j];
+            if (((mask >> j) & 1) && slots[j].keep >= 0) kl.p[j] = keep_buf[slots[j].keep];
         return kl;
     }
 };
 
-struct SsspLane {
+// Batched SSSP: kSs query slots over the graph's device-resident edges.
+struct SsspEngine {
     og_server *srv = nullptr;
     og_graph *g = nullptr;
-    DeviceState *lane = nullptr;
+    DeviceState *d = nullptr;
     uint32_t gi = 0;
     std::deque<Query> *pending = nullptr;
-    uint64_t runs = 0, rounds_run = 0;
-    Query cur{};
+    EngineFailure *failure = nullptr;
+    MemPool mem;
+    hipStream_t s = nullptr;
+    hipEvent_t ev_summary = nullptr;
+    uint32_t *dist = nullptr, *pend = nullptr, *alist = nullptr, *amask = nullptr, *extract = nullptr;
+    uint64_t *doff = nullptr;
+    uint32_t *bnv = nullptr, *bmask = nullptr, *bmin = nullptr;
+    uint64_t *bdeg = nullptr;
+    uint32_t nblk = 0;
+    SsCtl *ctl = nullptr, *h_ctl = nullptr;
+    unsigned long long *acc = nullptr, *h_acc = nullptr;
+    SlotState slots[kSs];
+    uint32_t used = 0, dirty = 0;
+    uint32_t round = 0;
+    std::atomic<uint64_t> rounds_run{0};
     std::thread th;
-    ~SsspLane() {
-        if (lane) device_state_free(lane);
+    ~SsspEngine() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+        if (ev_summary) (void)hipEventDestroy(ev_summary);
     }
+    void setup();
     void loop();
+    void step();
+    void finalize(uint32_t done);
+    void fail_held(og_status st, const std::string &msg);
 };
 
 }  // namespace og
@@ -300,8 +708,9 @@ struct SsspLane {
 struct og_server {
     std::vector<og_graph *> graphs;
     std::vector<std::deque<og::Query>> queues;          // [graph * 2 + kind]
+    std::vector<og::EngineFailure> failures;            // [graph * 2 + kind]
     std::vector<std::unique_ptr<og::BfsEngine>> bfs;    // [graph]
-    std::vector<std::vector<std::unique_ptr<og::SsspLane>>> lanes;  // [graph], empty when unweighted
+    std::vector<std::unique_ptr<og::SsspEngine>> sssp;  // [graph], null when unweighted
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     bool stop = false;
@@ -332,6 +741,29 @@ static void push_failure(og_server *srv, const Query &q, uint32_t gi, og_status 
     r.latency_ms = t - q.t_submit;
     srv->done.push_back(r);
     srv->completed++;
+}
+
+// Fails the queries an engine holds (in slots and queued) and marks its queue
+// failed, so that later submits for it return the error instead of waiting.
+// Caller holds srv->mu.
+template <class Slots>
+static void fail_engine(og_server *srv, uint32_t gi, std::deque<Query> *pending, EngineFailure *failure,
+                        Slots &slots, uint32_t nslots, og_status st, const std::string &msg) {
+    const double t = now_ms(srv);
+    failure->st = st == OG_OK ? OG_EHIP : st;
+    failure->msg = msg;
+    for (uint32_t j = 0; j < nslots; ++j)
+        if (slots[j].used) {
+            push_failure(srv, slots[j].q, gi, failure->st, t);
+            slots[j].used = false;
+            srv->in_flight--;
+        }
+    for (auto &q : *pending) {
+        push_failure(srv, q, gi, failure->st, t);
+        srv->pending--;
+    }
+    pending->clear();
+    srv->cv_done.notify_all();
 }
 
 // Results of the slots in `done` come from the accumulators copied at the
@@ -369,6 +801,8 @@ void BfsEngine::finalize(uint64_t done) {
         r.queue_ms = slots[j].t_admit - slots[j].q.t_submit;
         r.latency_ms = t - slots[j].q.t_submit;
         srv->done.push_back(r);
+        if (slots[j].keep >= 0) keep_used &= ~(1u << slots[j].keep);
+        slots[j].keep = -1;
         slots[j].used = false;
         srv->completed++;
         srv->in_flight--;
@@ -393,12 +827,21 @@ void BfsEngine::step() {
             dirty = 0;  // stream order: the sweep runs before this round's seeds and commit
         }
         while (!pending->empty() && (uint32_t)__builtin_popcountll(used | dirty) < kSlots) {
+            // a query keeping its levels needs one of the kKeepBufs buffers;
+            // admission stays in arrival order, so it waits for one
+            int kb = -1;
+            if (pending->front().flags & OG_QUERY_KEEP_VALUES) {
+                if (keep_used == (1u << kKeepBufs) - 1) break;
+                kb = __builtin_ctz(~keep_used);
+                keep_used |= 1u << kb;
+            }
             uint32_t j = 0;
             while (((used | dirty) >> j) & 1) ++j;
             slots[j].used = true;
             slots[j].q = pending->front();
             slots[j].start_round = round;
             slots[j].t_admit = t;
+            slots[j].keep = kb;
             pending->pop_front();
             used |= 1ull << j;
             sl.slot[sl.n] = j;
@@ -410,11 +853,8 @@ void BfsEngine::step() {
     }
     const uint64_t V = d->V;
     for (uint32_t i = 0; i < sl.n; ++i) {  // kept levels are written by the commits
-        const uint32_t j = sl.slot[i];
-        if (slots[j].q.flags & OG_QUERY_KEEP_VALUES) {
-            if (!keep_buf[j]) keep_buf[j] = (uint32_t *)mem.dalloc(V * 4);
-            OG_HIP(hipMemsetAsync(keep_buf[j], 0xFF, V * 4, s));
-        }
+        const int kb = slots[sl.slot[i]].keep;
+        if (kb >= 0) OG_HIP(hipMemsetAsync(keep_buf[kb], 0xFF, V * 4, s));
     }
     if (sl.n) {
         hipLaunchKernelGGL(k_srv_seed, dim3(1), dim3(64), 0, s, sl, next, f.flag, d_explored);
@@ -447,29 +887,18 @@ void BfsEngine::step() {
     // commit see everything this round offered).
     OG_HIP(hipEventSynchronize(ev_summary));
     ++round;
-    ++rounds_run;
-    pull_rounds += h_alive[1] ? 1 : 0;
+    rounds_run.fetch_add(1, std::memory_order_relaxed);
+    if (h_alive[1]) pull_rounds.fetch_add(1, std::memory_order_relaxed);
     // a slot is done when the round's commit left it no frontier vertex
     const uint64_t doneset = used & ~(uint64_t)*h_alive;
     if (doneset) finalize(doneset);
 }
 
-void BfsEngine::fail_held(og_status st) {
+void BfsEngine::fail_held(og_status st, const std::string &msg) {
     std::lock_guard<std::mutex> lk(srv->mu);
-    const double t = now_ms(srv);
-    for (uint32_t j = 0; j < kSlots; ++j)
-        if (slots[j].used) {
-            push_failure(srv, slots[j].q, gi, st, t);
-            slots[j].used = false;
-            srv->in_flight--;
-        }
-    for (auto &q : *pending) {
-        push_failure(srv, q, gi, st, t);
-        srv->pending--;
-    }
-    pending->clear();
+    fail_engine(srv, gi, pending, failure, slots, kSlots, st, msg);
     used = 0;
-    srv->cv_done.notify_all();
+    keep_used = 0;
 }
 
 void BfsEngine::loop() {
@@ -483,62 +912,178 @@ void BfsEngine::loop() {
             step();
         }
     } catch (const Error &e) {
-        // fail everything this engine holds; the server keeps serving others
-        fail_held(e.st);
+        // this graph's BFS queue fails (held and later queries get the error);
+        // the other graphs' engines keep serving
+        fail_held(e.st, e.msg);
+    } catch (...) {
+        fail_held(OG_ENOMEM_HOST, "BFS engine stopped: host allocation failed");
     }
 }
 
-void SsspLane::loop() {
-    try {
-        OG_HIP(hipSetDevice(lane->device));
-    } catch (const Error &) {
-        return;
+void SsspEngine::setup() {
+    const uint64_t V = d->V;
+    nblk = (uint32_t)((V + kSsWin - 1) / kSsWin);
+    OG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    OG_HIP(hipEventCreateWithFlags(&ev_summary, hipEventDisableTiming));
+    dist = (uint32_t *)mem.dalloc(V * kSs * 4);
+    pend = (uint32_t *)mem.dalloc(V * 4);
+    alist = (uint32_t *)mem.dalloc(V * 4);
+    amask = (uint32_t *)mem.dalloc(V * 4);
+    doff = (uint64_t *)mem.dalloc((V + 1) * 8);
+    extract = (uint32_t *)mem.dalloc(V * 4);
+    bnv = (uint32_t *)mem.dalloc((uint64_t)nblk * 4);
+    bdeg = (uint64_t *)mem.dalloc((uint64_t)nblk * 8);
+    bmask = (uint32_t *)mem.dalloc(2ull * nblk * 4);
+    bmin = (uint32_t *)mem.dalloc((uint64_t)kSs * nblk * 4);
+    ctl = (SsCtl *)mem.dalloc(sizeof(SsCtl));
+    h_ctl = (SsCtl *)mem.palloc(sizeof(SsCtl));
+    acc = (unsigned long long *)mem.dalloc(3 * kSs * 8);
+    h_acc = (unsigned long long *)mem.palloc(3 * kSs * 8);
+    OG_HIP(hipMemset(dist, 0xFF, V * kSs * 4));
+    OG_HIP(hipMemset(pend, 0, V * 4));
+    OG_HIP(hipMemset(acc, 0, 3 * kSs * 8));
+    SsCtl c{};
+    c.delta = d->tune.delta;
+    for (uint32_t j = 0; j < kSs; ++j) c.thresh[j] = 0xFFFFFFFFu;
+    OG_HIP(hipMemcpy(ctl, &c, sizeof c, hipMemcpyHostToDevice));
+}
+
+// Finished slots report the sums copied with the round's summary; kept
+// results are read from the distance column before it is reset.  The slots
+// turn dirty: their columns are reset by one sweep for many slots, when a
+// queued query needs a slot or half of them are dirty.
+void SsspEngine::finalize(uint32_t done) {
+    const uint64_t V = d->V;
+    std::vector<std::pair<uint64_t, std::vector<uint32_t>>> vals;
+    for (uint32_t j = 0; j < kSs; ++j) {
+        if (!((done >> j) & 1u) || !(slots[j].q.flags & OG_QUERY_KEEP_VALUES)) continue;
+        hipLaunchKernelGGL(k_ss_extract, dim3(sweep_grid(V, 8192)), dim3(kBlock), 0, s, dist, V, j, extract);
+        OG_HIP(hipGetLastError());
+        vals.push_back({slots[j].q.id, std::vector<uint32_t>(V)});
+        OG_HIP(hipMemcpyAsync(vals.back().second.data(), extract, V * 4, hipMemcpyDeviceToHost, s));
+        OG_HIP(hipStreamSynchronize(s));  // the extraction buffer is reused
     }
-    for (;;) {
-        {
-            std::unique_lock<std::mutex> lk(srv->mu);
-            srv->cv_work.wait(lk, [&] { return srv->stop || !pending->empty(); });
-            if (srv->stop) return;
-            cur = pending->front();
+    hipLaunchKernelGGL(k_ss_zero_acc, dim3(1), dim3(64), 0, s, done, acc);
+    OG_HIP(hipGetLastError());
+    dirty |= done;
+    const double t = now_ms(srv);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    for (uint32_t j = 0; j < kSs; ++j) {
+        if (!((done >> j) & 1u)) continue;
+        og_query_result r{};
+        r.id = slots[j].q.id;
+        r.graph = gi;
+        r.kind = OG_QUERY_SSSP;
+        r.source = slots[j].q.source;
+        r.status = OG_OK;
+        r.checksum = h_acc[j];
+        r.reached = h_acc[kSs + j];
+        r.traversed_edges = h_acc[2 * kSs + j];
+        r.rounds = round - slots[j].start_round;
+        r.queue_ms = slots[j].t_admit - slots[j].q.t_submit;
+        r.latency_ms = t - slots[j].q.t_submit;
+        srv->done.push_back(r);
+        slots[j].used = false;
+        srv->completed++;
+        srv->in_flight--;
+    }
+    for (auto &kv : vals) srv->kept[kv.first] = std::move(kv.second);
+    used &= ~done;
+    srv->cv_done.notify_all();
+}
+
+// One round: admit queued queries into free slots (resetting dirty columns
+// first when needed), then select / scan / write / relax on the engine's
+// stream; the host waits for the scan's summary only, so the round's write
+// and relax run while it reports the slots that finished.
+void SsspEngine::step() {
+    if (srv->params.flags & OG_SERVER_TEST_FAIL_SSSP) {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        srv->cv_work.wait(lk, [&] { return srv->stop || !pending->empty(); });
+        if (srv->stop) return;
+        lk.unlock();
+        raise(OG_EHIP, "forced failure (OG_SERVER_TEST_FAIL_SSSP)");
+    }
+    SsSeeds sl{};
+    {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        srv->cv_work.wait(lk, [&] { return srv->stop || used || !pending->empty(); });
+        if (srv->stop) return;
+        const double t = now_ms(srv);
+        const uint32_t clean = kSs - (uint32_t)__builtin_popcount(used | dirty);
+        if (dirty && (pending->size() > clean || __builtin_popcount(dirty) >= (int)kSs / 2)) {
+            hipLaunchKernelGGL(k_ss_clear, dim3(sweep_grid(d->V, 16384)), dim3(kBlock), 0, s, dist, d->V, dirty);
+            OG_HIP(hipGetLastError());
+            dirty = 0;  // stream order: the reset runs before this round's seeds
+        }
+        while (!pending->empty() && (uint32_t)__builtin_popcount(used | dirty) < kSs) {
+            uint32_t j = 0;
+            while (((used | dirty) >> j) & 1u) ++j;
+            slots[j].used = true;
+            slots[j].q = pending->front();
+            slots[j].start_round = round;
+            slots[j].t_admit = t;
             pending->pop_front();
+            used |= 1u << j;
+            sl.slot[sl.n] = j;
+            sl.src[sl.n] = slots[j].q.source;
+            sl.n++;
             srv->pending--;
             srv->in_flight++;
         }
-        const double t_admit = now_ms(srv);
-        std::vector<uint32_t> vals;
-        og_run_stats st{};
-        og_status status = OG_OK;
-        try {
-            if (cur.flags & OG_QUERY_KEEP_VALUES) vals.resize(lane->V);
-            lane_run(g, lane, ALGO_SSSP, cur.source, vals.empty() ? nullptr : vals.data(), &st);
-        } catch (const Error &e) {
-            status = e.st;
-        } catch (...) {
-            status = OG_ENOMEM_HOST;
+    }
+    if (!used) return;
+    const uint64_t V = d->V;
+    if (sl.n) hipLaunchKernelGGL(k_ss_seed, dim3(1), dim3(64), 0, s, sl, dist, pend, d->d_ptr, acc, ctl);
+    hipLaunchKernelGGL(k_ss_select, dim3(nblk), dim3(kBlock), 0, s, pend, dist, d->d_ptr, V, ctl, bnv, bdeg, bmask,
+                       bmin, nblk);
+    hipLaunchKernelGGL(k_ss_scan, dim3(1), dim3(1024), 0, s, bnv, bdeg, bmask, bmin, nblk, used, ctl, doff);
+    OG_HIP(hipGetLastError());
+    OG_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(SsCtl), hipMemcpyDeviceToHost, s));
+    OG_HIP(hipMemcpyAsync(h_acc, acc, 3 * kSs * 8, hipMemcpyDeviceToHost, s));
+    OG_HIP(hipEventRecord(ev_summary, s));
+    hipLaunchKernelGGL(k_ss_write, dim3(nblk), dim3(kBlock), 0, s, pend, d->d_ptr, V, bnv, bdeg, alist, amask, doff);
+    if (d->packed)
+        hipLaunchKernelGGL((k_ss_relax<true, true>), dim3(kSsRelaxGrid), dim3(kBlock), 0, s, alist, amask, doff, ctl,
+                           d->d_ptr, (const void *)d->d_static, d->wbits, dist, pend, acc);
+    else
+        hipLaunchKernelGGL((k_ss_relax<true, false>), dim3(kSsRelaxGrid), dim3(kBlock), 0, s, alist, amask, doff, ctl,
+                           d->d_ptr, (const void *)d->d_static, d->wbits, dist, pend, acc);
+    OG_HIP(hipGetLastError());
+    OG_HIP(hipEventSynchronize(ev_summary));
+    ++round;
+    rounds_run.fetch_add(1, std::memory_order_relaxed);
+    // a slot is done when the select sweep found nothing pending for it; its
+    // sums (copied with the summary) are final: this round relaxes none of it
+    const uint32_t doneset = used & h_ctl->done;
+    if (doneset) finalize(doneset);
+    if (!used && dirty) {  // idle: reset the finished columns now, off the next query's path
+        hipLaunchKernelGGL(k_ss_clear, dim3(sweep_grid(V, 16384)), dim3(kBlock), 0, s, dist, V, dirty);
+        OG_HIP(hipGetLastError());
+        dirty = 0;
+    }
+}
+
+void SsspEngine::fail_held(og_status st, const std::string &msg) {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    fail_engine(srv, gi, pending, failure, slots, kSs, st, msg);
+    used = 0;
+}
+
+void SsspEngine::loop() {
+    try {
+        OG_HIP(hipSetDevice(d->device));
+        for (;;) {
+            {
+                std::lock_guard<std::mutex> lk(srv->mu);
+                if (srv->stop) return;
+            }
+            step();
         }
-        const double t = now_ms(srv);
-        std::lock_guard<std::mutex> lk(srv->mu);
-        og_query_result r{};
-        r.id = cur.id;
-        r.graph = gi;
-        r.kind = OG_QUERY_SSSP;
-        r.source = cur.source;
-        r.status = status;
-        if (status == OG_OK) {
-            r.checksum = st.checksum;
-            r.reached = st.reached;
-            r.traversed_edges = st.traversed_edges;
-            r.rounds = st.iterations;
-            if (!vals.empty()) srv->kept[cur.id] = std::move(vals);
-        }
-        r.queue_ms = t_admit - cur.t_submit;
-        r.latency_ms = t - cur.t_submit;
-        srv->done.push_back(r);
-        srv->completed++;
-        srv->in_flight--;
-        ++runs;
-        rounds_run += st.iterations;
-        srv->cv_done.notify_all();
+    } catch (const Error &e) {
+        fail_held(e.st, e.msg);
+    } catch (...) {
+        fail_held(OG_ENOMEM_HOST, "SSSP engine stopped: host allocation failed");
     }
 }
 
@@ -550,11 +1095,10 @@ static void server_free(og_server *srv) {
     srv->cv_work.notify_all();
     for (auto &e : srv->bfs)
         if (e && e->th.joinable()) e->th.join();
-    for (auto &ls : srv->lanes)
-        for (auto &l : ls)
-            if (l && l->th.joinable()) l->th.join();
+    for (auto &e : srv->sssp)
+        if (e && e->th.joinable()) e->th.join();
     srv->bfs.clear();
-    srv->lanes.clear();
+    srv->sssp.clear();
     for (og_graph *g : srv->graphs) g->busy.store(0);
     delete srv;
 }
@@ -589,14 +1133,12 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
                 raise(OG_EBUSY, "graph %u is running another call", locked);
             srv->graphs.push_back(graphs[locked]);
         }
-        // Admission: the BFS engines' frontier masks (24 V bytes per graph:
-        // visit / front / next, 64 slots) and, per weighted graph, K SSSP
-        // lanes of about 17 V bytes each (values, flags, list, degree
-        // prefix): K is the largest of 4, 2, 1 that fits the budget.
-        uint64_t fixed = 0, per_lane = 0;
+        // Admission: every graph's BFS engine state and every weighted
+        // graph's SSSP engine state must fit the budget.
+        uint64_t need = 0;
         for (og_graph *g : srv->graphs) {
-            fixed += 24 * g->V;
-            if (g->kind == OG_KIND_WEIGHTED) per_lane += lane_bytes(g->V);
+            need += bfs_state_bytes(g->V);
+            if (g->kind == OG_KIND_WEIGHTED) need += sssp_state_bytes(g->V);
         }
         uint64_t budget = srv->params.state_budget_bytes;
         if (!budget) {
@@ -604,15 +1146,13 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             OG_HIP(hipMemGetInfo(&fr, &tot));
             budget = fr / 4;
         }
-        uint32_t K = kMaxLanes;
-        while (K > 0 && fixed + (uint64_t)K * per_lane > budget) K >>= 1;
-        if (fixed > budget || (per_lane && K == 0))
-            raise(OG_EBUDGET,
-                  "state budget %llu bytes holds no query state (%llu bytes of BFS masks + %llu per SSSP lane set)",
-                  (unsigned long long)budget, (unsigned long long)fixed, (unsigned long long)per_lane);
+        if (need > budget)
+            raise(OG_EBUDGET, "state budget %llu bytes cannot hold the engines' query state (%llu bytes)",
+                  (unsigned long long)budget, (unsigned long long)need);
         srv->queues.resize(2 * num_graphs);
+        srv->failures.resize(2 * num_graphs);
         srv->bfs.resize(num_graphs);
-        srv->lanes.resize(num_graphs);
+        srv->sssp.resize(num_graphs);
         for (uint32_t gi = 0; gi < num_graphs; ++gi) {
             og_graph *g = srv->graphs[gi];
             const uint64_t V = g->V;
@@ -623,12 +1163,14 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             e->d = g->dev;
             e->gi = gi;
             e->pending = &srv->queues[2 * gi];
+            e->failure = &srv->failures[2 * gi];
             OG_HIP(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
             OG_HIP(hipEventCreateWithFlags(&e->ev_summary, hipEventDisableTiming));
             frontier_alloc(g->dev, g, e->f, e->mem);
             e->visit = (uint64_t *)e->mem.dalloc(V * 8);
             e->front = (uint64_t *)e->mem.dalloc(V * 8);
             e->next = (uint64_t *)e->mem.dalloc(V * 8);
+            for (auto &kb : e->keep_buf) kb = (uint32_t *)e->mem.dalloc(V * 4);
             OG_HIP(hipMemset(e->visit, 0, V * 8));
             OG_HIP(hipMemset(e->front, 0, V * 8));
             OG_HIP(hipMemset(e->next, 0, V * 8));
@@ -643,24 +1185,24 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             e->d_acc = (unsigned long long *)e->mem.dalloc(3 * 64 * 8);
             e->h_acc = (unsigned long long *)e->mem.palloc(3 * 64 * 8);
             OG_HIP(hipMemset(e->d_acc, 0, 3 * 64 * 8));
-            srv->state_bytes += 24 * V;
+            srv->state_bytes += bfs_state_bytes(V);
             srv->bfs[gi] = std::move(e);
             if (g->kind != OG_KIND_WEIGHTED) continue;
-            for (uint32_t k = 0; k < K; ++k) {
-                std::unique_ptr<SsspLane> l(new SsspLane);
-                l->srv = srv;
-                l->g = g;
-                l->gi = gi;
-                l->pending = &srv->queues[2 * gi + 1];
-                l->lane = lane_state_create(g);
-                srv->state_bytes += l->lane->device_bytes();
-                srv->lanes[gi].push_back(std::move(l));
-            }
+            std::unique_ptr<SsspEngine> x(new SsspEngine);
+            x->srv = srv;
+            x->g = g;
+            x->d = g->dev;
+            x->gi = gi;
+            x->pending = &srv->queues[2 * gi + 1];
+            x->failure = &srv->failures[2 * gi + 1];
+            x->setup();
+            srv->state_bytes += sssp_state_bytes(V);
+            srv->sssp[gi] = std::move(x);
         }
         OG_HIP(hipDeviceSynchronize());
         for (auto &e : srv->bfs) e->th = std::thread(&BfsEngine::loop, e.get());
-        for (auto &ls : srv->lanes)
-            for (auto &l : ls) l->th = std::thread(&SsspLane::loop, l.get());
+        for (auto &x : srv->sssp)
+            if (x) x->th = std::thread(&SsspEngine::loop, x.get());
         return OG_OK;
     });
     if (st != OG_OK) {
@@ -680,11 +1222,14 @@ extern "C" og_status og_server_submit(og_server *srv, uint32_t graph, uint32_t k
     if (!srv) return fail(OG_EINVAL, "server is NULL");
     if (graph >= srv->graphs.size()) return fail(OG_EINVAL, "graph index %u out of range", graph);
     if (kind > 1) return fail(OG_EINVAL, "kind must be OG_QUERY_BFS or OG_QUERY_SSSP");
-    if (kind == OG_QUERY_SSSP && srv->lanes[graph].empty())
+    if (kind == OG_QUERY_SSSP && !srv->sssp[graph])
         return fail(OG_EUNSUPPORTED, "graph %u has no weights: SSSP queries are not served", graph);
     if (source >= srv->graphs[graph]->V) return fail(OG_EINVAL, "source %u >= V", source);
     std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->stop) return fail(OG_ESTATE, "server is stopping");
+    const EngineFailure &ef = srv->failures[2 * graph + kind];
+    if (ef.st != OG_OK)
+        return fail(ef.st, "the %s engine of graph %u has stopped: %s", kind ? "SSSP" : "BFS", graph, ef.msg.c_str());
     const uint32_t cap = srv->params.max_pending ? srv->params.max_pending : 1u << 20;
     if (srv->pending >= cap) {
         srv->rejected++;
@@ -737,9 +1282,9 @@ extern "C" og_status og_server_get_info(og_server *srv, og_server_info *info) {
     info->num_graphs = (uint32_t)srv->graphs.size();
     for (uint32_t gi = 0; gi < srv->graphs.size(); ++gi) {
         info->bfs_slots[gi] = kSlots;
-        info->sssp_slots[gi] = (uint32_t)srv->lanes[gi].size();
-        info->rounds += srv->bfs[gi]->rounds_run;
-        for (auto &l : srv->lanes[gi]) info->rounds += l->rounds_run;
+        info->sssp_slots[gi] = srv->sssp[gi] ? kSs : 0;
+        info->rounds += srv->bfs[gi]->rounds_run.load(std::memory_order_relaxed);
+        if (srv->sssp[gi]) info->rounds += srv->sssp[gi]->rounds_run.load(std::memory_order_relaxed);
     }
     info->state_bytes = srv->state_bytes;
     info->submitted = srv->submitted;

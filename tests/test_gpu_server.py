@@ -1,6 +1,7 @@
 """Continuous-batching server: each query's result is the same alone and in
-the mix (BFS slots batched with others, SSSP on concurrent lanes), equals the
-single-query path and the oracle; admission and reservation rules hold."""
+the mix (BFS slots batched 64 to an engine, SSSP slots 16 to an engine with
+their own near-far thresholds), equals the single-query path and the oracle;
+admission, reservation and failure rules hold."""
 import sys
 import time
 
@@ -46,15 +47,17 @@ class _Graphs(list):
         self.in_edges = in_edges
 
 
-def _state_bytes(graphs, K):
-    """Admission rule of og_server_create: the BFS engines' masks (24 V bytes
-    per graph) + K SSSP lanes of 17 V bytes per weighted graph."""
+def _state_bytes(graphs):
+    """Admission rule of og_server_create: per graph the BFS engine's masks
+    (24 V bytes) and 4 kept-levels buffers (16 V), per weighted graph the
+    SSSP engine's 16 distance columns (64 V), pending masks (4 V), active
+    list and masks (8 V), degree prefix (8 V + 8) and extraction buffer (4 V)."""
     total = 0
     for g in graphs:
         V = g.num_vertices
-        total += 24 * V
+        total += 40 * V
         if g.info()["kind"] == og.KIND_WEIGHTED:
-            total += 17 * V * K
+            total += 88 * V + 8
     return total
 
 
@@ -90,11 +93,11 @@ def _oracle(graphs, gi, kind, src):
 def test_alone_and_in_the_mix(gpu, graphs):
     qs = _queries(graphs, 60, 7)
     _say("server mix of", len(qs))
-    # small budget: one SSSP lane per weighted graph, so SSSP queries queue;
-    # BFS queries join the 64-slot batches mid-flight
-    with og.Server(graphs, state_budget_bytes=_state_bytes(graphs, 1)) as srv:
+    # the exact budget: queries join the BFS and SSSP batches mid-flight;
+    # kept BFS levels use 4 buffers per graph, so those queries queue
+    with og.Server(graphs, state_budget_bytes=_state_bytes(graphs)) as srv:
         info = srv.info()
-        assert info["bfs_slots"] == [64, 64, 64] and info["sssp_slots"] == [1, 1, 0]
+        assert info["bfs_slots"] == [64, 64, 64] and info["sssp_slots"] == [16, 16, 0]
         ids = [srv.submit(gi, k, s, keep=True) for gi, k, s in qs]
         res = _wait_all(srv, ids)
         mix = {i: srv.take_values(i, qs[n][0]) for n, i in enumerate(ids)}
@@ -134,12 +137,12 @@ def test_admission_and_reservation(gpu, graphs):
     with pytest.raises(og.OgError) as e:
         og.Server([g], state_budget_bytes=16)
     assert e.value.name == "OG_EBUDGET"
-    with pytest.raises(og.OgError) as e:  # the masks alone do not fit
-        og.Server([g], state_budget_bytes=_state_bytes([g], 1) - 1)
+    with pytest.raises(og.OgError) as e:  # one byte short
+        og.Server([g], state_budget_bytes=_state_bytes([g]) - 1)
     assert e.value.name == "OG_EBUDGET"
-    srv = og.Server([g], state_budget_bytes=_state_bytes([g], 2) + 1, max_pending=4)
-    assert srv.info()["bfs_slots"] == [64] and srv.info()["sssp_slots"] == [2]
-    assert srv.info()["state_bytes"] >= _state_bytes([g], 2)
+    srv = og.Server([g], state_budget_bytes=_state_bytes([g]), max_pending=4)
+    assert srv.info()["bfs_slots"] == [64] and srv.info()["sssp_slots"] == [16]
+    assert srv.info()["state_bytes"] == _state_bytes([g])
     with pytest.raises(og.OgError) as e:  # reserved while served
         g.bfs(0)
     assert e.value.name == "OG_EBUSY"
@@ -185,3 +188,78 @@ def test_burst_of_full_batches(gpu, graphs):
     for n, s in enumerate(srcs):
         _, st = g.bfs(int(s), want_values=False)
         assert st["checksum"] == res[ids[n]]["checksum"], (n, int(s))
+
+
+def test_sssp_batches_alone_and_in_the_mix(gpu, graphs):
+    """More SSSP queries than slots at once (repeated sources among them),
+    then one at a time: each query's distances equal the oracle's and its
+    result alone, whatever else shares its rounds."""
+    gi = 0
+    g = graphs[gi]
+    ptr, to, w, _ = g.export()
+    deg = np.diff(np.append(ptr.astype(np.int64), len(to)))
+    cand = np.flatnonzero(deg > 0)
+    srcs = [int(x) for x in np.random.default_rng(19).choice(cand, 36, replace=False)] + [int(cand[0])] * 3
+    _say("sssp burst of", len(srcs), "on graph 0")
+    with og.Server([g, graphs[1]]) as srv:
+        ids = [srv.submit(0, og.QUERY_SSSP, s, keep=True) for s in srcs]
+        ids += [srv.submit(1, og.QUERY_SSSP, 5, keep=True)]  # the other engine runs beside it
+        res = _wait_all(srv, ids)
+        vals = {i: srv.take_values(i, 0 if n < len(srcs) else 1) for n, i in enumerate(ids)}
+        alone = {}
+        for s in srcs[:4]:
+            i = srv.submit(0, og.QUERY_SSSP, s, keep=True)
+            alone[s] = (_wait_all(srv, [i])[i], srv.take_values(i, 0))
+    for n, s in enumerate(srcs):
+        r = res[ids[n]]
+        assert r["status"] == 0
+        ref = oracle.sssp(ptr, to, w, s)
+        assert np.array_equal(vals[ids[n]], ref), (n, s)
+        assert r["checksum"] == og.levels_checksum(ref)
+        assert r["reached"] == int((ref != og.UNREACHED).sum())
+        assert r["traversed_edges"] == int(deg[ref != og.UNREACHED].sum())
+        if s in alone:
+            assert alone[s][0]["checksum"] == r["checksum"]
+            assert np.array_equal(alone[s][1], vals[ids[n]])
+    p1, t1, w1, _ = graphs[1].export()
+    assert np.array_equal(vals[ids[-1]], oracle.sssp(p1, t1, w1, 5))
+
+
+@pytest.mark.parametrize("delta", [0, 1, 32])
+def test_sssp_batch_bucket_widths(gpu, delta):
+    """Per-slot thresholds at every bucket width, including none (delta 0:
+    every pending entry relaxes each round) and width 1 (a threshold move
+    for nearly every distance)."""
+    g = og.Graph.rmat(11, 8, seed=61, weight_seed=62)
+    g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0)
+    g.set_tuning(delta=delta)
+    ptr, to, w, _ = g.export()
+    cand = np.flatnonzero(np.diff(np.append(ptr.astype(np.int64), len(to))) > 0)
+    srcs = [int(x) for x in np.random.default_rng(63).choice(cand, 20, replace=False)]
+    with og.Server([g]) as srv:
+        ids = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs]
+        res = _wait_all(srv, ids)
+    for n, s in enumerate(srcs):
+        assert res[ids[n]]["checksum"] == og.levels_checksum(oracle.sssp(ptr, to, w, s)), (delta, s)
+    g.close()
+
+
+def test_failed_engine_refuses_its_queue(gpu, graphs):
+    """An engine that stops (forced here from the host: the SSSP engines of a
+    server created with og.SERVER_TEST_FAIL_SSSP raise at their first round,
+    as after a device error, without touching the GPU) fails the queries it
+    held and every later submit to it; the BFS engines keep serving."""
+    g = graphs[0]
+    with og.Server([g], flags=og.SERVER_TEST_FAIL_SSSP) as srv:
+        i = srv.submit(0, og.QUERY_SSSP, 0)
+        r = _wait_all(srv, [i])[i]
+        assert og.STATUS[r["status"]] == "OG_EHIP"
+        with pytest.raises(og.OgError) as e:
+            srv.submit(0, og.QUERY_SSSP, 1)
+        assert e.value.name == "OG_EHIP" and "stopped" in str(e.value)
+        j = srv.submit(0, og.QUERY_BFS, 0)
+        rb = _wait_all(srv, [j])[j]
+        assert rb["status"] == 0
+        assert srv.info()["in_flight"] == 0 and srv.info()["pending"] == 0
+    _, st = g.bfs(0, want_values=False)
+    assert st["checksum"] == rb["checksum"]
