@@ -421,9 +421,10 @@ def wl_serve(args):
     service = sample_plain("diropt", og.PLAN_DIRECTION_OPT)
 
     # per-query state (og_server_create's admission rule): per graph the BFS
-    # engine (40 V bytes), per weighted graph the 16-slot SSSP engine (88 V + 8)
-    full = sum(40 * g.num_vertices + (88 * g.num_vertices + 8 if g.info()["kind"] == og.KIND_WEIGHTED else 0)
-               for g in graphs)
+    # engine (40 V bytes), per weighted graph the 16-slot SSSP engine (88 V + 8
+    # + the relax's chunk table)
+    full = sum(40 * g.num_vertices + (88 * g.num_vertices + 8 + 4 * (g.num_edges // 512 + 2)
+                                      if g.info()["kind"] == og.KIND_WEIGHTED else 0) for g in graphs)
     budget = int(args.serve_budget_gib * (1 << 30)) if args.serve_budget_gib else full
 
     def replay(t_arr, gis, kinds, svc, seed):
@@ -441,7 +442,7 @@ def wl_serve(args):
         t_arr, gis, kinds, srcs = trace(rate, seed, bfs_share)
         plain_qps, plat = replay(t_arr, gis, kinds, service, seed + 4)
         td_qps, _ = replay(t_arr, gis, kinds, service_td, seed + 4)
-        srv = og.Server(graphs, state_budget_bytes=budget)
+        srv = og.Server(graphs, state_budget_bytes=budget, sssp_delta=args.serve_sssp_delta)
         info0 = srv.info()
         lat, done, submit_at = [], {}, {}
         t0 = time.perf_counter()
@@ -526,6 +527,7 @@ def wl_serve(args):
                       {"workload": "served bfs/sssp queries, rmat " + "/".join(map(str, scales)) + " ef16",
                        "rates": rates, "mixes": mixes, "duration_s": args.duration, "state_budget_bytes": budget,
                        "p99_target_ms": args.p99_target_ms, "sssp_delta": args.delta,
+                       "served_sssp_delta": args.serve_sssp_delta,
                        "split_weight": args.split_weight})
     out.update({"serve": per_mix,
                 "parity": [{"mix": k, "rate": r, "checked": v["checked"], "mismatches": v["mismatches"]}
@@ -591,6 +593,8 @@ def main():
     ap.add_argument("--verify", type=int, default=100, help="serve: served results re-run on the plain path")
     ap.add_argument("--serve-mix", default="0.7,1.0", help="serve: BFS shares of the query mixes to run")
     ap.add_argument("--p99-target-ms", type=float, default=50.0, help="serve: p99 latency target")
+    ap.add_argument("--serve-sssp-delta", type=int, default=32,
+                    help="serve: bucket width of the server's batched SSSP slots (whole rows)")
     args = ap.parse_args()
     if args.scale is None:
         args.scale = {"sssp-ooc": 24, "bfs-hbm": 20, "cc-ooc": 25, "pr-ooc": 25, "msbfs": 26, "serve": 0}[args.workload]

@@ -412,10 +412,17 @@ typedef struct og_server_params {
                                      buffers (40 V bytes), per weighted graph
                                      the SSSP engine's 16 distance columns,
                                      pending masks, active list and degree
-                                     prefix (88 V + 8 bytes);
+                                     prefix (88 V + 8 bytes) and chunk table
+                                     (4 (E / 512 + 2) bytes);
                                      0 = a quarter of free HBM                 */
     uint32_t max_pending;         /* queued queries before submit returns OG_EBUSY; 0 = 2^20 */
     uint32_t flags;               /* OG_SERVER_* bits                       */
+    uint32_t sssp_delta;          /* bucket width of the SSSP slots' near-far
+                                     rounds; 0 = the graph's tuning delta.
+                                     The slots relax whole rows (no
+                                     light/heavy split), for which 16-64 does
+                                     the least work with weights in [1, 63] */
+    uint32_t reserved;
 } og_server_params;
 
 /* Testing only: the server's SSSP engines stop at their first round with

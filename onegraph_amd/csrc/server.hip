@@ -246,7 +246,7 @@ __global__ void k_srv_zero_acc(uint64_t done, unsigned long long *acc) {
 constexpr uint32_t kSs = 16;                  // SSSP slots per graph (distances per 64-byte vertex line)
 constexpr uint32_t kSsWin = 8192;             // vertices per workgroup of the select / write sweeps
 constexpr uint32_t kSsPer = kSsWin / kBlock;  // per thread: vertices base + t + kBlock * k, k < kSsPer
-constexpr uint32_t kSsGroupEdges = 4;         // edges per 4-lane group per chunk
+constexpr uint32_t kSsGroupEdges = 8;         // edges per 4-lane group per chunk
 constexpr uint32_t kSsChunk = (kBlock / 4) * kSsGroupEdges;  // active edges per relax workgroup iteration
 constexpr unsigned kSsRelaxGrid = 4096;       // relax workgroups (they stride over the round's edges)
 
@@ -430,11 +430,13 @@ __global__ __launch_bounds__(1024) void k_ss_scan(uint32_t *__restrict__ bnv, ui
 }
 
 // Write sweep: the active list in block order (vertex, near mask), its
-// degree prefix, and pend[v] back to the far slots only.
+// degree prefix, the list entry holding the first edge of every relax chunk
+// (cstart[c], chunks of kSsChunk edges), and pend[v] back to the far slots.
 __global__ __launch_bounds__(kBlock) void k_ss_write(uint32_t *__restrict__ pend, const uint64_t *__restrict__ ptr,
                                                      uint64_t V, const uint32_t *__restrict__ bnv,
                                                      const uint64_t *__restrict__ bdeg, uint32_t *__restrict__ alist,
-                                                     uint32_t *__restrict__ amask, uint64_t *__restrict__ doff) {
+                                                     uint32_t *__restrict__ amask, uint64_t *__restrict__ doff,
+                                                     uint32_t *__restrict__ cstart) {
     __shared__ uint64_t lds[2 * kBlock / kWave];
     const uint64_t base = (uint64_t)blockIdx.x * kSsWin + threadIdx.x;
     uint32_t cnt = 0;
@@ -457,50 +459,48 @@ __global__ __launch_bounds__(kBlock) void k_ss_write(uint32_t *__restrict__ pend
         if (v >= V) break;
         const uint32_t p = pend[v];
         if (!(p >> 16)) continue;
+        const uint64_t dv = ptr[v + 1] - ptr[v];  // >= 1 for every listed vertex
         alist[pos] = (uint32_t)v;
         amask[pos] = p >> 16;
         doff[pos] = dpos;
+        // chunks of the relax whose first edge is one of this vertex's
+        for (uint64_t c = (dpos + kSsChunk - 1) / kSsChunk; c * kSsChunk < dpos + dv; ++c) cstart[c] = (uint32_t)pos;
         ++pos;
-        dpos += ptr[v + 1] - ptr[v];
+        dpos += dv;
         pend[v] = p & 0xFFFFu;
         --cnt;
     }
 }
 
-// Relax: workgroups stride over the round's active edge space in chunks of
-// kSsChunk; a chunk's slice of the degree prefix is staged in LDS and each
-// edge finds its source by binary search there.  Four lanes share an edge;
-// lane l reads the 16-byte pieces of the source's and the target's lines
-// holding slots 4l..4l+3 (when u is near for any of them) and offers them.
+// Relax: workgroups stride over the round's chunks of kSsChunk active
+// edges; a chunk's list entries start at cstart[c] (written with the list),
+// its slice of the degree prefix is staged in LDS and each edge finds its
+// source by binary search there.  Four lanes share an edge; lane l reads the
+// 16-byte pieces of the source's and the target's lines holding slots
+// 4l..4l+3 (when u is near for any of them) and offers them.  Each lane sums
+// its four slots' result terms in registers; the workgroup adds them up once
+// at the end.
 template <bool W, bool PK>
 __global__ __launch_bounds__(kBlock) void k_ss_relax(const uint32_t *__restrict__ alist,
                                                      const uint32_t *__restrict__ amask,
-                                                     const uint64_t *__restrict__ doff, const SsCtl *__restrict__ ctl,
+                                                     const uint64_t *__restrict__ doff,
+                                                     const uint32_t *__restrict__ cstart, const SsCtl *__restrict__ ctl,
                                                      const uint64_t *__restrict__ ptr, const void *__restrict__ edges,
                                                      uint32_t wbits, uint32_t *dist, uint32_t *pend,
                                                      unsigned long long *acc) {
     __shared__ uint64_t s_off[kSsChunk + 2];
-    __shared__ uint32_t s_i[2];
-    __shared__ unsigned long long s_acc[3 * kSs];
-    if (threadIdx.x < 3 * kSs) s_acc[threadIdx.x] = 0;
+    __shared__ unsigned long long s_acc[kBlock / kWave][3 * kSs];
     const uint32_t n = ctl->nactive;
     const uint64_t D = ctl->dsum;
+    const uint64_t nch = (D + kSsChunk - 1) / kSsChunk;
     const uint32_t g = threadIdx.x >> 2, l = threadIdx.x & 3u;
-    for (uint64_t q0 = (uint64_t)blockIdx.x * kSsChunk; q0 < D; q0 += (uint64_t)gridDim.x * kSsChunk) {
-        const uint64_t q1 = min(q0 + kSsChunk, D);
-        __syncthreads();  // s_off / s_i of the previous chunk are no longer read
-        if (threadIdx.x < 2) {
-            const uint64_t q = threadIdx.x == 0 ? q0 : q1 - 1;
-            uint32_t lo = 0, hi = n;  // largest i with doff[i] <= q; doff[0] = 0
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (doff[mid] <= q) lo = mid; else hi = mid;
-            }
-            s_i[threadIdx.x] = lo;
-        }
-        __syncthreads();
-        const uint32_t i0 = s_i[0];
-        const uint32_t nv = s_i[1] - i0 + 1;
+    unsigned long long a_dig[4] = {0, 0, 0, 0}, a_rch[4] = {0, 0, 0, 0}, a_trv[4] = {0, 0, 0, 0};
+    for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint64_t q0 = c * kSsChunk, q1 = min(q0 + kSsChunk, D);
+        const uint32_t i0 = cstart[c];
+        const uint32_t i1 = q1 == D ? n - 1 : cstart[c + 1];  // the entry holding q1 (or the last one)
+        const uint32_t nv = i1 - i0 + 1;                       // <= kSsChunk + 1
+        __syncthreads();  // the previous chunk's slice is no longer read
         for (uint32_t k = threadIdx.x; k <= nv; k += kBlock) s_off[k] = doff[i0 + k];
         __syncthreads();
         uint32_t to[kSsGroupEdges], wt[kSsGroupEdges], sub[kSsGroupEdges];
@@ -530,25 +530,23 @@ __global__ __launch_bounds__(kBlock) void k_ss_relax(const uint32_t *__restrict_
         for (uint32_t e = 0; e < kSsGroupEdges; ++e) {
             uint32_t imp = 0;
             if (sub[e]) {
+                const uint64_t key = (uint64_t)to[e] << 32;
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     if (!((sub[e] >> b) & 1u)) continue;
                     const uint64_t s64 = (uint64_t)quad_at(du[e], b) + wt[e];
                     const uint32_t nd = s64 > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)s64;
                     if (nd >= quad_at(dt[e], b)) continue;
-                    const uint32_t j = 4 * l + b;
-                    const uint32_t old = atomicMin(dist + (uint64_t)to[e] * kSs + j, nd);
+                    const uint32_t old = atomicMin(dist + (uint64_t)to[e] * kSs + 4 * l + b, nd);
                     if (nd >= old) continue;
-                    imp |= 1u << j;
-                    const uint64_t key = (uint64_t)to[e] << 32;
-                    unsigned long long dd = srv_mix64(key | nd);
+                    imp |= 1u << (4 * l + b);
+                    a_dig[b] += srv_mix64(key | nd);
                     if (old == OG_UNREACHED) {
-                        atomicAdd(&s_acc[kSs + j], 1ull);
-                        atomicAdd(&s_acc[2 * kSs + j], (unsigned long long)(ptr[to[e] + 1] - ptr[to[e]]));
+                        a_rch[b] += 1;
+                        a_trv[b] += ptr[to[e] + 1] - ptr[to[e]];
                     } else {
-                        dd -= srv_mix64(key | old);
+                        a_dig[b] -= srv_mix64(key | old);
                     }
-                    atomicAdd(&s_acc[j], dd);
                 }
             }
             imp |= __shfl_xor(imp, 1, kWave);
@@ -556,8 +554,30 @@ __global__ __launch_bounds__(kBlock) void k_ss_relax(const uint32_t *__restrict_
             if (l == 0 && imp) atomicOr(pend + to[e], imp);
         }
     }
+    // lanes with the same l hold the same four slots: sum over the wave's 16
+    // such lanes, then over the waves
+#pragma unroll
+    for (int m = 4; m < kWave; m <<= 1)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            a_dig[b] += shfl_xor_u64(a_dig[b], m);
+            a_rch[b] += shfl_xor_u64(a_rch[b], m);
+            a_trv[b] += shfl_xor_u64(a_trv[b], m);
+        }
+    const uint32_t w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    if (lane < 4)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            s_acc[w][4 * lane + b] = a_dig[b];
+            s_acc[w][kSs + 4 * lane + b] = a_rch[b];
+            s_acc[w][2 * kSs + 4 * lane + b] = a_trv[b];
+        }
     __syncthreads();
-    if (threadIdx.x < 3 * kSs && s_acc[threadIdx.x]) atomicAdd(acc + threadIdx.x, s_acc[threadIdx.x]);
+    if (threadIdx.x < 3 * kSs) {
+        unsigned long long t = 0;
+        for (uint32_t k = 0; k < kBlock / kWave; ++k) t += s_acc[k][threadIdx.x];
+        if (t) atomicAdd(acc + threadIdx.x, t);
+    }
 }
 
 // Resets the distance columns of the slots in `dirty` to "unreached".
@@ -611,9 +631,11 @@ constexpr uint32_t kKeepBufs = 4;   // BFS: kept-levels buffers per graph (4 V b
 // Device bytes of per-query state the admission rule charges: the BFS
 // engine's masks and kept-levels buffers, and the SSSP engine's distance
 // lines (4 kSs V), pending masks, active list with masks and degree prefix,
-// and the extraction buffer of kept results.
+// the extraction buffer of kept results and the relax's chunk table.
 static uint64_t bfs_state_bytes(uint64_t V) { return 24 * V + kKeepBufs * 4 * V; }
-static uint64_t sssp_state_bytes(uint64_t V) { return 4ull * kSs * V + 4 * V + 8 * V + 8 * (V + 1) + 4 * V; }
+static uint64_t sssp_state_bytes(uint64_t V, uint64_t E) {
+    return 4ull * kSs * V + 4 * V + 8 * V + 8 * (V + 1) + 4 * V + 4 * (E / kSsChunk + 2);
+}
 
 // What an engine thread marks when it fails: its queue stops taking queries.
 struct EngineFailure {
@@ -678,6 +700,7 @@ struct SsspEngine {
     hipStream_t s = nullptr;
     hipEvent_t ev_summary = nullptr;
     uint32_t *dist = nullptr, *pend = nullptr, *alist = nullptr, *amask = nullptr, *extract = nullptr;
+    uint32_t *cstart = nullptr;
     uint64_t *doff = nullptr;
     uint32_t *bnv = nullptr, *bmask = nullptr, *bmin = nullptr;
     uint64_t *bdeg = nullptr;
@@ -931,6 +954,7 @@ void SsspEngine::setup() {
     amask = (uint32_t *)mem.dalloc(V * 4);
     doff = (uint64_t *)mem.dalloc((V + 1) * 8);
     extract = (uint32_t *)mem.dalloc(V * 4);
+    cstart = (uint32_t *)mem.dalloc((d->E / kSsChunk + 2) * 4);
     bnv = (uint32_t *)mem.dalloc((uint64_t)nblk * 4);
     bdeg = (uint64_t *)mem.dalloc((uint64_t)nblk * 8);
     bmask = (uint32_t *)mem.dalloc(2ull * nblk * 4);
@@ -943,7 +967,7 @@ void SsspEngine::setup() {
     OG_HIP(hipMemset(pend, 0, V * 4));
     OG_HIP(hipMemset(acc, 0, 3 * kSs * 8));
     SsCtl c{};
-    c.delta = d->tune.delta;
+    c.delta = srv->params.sssp_delta ? srv->params.sssp_delta : d->tune.delta;
     for (uint32_t j = 0; j < kSs; ++j) c.thresh[j] = 0xFFFFFFFFu;
     OG_HIP(hipMemcpy(ctl, &c, sizeof c, hipMemcpyHostToDevice));
 }
@@ -1042,13 +1066,14 @@ void SsspEngine::step() {
     OG_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(SsCtl), hipMemcpyDeviceToHost, s));
     OG_HIP(hipMemcpyAsync(h_acc, acc, 3 * kSs * 8, hipMemcpyDeviceToHost, s));
     OG_HIP(hipEventRecord(ev_summary, s));
-    hipLaunchKernelGGL(k_ss_write, dim3(nblk), dim3(kBlock), 0, s, pend, d->d_ptr, V, bnv, bdeg, alist, amask, doff);
+    hipLaunchKernelGGL(k_ss_write, dim3(nblk), dim3(kBlock), 0, s, pend, d->d_ptr, V, bnv, bdeg, alist, amask, doff,
+                       cstart);
     if (d->packed)
-        hipLaunchKernelGGL((k_ss_relax<true, true>), dim3(kSsRelaxGrid), dim3(kBlock), 0, s, alist, amask, doff, ctl,
-                           d->d_ptr, (const void *)d->d_static, d->wbits, dist, pend, acc);
+        hipLaunchKernelGGL((k_ss_relax<true, true>), dim3(kSsRelaxGrid), dim3(kBlock), 0, s, alist, amask, doff,
+                           cstart, ctl, d->d_ptr, (const void *)d->d_static, d->wbits, dist, pend, acc);
     else
-        hipLaunchKernelGGL((k_ss_relax<true, false>), dim3(kSsRelaxGrid), dim3(kBlock), 0, s, alist, amask, doff, ctl,
-                           d->d_ptr, (const void *)d->d_static, d->wbits, dist, pend, acc);
+        hipLaunchKernelGGL((k_ss_relax<true, false>), dim3(kSsRelaxGrid), dim3(kBlock), 0, s, alist, amask, doff,
+                           cstart, ctl, d->d_ptr, (const void *)d->d_static, d->wbits, dist, pend, acc);
     OG_HIP(hipGetLastError());
     OG_HIP(hipEventSynchronize(ev_summary));
     ++round;
@@ -1138,7 +1163,7 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
         uint64_t need = 0;
         for (og_graph *g : srv->graphs) {
             need += bfs_state_bytes(g->V);
-            if (g->kind == OG_KIND_WEIGHTED) need += sssp_state_bytes(g->V);
+            if (g->kind == OG_KIND_WEIGHTED) need += sssp_state_bytes(g->V, g->E);
         }
         uint64_t budget = srv->params.state_budget_bytes;
         if (!budget) {
@@ -1196,7 +1221,7 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             x->pending = &srv->queues[2 * gi + 1];
             x->failure = &srv->failures[2 * gi + 1];
             x->setup();
-            srv->state_bytes += sssp_state_bytes(V);
+            srv->state_bytes += sssp_state_bytes(V, g->E);
             srv->sssp[gi] = std::move(x);
         }
         OG_HIP(hipDeviceSynchronize());

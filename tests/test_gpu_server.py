@@ -51,13 +51,14 @@ def _state_bytes(graphs):
     """Admission rule of og_server_create: per graph the BFS engine's masks
     (24 V bytes) and 4 kept-levels buffers (16 V), per weighted graph the
     SSSP engine's 16 distance columns (64 V), pending masks (4 V), active
-    list and masks (8 V), degree prefix (8 V + 8) and extraction buffer (4 V)."""
+    list and masks (8 V), degree prefix (8 V + 8), extraction buffer (4 V) and
+    the relax's chunk table (4 (E / 512 + 2))."""
     total = 0
     for g in graphs:
         V = g.num_vertices
         total += 40 * V
         if g.info()["kind"] == og.KIND_WEIGHTED:
-            total += 88 * V + 8
+            total += 88 * V + 8 + 4 * (g.num_edges // 512 + 2)
     return total
 
 
@@ -239,8 +240,13 @@ def test_sssp_batch_bucket_widths(gpu, delta):
     with og.Server([g]) as srv:
         ids = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs]
         res = _wait_all(srv, ids)
+    with og.Server([g], sssp_delta=7) as srv:  # the server's own width over the graph's
+        ids2 = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs]
+        res2 = _wait_all(srv, ids2)
     for n, s in enumerate(srcs):
-        assert res[ids[n]]["checksum"] == og.levels_checksum(oracle.sssp(ptr, to, w, s)), (delta, s)
+        want = og.levels_checksum(oracle.sssp(ptr, to, w, s))
+        assert res[ids[n]]["checksum"] == want, (delta, s)
+        assert res2[ids2[n]]["checksum"] == want, (7, s)
     g.close()
 
 
