@@ -413,7 +413,7 @@ typedef struct og_server_params {
                                      the SSSP engine's 16 distance columns,
                                      pending masks, active list and degree
                                      prefix (88 V + 8 bytes) and chunk table
-                                     (4 (E / 512 + 2) bytes);
+                                     (4 (16 E / 1024 + 2) bytes);
                                      0 = a quarter of free HBM                 */
     uint32_t max_pending;         /* queued queries before submit returns OG_EBUSY; 0 = 2^20 */
     uint32_t flags;               /* OG_SERVER_* bits                       */
@@ -424,6 +424,12 @@ typedef struct og_server_params {
                                      the least work with weights in [1, 63] */
     uint32_t reserved;
 } og_server_params;
+
+/* SSSP slots share one near-far threshold: it moves (to the smallest far
+ * value of any slot + delta) only when no slot has anything near, so slots
+ * relax the same distance band together and share more edge reads; a slot
+ * admitted mid-flight starts at the running threshold. */
+#define OG_SERVER_SSSP_SHARED_THRESHOLD 1u
 
 /* Testing only: the server's SSSP engines stop at their first round with
  * OG_EHIP, as after a device error, without touching the GPU (the failure

@@ -246,9 +246,9 @@ __global__ void k_srv_zero_acc(uint64_t done, unsigned long long *acc) {
 constexpr uint32_t kSs = 16;                  // SSSP slots per graph (distances per 64-byte vertex line)
 constexpr uint32_t kSsWin = 8192;             // vertices per workgroup of the select / write sweeps
 constexpr uint32_t kSsPer = kSsWin / kBlock;  // per thread: vertices base + t + kBlock * k, k < kSsPer
-constexpr uint32_t kSsGroupEdges = 8;         // edges per 4-lane group per chunk
-constexpr uint32_t kSsChunk = (kBlock / 4) * kSsGroupEdges;  // active edges per relax workgroup iteration
-constexpr unsigned kSsRelaxGrid = 4096;       // relax workgroups (they stride over the round's edges)
+constexpr uint32_t kSsRun = 4;                // work items per lane per chunk (consecutive)
+constexpr uint32_t kSsChunk = kBlock * kSsRun;  // work items per relax workgroup iteration
+constexpr unsigned kSsRelaxGrid = 4096;       // relax workgroups (they stride over the round's items)
 
 struct SsCtl {
     uint32_t thresh[kSs];  // near-far threshold of each slot (0xFFFFFFFF: every pending entry is near)
@@ -268,10 +268,13 @@ struct SsSeeds {
 
 // New queries: slot j starts at its source (value 1, pending), its sums at
 // the source's own terms, its threshold at 1 + delta.
-__global__ void k_ss_seed(SsSeeds sl, uint32_t *dist, uint32_t *pend, const uint64_t *__restrict__ ptr,
+__global__ void k_ss_seed(SsSeeds sl, int ref, uint32_t *dist, uint32_t *pend, const uint64_t *__restrict__ ptr,
                           unsigned long long *acc, SsCtl *ctl) {
     const uint32_t i = threadIdx.x;
     if (i >= sl.n) return;
+    // shared thresholds: a slot joining running ones takes their threshold
+    // (its source, at 1, is below it); ref < 0 otherwise
+    const uint32_t tref = ref >= 0 ? ctl->thresh[ref] : 0u;
     const uint32_t j = sl.slot[i], v = sl.src[i];
     dist[(uint64_t)v * kSs + j] = 1;
     atomicOr(pend + v, 1u << j);
@@ -279,7 +282,8 @@ __global__ void k_ss_seed(SsSeeds sl, uint32_t *dist, uint32_t *pend, const uint
     acc[kSs + j] = 1;
     acc[2 * kSs + j] = ptr[v + 1] - ptr[v];
     const uint32_t dl = ctl->delta;
-    ctl->thresh[j] = dl ? (uint32_t)min((uint64_t)0xFFFFFFFFull, 1ull + dl) : 0xFFFFFFFFu;
+    const uint32_t t0 = dl ? (uint32_t)min((uint64_t)0xFFFFFFFFull, 1ull + dl) : 0xFFFFFFFFu;
+    ctl->thresh[j] = max(t0, tref);
 }
 
 __device__ __forceinline__ uint32_t quad_at(const uint4 &x, int b) {
@@ -337,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_ss_select(uint32_t *__restrict__ pen
         fm |= far;
         if (dv) {
             ++cnt;
-            deg += dv;
+            deg += dv * (uint32_t)__builtin_popcount(near);  // work items: (edge, near slot) pairs
         }
     }
     const uint64_t x[2] = {cnt, deg};
@@ -365,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_ss_select(uint32_t *__restrict__ pen
 // degree at doff[nactive].
 __global__ __launch_bounds__(1024) void k_ss_scan(uint32_t *__restrict__ bnv, uint64_t *__restrict__ bdeg,
                                                   const uint32_t *__restrict__ bmask, const uint32_t *__restrict__ bmin,
-                                                  uint32_t nblk, uint32_t used, SsCtl *__restrict__ ctl,
+                                                  uint32_t nblk, uint32_t used, uint32_t shared, SsCtl *__restrict__ ctl,
                                                   uint64_t *__restrict__ doff) {
     __shared__ uint64_t lds[2 * 1024 / kWave];
     __shared__ uint32_t s_or[2], s_min[kSs];
@@ -402,7 +406,9 @@ __global__ __launch_bounds__(1024) void k_ss_scan(uint32_t *__restrict__ bnv, ui
     }
     __syncthreads();
     const uint32_t near = s_or[0], far = s_or[1];
-    const uint32_t moving = far & ~near;  // slots whose threshold moves for the next round
+    // slots whose threshold moves for the next round: each slot on its own,
+    // or (shared) every slot at once when no slot has anything near
+    const uint32_t moving = shared ? (near ? 0u : far) : far & ~near;
     for (uint32_t m = moving; m;) {
         const int j = __builtin_ctz(m);
         m &= m - 1;
@@ -419,19 +425,31 @@ __global__ __launch_bounds__(1024) void k_ss_scan(uint32_t *__restrict__ bnv, ui
         cl.far = far;
         cl.done = used & ~(near | far);
         cl.rounds++;
-        for (uint32_t m = moving; m;) {
-            const int j = __builtin_ctz(m);
-            m &= m - 1;
-            cl.thresh[j] = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)s_min[j] + cl.delta);
+        if (shared && moving) {
+            uint32_t mn = 0xFFFFFFFFu;
+            for (uint32_t m = moving; m;) {
+                const int j = __builtin_ctz(m);
+                m &= m - 1;
+                mn = min(mn, s_min[j]);
+            }
+            const uint32_t T = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)mn + cl.delta);
+            for (uint32_t j = 0; j < kSs; ++j) cl.thresh[j] = T;
+        } else {
+            for (uint32_t m = moving; m;) {
+                const int j = __builtin_ctz(m);
+                m &= m - 1;
+                cl.thresh[j] = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)s_min[j] + cl.delta);
+            }
         }
         *ctl = cl;
         doff[t[0]] = t[1];
     }
 }
 
-// Write sweep: the active list in block order (vertex, near mask), its
-// degree prefix, the list entry holding the first edge of every relax chunk
-// (cstart[c], chunks of kSsChunk edges), and pend[v] back to the far slots.
+// Write sweep: the active list in block order (vertex, near mask), the prefix
+// of its work items (an entry has degree x near slots of them), the list
+// entry holding the first item of every relax chunk (cstart[c], chunks of
+// kSsChunk items), and pend[v] back to the far slots.
 __global__ __launch_bounds__(kBlock) void k_ss_write(uint32_t *__restrict__ pend, const uint64_t *__restrict__ ptr,
                                                      uint64_t V, const uint32_t *__restrict__ bnv,
                                                      const uint64_t *__restrict__ bdeg, uint32_t *__restrict__ alist,
@@ -444,9 +462,10 @@ __global__ __launch_bounds__(kBlock) void k_ss_write(uint32_t *__restrict__ pend
     for (uint32_t k = 0; k < kSsPer; ++k) {
         const uint64_t v = base + (uint64_t)k * kBlock;
         if (v >= V) break;
-        if (pend[v] >> 16) {
+        const uint32_t near = pend[v] >> 16;
+        if (near) {
             ++cnt;
-            deg += ptr[v + 1] - ptr[v];
+            deg += (ptr[v + 1] - ptr[v]) * (uint32_t)__builtin_popcount(near);
         }
     }
     const uint64_t x[2] = {cnt, deg};
@@ -459,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_ss_write(uint32_t *__restrict__ pend
         if (v >= V) break;
         const uint32_t p = pend[v];
         if (!(p >> 16)) continue;
-        const uint64_t dv = ptr[v + 1] - ptr[v];  // >= 1 for every listed vertex
+        const uint64_t dv = (ptr[v + 1] - ptr[v]) * (uint32_t)__builtin_popcount(p >> 16);  // >= 1 item
         alist[pos] = (uint32_t)v;
         amask[pos] = p >> 16;
         doff[pos] = dpos;
@@ -472,14 +491,14 @@ __global__ __launch_bounds__(kBlock) void k_ss_write(uint32_t *__restrict__ pend
     }
 }
 
-// Relax: workgroups stride over the round's chunks of kSsChunk active
-// edges; a chunk's list entries start at cstart[c] (written with the list),
-// its slice of the degree prefix is staged in LDS and each edge finds its
-// source by binary search there.  Four lanes share an edge; lane l reads the
-// 16-byte pieces of the source's and the target's lines holding slots
-// 4l..4l+3 (when u is near for any of them) and offers them.  Each lane sums
-// its four slots' result terms in registers; the workgroup adds them up once
-// at the end.
+// Relax: one work item per (edge, near slot) of the active list, an entry's
+// items edge-major (item e * m + k: edge e, k-th of its m near slots), so
+// neighbouring lanes offer the slots of one edge to one 64-byte target line
+// and no lane idles for a slot that is not near.  Workgroups stride over
+// chunks of kSsChunk items; a chunk's entries start at cstart[c] and its
+// slice of the item prefix is staged in LDS.  Each lane takes kSsRun
+// consecutive items: one binary search, then it walks on.  Result terms are
+// summed per wave in LDS and added to the slots' sums once per workgroup.
 template <bool W, bool PK>
 __global__ __launch_bounds__(kBlock) void k_ss_relax(const uint32_t *__restrict__ alist,
                                                      const uint32_t *__restrict__ amask,
@@ -490,11 +509,11 @@ __global__ __launch_bounds__(kBlock) void k_ss_relax(const uint32_t *__restrict_
                                                      unsigned long long *acc) {
     __shared__ uint64_t s_off[kSsChunk + 2];
     __shared__ unsigned long long s_acc[kBlock / kWave][3 * kSs];
+    const uint32_t wv = threadIdx.x / kWave;
+    for (uint32_t k = threadIdx.x; k < (kBlock / kWave) * 3 * kSs; k += kBlock) (&s_acc[0][0])[k] = 0;
     const uint32_t n = ctl->nactive;
     const uint64_t D = ctl->dsum;
     const uint64_t nch = (D + kSsChunk - 1) / kSsChunk;
-    const uint32_t g = threadIdx.x >> 2, l = threadIdx.x & 3u;
-    unsigned long long a_dig[4] = {0, 0, 0, 0}, a_rch[4] = {0, 0, 0, 0}, a_trv[4] = {0, 0, 0, 0};
     for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
         const uint64_t q0 = c * kSsChunk, q1 = min(q0 + kSsChunk, D);
         const uint32_t i0 = cstart[c];
@@ -503,75 +522,71 @@ __global__ __launch_bounds__(kBlock) void k_ss_relax(const uint32_t *__restrict_
         __syncthreads();  // the previous chunk's slice is no longer read
         for (uint32_t k = threadIdx.x; k <= nv; k += kBlock) s_off[k] = doff[i0 + k];
         __syncthreads();
-        uint32_t to[kSsGroupEdges], wt[kSsGroupEdges], sub[kSsGroupEdges];
-        uint4 du[kSsGroupEdges], dt[kSsGroupEdges];
-#pragma unroll
-        for (uint32_t e = 0; e < kSsGroupEdges; ++e) {
-            const uint64_t q = q0 + g + (uint64_t)e * (kBlock / 4);
-            to[e] = 0;
-            wt[e] = 0;
-            sub[e] = 0;
-            if (q < q1) {
-                uint32_t lo = 0, hi = nv;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_off[mid] <= q) lo = mid; else hi = mid;
-                }
-                const uint32_t u = alist[i0 + lo];
-                sub[e] = (amask[i0 + lo] >> (4 * l)) & 0xFu;
-                load_edge<W, PK>(edges, ptr[u] + (q - s_off[lo]), wbits, to[e], wt[e]);
-                if (sub[e]) du[e] = *(const uint4 *)(dist + (uint64_t)u * kSs + 4 * l);
+        const uint64_t w0 = q0 + (uint64_t)kSsRun * threadIdx.x;
+        uint32_t to[kSsRun], wt[kSsRun], slot[kSsRun], du[kSsRun], dt[kSsRun];
+        bool ok[kSsRun];
+        if (w0 < q1) {
+            uint32_t lo = 0, hi = nv;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_off[mid] <= w0) lo = mid; else hi = mid;
             }
-        }
+            uint32_t u = alist[i0 + lo], M = amask[i0 + lo];
+            uint32_t m = (uint32_t)__builtin_popcount(M);
+            uint64_t off = w0 - s_off[lo];
+            uint64_t e = off / m;
+            uint32_t mm = M;
+            for (uint32_t t = (uint32_t)(off - e * m); t; --t) mm &= mm - 1;
+            uint64_t rb = ptr[u], iend = s_off[lo + 1];
 #pragma unroll
-        for (uint32_t e = 0; e < kSsGroupEdges; ++e)
-            if (sub[e]) dt[e] = *(const uint4 *)(dist + (uint64_t)to[e] * kSs + 4 * l);
-#pragma unroll
-        for (uint32_t e = 0; e < kSsGroupEdges; ++e) {
-            uint32_t imp = 0;
-            if (sub[e]) {
-                const uint64_t key = (uint64_t)to[e] << 32;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    if (!((sub[e] >> b) & 1u)) continue;
-                    const uint64_t s64 = (uint64_t)quad_at(du[e], b) + wt[e];
-                    const uint32_t nd = s64 > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)s64;
-                    if (nd >= quad_at(dt[e], b)) continue;
-                    const uint32_t old = atomicMin(dist + (uint64_t)to[e] * kSs + 4 * l + b, nd);
-                    if (nd >= old) continue;
-                    imp |= 1u << (4 * l + b);
-                    a_dig[b] += srv_mix64(key | nd);
-                    if (old == OG_UNREACHED) {
-                        a_rch[b] += 1;
-                        a_trv[b] += ptr[to[e] + 1] - ptr[to[e]];
-                    } else {
-                        a_dig[b] -= srv_mix64(key | old);
-                    }
+            for (uint32_t r = 0; r < kSsRun; ++r) {
+                const uint64_t w = w0 + r;
+                ok[r] = w < q1;
+                if (!ok[r]) continue;
+                if (w >= iend) {  // the next entry's first item
+                    ++lo;
+                    u = alist[i0 + lo];
+                    M = amask[i0 + lo];
+                    mm = M;
+                    e = 0;
+                    rb = ptr[u];
+                    iend = s_off[lo + 1];
+                }
+                const uint32_t j = (uint32_t)__builtin_ctz(mm);
+                slot[r] = j;
+                load_edge<W, PK>(edges, rb + e, wbits, to[r], wt[r]);
+                du[r] = dist[(uint64_t)u * kSs + j];
+                mm &= mm - 1;
+                if (!mm) {
+                    mm = M;
+                    ++e;
                 }
             }
-            imp |= __shfl_xor(imp, 1, kWave);
-            imp |= __shfl_xor(imp, 2, kWave);
-            if (l == 0 && imp) atomicOr(pend + to[e], imp);
+#pragma unroll
+            for (uint32_t r = 0; r < kSsRun; ++r)
+                if (ok[r]) dt[r] = dist[(uint64_t)to[r] * kSs + slot[r]];
+#pragma unroll
+            for (uint32_t r = 0; r < kSsRun; ++r) {
+                if (!ok[r]) continue;
+                const uint64_t s64 = (uint64_t)du[r] + wt[r];
+                const uint32_t nd = s64 > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)s64;
+                if (nd >= dt[r]) continue;
+                const uint32_t j = slot[r];
+                const uint32_t old = atomicMin(dist + (uint64_t)to[r] * kSs + j, nd);
+                if (nd >= old) continue;
+                atomicOr(pend + to[r], 1u << j);
+                const uint64_t key = (uint64_t)to[r] << 32;
+                unsigned long long dd = srv_mix64(key | nd);
+                if (old == OG_UNREACHED) {
+                    atomicAdd(&s_acc[wv][kSs + j], 1ull);
+                    atomicAdd(&s_acc[wv][2 * kSs + j], (unsigned long long)(ptr[to[r] + 1] - ptr[to[r]]));
+                } else {
+                    dd -= srv_mix64(key | old);
+                }
+                atomicAdd(&s_acc[wv][j], dd);
+            }
         }
     }
-    // lanes with the same l hold the same four slots: sum over the wave's 16
-    // such lanes, then over the waves
-#pragma unroll
-    for (int m = 4; m < kWave; m <<= 1)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            a_dig[b] += shfl_xor_u64(a_dig[b], m);
-            a_rch[b] += shfl_xor_u64(a_rch[b], m);
-            a_trv[b] += shfl_xor_u64(a_trv[b], m);
-        }
-    const uint32_t w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    if (lane < 4)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            s_acc[w][4 * lane + b] = a_dig[b];
-            s_acc[w][kSs + 4 * lane + b] = a_rch[b];
-            s_acc[w][2 * kSs + 4 * lane + b] = a_trv[b];
-        }
     __syncthreads();
     if (threadIdx.x < 3 * kSs) {
         unsigned long long t = 0;
@@ -634,7 +649,7 @@ constexpr uint32_t kKeepBufs = 4;   // BFS: kept-levels buffers per graph (4 V b
 // the extraction buffer of kept results and the relax's chunk table.
 static uint64_t bfs_state_bytes(uint64_t V) { return 24 * V + kKeepBufs * 4 * V; }
 static uint64_t sssp_state_bytes(uint64_t V, uint64_t E) {
-    return 4ull * kSs * V + 4 * V + 8 * V + 8 * (V + 1) + 4 * V + 4 * (E / kSsChunk + 2);
+    return 4ull * kSs * V + 4 * V + 8 * V + 8 * (V + 1) + 4 * V + 4 * (kSs * E / kSsChunk + 2);
 }
 
 // What an engine thread marks when it fails: its queue stops taking queries.
@@ -954,7 +969,7 @@ void SsspEngine::setup() {
     amask = (uint32_t *)mem.dalloc(V * 4);
     doff = (uint64_t *)mem.dalloc((V + 1) * 8);
     extract = (uint32_t *)mem.dalloc(V * 4);
-    cstart = (uint32_t *)mem.dalloc((d->E / kSsChunk + 2) * 4);
+    cstart = (uint32_t *)mem.dalloc((kSs * d->E / kSsChunk + 2) * 4);  // items <= kSs per edge
     bnv = (uint32_t *)mem.dalloc((uint64_t)nblk * 4);
     bdeg = (uint64_t *)mem.dalloc((uint64_t)nblk * 8);
     bmask = (uint32_t *)mem.dalloc(2ull * nblk * 4);
@@ -1058,10 +1073,17 @@ void SsspEngine::step() {
     }
     if (!used) return;
     const uint64_t V = d->V;
-    if (sl.n) hipLaunchKernelGGL(k_ss_seed, dim3(1), dim3(64), 0, s, sl, dist, pend, d->d_ptr, acc, ctl);
+    const bool shared = (srv->params.flags & OG_SERVER_SSSP_SHARED_THRESHOLD) != 0;
+    if (sl.n) {
+        uint32_t running = used;
+        for (uint32_t i = 0; i < sl.n; ++i) running &= ~(1u << sl.slot[i]);
+        const int ref = shared && running ? __builtin_ctz(running) : -1;
+        hipLaunchKernelGGL(k_ss_seed, dim3(1), dim3(64), 0, s, sl, ref, dist, pend, d->d_ptr, acc, ctl);
+    }
     hipLaunchKernelGGL(k_ss_select, dim3(nblk), dim3(kBlock), 0, s, pend, dist, d->d_ptr, V, ctl, bnv, bdeg, bmask,
                        bmin, nblk);
-    hipLaunchKernelGGL(k_ss_scan, dim3(1), dim3(1024), 0, s, bnv, bdeg, bmask, bmin, nblk, used, ctl, doff);
+    hipLaunchKernelGGL(k_ss_scan, dim3(1), dim3(1024), 0, s, bnv, bdeg, bmask, bmin, nblk, used, shared ? 1u : 0u,
+                       ctl, doff);
     OG_HIP(hipGetLastError());
     OG_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(SsCtl), hipMemcpyDeviceToHost, s));
     OG_HIP(hipMemcpyAsync(h_acc, acc, 3 * kSs * 8, hipMemcpyDeviceToHost, s));
