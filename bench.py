@@ -593,7 +593,7 @@ def main():
     ap.add_argument("--verify", type=int, default=100, help="serve: served results re-run on the plain path")
     ap.add_argument("--serve-mix", default="0.7,1.0", help="serve: BFS shares of the query mixes to run")
     ap.add_argument("--p99-target-ms", type=float, default=50.0, help="serve: p99 latency target")
-    ap.add_argument("--serve-sssp-delta", type=int, default=32,
+    ap.add_argument("--serve-sssp-delta", type=int, default=64,
                     help="serve: bucket width of the server's batched SSSP slots (whole rows)")
     args = ap.parse_args()
     if args.scale is None:
