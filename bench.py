@@ -421,9 +421,9 @@ def wl_serve(args):
     service = sample_plain("diropt", og.PLAN_DIRECTION_OPT)
 
     # per-query state (og_server_create's admission rule): per graph the BFS
-    # engine (40 V bytes), per weighted graph the 16-slot SSSP engine (88 V + 8
+    # engine (40 V bytes), per weighted graph the 16-slot SSSP engine (92 V + 8
     # + the relax's chunk table)
-    full = sum(40 * g.num_vertices + (88 * g.num_vertices + 8 + 4 * (16 * g.num_edges // 1024 + 2)
+    full = sum(40 * g.num_vertices + (92 * g.num_vertices + 8 + 4 * (16 * g.num_edges // 1024 + 2)
                                       if g.info()["kind"] == og.KIND_WEIGHTED else 0) for g in graphs)
     budget = int(args.serve_budget_gib * (1 << 30)) if args.serve_budget_gib else full
 

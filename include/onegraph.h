@@ -393,9 +393,9 @@ og_status og_pagerank(og_graph *g, const og_pr_params *params, float *ranks_f32,
  * others run join at the next round and share its kernels.  SSSP queries are
  * batched the same way: each weighted graph has an SSSP engine with 16 query
  * slots whose distances lie side by side (one 64-byte line per vertex); each
- * slot runs near-far rounds with its own threshold (bucket width = the
- * graph's tuning delta) and a round's edge reads serve every slot that is
- * near at the source.  Results equal og_bfs / og_sssp on the same graph and
+ * slot runs near-far rounds with its own threshold (light/heavy rounds on a
+ * graph prepared with split_weight) and a round's edge reads serve every
+ * slot active at the source.  Results equal og_bfs / og_sssp on the same graph and
  * source. */
 
 #define OG_SERVER_MAX_GRAPHS 8
@@ -411,25 +411,20 @@ typedef struct og_server_params {
                                      BFS engine's masks and 4 kept-levels
                                      buffers (40 V bytes), per weighted graph
                                      the SSSP engine's 16 distance columns,
-                                     pending masks, active list and degree
-                                     prefix (88 V + 8 bytes) and chunk table
-                                     (4 (16 E / 1024 + 2) bytes);
+                                     light / heavy pending masks, active list
+                                     and work prefix (92 V + 8 bytes) and
+                                     chunk table (4 (16 E / 1024 + 2) bytes);
                                      0 = a quarter of free HBM                 */
     uint32_t max_pending;         /* queued queries before submit returns OG_EBUSY; 0 = 2^20 */
     uint32_t flags;               /* OG_SERVER_* bits                       */
     uint32_t sssp_delta;          /* bucket width of the SSSP slots' near-far
                                      rounds; 0 = the graph's tuning delta.
-                                     The slots relax whole rows (no
-                                     light/heavy split), for which 16-64 does
-                                     the least work with weights in [1, 63] */
+                                     On graphs prepared with split_weight the
+                                     slots run light/heavy rounds (wide
+                                     buckets pay: 256 with weights in [1, 63]),
+                                     else they relax whole rows (16-64)       */
     uint32_t reserved;
 } og_server_params;
-
-/* SSSP slots share one near-far threshold: it moves (to the smallest far
- * value of any slot + delta) only when no slot has anything near, so slots
- * relax the same distance band together and share more edge reads; a slot
- * admitted mid-flight starts at the running threshold. */
-#define OG_SERVER_SSSP_SHARED_THRESHOLD 1u
 
 /* Testing only: the server's SSSP engines stop at their first round with
  * OG_EHIP, as after a device error, without touching the GPU (the failure

@@ -89,7 +89,6 @@ class PrStats(C.Structure):
 SERVER_MAX_GRAPHS = 8
 QUERY_BFS, QUERY_SSSP = 0, 1
 QUERY_KEEP_VALUES = 1
-SERVER_SSSP_SHARED_THRESHOLD = 1  # og.h OG_SERVER_SSSP_SHARED_THRESHOLD
 SERVER_TEST_FAIL_SSSP = 0x80000000  # testing: SSSP engines stop at their first round (og.h OG_SERVER_TEST_FAIL_SSSP)
 
 

@@ -224,18 +224,24 @@ __global__ void k_srv_zero_acc(uint64_t done, unsigned long long *acc) {
 // slot runs near-far SSSP with its own threshold thresh[j]: a round relaxes
 // the pending (v, j) with dist < thresh[j]; when a slot has nothing near
 // left but far entries, its threshold moves to (smallest far value + delta)
-// for the next round.  The slots of a round share the edge reads: an active
-// vertex carries the mask of its near slots, and one pass over its out-edges
-// offers every near slot's distance at once.  Rounds:
-//   select  one sweep over pend: per pending vertex, the near mask from its
-//           distance line; per-block counts, masks and per-slot far minima;
-//   scan    one workgroup: list offsets, the slots' near / far / done masks
-//           and next thresholds, the summary the host reads;
-//   write   the active list (vertex, near mask) and its degree prefix;
-//   relax   edge-parallel over the active list: four lanes per edge, lane l
-//           covering slots 4l..4l+3 (one 16-byte piece of the target's
-//           line), atomicMin per improved slot, one atomicOr of the improved
-//           slots into the target's pending mask.
+// for the next round.  On a graph stored with the light/heavy split (rows
+// list edges of weight < split_weight first, pmid[v] = start of the heavy
+// ones; delta-stepping, Meyer & Sanders) a near (v, j) relaxes only its light
+// edges and turns heavy-pending (hpend[v] bit j); once slot j has nothing
+// near in its bucket, one heavy round relaxes the heavy edges of all its
+// heavy-pending vertices, then its light rounds go on.  The slots of a round
+// share the edge reads: an active vertex carries the masks of its light and
+// heavy slots, and one pass over its edges offers every such slot.  Rounds:
+//   select  one sweep over the pending masks: per pending vertex, the near
+//           mask from its distance line (light-mode slots) and the
+//           heavy-pending slots of heavy-mode slots; per-block counts,
+//           masks and per-slot far minima;
+//   scan    one workgroup: list offsets, each slot's next mode (light,
+//           heavy, threshold move, done), the summary the host reads;
+//   write   the active list (vertex, light | heavy << 16 slot masks), the
+//           prefix of its work items and the chunk table of the relax;
+//   relax   one lane per (edge, slot) work item, atomicMin per improved
+//           slot, one atomicOr of it into the target's pending mask.
 // Results need no extra pass: every successful atomicMin of (v, j) from old
 // to new adds mix64(v << 32 | new) - mix64(v << 32 | old) (old = unreached
 // adds nothing and counts v as reached, with its out-degree as traversed) to
@@ -254,8 +260,10 @@ struct SsCtl {
     uint32_t thresh[kSs];  // near-far threshold of each slot (0xFFFFFFFF: every pending entry is near)
     uint32_t delta;        // bucket width (0: no near-far order)
     uint32_t nactive;      // vertices in this round's active list
-    uint64_t dsum;         // their degree sum (the relax's edge space)
-    uint32_t near, far;    // slots with near / far pending entries after the select sweep
+    uint64_t dsum;         // their work items (the relax's item space)
+    uint32_t near, far;    // slots with near / far light-pending entries after the select sweep
+    uint32_t hp;           // slots with heavy-pending entries after the select sweep
+    uint32_t heavy;        // slots in a heavy round (the select sweep reads it; the scan sets the next)
     uint32_t done;         // slots in use with nothing pending: finished
     uint32_t rounds;       // select sweeps so far
 };
@@ -266,60 +274,75 @@ struct SsSeeds {
     uint32_t src[kSs];
 };
 
-// New queries: slot j starts at its source (value 1, pending), its sums at
-// the source's own terms, its threshold at 1 + delta.
-__global__ void k_ss_seed(SsSeeds sl, int ref, uint32_t *dist, uint32_t *pend, const uint64_t *__restrict__ ptr,
+// New queries: slot j starts at its source (value 1, pending, light mode),
+// its sums at the source's own terms, its threshold at 1 + delta.
+__global__ void k_ss_seed(SsSeeds sl, uint32_t *dist, uint32_t *pend, const uint64_t *__restrict__ ptr,
                           unsigned long long *acc, SsCtl *ctl) {
     const uint32_t i = threadIdx.x;
     if (i >= sl.n) return;
-    // shared thresholds: a slot joining running ones takes their threshold
-    // (its source, at 1, is below it); ref < 0 otherwise
-    const uint32_t tref = ref >= 0 ? ctl->thresh[ref] : 0u;
     const uint32_t j = sl.slot[i], v = sl.src[i];
     dist[(uint64_t)v * kSs + j] = 1;
     atomicOr(pend + v, 1u << j);
+    atomicAnd(&ctl->heavy, ~(1u << j));
     acc[j] = srv_mix64(((uint64_t)v << 32) | 1u);
     acc[kSs + j] = 1;
     acc[2 * kSs + j] = ptr[v + 1] - ptr[v];
     const uint32_t dl = ctl->delta;
-    const uint32_t t0 = dl ? (uint32_t)min((uint64_t)0xFFFFFFFFull, 1ull + dl) : 0xFFFFFFFFu;
-    ctl->thresh[j] = max(t0, tref);
+    ctl->thresh[j] = dl ? (uint32_t)min((uint64_t)0xFFFFFFFFull, 1ull + dl) : 0xFFFFFFFFu;
 }
 
 __device__ __forceinline__ uint32_t quad_at(const uint4 &x, int b) {
     return b == 0 ? x.x : b == 1 ? x.y : b == 2 ? x.z : x.w;
 }
 
-// Select sweep: near mask of every pending vertex, pend[v] = near << 16 |
-// far (near listed only for vertices with out-edges); per block the count and degree sum of vertices with a near slot, the
-// OR of near and far masks, and per slot the smallest far value (LDS
-// atomicMin, tried only when it would lower the block's minimum).
-__global__ __launch_bounds__(kBlock) void k_ss_select(uint32_t *__restrict__ pend, const uint32_t *__restrict__ dist,
-                                                      const uint64_t *__restrict__ ptr, uint64_t V,
+// Light and heavy edge counts of row v (no split: the whole row is light).
+__device__ __forceinline__ void row_parts(const uint64_t *__restrict__ ptr, const uint64_t *__restrict__ pmid,
+                                          uint64_t v, uint64_t &ld, uint64_t &hd) {
+    const uint64_t b = ptr[v], e = ptr[v + 1];
+    const uint64_t m = pmid ? pmid[v] : e;
+    ld = m - b;
+    hd = e - m;
+}
+
+// Select sweep.  Per pending vertex: L = light-mode slots with a pending
+// entry below their threshold (their far entries stay pending; the block's
+// minimum far value per slot is kept with LDS atomicMin, tried only when it
+// would lower it), H = heavy-pending slots of heavy-mode slots.  Selected
+// slots move to the upper halves (pend = L << 16 | far, hpend = H << 16 |
+// still heavy-pending) when the vertex has edges of that kind; near (v, j)
+// turn heavy-pending when v has heavy edges.  Per block: listed vertices,
+// their work items, and the OR of near / far / heavy-pending masks.
+__global__ __launch_bounds__(kBlock) void k_ss_select(uint32_t *__restrict__ pend, uint32_t *__restrict__ hpend,
+                                                      const uint32_t *__restrict__ dist,
+                                                      const uint64_t *__restrict__ ptr,
+                                                      const uint64_t *__restrict__ pmid, uint64_t V,
                                                       const SsCtl *__restrict__ ctl, uint32_t *__restrict__ bnv,
                                                       uint64_t *__restrict__ bdeg, uint32_t *__restrict__ bmask,
                                                       uint32_t *__restrict__ bmin, uint32_t nblk) {
     __shared__ uint32_t s_thr[kSs], s_min[kSs];
     __shared__ uint64_t lds[2 * kBlock / kWave];
-    __shared__ uint32_t s_or[2];
+    __shared__ uint32_t s_or[3];
     if (threadIdx.x < kSs) {
         s_thr[threadIdx.x] = ctl->thresh[threadIdx.x];
         s_min[threadIdx.x] = 0xFFFFFFFFu;
     }
-    if (threadIdx.x < 2) s_or[threadIdx.x] = 0;
+    if (threadIdx.x < 3) s_or[threadIdx.x] = 0;
     __syncthreads();
+    const uint32_t heavy = ctl->heavy;
     const uint64_t base = (uint64_t)blockIdx.x * kSsWin + threadIdx.x;
-    uint32_t cnt = 0, nm = 0, fm = 0;
-    uint64_t deg = 0;
+    uint32_t cnt = 0, nm = 0, fm = 0, hm = 0;
+    uint64_t items = 0;
     for (uint32_t k = 0; k < kSsPer; ++k) {
         const uint64_t v = base + (uint64_t)k * kBlock;
         if (v >= V) break;
         const uint32_t p = pend[v] & 0xFFFFu;
-        if (!p) continue;
+        const uint32_t h = pmid ? hpend[v] & 0xFFFFu : 0u;
+        if (!(p | h)) continue;
+        const uint32_t pl = p & ~heavy;  // light candidates (heavy-mode slots keep theirs pending)
         uint32_t near = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const uint32_t qb = (p >> (4 * q)) & 0xFu;
+            const uint32_t qb = (pl >> (4 * q)) & 0xFu;
             if (!qb) continue;
             const uint4 x = *(const uint4 *)(dist + v * kSs + 4 * q);
 #pragma unroll
@@ -332,26 +355,35 @@ __global__ __launch_bounds__(kBlock) void k_ss_select(uint32_t *__restrict__ pen
             }
         }
         const uint32_t far = p & ~near;
-        // a near vertex without out-edges has nothing to relax: its near
-        // entries are consumed here and it is not listed (so every listed
-        // vertex owns at least one edge of the relax's edge space)
-        const uint64_t dv = near ? ptr[v + 1] - ptr[v] : 0;
-        pend[v] = (dv ? near << 16 : 0u) | far;
+        const uint32_t H = h & heavy;
+        uint64_t ld = 0, hd = 0;
+        if (near | H) row_parts(ptr, pmid, v, ld, hd);
+        // slots selected with no edges of their kind have nothing to relax:
+        // their entries are consumed here and do not enter the list (so
+        // every listed vertex owns at least one work item)
+        const uint32_t L = ld ? near : 0u, Hs = hd ? H : 0u;
+        pend[v] = (L << 16) | far;
+        const uint32_t hrest = (h & ~H) | (hd ? near : 0u);
+        if (pmid) hpend[v] = (Hs << 16) | hrest;
         nm |= near;
         fm |= far;
-        if (dv) {
+        hm |= hrest;
+        const uint64_t it = ld * (uint32_t)__builtin_popcount(L) + hd * (uint32_t)__builtin_popcount(Hs);
+        if (it) {
             ++cnt;
-            deg += dv * (uint32_t)__builtin_popcount(near);  // work items: (edge, near slot) pairs
+            items += it;
         }
     }
-    const uint64_t x[2] = {cnt, deg};
+    const uint64_t x[2] = {cnt, items};
     uint64_t ex[2], t[2];
     block_scan_n<2>(x, ex, t, lds);
     nm = (uint32_t)wave_or(nm);
     fm = (uint32_t)wave_or(fm);
+    hm = (uint32_t)wave_or(hm);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         if (nm) atomicOr(&s_or[0], nm);
         if (fm) atomicOr(&s_or[1], fm);
+        if (hm) atomicOr(&s_or[2], hm);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -359,32 +391,36 @@ __global__ __launch_bounds__(kBlock) void k_ss_select(uint32_t *__restrict__ pen
         bdeg[blockIdx.x] = t[1];
         bmask[blockIdx.x] = s_or[0];
         bmask[nblk + blockIdx.x] = s_or[1];
+        bmask[2 * nblk + blockIdx.x] = s_or[2];
     }
     if (threadIdx.x < kSs) bmin[(uint64_t)threadIdx.x * nblk + blockIdx.x] = s_min[threadIdx.x];
 }
 
-// Scan: exclusive offsets of the blocks' list counts and degree sums (in
-// place), the slots' near / far masks, the threshold moves of slots with far
-// entries only, the done mask of the slots in `used`, and the list's total
-// degree at doff[nactive].
+// Scan: exclusive offsets of the blocks' list counts and work items (in
+// place), and each slot's next round: after a heavy round, light; with
+// anything near, light again; else a heavy round when it has heavy-pending
+// vertices; else, with far entries only, its threshold moves to the
+// smallest far value + delta; with nothing at all it is done.  Publishes
+// the list's total work at doff[nactive].
 __global__ __launch_bounds__(1024) void k_ss_scan(uint32_t *__restrict__ bnv, uint64_t *__restrict__ bdeg,
                                                   const uint32_t *__restrict__ bmask, const uint32_t *__restrict__ bmin,
-                                                  uint32_t nblk, uint32_t used, uint32_t shared, SsCtl *__restrict__ ctl,
+                                                  uint32_t nblk, uint32_t used, SsCtl *__restrict__ ctl,
                                                   uint64_t *__restrict__ doff) {
     __shared__ uint64_t lds[2 * 1024 / kWave];
-    __shared__ uint32_t s_or[2], s_min[kSs];
-    if (threadIdx.x < 2) s_or[threadIdx.x] = 0;
+    __shared__ uint32_t s_or[3], s_min[kSs];
+    if (threadIdx.x < 3) s_or[threadIdx.x] = 0;
     if (threadIdx.x < kSs) s_min[threadIdx.x] = 0xFFFFFFFFu;
     __syncthreads();
     const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
     const uint32_t b0 = min(nblk, threadIdx.x * per), b1 = min(nblk, b0 + per);
     uint64_t c = 0, dg = 0;
-    uint32_t nm = 0, fm = 0;
+    uint32_t nm = 0, fm = 0, hm = 0;
     for (uint32_t b = b0; b < b1; ++b) {
         c += bnv[b];
         dg += bdeg[b];
         nm |= bmask[b];
         fm |= bmask[nblk + b];
+        hm |= bmask[2 * nblk + b];
     }
     const uint64_t x[2] = {c, dg};
     uint64_t ex[2], t[2];
@@ -400,15 +436,18 @@ __global__ __launch_bounds__(1024) void k_ss_scan(uint32_t *__restrict__ bnv, ui
     }
     nm = (uint32_t)wave_or(nm);
     fm = (uint32_t)wave_or(fm);
+    hm = (uint32_t)wave_or(hm);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         if (nm) atomicOr(&s_or[0], nm);
         if (fm) atomicOr(&s_or[1], fm);
+        if (hm) atomicOr(&s_or[2], hm);
     }
     __syncthreads();
-    const uint32_t near = s_or[0], far = s_or[1];
-    // slots whose threshold moves for the next round: each slot on its own,
-    // or (shared) every slot at once when no slot has anything near
-    const uint32_t moving = shared ? (near ? 0u : far) : far & ~near;
+    const uint32_t near = s_or[0], far = s_or[1], hp = s_or[2];
+    const uint32_t was_heavy = ctl->heavy;
+    const uint32_t light = ~was_heavy;
+    const uint32_t next_heavy = light & ~near & hp;
+    const uint32_t moving = light & ~near & ~hp & far;  // threshold moves for the next round
     for (uint32_t m = moving; m;) {
         const int j = __builtin_ctz(m);
         m &= m - 1;
@@ -423,52 +462,51 @@ __global__ __launch_bounds__(1024) void k_ss_scan(uint32_t *__restrict__ bnv, ui
         cl.dsum = t[1];
         cl.near = near;
         cl.far = far;
-        cl.done = used & ~(near | far);
+        cl.hp = hp;
+        cl.done = used & light & ~(near | far | hp);
+        cl.heavy = next_heavy;
         cl.rounds++;
-        if (shared && moving) {
-            uint32_t mn = 0xFFFFFFFFu;
-            for (uint32_t m = moving; m;) {
-                const int j = __builtin_ctz(m);
-                m &= m - 1;
-                mn = min(mn, s_min[j]);
-            }
-            const uint32_t T = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)mn + cl.delta);
-            for (uint32_t j = 0; j < kSs; ++j) cl.thresh[j] = T;
-        } else {
-            for (uint32_t m = moving; m;) {
-                const int j = __builtin_ctz(m);
-                m &= m - 1;
-                cl.thresh[j] = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)s_min[j] + cl.delta);
-            }
+        for (uint32_t m = moving; m;) {
+            const int j = __builtin_ctz(m);
+            m &= m - 1;
+            cl.thresh[j] = (uint32_t)min((uint64_t)0xFFFFFFFFull, (uint64_t)s_min[j] + cl.delta);
         }
         *ctl = cl;
         doff[t[0]] = t[1];
     }
 }
 
-// Write sweep: the active list in block order (vertex, near mask), the prefix
-// of its work items (an entry has degree x near slots of them), the list
-// entry holding the first item of every relax chunk (cstart[c], chunks of
-// kSsChunk items), and pend[v] back to the far slots.
-__global__ __launch_bounds__(kBlock) void k_ss_write(uint32_t *__restrict__ pend, const uint64_t *__restrict__ ptr,
-                                                     uint64_t V, const uint32_t *__restrict__ bnv,
+// Write sweep: the active list in block order (vertex, light | heavy << 16
+// slot masks), the prefix of its work items (light edges x light slots, then
+// heavy edges x heavy slots), the list entry holding the first item of every
+// relax chunk (cstart[c], chunks of kSsChunk items), and the pending masks
+// back to their lower halves.
+__global__ __launch_bounds__(kBlock) void k_ss_write(uint32_t *__restrict__ pend, uint32_t *__restrict__ hpend,
+                                                     const uint64_t *__restrict__ ptr,
+                                                     const uint64_t *__restrict__ pmid, uint64_t V,
+                                                     const uint32_t *__restrict__ bnv,
                                                      const uint64_t *__restrict__ bdeg, uint32_t *__restrict__ alist,
                                                      uint32_t *__restrict__ amask, uint64_t *__restrict__ doff,
                                                      uint32_t *__restrict__ cstart) {
     __shared__ uint64_t lds[2 * kBlock / kWave];
     const uint64_t base = (uint64_t)blockIdx.x * kSsWin + threadIdx.x;
+    auto items_of = [&](uint64_t v, uint32_t L, uint32_t H) -> uint64_t {
+        uint64_t ld, hd;
+        row_parts(ptr, pmid, v, ld, hd);
+        return ld * (uint32_t)__builtin_popcount(L) + hd * (uint32_t)__builtin_popcount(H);
+    };
     uint32_t cnt = 0;
-    uint64_t deg = 0;
+    uint64_t items = 0;
     for (uint32_t k = 0; k < kSsPer; ++k) {
         const uint64_t v = base + (uint64_t)k * kBlock;
         if (v >= V) break;
-        const uint32_t near = pend[v] >> 16;
-        if (near) {
+        const uint32_t L = pend[v] >> 16, H = pmid ? hpend[v] >> 16 : 0u;
+        if (L | H) {
             ++cnt;
-            deg += (ptr[v + 1] - ptr[v]) * (uint32_t)__builtin_popcount(near);
+            items += items_of(v, L, H);
         }
     }
-    const uint64_t x[2] = {cnt, deg};
+    const uint64_t x[2] = {cnt, items};
     uint64_t ex[2], t[2];
     block_scan_n<2>(x, ex, t, lds);
     if (!t[0]) return;
@@ -476,25 +514,28 @@ __global__ __launch_bounds__(kBlock) void k_ss_write(uint32_t *__restrict__ pend
     for (uint32_t k = 0; k < kSsPer && cnt; ++k) {
         const uint64_t v = base + (uint64_t)k * kBlock;
         if (v >= V) break;
-        const uint32_t p = pend[v];
-        if (!(p >> 16)) continue;
-        const uint64_t dv = (ptr[v + 1] - ptr[v]) * (uint32_t)__builtin_popcount(p >> 16);  // >= 1 item
+        const uint32_t p = pend[v], hp = pmid ? hpend[v] : 0u;
+        const uint32_t L = p >> 16, H = hp >> 16;
+        if (!(L | H)) continue;
+        const uint64_t it = items_of(v, L, H);  // >= 1
         alist[pos] = (uint32_t)v;
-        amask[pos] = p >> 16;
+        amask[pos] = L | (H << 16);
         doff[pos] = dpos;
-        // chunks of the relax whose first edge is one of this vertex's
-        for (uint64_t c = (dpos + kSsChunk - 1) / kSsChunk; c * kSsChunk < dpos + dv; ++c) cstart[c] = (uint32_t)pos;
+        // chunks of the relax whose first item is one of this vertex's
+        for (uint64_t c = (dpos + kSsChunk - 1) / kSsChunk; c * kSsChunk < dpos + it; ++c) cstart[c] = (uint32_t)pos;
         ++pos;
-        dpos += dv;
-        pend[v] = p & 0xFFFFu;
+        dpos += it;
+        if (L) pend[v] = p & 0xFFFFu;
+        if (H) hpend[v] = hp & 0xFFFFu;
         --cnt;
     }
 }
 
-// Relax: one work item per (edge, near slot) of the active list, an entry's
-// items edge-major (item e * m + k: edge e, k-th of its m near slots), so
+// Relax: one work item per (edge, slot) of the active list.  An entry's
+// items come light part first (edge e of the light edges with the k-th of
+// its light slots at item e * mL + k), then the heavy part likewise, so
 // neighbouring lanes offer the slots of one edge to one 64-byte target line
-// and no lane idles for a slot that is not near.  Workgroups stride over
+// and no lane idles for a slot that is not active.  Workgroups stride over
 // chunks of kSsChunk items; a chunk's entries start at cstart[c] and its
 // slice of the item prefix is staged in LDS.  Each lane takes kSsRun
 // consecutive items: one binary search, then it walks on.  Result terms are
@@ -504,7 +545,8 @@ __global__ __launch_bounds__(kBlock) void k_ss_relax(const uint32_t *__restrict_
                                                      const uint32_t *__restrict__ amask,
                                                      const uint64_t *__restrict__ doff,
                                                      const uint32_t *__restrict__ cstart, const SsCtl *__restrict__ ctl,
-                                                     const uint64_t *__restrict__ ptr, const void *__restrict__ edges,
+                                                     const uint64_t *__restrict__ ptr,
+                                                     const uint64_t *__restrict__ pmid, const void *__restrict__ edges,
                                                      uint32_t wbits, uint32_t *dist, uint32_t *pend,
                                                      unsigned long long *acc) {
     __shared__ uint64_t s_off[kSsChunk + 2];
@@ -523,68 +565,94 @@ __global__ __launch_bounds__(kBlock) void k_ss_relax(const uint32_t *__restrict_
         for (uint32_t k = threadIdx.x; k <= nv; k += kBlock) s_off[k] = doff[i0 + k];
         __syncthreads();
         const uint64_t w0 = q0 + (uint64_t)kSsRun * threadIdx.x;
+        if (w0 >= q1) continue;
         uint32_t to[kSsRun], wt[kSsRun], slot[kSsRun], du[kSsRun], dt[kSsRun];
         bool ok[kSsRun];
-        if (w0 < q1) {
-            uint32_t lo = 0, hi = nv;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_off[mid] <= w0) lo = mid; else hi = mid;
+        uint32_t lo = 0, hi = nv;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_off[mid] <= w0) lo = mid; else hi = mid;
+        }
+        // walking state: the current part (light or heavy) of entry lo
+        uint32_t u = 0, ms = 0, M = 0, mm = 0;
+        uint64_t rb = 0, e = 0, pend_item = 0, lpart = 0;
+        bool hv = false;
+        auto enter = [&](uint32_t idx) {  // entry idx; the light part if it has items, else the heavy part
+            u = alist[i0 + idx];
+            ms = amask[i0 + idx];
+            uint64_t ld, hd;
+            row_parts(ptr, pmid, u, ld, hd);
+            lpart = ld * (uint32_t)__builtin_popcount(ms & 0xFFFFu);
+            hv = lpart == 0;
+            M = hv ? ms >> 16 : ms & 0xFFFFu;
+            rb = hv ? pmid[u] : ptr[u];
+            pend_item = hv ? s_off[idx + 1] : s_off[idx] + lpart;
+        };
+        enter(lo);
+        {
+            uint64_t off = w0 - (hv ? s_off[lo] : s_off[lo]);
+            if (!hv && off >= lpart) {  // w0 lies in the heavy part
+                off -= lpart;
+                hv = true;
+                M = ms >> 16;
+                rb = pmid[u];
+                pend_item = s_off[lo + 1];
             }
-            uint32_t u = alist[i0 + lo], M = amask[i0 + lo];
-            uint32_t m = (uint32_t)__builtin_popcount(M);
-            uint64_t off = w0 - s_off[lo];
-            uint64_t e = off / m;
-            uint32_t mm = M;
+            const uint32_t m = (uint32_t)__builtin_popcount(M);
+            e = off / m;
+            mm = M;
             for (uint32_t t = (uint32_t)(off - e * m); t; --t) mm &= mm - 1;
-            uint64_t rb = ptr[u], iend = s_off[lo + 1];
+        }
 #pragma unroll
-            for (uint32_t r = 0; r < kSsRun; ++r) {
-                const uint64_t w = w0 + r;
-                ok[r] = w < q1;
-                if (!ok[r]) continue;
-                if (w >= iend) {  // the next entry's first item
-                    ++lo;
-                    u = alist[i0 + lo];
-                    M = amask[i0 + lo];
-                    mm = M;
-                    e = 0;
-                    rb = ptr[u];
-                    iend = s_off[lo + 1];
-                }
-                const uint32_t j = (uint32_t)__builtin_ctz(mm);
-                slot[r] = j;
-                load_edge<W, PK>(edges, rb + e, wbits, to[r], wt[r]);
-                du[r] = dist[(uint64_t)u * kSs + j];
-                mm &= mm - 1;
-                if (!mm) {
-                    mm = M;
-                    ++e;
-                }
-            }
-#pragma unroll
-            for (uint32_t r = 0; r < kSsRun; ++r)
-                if (ok[r]) dt[r] = dist[(uint64_t)to[r] * kSs + slot[r]];
-#pragma unroll
-            for (uint32_t r = 0; r < kSsRun; ++r) {
-                if (!ok[r]) continue;
-                const uint64_t s64 = (uint64_t)du[r] + wt[r];
-                const uint32_t nd = s64 > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)s64;
-                if (nd >= dt[r]) continue;
-                const uint32_t j = slot[r];
-                const uint32_t old = atomicMin(dist + (uint64_t)to[r] * kSs + j, nd);
-                if (nd >= old) continue;
-                atomicOr(pend + to[r], 1u << j);
-                const uint64_t key = (uint64_t)to[r] << 32;
-                unsigned long long dd = srv_mix64(key | nd);
-                if (old == OG_UNREACHED) {
-                    atomicAdd(&s_acc[wv][kSs + j], 1ull);
-                    atomicAdd(&s_acc[wv][2 * kSs + j], (unsigned long long)(ptr[to[r] + 1] - ptr[to[r]]));
+        for (uint32_t r = 0; r < kSsRun; ++r) {
+            const uint64_t w = w0 + r;
+            ok[r] = w < q1;
+            if (!ok[r]) continue;
+            if (w >= pend_item) {  // the next part: this entry's heavy part, or the next entry
+                if (!hv && pend_item < s_off[lo + 1]) {
+                    hv = true;
+                    M = ms >> 16;
+                    rb = pmid[u];
+                    pend_item = s_off[lo + 1];
                 } else {
-                    dd -= srv_mix64(key | old);
+                    ++lo;
+                    enter(lo);
                 }
-                atomicAdd(&s_acc[wv][j], dd);
+                mm = M;
+                e = 0;
             }
+            const uint32_t j = (uint32_t)__builtin_ctz(mm);
+            slot[r] = j;
+            load_edge<W, PK>(edges, rb + e, wbits, to[r], wt[r]);
+            du[r] = dist[(uint64_t)u * kSs + j];
+            mm &= mm - 1;
+            if (!mm) {
+                mm = M;
+                ++e;
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kSsRun; ++r)
+            if (ok[r]) dt[r] = dist[(uint64_t)to[r] * kSs + slot[r]];
+#pragma unroll
+        for (uint32_t r = 0; r < kSsRun; ++r) {
+            if (!ok[r]) continue;
+            const uint64_t s64 = (uint64_t)du[r] + wt[r];
+            const uint32_t nd = s64 > 0xFFFFFFFEull ? 0xFFFFFFFEu : (uint32_t)s64;
+            if (nd >= dt[r]) continue;
+            const uint32_t j = slot[r];
+            const uint32_t old = atomicMin(dist + (uint64_t)to[r] * kSs + j, nd);
+            if (nd >= old) continue;
+            atomicOr(pend + to[r], 1u << j);
+            const uint64_t key = (uint64_t)to[r] << 32;
+            unsigned long long dd = srv_mix64(key | nd);
+            if (old == OG_UNREACHED) {
+                atomicAdd(&s_acc[wv][kSs + j], 1ull);
+                atomicAdd(&s_acc[wv][2 * kSs + j], (unsigned long long)(ptr[to[r] + 1] - ptr[to[r]]));
+            } else {
+                dd -= srv_mix64(key | old);
+            }
+            atomicAdd(&s_acc[wv][j], dd);
         }
     }
     __syncthreads();
@@ -645,11 +713,12 @@ constexpr uint32_t kKeepBufs = 4;   // BFS: kept-levels buffers per graph (4 V b
 
 // Device bytes of per-query state the admission rule charges: the BFS
 // engine's masks and kept-levels buffers, and the SSSP engine's distance
-// lines (4 kSs V), pending masks, active list with masks and degree prefix,
+// lines (4 kSs V), light and heavy pending masks, active list with masks and
+// work prefix,
 // the extraction buffer of kept results and the relax's chunk table.
 static uint64_t bfs_state_bytes(uint64_t V) { return 24 * V + kKeepBufs * 4 * V; }
 static uint64_t sssp_state_bytes(uint64_t V, uint64_t E) {
-    return 4ull * kSs * V + 4 * V + 8 * V + 8 * (V + 1) + 4 * V + 4 * (kSs * E / kSsChunk + 2);
+    return 4ull * kSs * V + 8 * V + 8 * V + 8 * (V + 1) + 4 * V + 4 * (kSs * E / kSsChunk + 2);
 }
 
 // What an engine thread marks when it fails: its queue stops taking queries.
@@ -714,8 +783,8 @@ struct SsspEngine {
     MemPool mem;
     hipStream_t s = nullptr;
     hipEvent_t ev_summary = nullptr;
-    uint32_t *dist = nullptr, *pend = nullptr, *alist = nullptr, *amask = nullptr, *extract = nullptr;
-    uint32_t *cstart = nullptr;
+    uint32_t *dist = nullptr, *pend = nullptr, *hpend = nullptr, *alist = nullptr, *amask = nullptr;
+    uint32_t *extract = nullptr, *cstart = nullptr;
     uint64_t *doff = nullptr;
     uint32_t *bnv = nullptr, *bmask = nullptr, *bmin = nullptr;
     uint64_t *bdeg = nullptr;
@@ -965,6 +1034,7 @@ void SsspEngine::setup() {
     OG_HIP(hipEventCreateWithFlags(&ev_summary, hipEventDisableTiming));
     dist = (uint32_t *)mem.dalloc(V * kSs * 4);
     pend = (uint32_t *)mem.dalloc(V * 4);
+    hpend = (uint32_t *)mem.dalloc(V * 4);
     alist = (uint32_t *)mem.dalloc(V * 4);
     amask = (uint32_t *)mem.dalloc(V * 4);
     doff = (uint64_t *)mem.dalloc((V + 1) * 8);
@@ -980,6 +1050,7 @@ void SsspEngine::setup() {
     h_acc = (unsigned long long *)mem.palloc(3 * kSs * 8);
     OG_HIP(hipMemset(dist, 0xFF, V * kSs * 4));
     OG_HIP(hipMemset(pend, 0, V * 4));
+    OG_HIP(hipMemset(hpend, 0, V * 4));
     OG_HIP(hipMemset(acc, 0, 3 * kSs * 8));
     SsCtl c{};
     c.delta = srv->params.sssp_delta ? srv->params.sssp_delta : d->tune.delta;
@@ -1073,29 +1144,23 @@ void SsspEngine::step() {
     }
     if (!used) return;
     const uint64_t V = d->V;
-    const bool shared = (srv->params.flags & OG_SERVER_SSSP_SHARED_THRESHOLD) != 0;
-    if (sl.n) {
-        uint32_t running = used;
-        for (uint32_t i = 0; i < sl.n; ++i) running &= ~(1u << sl.slot[i]);
-        const int ref = shared && running ? __builtin_ctz(running) : -1;
-        hipLaunchKernelGGL(k_ss_seed, dim3(1), dim3(64), 0, s, sl, ref, dist, pend, d->d_ptr, acc, ctl);
-    }
-    hipLaunchKernelGGL(k_ss_select, dim3(nblk), dim3(kBlock), 0, s, pend, dist, d->d_ptr, V, ctl, bnv, bdeg, bmask,
-                       bmin, nblk);
-    hipLaunchKernelGGL(k_ss_scan, dim3(1), dim3(1024), 0, s, bnv, bdeg, bmask, bmin, nblk, used, shared ? 1u : 0u,
-                       ctl, doff);
+    const uint64_t *pmid = d->d_pmid;  // rows stored light edges first: light / heavy rounds
+    if (sl.n) hipLaunchKernelGGL(k_ss_seed, dim3(1), dim3(64), 0, s, sl, dist, pend, d->d_ptr, acc, ctl);
+    hipLaunchKernelGGL(k_ss_select, dim3(nblk), dim3(kBlock), 0, s, pend, hpend, dist, d->d_ptr, pmid, V, ctl, bnv,
+                       bdeg, bmask, bmin, nblk);
+    hipLaunchKernelGGL(k_ss_scan, dim3(1), dim3(1024), 0, s, bnv, bdeg, bmask, bmin, nblk, used, ctl, doff);
     OG_HIP(hipGetLastError());
     OG_HIP(hipMemcpyAsync(h_ctl, ctl, sizeof(SsCtl), hipMemcpyDeviceToHost, s));
     OG_HIP(hipMemcpyAsync(h_acc, acc, 3 * kSs * 8, hipMemcpyDeviceToHost, s));
     OG_HIP(hipEventRecord(ev_summary, s));
-    hipLaunchKernelGGL(k_ss_write, dim3(nblk), dim3(kBlock), 0, s, pend, d->d_ptr, V, bnv, bdeg, alist, amask, doff,
-                       cstart);
+    hipLaunchKernelGGL(k_ss_write, dim3(nblk), dim3(kBlock), 0, s, pend, hpend, d->d_ptr, pmid, V, bnv, bdeg, alist,
+                       amask, doff, cstart);
     if (d->packed)
         hipLaunchKernelGGL((k_ss_relax<true, true>), dim3(kSsRelaxGrid), dim3(kBlock), 0, s, alist, amask, doff,
-                           cstart, ctl, d->d_ptr, (const void *)d->d_static, d->wbits, dist, pend, acc);
+                           cstart, ctl, d->d_ptr, pmid, (const void *)d->d_static, d->wbits, dist, pend, acc);
     else
         hipLaunchKernelGGL((k_ss_relax<true, false>), dim3(kSsRelaxGrid), dim3(kBlock), 0, s, alist, amask, doff,
-                           cstart, ctl, d->d_ptr, (const void *)d->d_static, d->wbits, dist, pend, acc);
+                           cstart, ctl, d->d_ptr, pmid, (const void *)d->d_static, d->wbits, dist, pend, acc);
     OG_HIP(hipGetLastError());
     OG_HIP(hipEventSynchronize(ev_summary));
     ++round;

@@ -50,15 +50,16 @@ class _Graphs(list):
 def _state_bytes(graphs):
     """Admission rule of og_server_create: per graph the BFS engine's masks
     (24 V bytes) and 4 kept-levels buffers (16 V), per weighted graph the
-    SSSP engine's 16 distance columns (64 V), pending masks (4 V), active
-    list and masks (8 V), degree prefix (8 V + 8), extraction buffer (4 V) and
+    SSSP engine's 16 distance columns (64 V), light and heavy pending masks
+    (8 V), active list and masks (8 V), work prefix (8 V + 8), extraction
+    buffer (4 V) and
     the relax's chunk table (4 (16 E / 1024 + 2))."""
     total = 0
     for g in graphs:
         V = g.num_vertices
         total += 40 * V
         if g.info()["kind"] == og.KIND_WEIGHTED:
-            total += 88 * V + 8 + 4 * (16 * g.num_edges // 1024 + 2)
+            total += 92 * V + 8 + 4 * (16 * g.num_edges // 1024 + 2)
     return total
 
 
@@ -243,7 +244,9 @@ def test_sssp_batch_bucket_widths(gpu, delta):
     with og.Server([g], sssp_delta=7) as srv:  # the server's own width over the graph's
         ids2 = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs]
         res2 = _wait_all(srv, ids2)
-    with og.Server([g], flags=og.SERVER_SSSP_SHARED_THRESHOLD) as srv:  # one threshold for all slots
+    g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0, split_weight=8)
+    g.set_tuning(delta=delta)
+    with og.Server([g], sssp_delta=256 if delta == 32 else 0) as srv:  # light / heavy rounds
         ids3 = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs[:10]]
         time.sleep(0.002)
         ids3 += [srv.submit(0, og.QUERY_SSSP, s) for s in srcs[10:]]  # joining running slots
@@ -252,7 +255,7 @@ def test_sssp_batch_bucket_widths(gpu, delta):
         want = og.levels_checksum(oracle.sssp(ptr, to, w, s))
         assert res[ids[n]]["checksum"] == want, (delta, s)
         assert res2[ids2[n]]["checksum"] == want, (7, s)
-        assert res3[ids3[n]]["checksum"] == want, ("shared", s)
+        assert res3[ids3[n]]["checksum"] == want, ("split", s)
     g.close()
 
 
