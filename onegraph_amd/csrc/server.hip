@@ -1042,7 +1042,7 @@ void SsspEngine::setup() {
     cstart = (uint32_t *)mem.dalloc((kSs * d->E / kSsChunk + 2) * 4);  // items <= kSs per edge
     bnv = (uint32_t *)mem.dalloc((uint64_t)nblk * 4);
     bdeg = (uint64_t *)mem.dalloc((uint64_t)nblk * 8);
-    bmask = (uint32_t *)mem.dalloc(2ull * nblk * 4);
+    bmask = (uint32_t *)mem.dalloc(3ull * nblk * 4);  // per block: near, far, heavy-pending slot masks
     bmin = (uint32_t *)mem.dalloc((uint64_t)kSs * nblk * 4);
     ctl = (SsCtl *)mem.dalloc(sizeof(SsCtl));
     h_ctl = (SsCtl *)mem.palloc(sizeof(SsCtl));

@@ -278,3 +278,21 @@ def test_failed_engine_refuses_its_queue(gpu, graphs):
         assert srv.info()["in_flight"] == 0 and srv.info()["pending"] == 0
     _, st = g.bfs(0, want_values=False)
     assert st["checksum"] == rb["checksum"]
+
+
+@pytest.mark.parametrize("split", [0, 8])
+def test_sssp_batch_many_sweep_blocks(gpu, split):
+    """A graph spanning several select / write workgroups (V = 2^16: 8 blocks
+    of 8192 vertices), with and without light/heavy rows, 24 queries."""
+    g = og.Graph.rmat(16, 8, seed=71, weight_seed=72)
+    g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0, split_weight=split)
+    ptr, to, w, _ = g.export()
+    cand = np.flatnonzero(np.diff(np.append(ptr.astype(np.int64), len(to))) > 0)
+    srcs = [int(x) for x in np.random.default_rng(73).choice(cand, 24, replace=False)]
+    _say("sssp batch rmat16 split", split)
+    with og.Server([g], sssp_delta=256 if split else 64) as srv:
+        ids = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs]
+        res = _wait_all(srv, ids)
+    for n, s in enumerate(srcs):
+        assert res[ids[n]]["checksum"] == og.levels_checksum(oracle.sssp(ptr, to, w, s)), (split, s)
+    g.close()
