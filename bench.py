@@ -527,7 +527,8 @@ def wl_serve(args):
                       {"workload": "served bfs/sssp queries, rmat " + "/".join(map(str, scales)) + " ef16",
                        "rates": rates, "mixes": mixes, "duration_s": args.duration, "state_budget_bytes": budget,
                        "p99_target_ms": args.p99_target_ms, "sssp_delta": args.delta,
-                       "served_sssp_delta": args.serve_sssp_delta,
+                       "served_sssp_delta": args.serve_sssp_delta or args.delta,
+                       "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                        "split_weight": args.split_weight})
     out.update({"serve": per_mix,
                 "parity": [{"mix": k, "rate": r, "checked": v["checked"], "mismatches": v["mismatches"]}
@@ -593,11 +594,20 @@ def main():
     ap.add_argument("--verify", type=int, default=100, help="serve: served results re-run on the plain path")
     ap.add_argument("--serve-mix", default="0.7,1.0", help="serve: BFS shares of the query mixes to run")
     ap.add_argument("--p99-target-ms", type=float, default=50.0, help="serve: p99 latency target")
-    ap.add_argument("--serve-sssp-delta", type=int, default=64,
-                    help="serve: bucket width of the server's batched SSSP slots (whole rows)")
+    ap.add_argument("--serve-sssp-delta", type=int, default=0,
+                    help="serve: bucket width of the server's batched SSSP slots (0: the graphs' tuning delta, "
+                         "256 with the light/heavy split)")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="serve: GPU_MAX_HW_QUEUES for the process (each engine's stream gets its own hardware "
+                         "queue instead of sharing one of HIP's default 4; at most 32)")
     args = ap.parse_args()
     if args.scale is None:
         args.scale = {"sssp-ooc": 24, "bfs-hbm": 20, "cc-ooc": 25, "pr-ooc": 25, "msbfs": 26, "serve": 0}[args.workload]
+    if args.workload == "serve" and args.hw_queues:
+        # before the first HIP call: the server's six engine streams (BFS and
+        # SSSP per graph) each get a hardware queue, so one engine's kernels do
+        # not wait behind another's in a shared queue
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, args.hw_queues))
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); --gpus {args.gpus} refused",
               file=sys.stderr)
