@@ -254,7 +254,7 @@ constexpr uint32_t kSsWin = 8192;             // vertices per workgroup of the s
 constexpr uint32_t kSsPer = kSsWin / kBlock;  // per thread: vertices base + t + kBlock * k, k < kSsPer
 constexpr uint32_t kSsRun = 4;                // work items per lane per chunk (consecutive)
 constexpr uint32_t kSsChunk = kBlock * kSsRun;  // work items per relax workgroup iteration
-constexpr unsigned kSsRelaxGrid = 4096;       // relax workgroups (they stride over the round's items)
+constexpr unsigned kSsRelaxGridCap = 1u << 20;  // relax workgroups at most (one chunk each below it)
 
 struct SsCtl {
     uint32_t thresh[kSs];  // near-far threshold of each slot (0xFFFFFFFF: every pending entry is near)
@@ -1104,8 +1104,9 @@ void SsspEngine::finalize(uint32_t done) {
 
 // One round: admit queued queries into free slots (resetting dirty columns
 // first when needed), then select / scan / write / relax on the engine's
-// stream; the host waits for the scan's summary only, so the round's write
-// and relax run while it reports the slots that finished.
+// stream; the host waits for the scan's summary only (it sizes the relax),
+// so the round's write and relax run while it reports the slots that
+// finished.
 void SsspEngine::step() {
     if (srv->params.flags & OG_SERVER_TEST_FAIL_SSSP) {
         std::unique_lock<std::mutex> lk(srv->mu);
@@ -1155,14 +1156,23 @@ void SsspEngine::step() {
     OG_HIP(hipEventRecord(ev_summary, s));
     hipLaunchKernelGGL(k_ss_write, dim3(nblk), dim3(kBlock), 0, s, pend, hpend, d->d_ptr, pmid, V, bnv, bdeg, alist,
                        amask, doff, cstart);
-    if (d->packed)
-        hipLaunchKernelGGL((k_ss_relax<true, true>), dim3(kSsRelaxGrid), dim3(kBlock), 0, s, alist, amask, doff,
-                           cstart, ctl, d->d_ptr, pmid, (const void *)d->d_static, d->wbits, dist, pend, acc);
-    else
-        hipLaunchKernelGGL((k_ss_relax<true, false>), dim3(kSsRelaxGrid), dim3(kBlock), 0, s, alist, amask, doff,
-                           cstart, ctl, d->d_ptr, pmid, (const void *)d->d_static, d->wbits, dist, pend, acc);
     OG_HIP(hipGetLastError());
+    // The relax is sized from the summary: one chunk per workgroup, so its
+    // workgroups retire as they finish and the other engines' kernels (own
+    // streams and hardware queues) get CUs in between, instead of waiting
+    // behind a grid that holds every CU until the whole relax is done.
     OG_HIP(hipEventSynchronize(ev_summary));
+    const uint64_t nch = (h_ctl->dsum + kSsChunk - 1) / kSsChunk;
+    if (nch) {
+        const dim3 grid((unsigned)std::min<uint64_t>(nch, kSsRelaxGridCap));
+        if (d->packed)
+            hipLaunchKernelGGL((k_ss_relax<true, true>), grid, dim3(kBlock), 0, s, alist, amask, doff, cstart, ctl,
+                               d->d_ptr, pmid, (const void *)d->d_static, d->wbits, dist, pend, acc);
+        else
+            hipLaunchKernelGGL((k_ss_relax<true, false>), grid, dim3(kBlock), 0, s, alist, amask, doff, cstart, ctl,
+                               d->d_ptr, pmid, (const void *)d->d_static, d->wbits, dist, pend, acc);
+        OG_HIP(hipGetLastError());
+    }
     ++round;
     rounds_run.fetch_add(1, std::memory_order_relaxed);
     // a slot is done when the select sweep found nothing pending for it; its
