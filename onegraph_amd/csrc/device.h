@@ -145,6 +145,8 @@ struct Frontier {
     uint64_t *hub = nullptr;         // hub tiles (tile << 32 | vertex)
     uint64_t hub_cap = 0;
     uint64_t *doff = nullptr;        // V + 1 degree prefix of the dynamic list
+    uint32_t *dchunk = nullptr;      // list entry holding edge c * kDynEdgesPerBlock of the dynamic edge space
+    uint64_t dchunk_cap = 0;
     uint32_t *bcnt = nullptr;        // kBcntRows * nblk per-block counts
     uint64_t *bdeg = nullptr;        // 2 * nblk per-block degree sums
     uint32_t nblk = 0;

@@ -384,7 +384,9 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
 
 // Compaction pass 3: emits the lists in vertex order from the codes of pass
 // 1 (bins 0/1 need no reads; hub tiles and the dynamic degree prefix read
-// the row bounds again, kLoadBatch at a time), and rewrites the window:
+// the row bounds again, kLoadBatch at a time) with the chunk table of the
+// dynamic relax (dchunk[c]: the list entry holding edge c *
+// kDynEdgesPerBlock of the dynamic edge space), and rewrites the window:
 // 1 for held-back vertices, 0 elsewhere.  m_dlist / m_doff, when not null,
 // receive a copy of the dynamic list and its degree prefix: pinned host
 // memory the staged loader gathers from without a copy back.
@@ -397,7 +399,8 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                                                            uint32_t *__restrict__ list, uint64_t *__restrict__ hub,
                                                            uint64_t hub_cap, uint64_t *__restrict__ doff,
                                                            uint32_t *__restrict__ m_dlist,
-                                                           uint64_t *__restrict__ m_doff) {
+                                                           uint64_t *__restrict__ m_doff,
+                                                           uint32_t *__restrict__ dchunk, uint64_t dchunk_cap) {
     __shared__ uint64_t lds[3 * kBlock / kWave];
     __shared__ uint32_t s_win[4 * kFlagVecs * kBlock];  // word i of thread t at [i * kBlock + t]
     if (!info->live) return;  // nothing is flagged: no list, and the window is already clear
@@ -488,6 +491,11 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                     }
                     list[dyn_base + p3] = (uint32_t)v;
                     doff[p3] = pd;
+                    // the dynamic relax's chunks of kDynEdgesPerBlock edges
+                    // whose first edge is one of v's: they start at entry p3
+                    for (uint64_t c = (pd + kDynEdgesPerBlock - 1) / kDynEdgesPerBlock;
+                         c * kDynEdgesPerBlock < pd + deg && c < dchunk_cap; ++c)
+                        dchunk[c] = (uint32_t)p3;
                     if (m_dlist) {  // pinned host mirrors for the host-side gather
                         m_dlist[p3] = (uint32_t)v;
                         m_doff[p3] = pd;
@@ -722,7 +730,10 @@ __global__ __launch_bounds__(kBlock) void k_relax_hub(const uint64_t *__restrict
 // Each workgroup takes chunks of kDynEdgesPerBlock consecutive edges, loads
 // the covering slice of the degree prefix into LDS and maps each edge to its
 // source with a binary search there (every listed vertex has >= 1 edge, so a
-// chunk covers at most kDynEdgesPerBlock vertices).
+// chunk covers at most kDynEdgesPerBlock vertices).  With the compaction's
+// chunk table (dchunk, zero-copy rounds: chunks start at edge 0) the slice's
+// first and last entries are two loads; the staged loader's slots start
+// anywhere, so it finds them by binary search over the degree prefix.
 // LDS of the dynamic relax body: the degree-prefix slice and its bounds
 struct DynLds {
     uint64_t off[kDynEdgesPerBlock + 2];
@@ -737,8 +748,8 @@ __device__ __forceinline__ uint32_t relax_dyn_body(const uint32_t *__restrict__ 
                                                    uint64_t dbase, const void *__restrict__ resident,
                                                    const int64_t *__restrict__ rdelta, uint32_t wbits,
                                                    const RelaxArgs &a, const uint64_t *__restrict__ lpos,
-                                                   uint64_t nl, uint64_t vsplit, DynLds &L, uint32_t bid,
-                                                   uint32_t nb, uint32_t &hits) {
+                                                   uint64_t nl, uint64_t vsplit, const uint32_t *__restrict__ dchunk,
+                                                   DynLds &L, uint32_t bid, uint32_t nb, uint32_t &hits) {
     uint64_t *s_off = L.off;
     uint32_t *s_i = L.i;
     const uint32_t ndyn = info->n[3];
@@ -748,7 +759,13 @@ __device__ __forceinline__ uint32_t relax_dyn_body(const uint32_t *__restrict__ 
     uint32_t c = 0, h = 0;
     for (uint64_t q0 = c0 + (uint64_t)bid * kDynEdgesPerBlock; q0 < c1; q0 += (uint64_t)nb * kDynEdgesPerBlock) {
         const uint64_t q1 = min(q0 + kDynEdgesPerBlock, c1);
-        if (threadIdx.x < 2) {
+        if (dchunk) {  // chunks aligned to the table (c0 = 0): two loads
+            if (threadIdx.x < 2) {
+                const uint64_t k = q0 / kDynEdgesPerBlock + threadIdx.x;
+                // the entry holding q1 (or the last entry): one past the chunk's last edge at most
+                s_i[threadIdx.x] = threadIdx.x == 1 && q1 >= info->dyn_deg ? ndyn - 1 : dchunk[k];
+            }
+        } else if (threadIdx.x < 2) {
             const uint64_t q = threadIdx.x == 0 ? q0 : q1 - 1;
             uint32_t lo = 0, hi = ndyn;  // largest i with doff[i] <= q; doff[0] = 0
             while (hi - lo > 1) {
@@ -834,13 +851,13 @@ __global__ __launch_bounds__(kBlock) void k_relax_dyn(const uint32_t *__restrict
                                                       const void *__restrict__ resident,
                                                       const int64_t *__restrict__ rdelta,
                                                       uint32_t wbits, RelaxArgs a, const uint64_t *__restrict__ lpos,
-                                                      uint64_t nl, uint64_t vsplit, unsigned long long *upd,
-                                                      unsigned long long *hits) {
+                                                      uint64_t nl, uint64_t vsplit, const uint32_t *__restrict__ dchunk,
+                                                      unsigned long long *upd, unsigned long long *hits) {
     __shared__ DynLds L;
     uint32_t h = 0;
     const uint32_t c = relax_dyn_body<ALGO, W, PACKED, DIRECT>(list, info, doff, stage, c0, c1max, ptr, dbase, resident,
-                                                               rdelta, wbits, a, lpos, nl, vsplit, L, blockIdx.x,
-                                                               gridDim.x, h);
+                                                               rdelta, wbits, a, lpos, nl, vsplit, dchunk, L,
+                                                               blockIdx.x, gridDim.x, h);
     wave_add(upd, c);
     if (DIRECT) wave_add(hits, h);
 }
@@ -870,6 +887,7 @@ struct RoundRelax {
     const void *resident;       // resident chunks in HBM
     const int64_t *rdelta;
     const uint64_t *lpos;       // split layout of the host copy (null: rows whole)
+    const uint32_t *dchunk;     // chunk table of the dynamic edge space
     uint64_t nl, vsplit;
     unsigned long long *counters;  // [0] static updates, [1] dynamic updates, [5] HBM-served dynamic edges
 };
@@ -894,8 +912,8 @@ __global__ __launch_bounds__(kBlock) void k_relax_round(RoundRelax r, RelaxArgs 
         uint32_t h = 0;
         const uint32_t c = relax_dyn_body<ALGO, W, PK, true>(r.list, r.info, r.doff, r.stage, 0, ~0ull, r.ptr, r.dbase,
                                                              r.resident, r.rdelta, a.wbits, a, r.lpos, r.nl, r.vsplit,
-                                                             *(DynLds *)s_fuse, r.cd * k + (j - kFuseStatic), r.cd * K,
-                                                             h);
+                                                             r.dchunk, *(DynLds *)s_fuse, r.cd * k + (j - kFuseStatic),
+                                                             r.cd * K, h);
         wave_add(r.counters + 1, c);
         wave_add(r.counters + 5, h);
     }
@@ -1135,6 +1153,11 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) 
     }
     f.list = (uint32_t *)m.dalloc(V * sizeof(uint32_t));
     f.doff = (uint64_t *)m.dalloc((V + 1) * sizeof(uint64_t));
+    // chunk table of the dynamic relax: the dynamic edge space of a round is
+    // at most the dynamic region's edges
+    const uint64_t dyn_edges = d->plan.vsplit < V ? g->E - g->ptr[d->plan.vsplit] : 0;
+    f.dchunk_cap = dyn_edges / kDynEdgesPerBlock + 2;
+    f.dchunk = (uint32_t *)m.dalloc(f.dchunk_cap * sizeof(uint32_t));
     f.nblk = (uint32_t)((d->Vpad + kVertsPerBlock - 1) / kVertsPerBlock);
     f.V = V;
     f.bcnt = (uint32_t *)m.dalloc((uint64_t)kBcntRows * f.nblk * sizeof(uint32_t));
@@ -1172,7 +1195,8 @@ void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t
     hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff, f.ctl,
                        m_doff);
     hipLaunchKernelGGL(k_frontier_write, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr, pmid,
-                       d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist, m_doff);
+                       d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist, m_doff,
+                       f.dchunk, f.dchunk_cap);
     OG_HIP(hipGetLastError());
     if (to_host) OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
 }
@@ -1814,7 +1838,8 @@ static void launch_dyn(DeviceState *d, unsigned grid, const void *src, uint64_t 
                        d->fr.info, d->fr.doff, src, c0, c1, d->d_ptr, d->dyn_base, (const void *)d->d_staging,
                        DIRECT ? (const int64_t *)d->d_rdelta : nullptr, d->wbits, relax_args(d),
                        DIRECT ? (const uint64_t *)d->d_lpos : nullptr, d->split_nl, d->plan.vsplit,
-                       d->fr.counters + 1, d->fr.counters + 5);
+                       DIRECT && c0 == 0 ? (const uint32_t *)d->fr.dchunk : nullptr, d->fr.counters + 1,
+                       d->fr.counters + 5);
 }
 
 // The fused round relax (k_relax_round) of a device-driven round on the
@@ -1846,6 +1871,7 @@ static void launch_round_fmt(DeviceState *d, bool with_dyn) {
     r.resident = d->d_staging;
     r.rdelta = d->d_rdelta;
     r.lpos = d->d_lpos;
+    r.dchunk = d->fr.dchunk;
     r.nl = d->split_nl;
     r.vsplit = d->plan.vsplit;
     r.counters = d->fr.counters;
@@ -2721,7 +2747,7 @@ static void noalloc_bfs(const uint64_t *d_ptr, const uint32_t *d_cols, uint64_t 
                            (uint64_t *)nullptr);
         hipLaunchKernelGGL(k_frontier_write, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, d_ptr,
                            (const uint64_t *)nullptr, tn, bcnt, bdeg, nblk, info, list, hub, kNoAllocHubCap, doff,
-                           (uint32_t *)nullptr, (uint64_t *)nullptr);
+                           (uint32_t *)nullptr, (uint64_t *)nullptr, (uint32_t *)nullptr, (uint64_t)0);
     };
     RoundRelax r{};
     r.list = list;
