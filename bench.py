@@ -580,6 +580,8 @@ def main():
     ap.add_argument("--sweep-dyn-grid", default="", help="comma list: time SSSP at each dyn_grid and report")
     ap.add_argument("--link-probe", action="store_true",
                     help="only measure shader reads of pinned host memory and hipMemcpy (host-link bound)")
+    ap.add_argument("--gather-probe", action="store_true",
+                    help="only measure random 4-byte reads from device tables of 4 MiB .. 4 GiB")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--workload", default="sssp-ooc",
@@ -631,6 +633,12 @@ def main():
     if og.device_count() < 1:
         print("bench.py needs a GPU: no HIP device is visible", file=sys.stderr)
         sys.exit(1)
+    if args.gather_probe:
+        # random 4-byte reads from tables the size of the relax's value arrays
+        # (C1 4 MiB, C2 64 MiB, C4 256 MiB) and from HBM proper (4 GiB)
+        print(json.dumps({"gather_probe": [og.probe_hbm_gather(b) for b in (4 << 20, 64 << 20, 256 << 20, 4 << 30)],
+                          "hbm_peak_gbs": HBM_PEAK_GBS}))
+        return
     if args.link_probe:
         # what the host link gives shader reads (the zero-copy loader's bound)
         print(json.dumps({"link_probe": og.probe_host_link(1 << 30), "peak_gbs": LINK_PEAK_GBS}))

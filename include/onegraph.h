@@ -358,6 +358,15 @@ og_status og_bfs_device_noalloc(const uint64_t *row_ptr, const uint32_t *cols, u
                                 uint64_t num_edges, uint32_t source, uint32_t *levels, void *stream,
                                 og_run_stats *stats);
 
+/* Random-gather probe (no reference counterpart: a measurement helper for
+ * the relax kernels, whose per-edge target reads are random 4-byte
+ * accesses).  Times 3 launches of 134 M random 4-byte reads each
+ * (SplitMix64-drawn indices, 8 in flight per lane) from a device table of
+ * table_bytes:
+ * *gather_gbs = useful bytes per second / 1e9, *mgathers_per_s = million
+ * reads per second. */
+og_status og_probe_hbm_gather(uint64_t table_bytes, double *gather_gbs, double *mgathers_per_s);
+
 /* Digest of a BFS/SSSP result array: sum mod 2^64 of mix64(v << 32 | x[v])
  * over entries x[v] != OG_UNREACHED (mix64 = SplitMix64 finaliser).  The
  * device computes the same digest into og_run_stats.checksum. */

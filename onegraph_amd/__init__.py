@@ -166,6 +166,7 @@ def lib():
         "og_levels_checksum": (C.c_int, [_u32p, C.c_uint64, _u64p]),
         "og_bfs_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, C.c_void_p,
                                             C.c_void_p, C.POINTER(RunStats)]),
+        "og_probe_hbm_gather": (C.c_int, [C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "og_probe_host_link": (C.c_int, [C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]),
         "og_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
@@ -231,6 +232,14 @@ def probe_host_link(nbytes=1 << 30):
     c, sc, mc = C.c_double(), C.c_double(), C.c_double()
     _check(lib().og_probe_host_link(nbytes, C.byref(c), C.byref(sc), C.byref(mc)))
     return {"contiguous_gbs": c.value, "scattered_rows_gbs": sc.value, "memcpy_gbs": mc.value}
+
+
+def probe_hbm_gather(table_bytes):
+    """Random 4-byte reads from a device table of table_bytes: useful GB/s and
+    million reads per second (the random-access ceiling of the relax)."""
+    g, m = C.c_double(), C.c_double()
+    _check(lib().og_probe_hbm_gather(table_bytes, C.byref(g), C.byref(m)))
+    return {"table_bytes": table_bytes, "gather_gbs": g.value, "mgathers_per_s": m.value}
 
 
 def version():

@@ -2412,6 +2412,64 @@ __global__ __launch_bounds__(kBlock) void k_probe_rows(const uint32_t *__restric
 
 }  // namespace og
 
+namespace og {
+
+// Random 4-byte gathers from a table of n words: each lane draws its indices
+// with SplitMix64 from its own counter and keeps kGatherIlp loads in flight.
+constexpr int kGatherIlp = 8;
+__global__ __launch_bounds__(kBlock) void k_probe_gather(const uint32_t *__restrict__ t, uint64_t n, uint64_t per,
+                                                         uint64_t seed, uint32_t *out) {
+    const uint64_t tid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t acc = 0;
+    for (uint64_t i = 0; i < per; i += kGatherIlp) {
+        uint32_t x[kGatherIlp];
+#pragma unroll
+        for (int k = 0; k < kGatherIlp; ++k) x[k] = t[d_mix64(seed + tid * per + i + k) % n];
+#pragma unroll
+        for (int k = 0; k < kGatherIlp; ++k) acc ^= x[k];
+    }
+    if (acc == 0x9E3779B9u) out[0] = acc;  // keeps the reads alive
+}
+
+}  // namespace og
+
+// Random-gather probe (no reference counterpart: a measurement helper for the
+// relax kernels, whose per-edge target reads are random 4-byte accesses).
+extern "C" og_status og_probe_hbm_gather(uint64_t table_bytes, double *gather_gbs, double *mgathers_per_s) {
+    clear_error();
+    if (table_bytes < (1u << 16) || table_bytes > (64ull << 30)) return fail(OG_EINVAL, "table must be 64 KiB .. 64 GiB");
+    return guard([&]() -> og_status {
+        if (!hip_available()) raise(OG_EHIP, "no HIP device is visible");
+        MemPool m;
+        const uint64_t n = table_bytes / 4;
+        uint32_t *t = (uint32_t *)m.dalloc(n * 4);
+        uint32_t *dout = (uint32_t *)m.dalloc(4);
+        hipLaunchKernelGGL(k_iota, dim3(fill_grid(n)), dim3(256), 0, 0, t, n);
+        const unsigned grid = 8192;
+        const uint64_t per = 8 * kGatherIlp;  // gathers per lane
+        const double total = (double)grid * kBlock * per;
+        hipEvent_t a, b;
+        OG_HIP(hipEventCreate(&a));
+        OG_HIP(hipEventCreate(&b));
+        hipLaunchKernelGGL(k_probe_gather, dim3(grid), dim3(kBlock), 0, 0, t, n, per, 1ull, dout);
+        OG_HIP(hipDeviceSynchronize());
+        OG_HIP(hipEventRecord(a, 0));
+        for (int r = 0; r < 3; ++r)
+            hipLaunchKernelGGL(k_probe_gather, dim3(grid), dim3(kBlock), 0, 0, t, n, per, 2ull + r, dout);
+        OG_HIP(hipEventRecord(b, 0));
+        OG_HIP(hipEventSynchronize(b));
+        OG_HIP(hipGetLastError());
+        float ms = 0;
+        OG_HIP(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        const double per_s = 3.0 * total / (ms * 1e-3);
+        if (gather_gbs) *gather_gbs = per_s * 4 / 1e9;
+        if (mgathers_per_s) *mgathers_per_s = per_s / 1e6;
+        return OG_OK;
+    });
+}
+
 extern "C" og_status og_probe_host_link(uint64_t bytes, double *contig_gbs, double *scattered_gbs,
                                         double *memcpy_gbs) {
     clear_error();
