@@ -52,8 +52,6 @@ struct RelaxArgs {
     const uint64_t *pull; // multi-source, optional: sources relaxed bottom-up this round
                           // (left out of the top-down relax)
     const uint64_t *pmid; // light/heavy split, optional: end of each row's light edges
-    uint32_t *vis;        // BFS, optional: visited bitmap (bit v % 32 of word v / 32); a
-                          // vertex's level is written by the one lane whose atomicOr sets its bit
 };
 
 // Edge range of u in a round: the whole row, or with the light/heavy split
@@ -242,7 +240,6 @@ struct DeviceState {
     std::vector<uint64_t> h_pmid;    // host copy (the staged gather)
     uint32_t split_weight = 0;       // light: weight < split_weight (0: no split)
     const uint64_t *run_pmid = nullptr;  // d_pmid while a run uses the split (SSSP near-far), else null
-    uint32_t *run_vis = nullptr;     // d_vis while a BFS run uses the visited bitmap, else null
     void *d_static = nullptr;        // static-region edges (USM: managed, whole graph)
     uint64_t static_used = 0;        // edges of the static vertices, = ptr[vsplit]
     void *d_staging = nullptr;       // nslots * slot_edges edges (zero-copy: the resident chunks)
@@ -273,7 +270,6 @@ struct DeviceState {
 
     // per-vertex state
     uint32_t *d_val = nullptr;       // V
-    uint32_t *d_vis = nullptr;       // BFS visited bitmap, (V + 31) / 32 words
     Frontier fr;                     // frontier buffers of single-source / batch runs
 
     // pinned host mirrors of the dynamic list

@@ -51,12 +51,6 @@ __global__ void k_set_source(uint32_t *val, uint8_t *flag, uint32_t src) {
     flag[src] = 1;
 }
 
-// BFS with the visited bitmap: cleared, then the source's bit set.
-__global__ void k_vis_reset(uint32_t *vis, uint64_t words, uint32_t src) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
-        vis[i] = i == (src >> 5) ? 1u << (src & 31u) : 0u;
-}
-
 // Per-thread flag window: kFlagsPerThread consecutive vertices.
 __device__ __forceinline__ void load_flags(const uint8_t *flag, uint64_t base, uint64_t Vpad, uint4 (&w)[kFlagVecs]) {
 #pragma unroll
@@ -571,19 +565,13 @@ __device__ __forceinline__ typename SrcOf<ALGO>::T load_src(const RelaxArgs &a, 
 template <int ALGO>
 __device__ __forceinline__ typename SrcOf<ALGO>::T peek(const RelaxArgs &a, uint32_t to) {
     if constexpr (ALGO == ALGO_MSBFS) return a.visit[to];
-    else if constexpr (ALGO == ALGO_BFS) {
-        // with the bitmap: 0 for a visited vertex (no offer beats it), else
-        // "unreached"; the bitmap is 32x smaller than the levels, so its
-        // words stay in L2
-        if (a.vis) return ((a.vis[to >> 5] >> (to & 31u)) & 1u) ? 0u : OG_UNREACHED;
-        return a.val[to];
-    } else return a.val[to];
+    else return a.val[to];
 }
 
 // Returns 1 when `to` improved (and is flagged for the next frontier).
 // BFS writes the same level from every writer in an iteration, so a plain
-// store is race-free (with the visited bitmap only the lane whose atomicOr
-// sets the vertex's bit writes); SSSP/CC use atomicMin so no smaller offer is lost
+// store is race-free (a visited bitmap claimed with atomicOr, one writer per
+// vertex, was measured slower on R-MAT 20: 14.7 vs 17.2 GTEPS); SSSP/CC use atomicMin so no smaller offer is lost
 // (the reference's check-then-write in K7/K11 can lose one).  Multi-source:
 // the sources of u's frontier that have not reached `to` are OR-ed into
 // next[to] (one 64-bit atomic for all 64 searches).
@@ -600,10 +588,6 @@ __device__ __forceinline__ uint32_t commit(const RelaxArgs &a, typename SrcOf<AL
         const uint32_t nd = offer<ALGO>(s, wt);
         if (nd >= seen) return 0;
         if (ALGO == ALGO_BFS) {
-            if (a.vis) {  // one writer per vertex: the lane that sets its bit
-                const uint32_t bit = 1u << (to & 31u);
-                if (atomicOr(a.vis + (to >> 5), bit) & bit) return 0;
-            }
             a.val[to] = nd;
             a.flag[to] = 1;
             return 1;
@@ -998,8 +982,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_pull(const uint64_t *__restrict_
                                                      const uint32_t *__restrict__ in_cols, uint64_t V,
                                                      const FrontierInfo *__restrict__ info,
                                                      const RoundCtl *__restrict__ ctl, uint32_t *val,
-                                                     uint8_t *__restrict__ flag, uint32_t *vis,
-                                                     unsigned long long *upd) {
+                                                     uint8_t *__restrict__ flag, unsigned long long *upd) {
     if (!info->pull) return;
     const uint32_t L = ctl->rounds;
     uint32_t c = 0;
@@ -1016,7 +999,6 @@ __global__ __launch_bounds__(kBlock) void k_bfs_pull(const uint64_t *__restrict_
             if (hit) {
                 val[v] = L + 1;
                 flag[v] = 1;
-                if (vis) atomicOr(vis + (v >> 5), 1u << (v & 31u));  // top-down rounds peek the bitmap
                 ++c;
                 break;
             }
@@ -1640,7 +1622,6 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
 
     // ---- vertex state
     d->d_val = (uint32_t *)d->dalloc(V * sizeof(uint32_t));
-    d->d_vis = (uint32_t *)d->dalloc((V + 31) / 32 * sizeof(uint32_t));
     OG_HIP(hipStreamSynchronize(d->s_compute));
     frontier_alloc(d.get(), g, d->fr, d->mem);
     pl.device_bytes = d->device_bytes();
@@ -1779,8 +1760,7 @@ static void gather(DeviceState *d, const og_graph *g, uint32_t ndyn, uint64_t c0
 }
 
 static RelaxArgs relax_args(DeviceState *d) {
-    return RelaxArgs{d->d_val, d->fr.flag, d->d_visit, d->d_front, d->d_next, d->wbits, nullptr, d->run_pmid,
-                     d->run_vis};
+    return RelaxArgs{d->d_val, d->fr.flag, d->d_visit, d->d_front, d->d_next, d->wbits, nullptr, d->run_pmid};
 }
 
 // Grids of device-count launches: the kernels stride over whatever count
@@ -2090,7 +2070,7 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
                     span(kClsStat, s, [&] {
                         hipLaunchKernelGGL(k_bfs_pull, dim3(capped((d->V + kBlock - 1) / kBlock, kGroupGridCap)),
                                            dim3(kBlock), 0, s, d->d_in_ptr, d->d_in_cols, d->V, d->fr.info,
-                                           d->fr.ctl, d->d_val, d->fr.flag, d->run_vis, d->fr.counters + 0);
+                                           d->fr.ctl, d->d_val, d->fr.flag, d->fr.counters + 0);
                         OG_HIP(hipGetLastError());
                     });
                 if (joined) OG_HIP(hipStreamWaitEvent(s, d->ev_static, 0));
@@ -2224,15 +2204,8 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
         hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid(V)), dim3(256), 0, s, d->d_val, V, OG_UNREACHED);
         hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d->d_val, d->fr.flag, src);
     }
-    d->run_vis = nullptr;
-    if (ALGO == ALGO_BFS && d->d_vis) {
-        const uint64_t words = (V + 31) / 32;
-        hipLaunchKernelGGL(k_vis_reset, dim3(fill_grid(words)), dim3(256), 0, s, d->d_vis, words, src);
-        d->run_vis = d->d_vis;
-    }
     OG_HIP(hipGetLastError());
     frontier_loop<ALGO, W>(g, d, st, tm, gather_ms, overload_wall, []() {}, true);
-    d->run_vis = nullptr;
     OG_HIP(hipStreamSynchronize(s));
     st.total_ms = total.ms();
 
