@@ -315,12 +315,17 @@ og_status og_set_tuning(og_graph *g, const og_tuning *t);
 #define OG_TUNE_BFS 0u
 #define OG_TUNE_SSSP 1u
 #define OG_TUNE_CC 2u
+#define OG_TUNE_MSBFS 3u   /* og_msbfs on one batch of 64 sources drawn uniformly
+                              (SplitMix64 seeded with `source`) among the
+                              vertices with out-edges */
 
 /* Autotuner: times `algo` from `source` on the prepared graph under
  * candidate tunings (coordinate descent: defer for SSSP / CC with a dynamic
  * region, delta for SSSP, then t1 with lpv0, t2, tile, and dyn_grid with a
  * dynamic region; the median of `runs` timed runs each), installs the fastest and
- * returns it with its time.  Results do not depend on the tuning. */
+ * returns it with its time.  Results do not depend on the tuning.  The
+ * workgroup size stays 256 (four 64-lane waves); the grid shapes tuned are
+ * the lanes per vertex of bin 0 and the dynamic relax's workgroups. */
 og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, uint32_t runs,
                       og_tuning *best, double *best_ms);
 

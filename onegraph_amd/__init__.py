@@ -74,7 +74,7 @@ class Tuning(C.Structure):
                 ("delta", C.c_uint32), ("defer", C.c_uint32), ("dyn_grid", C.c_uint32)]
 
 
-TUNE_BFS, TUNE_SSSP, TUNE_CC = 0, 1, 2
+TUNE_BFS, TUNE_SSSP, TUNE_CC, TUNE_MSBFS = 0, 1, 2, 3
 
 
 class PrParams(C.Structure):

@@ -2587,12 +2587,29 @@ static og_tuning tuning_of(const DeviceState *d) {
                      d->tune.dyn_grid};
 }
 
+// The multi-source tuning batch: 64 sources drawn uniformly (SplitMix64
+// seeded with `seed`) among the vertices with out-edges.
+static std::vector<uint32_t> tune_sources(const og_graph *g, uint32_t seed) {
+    std::vector<uint32_t> cand;
+    for (uint64_t v = 0; v < g->V; ++v)
+        if (g->ptr[v + 1] > g->ptr[v]) cand.push_back((uint32_t)v);
+    std::vector<uint32_t> out;
+    if (cand.empty()) return out;
+    const uint64_t key = rng_key(seed);
+    for (uint32_t i = 0; i < (uint32_t)kMsWidth; ++i) out.push_back(cand[rng_draw(key, i) % cand.size()]);
+    return out;
+}
+
 static double time_algo(og_graph *g, uint32_t algo, uint32_t src, uint32_t runs) {
     std::vector<double> ts;
     const bool w = g->kind == OG_KIND_WEIGHTED;
+    const std::vector<uint32_t> batch = algo == OG_TUNE_MSBFS ? tune_sources(g, src) : std::vector<uint32_t>();
     for (uint32_t r = 0; r < runs; ++r) {
         og_run_stats st;
-        if (algo == OG_TUNE_BFS) {
+        if (algo == OG_TUNE_MSBFS) {
+            if (w) run_msbfs<true>(g, batch.data(), (uint32_t)batch.size(), nullptr, nullptr, nullptr, &st);
+            else run_msbfs<false>(g, batch.data(), (uint32_t)batch.size(), nullptr, nullptr, nullptr, &st);
+        } else if (algo == OG_TUNE_BFS) {
             if (w) run_frontier<ALGO_BFS, true>(g, src, nullptr, &st);
             else run_frontier<ALGO_BFS, false>(g, src, nullptr, &st);
         } else if (algo == OG_TUNE_SSSP) {
@@ -2638,10 +2655,11 @@ extern "C" og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, ui
     clear_error();
     if (!g) return fail(OG_EINVAL, "graph is NULL");
     if (!g->dev) return fail(OG_ESTATE, "og_prepare has not been called for this graph");
-    if (algo > OG_TUNE_CC) return fail(OG_EINVAL, "algo must be OG_TUNE_BFS, OG_TUNE_SSSP or OG_TUNE_CC");
+    if (algo > OG_TUNE_MSBFS)
+        return fail(OG_EINVAL, "algo must be OG_TUNE_BFS, OG_TUNE_SSSP, OG_TUNE_CC or OG_TUNE_MSBFS");
     if (g->kind == OG_KIND_PULL) return fail(OG_EUNSUPPORTED, "pull-layout graphs serve PageRank only");
     if (algo == OG_TUNE_SSSP && g->kind != OG_KIND_WEIGHTED) return fail(OG_EUNSUPPORTED, "SSSP needs weights");
-    if (algo != OG_TUNE_CC && source >= g->V) return fail(OG_EINVAL, "source out of range");
+    if ((algo == OG_TUNE_BFS || algo == OG_TUNE_SSSP) && source >= g->V) return fail(OG_EINVAL, "source out of range");
     if (runs == 0) runs = 1;
     int expect = 0;
     if (!g->busy.compare_exchange_strong(expect, 1)) return fail(OG_EBUSY, "graph is running another call");
@@ -2662,7 +2680,7 @@ extern "C" og_status og_autotune(og_graph *g, uint32_t algo, uint32_t source, ui
         };
         const bool has_dyn = g->dev->plan.vsplit < g->V;
         for (int pass = 0; pass < 2; ++pass) {
-            if (algo != OG_TUNE_BFS && has_dyn)
+            if ((algo == OG_TUNE_SSSP || algo == OG_TUNE_CC) && has_dyn)
                 for (uint32_t df : {0u, 1u, 2u}) {
                     if (df == 2 && algo != OG_TUNE_SSSP) continue;
                     og_tuning c = cur;

@@ -76,6 +76,27 @@ def test_autotune(gpu):
     g.close()
 
 
+def test_autotune_multi_source(gpu):
+    """OG_TUNE_MSBFS tunes on a seeded 64-source batch; the installed tuning
+    leaves every source's levels equal to the oracle's (zero-copy plan, so the
+    dynamic grid is among the candidates)."""
+    g = og.Graph.rmat(13, 16, seed=66)
+    ptr, to, _, _ = g.export()
+    g.prepare("ascetic", edge_budget_bytes=len(to) * 4 // 2, advisek=0.8, flags=og.PLAN_ZERO_COPY)
+    best, ms = g.autotune(og.TUNE_MSBFS, 3004, runs=1)
+    assert ms > 0 and g.tuning() == best
+    cand = np.flatnonzero(np.diff(np.append(ptr.astype(np.int64), len(to))) > 0)
+    srcs = np.random.default_rng(67).choice(cand, 70, replace=False).astype(np.uint32)
+    lv, cs, _, _ = g.msbfs(srcs)
+    for i in (0, 33, 69):
+        ref, _, _ = oracle.bfs(ptr, to, int(srcs[i]))
+        assert np.array_equal(lv[i], ref)
+        assert og.levels_checksum(ref) == int(cs[i])
+    with pytest.raises(og.OgError):
+        g.autotune(7, 0)
+    g.close()
+
+
 def test_pagerank_after_retune(gpu):
     g = og.Graph.rmat(11, 16, seed=65).to_pull()
     ptr, cols, _, od = g.export()
