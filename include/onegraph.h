@@ -44,7 +44,8 @@ typedef enum og_status {
     OG_EHIP = 7,            /* HIP runtime error, including "no GPU present"       */
     OG_EBUSY = 8,           /* object already in use / server queue full           */
     OG_EUNSUPPORTED = 9,    /* algorithm and graph kind do not fit together        */
-    OG_ESTATE = 10          /* call order violated (e.g. run before og_prepare)    */
+    OG_ESTATE = 10,         /* call order violated (e.g. run before og_prepare)    */
+    OG_ETIMEDOUT = 11       /* served query not started within its deadline         */
 } og_status;
 
 /* Graph kinds = the three on-disk CSR layouts of the reference. */
@@ -431,13 +432,16 @@ typedef struct og_server_params {
                                      0 = a quarter of free HBM                 */
     uint32_t max_pending;         /* queued queries before submit returns OG_EBUSY; 0 = 2^20 */
     uint32_t flags;               /* OG_SERVER_* bits                       */
+    uint32_t deadline_ms;         /* a query not admitted into a slot within
+                                     this many ms of its submit completes with
+                                     OG_ETIMEDOUT instead (0: no deadline);
+                                     admitted queries always run to the end */
     uint32_t sssp_delta;          /* bucket width of the SSSP slots' near-far
                                      rounds; 0 = the graph's tuning delta.
                                      On graphs prepared with split_weight the
                                      slots run light/heavy rounds (wide
                                      buckets pay: 256 with weights in [1, 63]),
                                      else they relax whole rows (16-64)       */
-    uint32_t reserved;
 } og_server_params;
 
 /* Testing only: the server's SSSP engines stop at their first round with

@@ -28,7 +28,7 @@ EDGES_SYMMETRIZE, EDGES_KEEP_SELF_LOOPS, EDGES_ONE_BASED, EDGES_WEIGHTED = 1, 2,
 STATUS = {
     0: "OG_OK", 1: "OG_EINVAL", 2: "OG_EIO", 3: "OG_EFORMAT", 4: "OG_EBUDGET",
     5: "OG_ENOMEM_HOST", 6: "OG_ENOMEM_DEVICE", 7: "OG_EHIP", 8: "OG_EBUSY",
-    9: "OG_EUNSUPPORTED", 10: "OG_ESTATE",
+    9: "OG_EUNSUPPORTED", 10: "OG_ESTATE", 11: "OG_ETIMEDOUT",
 }
 
 
@@ -94,7 +94,7 @@ SERVER_TEST_FAIL_SSSP = 0x80000000  # testing: SSSP engines stop at their first 
 
 class ServerParams(C.Structure):
     _fields_ = [("state_budget_bytes", C.c_uint64), ("max_pending", C.c_uint32), ("flags", C.c_uint32),
-                ("sssp_delta", C.c_uint32), ("reserved", C.c_uint32)]
+                ("deadline_ms", C.c_uint32), ("sssp_delta", C.c_uint32)]
 
 
 class QueryResult(C.Structure):
@@ -448,10 +448,10 @@ class Graph:
 class Server:
     """Continuous-batching query server over device-resident graphs."""
 
-    def __init__(self, graphs, state_budget_bytes=0, max_pending=0, flags=0, sssp_delta=0):
+    def __init__(self, graphs, state_budget_bytes=0, max_pending=0, flags=0, sssp_delta=0, deadline_ms=0):
         self._graphs = list(graphs)
         arr = (_P * len(self._graphs))(*[g._h.value for g in self._graphs])
-        p = ServerParams(state_budget_bytes, max_pending, flags, sssp_delta, 0)
+        p = ServerParams(state_budget_bytes, max_pending, flags, deadline_ms, sssp_delta)
         h = _P()
         _check(lib().og_server_create(arr, len(self._graphs), C.byref(p), C.byref(h)))
         self._h = h

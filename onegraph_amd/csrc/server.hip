@@ -850,6 +850,22 @@ static void push_failure(og_server *srv, const Query &q, uint32_t gi, og_status 
     srv->completed++;
 }
 
+// Queued queries whose deadline passed before a slot took them complete with
+// OG_ETIMEDOUT (the queue is in submit order, so the expired ones lead it).
+// Caller holds srv->mu.
+static void expire_queued(og_server *srv, uint32_t gi, std::deque<Query> *pending, double t) {
+    const uint32_t dl = srv->params.deadline_ms;
+    if (!dl) return;
+    bool any = false;
+    while (!pending->empty() && t - pending->front().t_submit > (double)dl) {
+        push_failure(srv, pending->front(), gi, OG_ETIMEDOUT, t);
+        pending->pop_front();
+        srv->pending--;
+        any = true;
+    }
+    if (any) srv->cv_done.notify_all();
+}
+
 // Fails the queries an engine holds (in slots and queued) and marks its queue
 // failed, so that later submits for it return the error instead of waiting.
 // Caller holds srv->mu.
@@ -926,6 +942,7 @@ void BfsEngine::step() {
         srv->cv_work.wait(lk, [&] { return srv->stop || used || !pending->empty(); });
         if (srv->stop) return;
         const double t = now_ms(srv);
+        expire_queued(srv, gi, pending, t);
         const uint32_t clean = kSlots - (uint32_t)__builtin_popcountll(used | dirty);
         if (dirty && (pending->size() > clean || __builtin_popcountll(dirty) >= (int)kSlots / 2)) {
             hipLaunchKernelGGL(k_srv_clear_visit, dim3(sweep_grid(d->V, 16384)), dim3(kBlock), 0, s, d->V, dirty,
@@ -1121,6 +1138,7 @@ void SsspEngine::step() {
         srv->cv_work.wait(lk, [&] { return srv->stop || used || !pending->empty(); });
         if (srv->stop) return;
         const double t = now_ms(srv);
+        expire_queued(srv, gi, pending, t);
         const uint32_t clean = kSs - (uint32_t)__builtin_popcount(used | dirty);
         if (dirty && (pending->size() > clean || __builtin_popcount(dirty) >= (int)kSs / 2)) {
             hipLaunchKernelGGL(k_ss_clear, dim3(sweep_grid(d->V, 16384)), dim3(kBlock), 0, s, dist, d->V, dirty);

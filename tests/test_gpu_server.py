@@ -296,3 +296,30 @@ def test_sssp_batch_many_sweep_blocks(gpu, split):
     for n, s in enumerate(srcs):
         assert res[ids[n]]["checksum"] == og.levels_checksum(oracle.sssp(ptr, to, w, s)), (split, s)
     g.close()
+
+
+def test_deadline_expires_queued_queries(gpu, graphs):
+    """With a 1 ms admission deadline, a burst of 3000 BFS queries (64 slots)
+    leaves most of them queued past it: they complete with OG_ETIMEDOUT, the
+    admitted ones with their usual results, and nothing is left pending.
+    Without a deadline none expire."""
+    g = graphs[0]
+    ptr, to, _, _ = g.export()
+    deg = np.diff(np.append(ptr.astype(np.int64), len(to)))
+    srcs = np.random.default_rng(23).choice(np.flatnonzero(deg > 0), 3000)
+    with og.Server([g], deadline_ms=1) as srv:
+        ids = [srv.submit(0, og.QUERY_BFS, int(s)) for s in srcs]
+        res = _wait_all(srv, ids, timeout=120)
+        info = srv.info()
+        assert info["pending"] == 0 and info["in_flight"] == 0 and info["completed"] == len(ids)
+    names = [og.STATUS[res[i]["status"]] for i in ids]
+    assert set(names) <= {"OG_OK", "OG_ETIMEDOUT"}
+    assert names.count("OG_ETIMEDOUT") > 0 and names.count("OG_OK") >= 64
+    ok = [n for n, i in enumerate(ids) if res[i]["status"] == 0][:20]
+    for n in ok:
+        _, st = g.bfs(int(srcs[n]), want_values=False)
+        assert st["checksum"] == res[ids[n]]["checksum"]
+    with og.Server([g]) as srv:
+        ids = [srv.submit(0, og.QUERY_BFS, int(s)) for s in srcs[:500]]
+        res = _wait_all(srv, ids, timeout=120)
+    assert all(res[i]["status"] == 0 for i in ids)
