@@ -22,9 +22,9 @@ namespace og {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;             // threads per block of the frontier kernels
-constexpr int kFlagVecs = 2;            // uint4 flag loads per thread
+constexpr int kFlagVecs = 1;            // uint4 flag loads per thread
 constexpr int kFlagsPerThread = 16 * kFlagVecs;
-constexpr int kVertsPerBlock = kBlock * kFlagsPerThread;   // 8192
+constexpr int kVertsPerBlock = kBlock * kFlagsPerThread;   // 4096
 constexpr int kDynEdgesPerThread = 8;   // dynamic relax: edges per lane
 constexpr int kDynEdgesPerBlock = kBlock * kDynEdgesPerThread;
 constexpr int kMaxSlots = 16;           // staging slots of the dynamic-region pipeline
