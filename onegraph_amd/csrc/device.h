@@ -22,9 +22,14 @@ namespace og {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;             // threads per block of the frontier kernels
-constexpr int kFlagVecs = 1;            // uint4 flag loads per thread
-constexpr int kFlagsPerThread = 16 * kFlagVecs;
-constexpr int kVertsPerBlock = kBlock * kFlagsPerThread;   // 4096
+// Frontier compaction windows: each thread of a count / write workgroup
+// takes FV uint4 loads of flags (16 * FV vertices), so a workgroup covers
+// kBlock * 16 * FV vertices.  Graphs up to kSmallFrontierV vertices use
+// FV = 1 (more, shorter-lived workgroups: their rounds are latency bound);
+// larger ones FV = 2 (half the blocks for the single-workgroup scan).
+constexpr uint64_t kSmallFrontierV = 1ull << 22;
+constexpr int frontier_fv(uint64_t V) { return V <= kSmallFrontierV ? 1 : 2; }
+constexpr uint64_t verts_per_block(int fv) { return (uint64_t)kBlock * 16 * fv; }
 constexpr int kDynEdgesPerThread = 8;   // dynamic relax: edges per lane
 constexpr int kDynEdgesPerBlock = kBlock * kDynEdgesPerThread;
 constexpr int kMaxSlots = 16;           // staging slots of the dynamic-region pipeline
@@ -150,6 +155,7 @@ struct Frontier {
     uint32_t *bcnt = nullptr;        // kBcntRows * nblk per-block counts
     uint64_t *bdeg = nullptr;        // 2 * nblk per-block degree sums
     uint32_t nblk = 0;
+    uint32_t fv = 2;                 // flag vectors per thread of the compaction (frontier_fv)
     uint64_t V = 0;
     FrontierInfo *info = nullptr;
     FrontierInfo *h_info = nullptr;  // pinned

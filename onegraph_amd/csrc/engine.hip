@@ -51,33 +51,21 @@ __global__ void k_set_source(uint32_t *val, uint8_t *flag, uint32_t src) {
     flag[src] = 1;
 }
 
-// Per-thread flag window: kFlagsPerThread consecutive vertices.
-__device__ __forceinline__ void load_flags(const uint8_t *flag, uint64_t base, uint64_t Vpad, uint4 (&w)[kFlagVecs]) {
+// Per-thread flag window: 16 * FV consecutive vertices.
+template <int FV>
+__device__ __forceinline__ void load_flags(const uint8_t *flag, uint64_t base, uint64_t Vpad, uint4 (&w)[FV]) {
 #pragma unroll
-    for (int k = 0; k < kFlagVecs; ++k) {
+    for (int k = 0; k < FV; ++k) {
         uint64_t at = base + 16ull * k;
         w[k] = at < Vpad ? *(const uint4 *)(flag + at) : make_uint4(0, 0, 0, 0);
     }
 }
 
-__device__ __forceinline__ uint32_t flag_at(const uint4 (&w)[kFlagVecs], int j) {
-    const uint4 &q = w[j >> 4];
-    const int b = j & 15;
-    uint32_t word = (b < 4) ? q.x : (b < 8) ? q.y : (b < 12) ? q.z : q.w;
-    return (word >> (8 * (b & 3))) & 0xFFu;
-}
-
-__device__ __forceinline__ bool any_flag(const uint4 (&w)[kFlagVecs]) {
-    uint32_t o = 0;
-#pragma unroll
-    for (int k = 0; k < kFlagVecs; ++k) o |= w[k].x | w[k].y | w[k].z | w[k].w;
-    return o != 0;
-}
-
+template <int FV>
 __device__ __forceinline__ void store_flags(uint8_t *flag, uint64_t base, uint64_t Vpad,
-                                            const uint32_t (&word)[4 * kFlagVecs]) {
+                                            const uint32_t (&word)[4 * FV]) {
 #pragma unroll
-    for (int k = 0; k < kFlagVecs; ++k) {
+    for (int k = 0; k < FV; ++k) {
         const uint64_t at = base + 16ull * k;
         if (at < Vpad)
             *(uint4 *)(flag + at) = make_uint4(word[4 * k], word[4 * k + 1], word[4 * k + 2], word[4 * k + 3]);
@@ -114,10 +102,11 @@ constexpr int kLoadBatch = 8;  // flagged vertices whose row bounds are read tog
 // then walk the set bits only (lowest first, i.e. in vertex order), which
 // keeps their code small: a per-slot unrolled switch over 32 slots does not
 // fit the instruction cache.
-__device__ __forceinline__ uint32_t window_mask(const uint4 (&w)[kFlagVecs], uint32_t lo) {
+template <int FV>
+__device__ __forceinline__ uint32_t window_mask(const uint4 (&w)[FV], uint32_t lo) {
     uint32_t m = 0;
 #pragma unroll
-    for (int k = 0; k < kFlagVecs; ++k) {
+    for (int k = 0; k < FV; ++k) {
         const uint32_t q[4] = {w[k].x, w[k].y, w[k].z, w[k].w};
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -140,6 +129,7 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
 // consecutive flag bytes (only the block totals depend on the mapping).  Row
 // bounds (and values, for near-far) of kLoadBatch flagged vertices are read
 // before any is used, so their latencies overlap.
+template <int FV>
 __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__ flag_a, uint8_t *__restrict__ hflag,
                                                            uint64_t Vpad, uint64_t vsplit,
                                                            const uint64_t *__restrict__ ptr,
@@ -149,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                                                            uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
                                                            uint32_t nblk) {
     __shared__ uint64_t lds[6 * kBlock / kWave];
-    __shared__ uint4 s_win[kVertsPerBlock / 16];  // the window's flag bytes
+    __shared__ uint4 s_win[(kBlock * 16 * FV) / 16];  // the window's flag bytes
     // a round enqueued after the traversal ended: no vertex is flagged and
     // the block counts of the last pass (all zero) stand
     if (ctl->halt && !ctl->live) return;
@@ -158,16 +148,16 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
     const bool heavy = pmid && ctl->mode;
     uint8_t *flag = heavy ? hflag : flag_a;
     const uint32_t *val = heavy ? nullptr : val_a;
-    const uint64_t wbase = (uint64_t)blockIdx.x * kVertsPerBlock;
-    uint4 w[kFlagVecs];
-    load_flags(flag, wbase + (uint64_t)threadIdx.x * kFlagsPerThread, Vpad, w);
+    const uint64_t wbase = (uint64_t)blockIdx.x * (kBlock * 16 * FV);
+    uint4 w[FV];
+    load_flags(flag, wbase + (uint64_t)threadIdx.x * (16 * FV), Vpad, w);
 #pragma unroll
-    for (int k = 0; k < kFlagVecs; ++k) s_win[threadIdx.x * kFlagVecs + k] = w[k];
+    for (int k = 0; k < FV; ++k) s_win[threadIdx.x * FV + k] = w[k];
     __syncthreads();
     const uint8_t *sb = (const uint8_t *)s_win;
     uint32_t m = 0;
 #pragma unroll
-    for (int j = 0; j < kFlagsPerThread; ++j) m |= (sb[j * kBlock + threadIdx.x] != 0 ? 1u : 0u) << j;
+    for (int j = 0; j < (16 * FV); ++j) m |= (sb[j * kBlock + threadIdx.x] != 0 ? 1u : 0u) << j;
     const uint64_t base = wbase + threadIdx.x;  // slot j: vertex base + j * kBlock
     uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0, cz = 0, ch = 0, cf = 0, cl = 0, mf = 0xFFFFFFFFu;
     uint64_t sdeg = 0, ddeg = 0;
@@ -184,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
             bool far[kLoadBatch];
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
-                jj[k] = kFlagsPerThread;  // empty slot
+                jj[k] = (16 * FV);  // empty slot
                 pb[k] = pe[k] = 0;
                 vv[k] = 0;
                 if (m) {
@@ -200,14 +190,14 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
                 far[k] = false;
-                if (jj[k] == kFlagsPerThread || !val) continue;
+                if (jj[k] == (16 * FV) || !val) continue;
                 const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + (uint64_t)jj[k] * kBlock >= vsplit;
                 far[k] = vv[k] >= thresh || lag;
                 if (!far[k]) row_range(ptr, pmid, heavy, base + (uint64_t)jj[k] * kBlock, pb[k], pe[k]);
             }
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
-                if (jj[k] == kFlagsPerThread) continue;
+                if (jj[k] == (16 * FV)) continue;
                 const uint64_t deg = pe[k] - pb[k];
                 const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + (uint64_t)jj[k] * kBlock >= vsplit;
                 cl += lag ? 1u : 0u;
@@ -390,6 +380,7 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
 // 1 for held-back vertices, 0 elsewhere.  m_dlist / m_doff, when not null,
 // receive a copy of the dynamic list and its degree prefix: pinned host
 // memory the staged loader gathers from without a copy back.
+template <int FV>
 __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__ flag_a, uint8_t *__restrict__ hflag,
                                                            uint64_t Vpad, const uint64_t *__restrict__ ptr,
                                                            const uint64_t *__restrict__ pmid, Tuning tn,
@@ -402,17 +393,17 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                                                            uint64_t *__restrict__ m_doff,
                                                            uint32_t *__restrict__ dchunk, uint64_t dchunk_cap) {
     __shared__ uint64_t lds[3 * kBlock / kWave];
-    __shared__ uint32_t s_win[4 * kFlagVecs * kBlock];  // word i of thread t at [i * kBlock + t]
+    __shared__ uint32_t s_win[4 * FV * kBlock];  // word i of thread t at [i * kBlock + t]
     if (!info->live) return;  // nothing is flagged: no list, and the window is already clear
     // the array and row ranges the count pass used (light/heavy split)
     const bool heavy = pmid && info->heavy;
     uint8_t *flag = heavy ? hflag : flag_a;
     const bool mark = pmid && !heavy;  // light round: relaxed vertices turn heavy-pending
-    const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kFlagsPerThread;
-    uint4 w[kFlagVecs];
+    const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * (16 * FV);
+    uint4 w[FV];
     load_flags(flag, base, Vpad, w);
 #pragma unroll
-    for (int k = 0; k < kFlagVecs; ++k) {
+    for (int k = 0; k < FV; ++k) {
         s_win[(4 * k + 0) * kBlock + threadIdx.x] = w[k].x;
         s_win[(4 * k + 1) * kBlock + threadIdx.x] = w[k].y;
         s_win[(4 * k + 2) * kBlock + threadIdx.x] = w[k].z;
@@ -433,7 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
             uint64_t pb[kLoadBatch], pe[kLoadBatch];
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
-                jj[k] = kFlagsPerThread;
+                jj[k] = (16 * FV);
                 bb[k] = -1;
                 pb[k] = pe[k] = 0;
                 if (m) {
@@ -446,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
             }
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k)
-                if (jj[k] != kFlagsPerThread) use(jj[k], bb[k], pe[k] - pb[k]);
+                if (jj[k] != (16 * FV)) use(jj[k], bb[k], pe[k] - pb[k]);
         }
     };
     walk([&](uint32_t, int b, uint64_t deg) {
@@ -508,12 +499,12 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
             }
         });
     }
-    uint32_t word[4 * kFlagVecs];
+    uint32_t word[4 * FV];
 #pragma unroll
-    for (int k = 0; k < 4 * kFlagVecs; ++k) word[k] = 0;
+    for (int k = 0; k < 4 * FV; ++k) word[k] = 0;
 #pragma unroll
-    for (int j = 0; j < kFlagsPerThread; ++j) word[j >> 2] |= ((keep >> j) & 1u) << (8 * (j & 3));
-    store_flags(flag, base, Vpad, word);
+    for (int j = 0; j < (16 * FV); ++j) word[j >> 2] |= ((keep >> j) & 1u) << (8 * (j & 3));
+    store_flags<FV>(flag, base, Vpad, word);
 }
 
 __global__ void k_ctl_reset(RoundCtl *ctl, uint32_t delta, uint32_t defer, uint64_t dir_opt_edges, uint64_t V,
@@ -1159,7 +1150,8 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) 
     const uint64_t dyn_edges = d->plan.vsplit < V ? g->E - g->ptr[d->plan.vsplit] : 0;
     f.dchunk_cap = dyn_edges / kDynEdgesPerBlock + 2;
     f.dchunk = (uint32_t *)m.dalloc(f.dchunk_cap * sizeof(uint32_t));
-    f.nblk = (uint32_t)((d->Vpad + kVertsPerBlock - 1) / kVertsPerBlock);
+    f.fv = (uint32_t)frontier_fv(V);
+    f.nblk = (uint32_t)((d->Vpad + verts_per_block(f.fv) - 1) / verts_per_block(f.fv));
     f.V = V;
     f.bcnt = (uint32_t *)m.dalloc((uint64_t)kBcntRows * f.nblk * sizeof(uint32_t));
     f.bdeg = (uint64_t *)m.dalloc(2ull * f.nblk * sizeof(uint64_t));
@@ -1191,16 +1183,25 @@ void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t
     // PCIe write per vertex: slower than one bulk copy back, so unused)
     uint32_t *m_dlist = nullptr;
     uint64_t *m_doff = nullptr;
-    hipLaunchKernelGGL(k_frontier_count, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->plan.vsplit,
-                       d->d_ptr, pmid, d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
+    if (f.fv == 1)
+        hipLaunchKernelGGL(k_frontier_count<1>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad,
+                           d->plan.vsplit, d->d_ptr, pmid, d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
+    else
+        hipLaunchKernelGGL(k_frontier_count<2>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad,
+                           d->plan.vsplit, d->d_ptr, pmid, d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
     // one lane per block of counts up to 1024 lanes: a small graph's scan
     // runs on a wave or two instead of sixteen
     const unsigned scan_threads = (unsigned)std::min<uint64_t>(1024, (std::max<uint64_t>(f.nblk, 1) + kWave - 1) / kWave * kWave);
     hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(scan_threads), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff,
                        f.ctl, m_doff);
-    hipLaunchKernelGGL(k_frontier_write, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr, pmid,
-                       d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist, m_doff,
-                       f.dchunk, f.dchunk_cap);
+    if (f.fv == 1)
+        hipLaunchKernelGGL(k_frontier_write<1>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr,
+                           pmid, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist,
+                           m_doff, f.dchunk, f.dchunk_cap);
+    else
+        hipLaunchKernelGGL(k_frontier_write<2>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr,
+                           pmid, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist,
+                           m_doff, f.dchunk, f.dchunk_cap);
     OG_HIP(hipGetLastError());
     if (to_host) OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
 }
@@ -2750,7 +2751,8 @@ constexpr uint64_t na_up(uint64_t x) { return (x + kNoAlign - 1) / kNoAlign * kN
 // of degree d > t2 takes ceil(d / tile) <= d / tile + 1 tiles, and at most
 // E / (t2 + 1) vertices exceed t2.
 constexpr uint64_t kNoAllocHubCap = kNoAllocMaxE / 2048 + kNoAllocMaxE / 4097 + 1;
-constexpr uint64_t kNoAllocMaxBlk = (kNoAllocMaxV + kVertsPerBlock - 1) / kVertsPerBlock;
+static_assert(frontier_fv(kNoAllocMaxV) == 1, "the allocation-free BFS runs the one-vector compaction");
+constexpr uint64_t kNoAllocMaxBlk = (kNoAllocMaxV + verts_per_block(1) - 1) / verts_per_block(1);
 constexpr uint64_t kNaFlag = 0;
 constexpr uint64_t kNaList = kNaFlag + na_up(kNoAllocMaxV);
 constexpr uint64_t kNaHub = kNaList + na_up(4 * kNoAllocMaxV);
@@ -2801,7 +2803,7 @@ static void noalloc_bfs(const uint64_t *d_ptr, const uint32_t *d_cols, uint64_t 
         raise(OG_EINVAL, "row offsets must run from 0 to E=%llu over V+1 entries (got %llu .. %llu)",
               (unsigned long long)E, (unsigned long long)ends[0], (unsigned long long)ends[1]);
     const uint64_t Vpad = (V + 15) / 16 * 16;
-    const uint32_t nblk = (uint32_t)((Vpad + kVertsPerBlock - 1) / kVertsPerBlock);
+    const uint32_t nblk = (uint32_t)((Vpad + verts_per_block(1) - 1) / verts_per_block(1));
     uint8_t *flag = base + kNaFlag;
     uint32_t *list = (uint32_t *)(base + kNaList);
     uint64_t *hub = (uint64_t *)(base + kNaHub);
@@ -2821,11 +2823,11 @@ static void noalloc_bfs(const uint64_t *d_ptr, const uint32_t *d_cols, uint64_t 
     hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d_levels, flag, src);
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, ctl, 0u, 0u, (uint64_t)0, V, 0u);
     auto compact = [&] {
-        hipLaunchKernelGGL(k_frontier_count, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, V,
+        hipLaunchKernelGGL(k_frontier_count<1>, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, V,
                            d_ptr, (const uint64_t *)nullptr, tn, (const uint32_t *)nullptr, ctl, bcnt, bdeg, nblk);
         hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, bcnt, bdeg, nblk, info, doff, ctl,
                            (uint64_t *)nullptr);
-        hipLaunchKernelGGL(k_frontier_write, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, d_ptr,
+        hipLaunchKernelGGL(k_frontier_write<1>, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, d_ptr,
                            (const uint64_t *)nullptr, tn, bcnt, bdeg, nblk, info, list, hub, kNoAllocHubCap, doff,
                            (uint32_t *)nullptr, (uint64_t *)nullptr, (uint32_t *)nullptr, (uint64_t)0);
     };
