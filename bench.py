@@ -594,7 +594,62 @@ def wl_serve(args):
         g.close()
 
 
-WORKLOADS = {"bfs-hbm": wl_bfs_hbm, "cc-ooc": wl_cc_ooc, "pr-ooc": wl_pr_ooc, "msbfs": wl_msbfs, "serve": wl_serve}
+def wl_sssp_batch(args):
+    """The query server's batched SSSP engine against single-source runs:
+    16 SSSP queries submitted at once to a server over one device-resident
+    R-MAT graph (scale 24 by default, ef 16, seeds 1006 / 2006, sources seed
+    5), for several bucket widths with and without the light/heavy split,
+    against 16 consecutive og_sssp runs (split 8, delta 256).  Every result's
+    checksum is compared with the single-source run's."""
+    import onegraph_amd as og
+
+    g = og.Graph.rmat(args.scale, args.edge_factor, seed=1006, weight_seed=2006)
+    ptr, to, w, _ = g.export()
+    E = len(to)
+    cand = np.flatnonzero(np.diff(np.append(ptr.astype(np.int64), E)) > 0)
+    srcs = [int(x) for x in np.random.default_rng(5).choice(cand, 16, replace=False)]
+    log(f"rmat scale {args.scale} E={E} ready")
+    g.prepare("ascetic", edge_budget_bytes=E * 8, advisek=1.0, split_weight=8)
+    g.set_tuning(delta=256)
+    g.sssp(srcs[0], want_values=False)
+    ref, t = {}, time.perf_counter()
+    for s in srcs:
+        ref[s] = g.sssp(s, want_values=False)[1]["checksum"]
+    plain_ms = 1e3 * (time.perf_counter() - t)
+    rows = []
+    for split, sd in ((0, 64), (8, 64), (8, 128), (8, 256), (8, 512)):
+        g.prepare("ascetic", edge_budget_bytes=E * 8, advisek=1.0, split_weight=split)
+        with og.Server([g], sssp_delta=sd) as srv:
+            first = srv.submit(0, og.QUERY_SSSP, srcs[0])
+            while not srv.poll(16, timeout_ms=100):
+                pass
+            r0 = srv.info()["rounds"]
+            t = time.perf_counter()
+            ids = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs]
+            got = {}
+            while len(got) < len(ids):
+                for r in srv.poll(16, timeout_ms=100):
+                    got[r["id"]] = r
+                if time.perf_counter() - t > 120:
+                    raise RuntimeError("batched SSSP did not finish")
+            ms = 1e3 * (time.perf_counter() - t)
+            rows.append({"split_weight": split, "delta": sd, "ms": round(ms, 2), "rounds": srv.info()["rounds"] - r0,
+                         "checksums_equal": all(got[i]["checksum"] == ref[s] for i, s in zip(ids, srcs))})
+        log(f"batched split {split} delta {sd}: {rows[-1]}")
+    best = min(rows, key=lambda r: r["ms"])
+    out = base_record(round(16 / (best["ms"] * 1e-3), 2), "queries/s", 1, 1, best["ms"], "u32",
+                      {"workload": f"16 concurrent sssp queries, batched server engine, rmat{args.scale} ef16",
+                       "V": g.num_vertices, "E": E, "graph_in_hbm": True})
+    out.update({"batched": rows, "plain_16_ms": round(plain_ms, 2),
+                "plain_queries_per_s": round(16 / (plain_ms * 1e-3), 2),
+                "speedup_vs_plain": round(plain_ms / best["ms"], 2),
+                "parity": {"checksums_equal": all(r["checksums_equal"] for r in rows)}})
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
+WORKLOADS = {"bfs-hbm": wl_bfs_hbm, "cc-ooc": wl_cc_ooc, "pr-ooc": wl_pr_ooc, "msbfs": wl_msbfs, "serve": wl_serve,
+             "sssp-batch": wl_sssp_batch}
 
 
 def main():
@@ -637,7 +692,7 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--workload", default="sssp-ooc",
-                    choices=["sssp-ooc", "bfs-hbm", "cc-ooc", "pr-ooc", "msbfs", "serve"],
+                    choices=["sssp-ooc", "bfs-hbm", "cc-ooc", "pr-ooc", "msbfs", "serve", "sssp-batch"],
                     help="sssp-ooc is the headline line; the others measure the remaining configurations")
     ap.add_argument("--sources", type=int, default=1024, help="msbfs: number of sources")
     ap.add_argument("--plain-sample", type=int, default=32,
@@ -658,7 +713,8 @@ def main():
                          "queue instead of sharing one of HIP's default 4; at most 32)")
     args = ap.parse_args()
     if args.scale is None:
-        args.scale = {"sssp-ooc": 24, "bfs-hbm": 20, "cc-ooc": 25, "pr-ooc": 25, "msbfs": 26, "serve": 0}[args.workload]
+        args.scale = {"sssp-ooc": 24, "bfs-hbm": 20, "cc-ooc": 25, "pr-ooc": 25, "msbfs": 26, "serve": 0,
+                      "sssp-batch": 24}[args.workload]
     if args.workload == "serve" and args.hw_queues:
         # before the first HIP call: the server's six engine streams (BFS and
         # SSSP per graph) each get a hardware queue, so one engine's kernels do
