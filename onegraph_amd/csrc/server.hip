@@ -254,7 +254,8 @@ constexpr uint32_t kSsWin = 8192;             // vertices per workgroup of the s
 constexpr uint32_t kSsPer = kSsWin / kBlock;  // per thread: vertices base + t + kBlock * k, k < kSsPer
 constexpr uint32_t kSsRun = 4;                // work items per lane per chunk (consecutive)
 constexpr uint32_t kSsChunk = kBlock * kSsRun;  // work items per relax workgroup iteration
-constexpr unsigned kSsRelaxGridCap = 1u << 20;  // relax workgroups at most (one chunk each below it)
+constexpr unsigned kSsRelaxGridCap = 4096;    // relax workgroups at most (one chunk each below it); each
+                                              // workgroup adds its per-slot sums with 48 global atomics
 
 struct SsCtl {
     uint32_t thresh[kSs];  // near-far threshold of each slot (0xFFFFFFFF: every pending entry is near)
@@ -1175,10 +1176,9 @@ void SsspEngine::step() {
     hipLaunchKernelGGL(k_ss_write, dim3(nblk), dim3(kBlock), 0, s, pend, hpend, d->d_ptr, pmid, V, bnv, bdeg, alist,
                        amask, doff, cstart);
     OG_HIP(hipGetLastError());
-    // The relax is sized from the summary: one chunk per workgroup, so its
-    // workgroups retire as they finish and the other engines' kernels (own
-    // streams and hardware queues) get CUs in between, instead of waiting
-    // behind a grid that holds every CU until the whole relax is done.
+    // The relax is sized from the summary: one chunk per workgroup up to
+    // kSsRelaxGridCap workgroups (a small round launches only what it needs,
+    // so the other engines' kernels get CUs beside it), striding beyond.
     OG_HIP(hipEventSynchronize(ev_summary));
     const uint64_t nch = (h_ctl->dsum + kSsChunk - 1) / kSsChunk;
     if (nch) {
