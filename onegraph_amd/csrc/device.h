@@ -75,6 +75,18 @@ __device__ __forceinline__ void row_range(const uint64_t *__restrict__ ptr, cons
     }
 }
 
+// Degree of u's row in the round's range.  With a degree table (4 bytes per
+// vertex; with the split, light degrees then heavy degrees at + Vpad) one
+// load replaces the two 8-byte row bounds.
+__device__ __forceinline__ uint64_t round_deg(const uint32_t *__restrict__ deg, uint64_t Vpad,
+                                              const uint64_t *__restrict__ ptr, const uint64_t *__restrict__ pmid,
+                                              bool heavy, uint64_t u) {
+    if (deg) return deg[(heavy ? Vpad : 0) + u];
+    uint64_t b, e;
+    row_range(ptr, pmid, heavy, u, b, e);
+    return e - b;
+}
+
 // Degree-bin thresholds and shapes for the static relax (autotunable).
 struct Tuning {
     uint32_t t1 = 32;      // bin 0: 1 <= deg <= t1, lpv0 lanes per vertex
@@ -136,7 +148,7 @@ struct RoundCtl {
     uint32_t pad_;
 };
 
-constexpr int kBcntRows = 7;            // per-block rows: bins 0-3, zero|hub, far|lagging, min far
+constexpr int kBcntRows = 8;            // per-block rows: bins 0-3, zero|hub, far|lagging, min far, any flag
 constexpr uint8_t kCode0 = 2;           // count pass: flag byte = kCode0 + bin of the vertex
 
 // Buffers of one frontier traversal: next-frontier flags, the compacted bin
@@ -243,6 +255,9 @@ struct DeviceState {
     // graph on the device
     uint64_t *d_ptr = nullptr;       // V + 1 (device copy; USM: managed)
     uint64_t *d_pmid = nullptr;      // light/heavy split: V, end of the light edges of each row
+    uint32_t *d_deg = nullptr;       // degree table of the compaction: Vpad whole-row degrees, or with
+                                     // the split Vpad light then Vpad heavy degrees (null: row bounds)
+    bool deg_split = false;          // d_deg holds light and heavy degrees
     std::vector<uint64_t> h_pmid;    // host copy (the staged gather)
     uint32_t split_weight = 0;       // light: weight < split_weight (0: no split)
     const uint64_t *run_pmid = nullptr;  // d_pmid while a run uses the split (SSSP near-far), else null

@@ -121,6 +121,10 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
                             FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
                             uint64_t *__restrict__ m_doff, uint64_t *lds);
 
+// Lanes of the one-workgroup scan, one per block of counts (measured: runs
+// of 8 blocks on 256 lanes, with their loads batched, scan slower).
+constexpr uint32_t kScanThreads = 1024;
+
 // Compaction pass 1: classifies every flagged vertex of the block's window,
 // stores kCode0 + bin in its flag byte and publishes the per-block counts.
 // The window's flags are loaded 16 bytes per lane and transposed through
@@ -133,7 +137,8 @@ template <int FV>
 __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__ flag_a, uint8_t *__restrict__ hflag,
                                                            uint64_t Vpad, uint64_t vsplit,
                                                            const uint64_t *__restrict__ ptr,
-                                                           const uint64_t *__restrict__ pmid, Tuning tn,
+                                                           const uint64_t *__restrict__ pmid,
+                                                           const uint32_t *__restrict__ deg, Tuning tn,
                                                            const uint32_t *__restrict__ val_a,
                                                            const RoundCtl *__restrict__ ctl,
                                                            uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
@@ -151,6 +156,23 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
     const uint64_t wbase = (uint64_t)blockIdx.x * (kBlock * 16 * FV);
     uint4 w[FV];
     load_flags(flag, wbase + (uint64_t)threadIdx.x * (16 * FV), Vpad, w);
+    uint32_t nz = 0;
+#pragma unroll
+    for (int k = 0; k < FV; ++k) nz |= w[k].x | w[k].y | w[k].z | w[k].w;
+    // a window without flags (most of them in a small round) publishes zero
+    // counts without the transpose and the block scans; the write pass
+    // skips it by row 7
+    if (!__syncthreads_or(nz != 0)) {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) bcnt[r * nblk + blockIdx.x] = 0;
+            bcnt[6 * nblk + blockIdx.x] = 0xFFFFFFFFu;
+            bcnt[7 * nblk + blockIdx.x] = 0;
+            bdeg[0 * nblk + blockIdx.x] = 0;
+            bdeg[1 * nblk + blockIdx.x] = 0;
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < FV; ++k) s_win[threadIdx.x * FV + k] = w[k];
     __syncthreads();
@@ -169,44 +191,44 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
         const uint32_t fbound = lagged ? ctl->fbound : 0u;
         while (m) {
             uint32_t jj[kLoadBatch];
-            uint64_t pb[kLoadBatch], pe[kLoadBatch];
+            uint64_t dg[kLoadBatch];
             uint32_t vv[kLoadBatch];
             bool far[kLoadBatch];
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
                 jj[k] = (16 * FV);  // empty slot
-                pb[k] = pe[k] = 0;
+                dg[k] = 0;
                 vv[k] = 0;
                 if (m) {
                     const uint32_t j = (uint32_t)__builtin_ctz(m);
                     m &= m - 1;
                     jj[k] = j;
                     if (val) vv[k] = val[base + (uint64_t)j * kBlock];
-                    else row_range(ptr, pmid, heavy, base + (uint64_t)j * kBlock, pb[k], pe[k]);
+                    else dg[k] = round_deg(deg, Vpad, ptr, pmid, heavy, base + (uint64_t)j * kBlock);
                 }
             }
-            // near-far: a held-back vertex needs only its value, so the row
-            // bounds are read for the near ones alone
+            // near-far: a held-back vertex needs only its value, so the
+            // degrees are read for the near ones alone
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
                 far[k] = false;
                 if (jj[k] == (16 * FV) || !val) continue;
                 const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + (uint64_t)jj[k] * kBlock >= vsplit;
                 far[k] = vv[k] >= thresh || lag;
-                if (!far[k]) row_range(ptr, pmid, heavy, base + (uint64_t)jj[k] * kBlock, pb[k], pe[k]);
+                if (!far[k]) dg[k] = round_deg(deg, Vpad, ptr, pmid, heavy, base + (uint64_t)jj[k] * kBlock);
             }
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
                 if (jj[k] == (16 * FV)) continue;
-                const uint64_t deg = pe[k] - pb[k];
+                const uint64_t d = dg[k];
                 const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + (uint64_t)jj[k] * kBlock >= vsplit;
                 cl += lag ? 1u : 0u;
-                const int b = bin_of(far[k], base + (uint64_t)jj[k] * kBlock, vsplit, deg, tn);
+                const int b = bin_of(far[k], base + (uint64_t)jj[k] * kBlock, vsplit, d, tn);
                 switch (b) {
-                    case 0: ++c0; sdeg += deg; break;
-                    case 1: ++c1; sdeg += deg; break;
-                    case 2: c2 += (uint32_t)((deg + tn.tile - 1) / tn.tile); ++ch; sdeg += deg; break;
-                    case 3: ++c3; ddeg += deg; break;
+                    case 0: ++c0; sdeg += d; break;
+                    case 1: ++c1; sdeg += d; break;
+                    case 2: c2 += (uint32_t)((d + tn.tile - 1) / tn.tile); ++ch; sdeg += d; break;
+                    case 3: ++c3; ddeg += d; break;
                     case 5: ++cf; mf = min(mf, vv[k]); break;
                     default: ++cz; break;
                 }
@@ -227,6 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
         bcnt[4 * nblk + blockIdx.x] = (uint32_t)t[2] | ((uint32_t)(t[2] >> 32) << 16);  // both < 2^16 per block
         bcnt[5 * nblk + blockIdx.x] = (uint32_t)t[5];  // far | lagging << 16, both <= 8192 per block
         bcnt[6 * nblk + blockIdx.x] = bm;
+        bcnt[7 * nblk + blockIdx.x] = 1;
         bdeg[0 * nblk + blockIdx.x] = t[3];
         bdeg[1 * nblk + blockIdx.x] = t[4];
     }
@@ -236,11 +259,13 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
 // separate launch rather than a last-workgroup-arrives tail in pass 1: on
 // gfx950 the device-scope release/acquire fences such a tail needs write
 // back and invalidate the whole L2 of each XCD, once per workgroup.
-__global__ __launch_bounds__(1024) void k_frontier_scan(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
-                                                        uint32_t nblk, FrontierInfo *__restrict__ info,
-                                                        uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
-                                                        uint64_t *__restrict__ m_doff) {
-    __shared__ uint64_t lds[8 * 1024 / kWave];
+__global__ __launch_bounds__(kScanThreads) void k_frontier_scan(uint32_t *__restrict__ bcnt,
+                                                                uint64_t *__restrict__ bdeg, uint32_t nblk,
+                                                                FrontierInfo *__restrict__ info,
+                                                                uint64_t *__restrict__ doff,
+                                                                RoundCtl *__restrict__ ctl,
+                                                                uint64_t *__restrict__ m_doff) {
+    __shared__ uint64_t lds[8 * kScanThreads / kWave];
     scan_counts(bcnt, bdeg, nblk, info, doff, ctl, m_doff, lds);
 }
 
@@ -383,7 +408,8 @@ __device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ 
 template <int FV>
 __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__ flag_a, uint8_t *__restrict__ hflag,
                                                            uint64_t Vpad, const uint64_t *__restrict__ ptr,
-                                                           const uint64_t *__restrict__ pmid, Tuning tn,
+                                                           const uint64_t *__restrict__ pmid,
+                                                           const uint32_t *__restrict__ deg, Tuning tn,
                                                            const uint32_t *__restrict__ bcnt,
                                                            const uint64_t *__restrict__ bdeg, uint32_t nblk,
                                                            const FrontierInfo *__restrict__ info,
@@ -395,6 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
     __shared__ uint64_t lds[3 * kBlock / kWave];
     __shared__ uint32_t s_win[4 * FV * kBlock];  // word i of thread t at [i * kBlock + t]
     if (!info->live) return;  // nothing is flagged: no list, and the window is already clear
+    if (!bcnt[7 * nblk + blockIdx.x]) return;  // the count pass found no flag in this window
     // the array and row ranges the count pass used (light/heavy split)
     const bool heavy = pmid && info->heavy;
     uint8_t *flag = heavy ? hflag : flag_a;
@@ -416,28 +443,28 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
     const uint32_t all = window_mask(w, kCode0);
     uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
     uint64_t ddeg = 0;
-    // walks the set bits kLoadBatch at a time; row bounds read for bins 2 / 3
+    // walks the set bits kLoadBatch at a time; degrees read for bins 2 / 3
     auto walk = [&](auto &&use) {
         for (uint32_t m = all; m;) {
             uint32_t jj[kLoadBatch];
             int bb[kLoadBatch];
-            uint64_t pb[kLoadBatch], pe[kLoadBatch];
+            uint64_t dg[kLoadBatch];
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
                 jj[k] = (16 * FV);
                 bb[k] = -1;
-                pb[k] = pe[k] = 0;
+                dg[k] = 0;
                 if (m) {
                     const uint32_t j = (uint32_t)__builtin_ctz(m);
                     m &= m - 1;
                     jj[k] = j;
                     bb[k] = bin_at(j);
-                    if (bb[k] == 2 || bb[k] == 3) row_range(ptr, pmid, heavy, base + j, pb[k], pe[k]);
+                    if (bb[k] == 2 || bb[k] == 3) dg[k] = round_deg(deg, Vpad, ptr, pmid, heavy, base + j);
                 }
             }
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k)
-                if (jj[k] != (16 * FV)) use(jj[k], bb[k], pe[k] - pb[k]);
+                if (jj[k] != (16 * FV)) use(jj[k], bb[k], dg[k]);
         }
     };
     walk([&](uint32_t, int b, uint64_t deg) {
@@ -1183,25 +1210,29 @@ void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t
     // PCIe write per vertex: slower than one bulk copy back, so unused)
     uint32_t *m_dlist = nullptr;
     uint64_t *m_doff = nullptr;
+    // the degree table when its layout is the run's (whole rows, or light
+    // and heavy parts), else the row bounds
+    const uint32_t *deg = d->d_deg && (pmid != nullptr) == d->deg_split ? d->d_deg : nullptr;
     if (f.fv == 1)
         hipLaunchKernelGGL(k_frontier_count<1>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad,
-                           d->plan.vsplit, d->d_ptr, pmid, d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
+                           d->plan.vsplit, d->d_ptr, pmid, deg, d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
     else
         hipLaunchKernelGGL(k_frontier_count<2>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad,
-                           d->plan.vsplit, d->d_ptr, pmid, d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
-    // one lane per block of counts up to 1024 lanes: a small graph's scan
-    // runs on a wave or two instead of sixteen
-    const unsigned scan_threads = (unsigned)std::min<uint64_t>(1024, (std::max<uint64_t>(f.nblk, 1) + kWave - 1) / kWave * kWave);
+                           d->plan.vsplit, d->d_ptr, pmid, deg, d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
+    // one lane per block of counts up to kScanThreads lanes: a small graph's
+    // scan runs on one wave
+    const unsigned scan_threads =
+        (unsigned)std::min<uint64_t>(kScanThreads, (std::max<uint64_t>(f.nblk, 1) + kWave - 1) / kWave * kWave);
     hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(scan_threads), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff,
                        f.ctl, m_doff);
     if (f.fv == 1)
         hipLaunchKernelGGL(k_frontier_write<1>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr,
-                           pmid, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist,
-                           m_doff, f.dchunk, f.dchunk_cap);
+                           pmid, deg, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff,
+                           m_dlist, m_doff, f.dchunk, f.dchunk_cap);
     else
         hipLaunchKernelGGL(k_frontier_write<2>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr,
-                           pmid, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist,
-                           m_doff, f.dchunk, f.dchunk_cap);
+                           pmid, deg, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff,
+                           m_dlist, m_doff, f.dchunk, f.dchunk_cap);
     OG_HIP(hipGetLastError());
     if (to_host) OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
 }
@@ -1303,51 +1334,67 @@ static void upload_packed(DeviceState *d, const og_graph *g, uint64_t vend, uint
 }
 
 // Zero-copy loading reads dynamic rows in place, so the staging share of the
-// edge budget (cap edges) holds part of the dynamic region in HBM: whole
-// chunks of kResChunk consecutive vertices, chosen by the link lines their
-// rows would cost per edge.  A row of b bytes at an arbitrary 4-byte offset
-// touches (b + kLinkLine - 4) / kLinkLine lines of kLinkLine bytes on average,
-// each fetched whole, so chunks of short rows are taken first (ties: lower
-// ids).  The chunk table (8 bytes per chunk) is charged to the same share.
+// edge budget (cap edges) holds part of the dynamic region in HBM: blocks of
+// kResChunk consecutive vertices, chosen by the link lines their rows would
+// cost per edge.  A row of b bytes at an arbitrary 4-byte offset touches
+// (b + kLinkLine - 4) / kLinkLine lines of kLinkLine bytes on average, each
+// fetched whole, so blocks of short rows are taken first (ties: lower ids).
+// With the light/heavy split each chunk has two blocks, its light parts and
+// its heavy parts, ranked by the same measure and chosen separately.
+// (Ranking every light block first, for the light rounds' repeated reads,
+// measured slower on R-MAT: the short heavy rows it leaves on the host cost
+// the heavy round more.)  The chunk table (8 bytes per block) is charged to
+// the same share.
 static void stage_resident(DeviceState *d, const og_graph *g, uint64_t vsplit, uint64_t cap,
                            const std::vector<uint64_t> &lpos) {
     const uint64_t V = g->V, eb = d->eb;
     const bool split = !lpos.empty();  // two blocks per chunk: light parts, heavy parts
+    const uint64_t nb = split ? 2 : 1;  // blocks per chunk
     const uint64_t c0 = vsplit >> kResChunkShift, c1 = (V + kResChunk - 1) >> kResChunkShift;
-    const uint64_t table_edges = ((split ? 2 : 1) * c1 * sizeof(int64_t) + eb - 1) / eb;
+    const uint64_t table_edges = (nb * c1 * sizeof(int64_t) + eb - 1) / eb;
     if (vsplit >= V || cap <= table_edges) return;
     cap -= table_edges;
     const uint64_t nch = c1 - c0;
     auto first = [&](uint64_t i) { return std::max<uint64_t>((c0 + i) << kResChunkShift, vsplit); };
     auto last = [&](uint64_t i) { return std::min<uint64_t>((c0 + i + 1) << kResChunkShift, V); };
-    std::vector<uint64_t> cedges(nch);
-    std::vector<double> score(nch);
+    // block k = nb * i + part of chunk i (part 0: light or whole rows, 1: heavy)
+    const uint64_t nblk = nb * nch;
+    std::vector<uint64_t> bedges(nblk);
+    std::vector<double> score(nblk);
+    auto lines_of = [&](uint64_t n) { return n ? (double)(n * eb + kLinkLine - 4) / kLinkLine : 0.0; };
     parallel_for(nch, 1 << 12, 0, [&](uint64_t b, uint64_t e, unsigned) {
         for (uint64_t i = b; i < e; ++i) {
-            double lines = 0;
+            double lines[2] = {0, 0};
+            uint64_t n[2] = {0, 0};
             for (uint64_t v = first(i); v < last(i); ++v) {
-                const uint64_t n = g->ptr[v + 1] - g->ptr[v];
-                if (n) lines += (double)(n * eb + kLinkLine - 4) / kLinkLine;
+                const uint64_t deg = g->ptr[v + 1] - g->ptr[v];
+                const uint64_t nl = split ? lpos[v - vsplit + 1] - lpos[v - vsplit] : deg;
+                lines[0] += lines_of(nl);
+                n[0] += nl;
+                lines[1] += lines_of(deg - nl);
+                n[1] += deg - nl;
             }
-            cedges[i] = g->ptr[last(i)] - g->ptr[first(i)];
-            score[i] = cedges[i] ? lines / (double)cedges[i] : 0.0;
+            for (uint64_t k = 0; k < nb; ++k) {
+                bedges[nb * i + k] = n[k];
+                score[nb * i + k] = n[k] ? lines[k] / (double)n[k] : 0.0;
+            }
         }
     });
-    std::vector<uint32_t> order(nch);  // nch <= 2^27: vertex ids are 32-bit
-    for (uint64_t i = 0; i < nch; ++i) order[i] = (uint32_t)i;
+    std::vector<uint32_t> order(nblk);  // nblk <= 2^28
+    for (uint64_t i = 0; i < nblk; ++i) order[i] = (uint32_t)i;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return score[x] > score[y]; });
-    std::vector<uint8_t> take(nch, 0);
+    std::vector<uint8_t> take(nblk, 0);
     uint64_t used = 0;
-    for (uint32_t i : order) {
-        if (!cedges[i] || used + cedges[i] > cap) continue;
-        take[i] = 1;
-        used += cedges[i];
+    for (uint32_t k : order) {
+        if (!bedges[k] || used + bedges[k] > cap) continue;
+        take[k] = 1;
+        used += bedges[k];
     }
     if (!used) return;
-    // The resident buffer holds the chosen chunks in vertex order (with the
-    // split layout, each chunk's light block then its heavy block); runs of
-    // host positions that follow each other are gathered in one piece.
-    // Positions are in the host copy of the dynamic region.
+    // The resident buffer holds the chosen blocks in block order (a chunk's
+    // light block before its heavy block); runs of host positions that follow
+    // each other are gathered in one piece.  Positions are in the host copy
+    // of the dynamic region (with the split: light parts, then heavy parts).
     struct Run {
         uint64_t src, dst, n;
     };
@@ -1360,24 +1407,18 @@ static void stage_resident(DeviceState *d, const og_graph *g, uint64_t vsplit, u
             runs.push_back({src, dst, n});
     };
     const uint64_t dbase = d->dyn_base, nl = split ? lpos[V - vsplit] : 0;
-    std::vector<int64_t> delta((split ? 2 : 1) * c1, kNotResident);
+    std::vector<int64_t> delta(nb * c1, kNotResident);
     uint64_t pos = 0;
-    for (uint64_t i = 0; i < nch; ++i) {
-        if (!take[i]) continue;
-        const uint64_t v0 = first(i), v1 = last(i);
-        if (!split) {
-            const uint64_t h0 = g->ptr[v0] - dbase;
-            delta[c0 + i] = (int64_t)pos - (int64_t)h0;
-            add_run(h0, pos, cedges[i]);
-        } else {
-            const uint64_t l0 = lpos[v0 - vsplit], l1 = lpos[v1 - vsplit];
-            const uint64_t hh0 = nl + (g->ptr[v0] - dbase) - l0;
-            delta[2 * (c0 + i)] = (int64_t)pos - (int64_t)l0;
-            delta[2 * (c0 + i) + 1] = (int64_t)(pos + (l1 - l0)) - (int64_t)hh0;
-            add_run(l0, pos, l1 - l0);
-            add_run(hh0, pos + (l1 - l0), cedges[i] - (l1 - l0));
-        }
-        pos += cedges[i];
+    for (uint64_t k = 0; k < nblk; ++k) {
+        if (!take[k]) continue;
+        const uint64_t i = k / nb, v0 = first(i);
+        uint64_t h0;  // host position of the block's first edge
+        if (!split) h0 = g->ptr[v0] - dbase;
+        else if (k % nb == 0) h0 = lpos[v0 - vsplit];
+        else h0 = nl + (g->ptr[v0] - dbase) - lpos[v0 - vsplit];
+        delta[nb * c0 + k] = (int64_t)pos - (int64_t)h0;
+        add_run(h0, pos, bedges[k]);
+        pos += bedges[k];
     }
     MemPool tmp;
     uint8_t *hb = (uint8_t *)tmp.palloc(used * eb);
@@ -1392,6 +1433,32 @@ static void stage_resident(DeviceState *d, const og_graph *g, uint64_t vsplit, u
     OG_HIP(hipMemcpyAsync(d->d_rdelta, delta.data(), delta.size() * sizeof(int64_t), hipMemcpyHostToDevice,
                           d->s_compute));
     OG_HIP(hipStreamSynchronize(d->s_compute));
+}
+
+// The compaction's degree table (DeviceState::d_deg): 4 bytes per vertex
+// and range in place of two 8-byte row bounds.  Not built when a degree
+// exceeds 32 bits (the passes then read the row bounds).
+static void upload_degrees(DeviceState *d, const og_graph *g) {
+    const uint64_t V = g->V, Vp = d->Vpad;
+    const bool split = d->split_weight != 0;
+    std::atomic<bool> wide{false};
+    std::vector<uint32_t> dg((split ? 2 : 1) * Vp, 0);
+    parallel_for(V, 1 << 16, 0, [&](uint64_t b, uint64_t e, unsigned) {
+        bool w = false;
+        for (uint64_t v = b; v < e; ++v) {
+            const uint64_t b0 = g->ptr[v], e0 = g->ptr[v + 1];
+            const uint64_t m = split ? d->h_pmid[v] : e0;
+            w |= (e0 - b0) >> 32 != 0;
+            dg[v] = (uint32_t)(m - b0);
+            if (split) dg[Vp + v] = (uint32_t)(e0 - m);
+        }
+        if (w) wide = true;
+    });
+    if (wide) return;
+    d->d_deg = (uint32_t *)d->dalloc(dg.size() * sizeof(uint32_t));
+    OG_HIP(hipMemcpyAsync(d->d_deg, dg.data(), dg.size() * sizeof(uint32_t), hipMemcpyHostToDevice, d->s_compute));
+    OG_HIP(hipStreamSynchronize(d->s_compute));
+    d->deg_split = split;
 }
 
 static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params *params, og_plan_info *info) {
@@ -1501,6 +1568,7 @@ static void prepare_impl(og_graph *g, og_approach approach, const og_plan_params
             OG_HIP(hipMemcpyAsync(d->d_pmid, d->h_pmid.data(), V * sizeof(uint64_t), hipMemcpyHostToDevice,
                                   d->s_compute));
         }
+        upload_degrees(d.get(), g);
         if (pl.static_edges) {
             d->d_static = d->dalloc(pl.static_edges * eb);
             d->static_used = g->ptr[pl.vsplit];
@@ -2824,11 +2892,13 @@ static void noalloc_bfs(const uint64_t *d_ptr, const uint32_t *d_cols, uint64_t 
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, ctl, 0u, 0u, (uint64_t)0, V, 0u);
     auto compact = [&] {
         hipLaunchKernelGGL(k_frontier_count<1>, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, V,
-                           d_ptr, (const uint64_t *)nullptr, tn, (const uint32_t *)nullptr, ctl, bcnt, bdeg, nblk);
-        hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(1024), 0, s, bcnt, bdeg, nblk, info, doff, ctl,
+                           d_ptr, (const uint64_t *)nullptr, (const uint32_t *)nullptr, tn, (const uint32_t *)nullptr,
+                           ctl, bcnt, bdeg, nblk);
+        hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(kScanThreads), 0, s, bcnt, bdeg, nblk, info, doff, ctl,
                            (uint64_t *)nullptr);
         hipLaunchKernelGGL(k_frontier_write<1>, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, d_ptr,
-                           (const uint64_t *)nullptr, tn, bcnt, bdeg, nblk, info, list, hub, kNoAllocHubCap, doff,
+                           (const uint64_t *)nullptr, (const uint32_t *)nullptr, tn, bcnt, bdeg, nblk, info, list,
+                           hub, kNoAllocHubCap, doff,
                            (uint32_t *)nullptr, (uint64_t *)nullptr, (uint32_t *)nullptr, (uint64_t)0);
     };
     RoundRelax r{};
