@@ -148,8 +148,20 @@ struct RoundCtl {
     uint32_t pad_;
 };
 
-constexpr int kBcntRows = 8;            // per-block rows: bins 0-3, zero|hub, far|lagging, min far, any flag
+constexpr int kBcntRows = 5;            // per-block rows: bins 0-3, any flag
+constexpr uint32_t kSbBlocks = 64;      // compaction blocks per super-block (one lane each in the write pass)
+constexpr int kSbVals = 9;              // super-block totals: bins 0|1, bins 2|3, dynamic degrees, zero, hub,
+                                        // far, static degrees, lagging, min far
 constexpr uint8_t kCode0 = 2;           // count pass: flag byte = kCode0 + bin of the vertex
+
+// Per-block and per-super-block counts of the compaction passes.
+struct BlockCounts {
+    uint32_t *bcnt;
+    uint64_t *bdeg;
+    uint64_t *sbt;
+    uint64_t *sbp;
+    uint32_t nblk, nsb;
+};
 
 // Buffers of one frontier traversal: next-frontier flags, the compacted bin
 // lists, the dynamic degree prefix and the per-block scan scratch.  A graph's
@@ -165,8 +177,13 @@ struct Frontier {
     uint32_t *dchunk = nullptr;      // list entry holding edge c * kDynEdgesPerBlock of the dynamic edge space
     uint64_t dchunk_cap = 0;
     uint32_t *bcnt = nullptr;        // kBcntRows * nblk per-block counts
-    uint64_t *bdeg = nullptr;        // 2 * nblk per-block degree sums
+    uint64_t *bdeg = nullptr;        // nblk per-block dynamic degree sums
+    uint64_t *sbt = nullptr;         // kSbVals * nsb super-block totals (added by the count pass; zero, the
+                                     // min row all ones, between compactions)
+    uint64_t *sbp = nullptr;         // 3 * nsb super-block exclusive prefixes (bins 0|1, 2|3, dynamic degrees)
     uint32_t nblk = 0;
+    uint32_t nsb = 0;
+    BlockCounts counts() const { return BlockCounts{bcnt, bdeg, sbt, sbp, nblk, nsb}; }
     uint32_t fv = 2;                 // flag vectors per thread of the compaction (frontier_fv)
     uint64_t V = 0;
     FrontierInfo *info = nullptr;

@@ -117,13 +117,18 @@ __device__ __forceinline__ uint32_t window_mask(const uint4 (&w)[FV], uint32_t l
     return m;
 }
 
-__device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg, uint32_t nblk,
-                            FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
-                            uint64_t *__restrict__ m_doff, uint64_t *lds);
+__device__ void scan_counts(const BlockCounts &bc, FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff,
+                            RoundCtl *__restrict__ ctl, uint64_t *__restrict__ m_doff, uint64_t *lds);
 
-// Lanes of the one-workgroup scan, one per block of counts (measured: runs
-// of 8 blocks on 256 lanes, with their loads batched, scan slower).
+// Most lanes of the one-workgroup scan, one per super-block
 constexpr uint32_t kScanThreads = 1024;
+
+// Super-block totals between compactions: sums zero, the min row all ones.
+__global__ void k_sb_init(uint64_t *sbt, uint32_t nsb) {
+    const uint64_t n = (uint64_t)kSbVals * nsb;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        sbt[i] = i / nsb == kSbVals - 1 ? ~0ull : 0ull;
+}
 
 // Compaction pass 1: classifies every flagged vertex of the block's window,
 // stores kCode0 + bin in its flag byte and publishes the per-block counts.
@@ -140,9 +145,8 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                                                            const uint64_t *__restrict__ pmid,
                                                            const uint32_t *__restrict__ deg, Tuning tn,
                                                            const uint32_t *__restrict__ val_a,
-                                                           const RoundCtl *__restrict__ ctl,
-                                                           uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg,
-                                                           uint32_t nblk) {
+                                                           const RoundCtl *__restrict__ ctl, BlockCounts bc) {
+    const uint32_t nblk = bc.nblk;
     __shared__ uint64_t lds[6 * kBlock / kWave];
     __shared__ uint4 s_win[(kBlock * 16 * FV) / 16];  // the window's flag bytes
     // a round enqueued after the traversal ended: no vertex is flagged and
@@ -165,11 +169,8 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
     if (!__syncthreads_or(nz != 0)) {
         if (threadIdx.x == 0) {
 #pragma unroll
-            for (int r = 0; r < 6; ++r) bcnt[r * nblk + blockIdx.x] = 0;
-            bcnt[6 * nblk + blockIdx.x] = 0xFFFFFFFFu;
-            bcnt[7 * nblk + blockIdx.x] = 0;
-            bdeg[0 * nblk + blockIdx.x] = 0;
-            bdeg[1 * nblk + blockIdx.x] = 0;
+            for (int r = 0; r < kBcntRows; ++r) bc.bcnt[r * nblk + blockIdx.x] = 0;
+            bc.bdeg[blockIdx.x] = 0;
         }
         return;
     }
@@ -242,16 +243,29 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
     block_scan_n<6>(x, ex, t, lds);
     const uint32_t bm = block_min_u32(mf, lds);
     if (threadIdx.x == 0) {
-        bcnt[0 * nblk + blockIdx.x] = (uint32_t)t[0];
-        bcnt[1 * nblk + blockIdx.x] = (uint32_t)(t[0] >> 32);
-        bcnt[2 * nblk + blockIdx.x] = (uint32_t)t[1];
-        bcnt[3 * nblk + blockIdx.x] = (uint32_t)(t[1] >> 32);
-        bcnt[4 * nblk + blockIdx.x] = (uint32_t)t[2] | ((uint32_t)(t[2] >> 32) << 16);  // both < 2^16 per block
-        bcnt[5 * nblk + blockIdx.x] = (uint32_t)t[5];  // far | lagging << 16, both <= 8192 per block
-        bcnt[6 * nblk + blockIdx.x] = bm;
-        bcnt[7 * nblk + blockIdx.x] = 1;
-        bdeg[0 * nblk + blockIdx.x] = t[3];
-        bdeg[1 * nblk + blockIdx.x] = t[4];
+        bc.bcnt[0 * nblk + blockIdx.x] = (uint32_t)t[0];
+        bc.bcnt[1 * nblk + blockIdx.x] = (uint32_t)(t[0] >> 32);
+        bc.bcnt[2 * nblk + blockIdx.x] = (uint32_t)t[1];
+        bc.bcnt[3 * nblk + blockIdx.x] = (uint32_t)(t[1] >> 32);
+        bc.bcnt[4 * nblk + blockIdx.x] = 1;
+        bc.bdeg[blockIdx.x] = t[4];
+    }
+    // the block's totals into its super-block's, one lane per value (packed
+    // pairs add without carries: each half stays below 2^32)
+    if (threadIdx.x < (unsigned)kSbVals) {
+        const uint32_t sb = blockIdx.x / kSbBlocks;
+        const uint64_t v[kSbVals] = {t[0], t[1], t[4], (uint32_t)t[2], t[2] >> 32, t[5] & 0xFFFFu, t[3],
+                                     t[5] >> 16, bm};
+        uint64_t x = 0;
+#pragma unroll
+        for (int i = 0; i < kSbVals; ++i)
+            if ((int)threadIdx.x == i) x = v[i];
+        unsigned long long *at = (unsigned long long *)bc.sbt + threadIdx.x * (uint64_t)bc.nsb + sb;
+        if (threadIdx.x == kSbVals - 1) {
+            if (x != 0xFFFFFFFFull) atomicMin(at, (unsigned long long)x);
+        } else if (x) {
+            atomicAdd(at, (unsigned long long)x);
+        }
     }
 }
 
@@ -259,14 +273,12 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
 // separate launch rather than a last-workgroup-arrives tail in pass 1: on
 // gfx950 the device-scope release/acquire fences such a tail needs write
 // back and invalidate the whole L2 of each XCD, once per workgroup.
-__global__ __launch_bounds__(kScanThreads) void k_frontier_scan(uint32_t *__restrict__ bcnt,
-                                                                uint64_t *__restrict__ bdeg, uint32_t nblk,
-                                                                FrontierInfo *__restrict__ info,
+__global__ __launch_bounds__(kScanThreads) void k_frontier_scan(BlockCounts bc, FrontierInfo *__restrict__ info,
                                                                 uint64_t *__restrict__ doff,
                                                                 RoundCtl *__restrict__ ctl,
                                                                 uint64_t *__restrict__ m_doff) {
     __shared__ uint64_t lds[8 * kScanThreads / kWave];
-    scan_counts(bcnt, bdeg, nblk, info, doff, ctl, m_doff, lds);
+    scan_counts(bc, info, doff, ctl, m_doff, lds);
 }
 
 // Compaction pass 2 body: one workgroup turns the per-block counts into
@@ -276,43 +288,42 @@ __global__ __launch_bounds__(kScanThreads) void k_frontier_scan(uint32_t *__rest
 // are held back, the near-far threshold (min held-back value + delta) for
 // the next compaction.  Each thread scans a run of consecutive blocks, so the
 // whole pass is one multi-value block scan.  `lds` holds 8 * blockDim / 64.
-__device__ void scan_counts(uint32_t *__restrict__ bcnt, uint64_t *__restrict__ bdeg, uint32_t nblk,
-                            FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff, RoundCtl *__restrict__ ctl,
-                            uint64_t *__restrict__ m_doff, uint64_t *lds) {
-    const uint32_t per = (nblk + blockDim.x - 1) / blockDim.x;
-    const uint32_t b0 = min(nblk, threadIdx.x * per), b1 = min(nblk, b0 + per);
+__device__ void scan_counts(const BlockCounts &bc, FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff,
+                            RoundCtl *__restrict__ ctl, uint64_t *__restrict__ m_doff, uint64_t *lds) {
+    const uint32_t nsb = bc.nsb;
+    const uint64_t *sbt = bc.sbt;
+    const uint32_t per = (nsb + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = min(nsb, threadIdx.x * per), b1 = min(nsb, b0 + per);
     uint64_t s01 = 0, s23 = 0, sdd = 0, zs = 0, hs = 0, fs = 0, ss = 0, ls = 0;
     uint32_t fm = 0xFFFFFFFFu;
     for (uint32_t b = b0; b < b1; ++b) {
-        s01 += (uint64_t)bcnt[0 * nblk + b] | ((uint64_t)bcnt[1 * nblk + b] << 32);
-        s23 += (uint64_t)bcnt[2 * nblk + b] | ((uint64_t)bcnt[3 * nblk + b] << 32);
-        const uint32_t zh = bcnt[4 * nblk + b];
-        zs += zh & 0xFFFFu;
-        hs += zh >> 16;
-        const uint32_t fl = bcnt[5 * nblk + b];
-        fs += fl & 0xFFFFu;
-        ls += fl >> 16;
-        fm = min(fm, bcnt[6 * nblk + b]);
-        ss += bdeg[b];
-        sdd += bdeg[nblk + b];
+        s01 += sbt[0 * nsb + b];
+        s23 += sbt[1 * nsb + b];
+        sdd += sbt[2 * nsb + b];
+        zs += sbt[3 * nsb + b];
+        hs += sbt[4 * nsb + b];
+        fs += sbt[5 * nsb + b];
+        ss += sbt[6 * nsb + b];
+        ls += sbt[7 * nsb + b];
+        fm = (uint32_t)min((uint64_t)fm, sbt[8 * nsb + b]);
     }
     const uint64_t x[8] = {s01, s23, sdd, zs, hs, fs, ss, ls};
     uint64_t ex[8], t[8];
     block_scan_n<8>(x, ex, t, lds);
     const uint32_t fmin = block_min_u32(fm, lds);
+    // super-block prefixes for the write pass, and the totals cleared for
+    // the next compaction
     uint64_t r01 = ex[0], r23 = ex[1], rdd = ex[2];
     for (uint32_t b = b0; b < b1; ++b) {
-        const uint64_t a01 = (uint64_t)bcnt[0 * nblk + b] | ((uint64_t)bcnt[1 * nblk + b] << 32);
-        const uint64_t a23 = (uint64_t)bcnt[2 * nblk + b] | ((uint64_t)bcnt[3 * nblk + b] << 32);
-        const uint64_t dd = bdeg[nblk + b];
-        bcnt[0 * nblk + b] = (uint32_t)r01;
-        bcnt[1 * nblk + b] = (uint32_t)(r01 >> 32);
-        bcnt[2 * nblk + b] = (uint32_t)r23;
-        bcnt[3 * nblk + b] = (uint32_t)(r23 >> 32);
-        bdeg[nblk + b] = rdd;
-        r01 += a01;
-        r23 += a23;
-        rdd += dd;
+        bc.sbp[0 * nsb + b] = r01;
+        bc.sbp[1 * nsb + b] = r23;
+        bc.sbp[2 * nsb + b] = rdd;
+        r01 += sbt[0 * nsb + b];
+        r23 += sbt[1 * nsb + b];
+        rdd += sbt[2 * nsb + b];
+#pragma unroll
+        for (int i = 0; i < kSbVals - 1; ++i) bc.sbt[i * nsb + b] = 0;
+        bc.sbt[(kSbVals - 1) * nsb + b] = ~0ull;
     }
     if (threadIdx.x == 0) {
         FrontierInfo fi;
@@ -410,9 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                                                            uint64_t Vpad, const uint64_t *__restrict__ ptr,
                                                            const uint64_t *__restrict__ pmid,
                                                            const uint32_t *__restrict__ deg, Tuning tn,
-                                                           const uint32_t *__restrict__ bcnt,
-                                                           const uint64_t *__restrict__ bdeg, uint32_t nblk,
-                                                           const FrontierInfo *__restrict__ info,
+                                                           BlockCounts bc, const FrontierInfo *__restrict__ info,
                                                            uint32_t *__restrict__ list, uint64_t *__restrict__ hub,
                                                            uint64_t hub_cap, uint64_t *__restrict__ doff,
                                                            uint32_t *__restrict__ m_dlist,
@@ -420,8 +429,34 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                                                            uint32_t *__restrict__ dchunk, uint64_t dchunk_cap) {
     __shared__ uint64_t lds[3 * kBlock / kWave];
     __shared__ uint32_t s_win[4 * FV * kBlock];  // word i of thread t at [i * kBlock + t]
+    __shared__ uint64_t s_pre[3];                // the block's exclusive prefixes (bins 0|1, 2|3, degrees)
+    const uint32_t nblk = bc.nblk;
     if (!info->live) return;  // nothing is flagged: no list, and the window is already clear
-    if (!bcnt[7 * nblk + blockIdx.x]) return;  // the count pass found no flag in this window
+    if (!bc.bcnt[4 * nblk + blockIdx.x]) return;  // the count pass found no flag in this window
+    // prefix = the super-block's + the counts of the blocks before this one
+    // in its super-block (one lane each; read before the block scan below,
+    // whose barriers publish s_pre)
+    if (threadIdx.x < kWave) {
+        static_assert(kSbBlocks <= kWave, "one lane per block of a super-block");
+        const uint32_t sb = blockIdx.x / kSbBlocks, q = sb * kSbBlocks + threadIdx.x;
+        uint64_t a01 = 0, a23 = 0, add = 0;
+        if (q < blockIdx.x) {
+            a01 = (uint64_t)bc.bcnt[0 * nblk + q] | ((uint64_t)bc.bcnt[1 * nblk + q] << 32);
+            a23 = (uint64_t)bc.bcnt[2 * nblk + q] | ((uint64_t)bc.bcnt[3 * nblk + q] << 32);
+            add = bc.bdeg[q];
+        }
+#pragma unroll
+        for (int m = kWave / 2; m >= 1; m >>= 1) {
+            a01 += shfl_xor_u64(a01, m);
+            a23 += shfl_xor_u64(a23, m);
+            add += shfl_xor_u64(add, m);
+        }
+        if (threadIdx.x == 0) {
+            s_pre[0] = bc.sbp[0 * bc.nsb + sb] + a01;
+            s_pre[1] = bc.sbp[1 * bc.nsb + sb] + a23;
+            s_pre[2] = bc.sbp[2 * bc.nsb + sb] + add;
+        }
+    }
     // the array and row ranges the count pass used (light/heavy split)
     const bool heavy = pmid && info->heavy;
     uint8_t *flag = heavy ? hflag : flag_a;
@@ -483,11 +518,11 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
     uint32_t keep = 0;  // slots that stay flagged: held back (bin 5) or a held dynamic list
     if (!info->pull) {  // bottom-up round: no lists, the frontier is its level
         const uint32_t n0 = info->n[0], n1 = info->n[1];
-        uint64_t p0 = (uint64_t)bcnt[0 * nblk + blockIdx.x] + (uint32_t)ex[0];
-        uint64_t p1 = (uint64_t)n0 + bcnt[1 * nblk + blockIdx.x] + (uint32_t)(ex[0] >> 32);
-        uint64_t p2 = (uint64_t)bcnt[2 * nblk + blockIdx.x] + (uint32_t)ex[1];
-        uint64_t p3 = (uint64_t)bcnt[3 * nblk + blockIdx.x] + (uint32_t)(ex[1] >> 32);
-        uint64_t pd = bdeg[1 * nblk + blockIdx.x] + ex[2];
+        uint64_t p0 = (uint64_t)(uint32_t)s_pre[0] + (uint32_t)ex[0];
+        uint64_t p1 = (uint64_t)n0 + (uint32_t)(s_pre[0] >> 32) + (uint32_t)(ex[0] >> 32);
+        uint64_t p2 = (uint64_t)(uint32_t)s_pre[1] + (uint32_t)ex[1];
+        uint64_t p3 = (uint64_t)(uint32_t)(s_pre[1] >> 32) + (uint32_t)(ex[1] >> 32);
+        uint64_t pd = s_pre[2] + ex[2];
         const uint64_t dyn_base = (uint64_t)n0 + n1;
         const bool held = info->dyn_held != 0;
         walk([&](uint32_t j, int b, uint64_t deg) {
@@ -1181,7 +1216,12 @@ void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m) 
     f.nblk = (uint32_t)((d->Vpad + verts_per_block(f.fv) - 1) / verts_per_block(f.fv));
     f.V = V;
     f.bcnt = (uint32_t *)m.dalloc((uint64_t)kBcntRows * f.nblk * sizeof(uint32_t));
-    f.bdeg = (uint64_t *)m.dalloc(2ull * f.nblk * sizeof(uint64_t));
+    f.bdeg = (uint64_t *)m.dalloc((uint64_t)f.nblk * sizeof(uint64_t));
+    f.nsb = (f.nblk + kSbBlocks - 1) / kSbBlocks;
+    f.sbt = (uint64_t *)m.dalloc((uint64_t)kSbVals * f.nsb * sizeof(uint64_t));
+    f.sbp = (uint64_t *)m.dalloc(3ull * f.nsb * sizeof(uint64_t));
+    OG_HIP(hipMemset(f.sbt, 0, (uint64_t)(kSbVals - 1) * f.nsb * sizeof(uint64_t)));
+    OG_HIP(hipMemset(f.sbt + (uint64_t)(kSbVals - 1) * f.nsb, 0xFF, (uint64_t)f.nsb * sizeof(uint64_t)));
     f.info = (FrontierInfo *)m.dalloc(sizeof(FrontierInfo));
     f.ctl = (RoundCtl *)m.dalloc(sizeof(RoundCtl));
     f.counters = (unsigned long long *)m.dalloc(16 * sizeof(unsigned long long));
@@ -1215,24 +1255,23 @@ void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t
     const uint32_t *deg = d->d_deg && (pmid != nullptr) == d->deg_split ? d->d_deg : nullptr;
     if (f.fv == 1)
         hipLaunchKernelGGL(k_frontier_count<1>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad,
-                           d->plan.vsplit, d->d_ptr, pmid, deg, d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
+                           d->plan.vsplit, d->d_ptr, pmid, deg, d->tune, val, f.ctl, f.counts());
     else
         hipLaunchKernelGGL(k_frontier_count<2>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad,
-                           d->plan.vsplit, d->d_ptr, pmid, deg, d->tune, val, f.ctl, f.bcnt, f.bdeg, f.nblk);
-    // one lane per block of counts up to kScanThreads lanes: a small graph's
-    // scan runs on one wave
+                           d->plan.vsplit, d->d_ptr, pmid, deg, d->tune, val, f.ctl, f.counts());
+    // one lane per super-block up to kScanThreads lanes (one wave up to
+    // 2^25 vertices)
     const unsigned scan_threads =
-        (unsigned)std::min<uint64_t>(kScanThreads, (std::max<uint64_t>(f.nblk, 1) + kWave - 1) / kWave * kWave);
-    hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(scan_threads), 0, s, f.bcnt, f.bdeg, f.nblk, f.info, f.doff,
-                       f.ctl, m_doff);
+        (unsigned)std::min<uint64_t>(kScanThreads, (std::max<uint64_t>(f.nsb, 1) + kWave - 1) / kWave * kWave);
+    hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(scan_threads), 0, s, f.counts(), f.info, f.doff, f.ctl, m_doff);
     if (f.fv == 1)
         hipLaunchKernelGGL(k_frontier_write<1>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr,
-                           pmid, deg, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff,
-                           m_dlist, m_doff, f.dchunk, f.dchunk_cap);
+                           pmid, deg, d->tune, f.counts(), f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist,
+                           m_doff, f.dchunk, f.dchunk_cap);
     else
         hipLaunchKernelGGL(k_frontier_write<2>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr,
-                           pmid, deg, d->tune, f.bcnt, f.bdeg, f.nblk, f.info, f.list, f.hub, f.hub_cap, f.doff,
-                           m_dlist, m_doff, f.dchunk, f.dchunk_cap);
+                           pmid, deg, d->tune, f.counts(), f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist,
+                           m_doff, f.dchunk, f.dchunk_cap);
     OG_HIP(hipGetLastError());
     if (to_host) OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
 }
@@ -2825,9 +2864,12 @@ constexpr uint64_t kNaFlag = 0;
 constexpr uint64_t kNaList = kNaFlag + na_up(kNoAllocMaxV);
 constexpr uint64_t kNaHub = kNaList + na_up(4 * kNoAllocMaxV);
 constexpr uint64_t kNaDoff = kNaHub + na_up(8 * kNoAllocHubCap);
+constexpr uint64_t kNoAllocMaxSb = (kNoAllocMaxBlk + kSbBlocks - 1) / kSbBlocks;
 constexpr uint64_t kNaBcnt = kNaDoff + na_up(8 * 16);
 constexpr uint64_t kNaBdeg = kNaBcnt + na_up(4 * kBcntRows * kNoAllocMaxBlk);
-constexpr uint64_t kNaInfo = kNaBdeg + na_up(8 * 2 * kNoAllocMaxBlk);
+constexpr uint64_t kNaSbt = kNaBdeg + na_up(8 * kNoAllocMaxBlk);
+constexpr uint64_t kNaSbp = kNaSbt + na_up(8 * kSbVals * kNoAllocMaxSb);
+constexpr uint64_t kNaInfo = kNaSbp + na_up(8 * 3 * kNoAllocMaxSb);
 constexpr uint64_t kNaCtl = kNaInfo + na_up(sizeof(FrontierInfo));
 constexpr uint64_t kNaCounters = kNaCtl + na_up(sizeof(RoundCtl));
 constexpr uint64_t kNoAllocScratchBytes = kNaCounters + na_up(16 * 8);
@@ -2876,8 +2918,9 @@ static void noalloc_bfs(const uint64_t *d_ptr, const uint32_t *d_cols, uint64_t 
     uint32_t *list = (uint32_t *)(base + kNaList);
     uint64_t *hub = (uint64_t *)(base + kNaHub);
     uint64_t *doff = (uint64_t *)(base + kNaDoff);
-    uint32_t *bcnt = (uint32_t *)(base + kNaBcnt);
-    uint64_t *bdeg = (uint64_t *)(base + kNaBdeg);
+    const uint32_t nsb = (nblk + kSbBlocks - 1) / kSbBlocks;
+    const BlockCounts bc{(uint32_t *)(base + kNaBcnt), (uint64_t *)(base + kNaBdeg), (uint64_t *)(base + kNaSbt),
+                         (uint64_t *)(base + kNaSbp), nblk, nsb};
     FrontierInfo *info = (FrontierInfo *)(base + kNaInfo);
     RoundCtl *ctl = (RoundCtl *)(base + kNaCtl);
     unsigned long long *ctr = (unsigned long long *)(base + kNaCounters);
@@ -2890,16 +2933,16 @@ static void noalloc_bfs(const uint64_t *d_ptr, const uint32_t *d_cols, uint64_t 
     hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid(V)), dim3(256), 0, s, d_levels, V, OG_UNREACHED);
     hipLaunchKernelGGL(k_set_source, dim3(1), dim3(1), 0, s, d_levels, flag, src);
     hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, ctl, 0u, 0u, (uint64_t)0, V, 0u);
+    hipLaunchKernelGGL(k_sb_init, dim3(1), dim3(64), 0, s, bc.sbt, nsb);
     auto compact = [&] {
         hipLaunchKernelGGL(k_frontier_count<1>, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, V,
                            d_ptr, (const uint64_t *)nullptr, (const uint32_t *)nullptr, tn, (const uint32_t *)nullptr,
-                           ctl, bcnt, bdeg, nblk);
-        hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(kScanThreads), 0, s, bcnt, bdeg, nblk, info, doff, ctl,
-                           (uint64_t *)nullptr);
+                           ctl, bc);
+        hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(kWave), 0, s, bc, info, doff, ctl, (uint64_t *)nullptr);
         hipLaunchKernelGGL(k_frontier_write<1>, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, d_ptr,
-                           (const uint64_t *)nullptr, (const uint32_t *)nullptr, tn, bcnt, bdeg, nblk, info, list,
-                           hub, kNoAllocHubCap, doff,
-                           (uint32_t *)nullptr, (uint64_t *)nullptr, (uint32_t *)nullptr, (uint64_t)0);
+                           (const uint64_t *)nullptr, (const uint32_t *)nullptr, tn, bc, info, list, hub,
+                           kNoAllocHubCap, doff, (uint32_t *)nullptr, (uint64_t *)nullptr, (uint32_t *)nullptr,
+                           (uint64_t)0);
     };
     RoundRelax r{};
     r.list = list;
