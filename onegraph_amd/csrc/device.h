@@ -346,7 +346,8 @@ struct DeviceState {
     uint64_t pr_partial_cap = 0;     // entries of d_partial
     bool pr_fp64 = false;
     std::shared_ptr<void> pr_plan;
-    std::shared_ptr<void> round_graph;  // captured batch of device-driven rounds (engine.hip)   // PageRank bins and chunk ranges (pagerank.hip)
+    std::shared_ptr<void> round_graph;  // captured batches of device-driven rounds (engine.hip)
+    uint32_t last_pairs[8] = {};        // relax + compaction pairs of the last device-driven run, per algorithm   // PageRank bins and chunk ranges (pagerank.hip)
 
     MemPool mem;                     // device + pinned allocations of this preparation
 

@@ -290,6 +290,9 @@ __global__ __launch_bounds__(kScanThreads) void k_frontier_scan(BlockCounts bc, 
 // whole pass is one multi-value block scan.  `lds` holds 8 * blockDim / 64.
 __device__ void scan_counts(const BlockCounts &bc, FrontierInfo *__restrict__ info, uint64_t *__restrict__ doff,
                             RoundCtl *__restrict__ ctl, uint64_t *__restrict__ m_doff, uint64_t *lds) {
+    // a round enqueued after the traversal ended (its count pass returned at
+    // once): the summary and the traversal state stand
+    if (ctl->halt && !ctl->live) return;
     const uint32_t nsb = bc.nsb;
     const uint64_t *sbt = bc.sbt;
     const uint32_t per = (nsb + blockDim.x - 1) / blockDim.x;
@@ -1803,35 +1806,42 @@ struct GraphSpan {
     }
 };
 
+// Captured batches of rounds of one preparation, by key (algorithm, layout,
+// rounds per batch); the oldest is dropped beyond kGraphCache.
+constexpr size_t kGraphCache = 8;
+struct RoundGraphCache {
+    std::vector<std::unique_ptr<RoundGraph>> g;
+};
+
 // The cached graph for `key`, captured on the compute stream by body(rg)
-// when the cache holds another one (or none).
+// when the cache does not hold it.
 template <class Body>
 static RoundGraph *round_graph_for(DeviceState *d, uint64_t key, Body body) {
-    auto *cur = static_cast<RoundGraph *>(d->round_graph.get());
-    if (cur && cur->key == key) return cur;
-    d->round_graph.reset();
-    auto *rg = new RoundGraph;
-    d->round_graph = std::shared_ptr<void>(rg, [](void *p) { delete static_cast<RoundGraph *>(p); });
+    if (!d->round_graph)
+        d->round_graph = std::shared_ptr<void>(new RoundGraphCache,
+                                               [](void *p) { delete static_cast<RoundGraphCache *>(p); });
+    auto &cache = static_cast<RoundGraphCache *>(d->round_graph.get())->g;
+    for (auto &c : cache)
+        if (c->key == key) return c.get();
+    std::unique_ptr<RoundGraph> rg(new RoundGraph);
     hipStream_t s = d->s_compute;
     OG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipGraph_t graph = nullptr;
     try {
-        body(rg);
+        body(rg.get());
     } catch (...) {
         (void)hipStreamEndCapture(s, &graph);
         if (graph) (void)hipGraphDestroy(graph);
-        d->round_graph.reset();
         throw;
     }
     OG_HIP(hipStreamEndCapture(s, &graph));
     const hipError_t e = hipGraphInstantiate(&rg->exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-        d->round_graph.reset();
-        OG_HIP(e);
-    }
+    OG_HIP(e);
     rg->key = key;
-    return rg;
+    if (cache.size() >= kGraphCache) cache.erase(cache.begin());
+    cache.push_back(std::move(rg));
+    return cache.back().get();
 }
 
 // Host gather of dynamic edges [c0, c1) into a pinned slot (reference
@@ -2108,7 +2118,8 @@ constexpr uint64_t edge_bytes_alg() {
     return ALGO == ALGO_MSBFS ? 12ull : (ALGO == ALGO_SSSP ? 12ull : 8ull);
 }
 
-constexpr int kRoundBatch = 4;  // device-driven rounds enqueued per host check
+constexpr int kRoundBatch = 4;     // device-driven rounds enqueued per host check
+constexpr int kRoundBatchMax = 64;  // largest first batch (sized from the previous run; a multiple of kRoundBatch)
 
 // Iterates compaction -> static relax || dynamic relax -> after() until no
 // vertex is active or held back.  Accumulates into st / tm.
@@ -2116,10 +2127,14 @@ constexpr int kRoundBatch = 4;  // device-driven rounds enqueued per host check
 // Without a host-side loader (whole graph in HBM, zero-copy dynamic region,
 // USM) the rounds are device-driven: every kernel reads its counts from the
 // round's FrontierInfo and the near-far threshold advances on the device, so
-// kRoundBatch rounds are enqueued at a time and the host reads the traversal
+// rounds are enqueued a batch at a time and the host reads the traversal
 // state once per batch (rounds enqueued past the end find no flags and do no
-// work).  The staged loader needs the host in every round (gather, copies),
-// so it synchronises on each round's summary.
+// work).  The first batch holds as many rounds as the previous run of the
+// algorithm on this preparation took, rounded up to a multiple of
+// kRoundBatch (a repeated traversal then needs one launch and no host round
+// trip: each costs about 50 us); later batches kRoundBatch.  The staged loader needs
+// the host in every round (gather, copies), so it synchronises on each
+// round's summary.
 // quiet_after: after() enqueues nothing (its rounds' timing spans may chain).
 template <int ALGO, bool W, class After>
 static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTimers &tm, double &gather_ms,
@@ -2155,8 +2170,8 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
         const bool overlap = has_dyn && !(d->flags & OG_PLAN_NO_OVERLAP);
         const bool fuse = overlap && !(d->flags & OG_PLAN_NO_FUSE);
         const bool fuse_static = !has_dyn && !(d->flags & OG_PLAN_NO_FUSE);
-        auto batch = [&](const auto &span) {
-            for (int r = 0; r < kRoundBatch; ++r) {
+        auto batch = [&](const auto &span, int rounds) {
+            for (int r = 0; r < rounds; ++r) {
                 bool joined = false;
                 if (fuse) {
                     span(kClsDyn, s, [&] { launch_round_relax<ALGO, W>(d, true); });
@@ -2188,15 +2203,18 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
             OG_HIP(hipMemcpyAsync(d->fr.h_ctl, d->fr.ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
         };
         if (!(d->flags & OG_PLAN_NO_GRAPH)) {
-            const uint64_t key = (uint64_t)ALGO | (uint64_t)W << 4 | (uint64_t)(delta != 0) << 5 |
-                                 (uint64_t)has_dyn << 6 | (uint64_t)dir_opt << 7 | (uint64_t)split << 16 |
-                                 (uint64_t)kRoundBatch << 8;
-            RoundGraph *rg = round_graph_for(d, key, [&](RoundGraph *n) {
-                GraphSpan gs{n};
-                gs.chain = (fuse || fuse_static) && quiet_after;  // one stream: relax span(s), then compaction
-                batch(gs);
-            });
+            const uint32_t guess = std::min<uint32_t>(d->last_pairs[ALGO], kRoundBatchMax);
+            int rounds = std::max<int>(kRoundBatch, (int)(guess + kRoundBatch - 1) / kRoundBatch * kRoundBatch);
             for (;;) {
+                const uint64_t key = (uint64_t)ALGO | (uint64_t)W << 4 | (uint64_t)(delta != 0) << 5 |
+                                     (uint64_t)has_dyn << 6 | (uint64_t)dir_opt << 7 | (uint64_t)split << 16 |
+                                     (uint64_t)rounds << 8;
+                RoundGraph *rg = round_graph_for(d, key, [&](RoundGraph *n) {
+                    GraphSpan gs{n};
+                    gs.chain = (fuse || fuse_static) && quiet_after;  // one stream: relax span(s), then compaction
+                    batch(gs, rounds);
+                });
+                rounds = kRoundBatch;
                 OG_HIP(hipGraphLaunch(rg->exec, s));
                 OG_HIP(hipStreamSynchronize(s));
                 for (const auto &m : rg->marks) {
@@ -2210,9 +2228,11 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
                 }
                 if (!d->fr.h_ctl->live) break;
             }
+            // relax + compaction pairs of this run, for the next run's first batch
+            d->last_pairs[ALGO] = d->fr.h_ctl->compactions ? d->fr.h_ctl->compactions - 1 : 0;
         } else {
             for (;;) {
-                batch(eager);
+                batch(eager, kRoundBatch);
                 OG_HIP(hipStreamSynchronize(s));
                 if (!d->fr.h_ctl->live) break;
             }
