@@ -1073,18 +1073,54 @@ __device__ __forceinline__ uint64_t d_mix64(uint64_t z) {
 
 // reached = #{v : val[v] != UNREACHED}, traversed = sum of their degrees,
 // checksum = digest of val.
+// deg / hdeg: the degree table (hdeg: heavy degrees with the split, else
+// null), or null for the row bounds.  With the table, each lane reads four
+// values and four degrees per 16-byte load.
 __global__ __launch_bounds__(kBlock) void k_reach_stats(const uint32_t *__restrict__ val,
-                                                        const uint64_t *__restrict__ ptr, uint64_t V,
+                                                        const uint64_t *__restrict__ ptr,
+                                                        const uint32_t *__restrict__ deg,
+                                                        const uint32_t *__restrict__ hdeg, uint64_t V,
                                                         unsigned long long *out) {
     __shared__ uint64_t lds[kBlock / kWave];
     uint64_t r = 0, t = 0, h = 0;
-    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < V; v += (uint64_t)gridDim.x * kBlock) {
-        const uint32_t x = val[v];
+    const uint64_t tid = (uint64_t)blockIdx.x * kBlock + threadIdx.x, nth = (uint64_t)gridDim.x * kBlock;
+    auto add = [&](uint64_t v, uint32_t x, uint64_t dv) {
         if (x != OG_UNREACHED) {
             ++r;
-            t += ptr[v + 1] - ptr[v];
+            t += dv;
             h += d_mix64((v << 32) | x);
         }
+    };
+    uint64_t v0 = 0;  // first vertex of the scalar tail
+    if (deg) {
+        // val and the table are 16-byte aligned allocations; the table is
+        // padded to a multiple of 16 entries
+        const uint64_t n4 = V / 4;
+        // two chunks per lane per step, their loads issued together
+        for (uint64_t q0 = tid; q0 < n4; q0 += 2 * nth) {
+            uint4 x[2], dl[2], dh[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const uint64_t q = q0 + k * nth;
+                const bool in = q < n4;
+                x[k] = in ? ((const uint4 *)val)[q] : make_uint4(OG_UNREACHED, OG_UNREACHED, OG_UNREACHED, OG_UNREACHED);
+                dl[k] = in ? ((const uint4 *)deg)[q] : make_uint4(0, 0, 0, 0);
+                dh[k] = in && hdeg ? ((const uint4 *)hdeg)[q] : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const uint64_t q = q0 + k * nth;
+                add(4 * q + 0, x[k].x, (uint64_t)dl[k].x + dh[k].x);
+                add(4 * q + 1, x[k].y, (uint64_t)dl[k].y + dh[k].y);
+                add(4 * q + 2, x[k].z, (uint64_t)dl[k].z + dh[k].z);
+                add(4 * q + 3, x[k].w, (uint64_t)dl[k].w + dh[k].w);
+            }
+        }
+        v0 = 4 * n4;
+    }
+    for (uint64_t v = v0 + tid; v < V; v += nth) {
+        const uint32_t x = val[v];
+        add(v, x, deg ? (uint64_t)deg[v] + (hdeg ? hdeg[v] : 0u) : (x != OG_UNREACHED ? ptr[v + 1] - ptr[v] : 0));
     }
     r = block_sum(r, lds);
     t = block_sum(t, lds);
@@ -2337,8 +2373,16 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
     OG_HIP(hipStreamSynchronize(s));
     st.total_ms = total.ms();
 
-    hipLaunchKernelGGL(k_reach_stats, dim3(std::min<uint64_t>(2048, (V + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       d->d_val, d->d_ptr, V, d->fr.counters + 2);
+    {
+        // one lane per four vertices with the degree table, at most 2048
+        // workgroups (three atomics each on the same three counters: 16384
+        // workgroups spent 0.6 ms on them)
+        const uint64_t lanes = d->d_deg ? (V + 3) / 4 : V;
+        const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(2048, (lanes + kBlock - 1) / kBlock));
+        hipLaunchKernelGGL(k_reach_stats, dim3(grid), dim3(kBlock), 0, s, d->d_val, d->d_ptr, d->d_deg,
+                           d->d_deg && d->deg_split ? d->d_deg + d->Vpad : (const uint32_t *)nullptr, V,
+                           d->fr.counters + 2);
+    }
     OG_HIP(hipGetLastError());
     OG_HIP(hipMemcpyAsync(d->fr.h_counters, d->fr.counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     if (out) OG_HIP(hipMemcpyAsync(out, d->d_val, V * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -2992,7 +3036,8 @@ static void noalloc_bfs(const uint64_t *d_ptr, const uint32_t *d_cols, uint64_t 
     }
     const double ms = total.ms();
     hipLaunchKernelGGL(k_reach_stats, dim3((unsigned)std::min<uint64_t>(2048, (V + kBlock - 1) / kBlock)),
-                       dim3(kBlock), 0, s, d_levels, d_ptr, V, ctr + 2);
+                       dim3(kBlock), 0, s, d_levels, d_ptr, (const uint32_t *)nullptr, (const uint32_t *)nullptr, V,
+                       ctr + 2);
     OG_HIP(hipGetLastError());
     unsigned long long hc[16];
     OG_HIP(hipMemcpyAsync(hc, ctr, sizeof hc, hipMemcpyDeviceToHost, s));
