@@ -76,12 +76,16 @@ __device__ __forceinline__ void row_range(const uint64_t *__restrict__ ptr, cons
 }
 
 // Degree of u's row in the round's range.  With a degree table (4 bytes per
-// vertex; with the split, light degrees then heavy degrees at + Vpad) one
-// load replaces the two 8-byte row bounds.
-__device__ __forceinline__ uint64_t round_deg(const uint32_t *__restrict__ deg, uint64_t Vpad,
+// vertex: deg the whole-row degrees, or with the split deg the light and
+// hdeg the heavy degrees) one or two loads replace the two 8-byte row
+// bounds; a round over whole rows of a split graph adds the two parts.
+__device__ __forceinline__ uint64_t round_deg(const uint32_t *__restrict__ deg, const uint32_t *__restrict__ hdeg,
                                               const uint64_t *__restrict__ ptr, const uint64_t *__restrict__ pmid,
                                               bool heavy, uint64_t u) {
-    if (deg) return deg[(heavy ? Vpad : 0) + u];
+    if (deg) {
+        if (pmid) return heavy ? hdeg[u] : deg[u];
+        return (uint64_t)deg[u] + (hdeg ? hdeg[u] : 0u);
+    }
     uint64_t b, e;
     row_range(ptr, pmid, heavy, u, b, e);
     return e - b;
@@ -275,6 +279,7 @@ struct DeviceState {
     uint32_t *d_deg = nullptr;       // degree table of the compaction: Vpad whole-row degrees, or with
                                      // the split Vpad light then Vpad heavy degrees (null: row bounds)
     bool deg_split = false;          // d_deg holds light and heavy degrees
+    const uint32_t *deg_heavy() const { return d_deg && deg_split ? d_deg + Vpad : nullptr; }
     std::vector<uint64_t> h_pmid;    // host copy (the staged gather)
     uint32_t split_weight = 0;       // light: weight < split_weight (0: no split)
     const uint64_t *run_pmid = nullptr;  // d_pmid while a run uses the split (SSSP near-far), else null

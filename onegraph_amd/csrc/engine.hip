@@ -143,7 +143,8 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                                                            uint64_t Vpad, uint64_t vsplit,
                                                            const uint64_t *__restrict__ ptr,
                                                            const uint64_t *__restrict__ pmid,
-                                                           const uint32_t *__restrict__ deg, Tuning tn,
+                                                           const uint32_t *__restrict__ deg,
+                                                           const uint32_t *__restrict__ hdeg, Tuning tn,
                                                            const uint32_t *__restrict__ val_a,
                                                            const RoundCtl *__restrict__ ctl, BlockCounts bc) {
     const uint32_t nblk = bc.nblk;
@@ -205,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                     m &= m - 1;
                     jj[k] = j;
                     if (val) vv[k] = val[base + (uint64_t)j * kBlock];
-                    else dg[k] = round_deg(deg, Vpad, ptr, pmid, heavy, base + (uint64_t)j * kBlock);
+                    else dg[k] = round_deg(deg, hdeg, ptr, pmid, heavy, base + (uint64_t)j * kBlock);
                 }
             }
             // near-far: a held-back vertex needs only its value, so the
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_count(uint8_t *__restrict__
                 if (jj[k] == (16 * FV) || !val) continue;
                 const bool lag = lagged && vv[k] < thresh && vv[k] >= fbound && base + (uint64_t)jj[k] * kBlock >= vsplit;
                 far[k] = vv[k] >= thresh || lag;
-                if (!far[k]) dg[k] = round_deg(deg, Vpad, ptr, pmid, heavy, base + (uint64_t)jj[k] * kBlock);
+                if (!far[k]) dg[k] = round_deg(deg, hdeg, ptr, pmid, heavy, base + (uint64_t)jj[k] * kBlock);
             }
 #pragma unroll
             for (int k = 0; k < kLoadBatch; ++k) {
@@ -423,7 +424,8 @@ template <int FV>
 __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__ flag_a, uint8_t *__restrict__ hflag,
                                                            uint64_t Vpad, const uint64_t *__restrict__ ptr,
                                                            const uint64_t *__restrict__ pmid,
-                                                           const uint32_t *__restrict__ deg, Tuning tn,
+                                                           const uint32_t *__restrict__ deg,
+                                                           const uint32_t *__restrict__ hdeg, Tuning tn,
                                                            BlockCounts bc, const FrontierInfo *__restrict__ info,
                                                            uint32_t *__restrict__ list, uint64_t *__restrict__ hub,
                                                            uint64_t hub_cap, uint64_t *__restrict__ doff,
@@ -497,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_frontier_write(uint8_t *__restrict__
                     m &= m - 1;
                     jj[k] = j;
                     bb[k] = bin_at(j);
-                    if (bb[k] == 2 || bb[k] == 3) dg[k] = round_deg(deg, Vpad, ptr, pmid, heavy, base + j);
+                    if (bb[k] == 2 || bb[k] == 3) dg[k] = round_deg(deg, hdeg, ptr, pmid, heavy, base + j);
                 }
             }
 #pragma unroll
@@ -1289,15 +1291,15 @@ void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t
     // PCIe write per vertex: slower than one bulk copy back, so unused)
     uint32_t *m_dlist = nullptr;
     uint64_t *m_doff = nullptr;
-    // the degree table when its layout is the run's (whole rows, or light
-    // and heavy parts), else the row bounds
-    const uint32_t *deg = d->d_deg && (pmid != nullptr) == d->deg_split ? d->d_deg : nullptr;
+    // the degree table (a split round needs its light and heavy parts)
+    const uint32_t *deg = d->d_deg && (!pmid || d->deg_split) ? d->d_deg : nullptr;
+    const uint32_t *hdeg = deg ? d->deg_heavy() : nullptr;
     if (f.fv == 1)
         hipLaunchKernelGGL(k_frontier_count<1>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad,
-                           d->plan.vsplit, d->d_ptr, pmid, deg, d->tune, val, f.ctl, f.counts());
+                           d->plan.vsplit, d->d_ptr, pmid, deg, hdeg, d->tune, val, f.ctl, f.counts());
     else
         hipLaunchKernelGGL(k_frontier_count<2>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad,
-                           d->plan.vsplit, d->d_ptr, pmid, deg, d->tune, val, f.ctl, f.counts());
+                           d->plan.vsplit, d->d_ptr, pmid, deg, hdeg, d->tune, val, f.ctl, f.counts());
     // one lane per super-block up to kScanThreads lanes (one wave up to
     // 2^25 vertices)
     const unsigned scan_threads =
@@ -1305,11 +1307,11 @@ void frontier_compact(DeviceState *d, Frontier &f, hipStream_t s, const uint32_t
     hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(scan_threads), 0, s, f.counts(), f.info, f.doff, f.ctl, m_doff);
     if (f.fv == 1)
         hipLaunchKernelGGL(k_frontier_write<1>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr,
-                           pmid, deg, d->tune, f.counts(), f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist,
+                           pmid, deg, hdeg, d->tune, f.counts(), f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist,
                            m_doff, f.dchunk, f.dchunk_cap);
     else
         hipLaunchKernelGGL(k_frontier_write<2>, dim3(f.nblk), dim3(kBlock), 0, s, f.flag, f.hflag, d->Vpad, d->d_ptr,
-                           pmid, deg, d->tune, f.counts(), f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist,
+                           pmid, deg, hdeg, d->tune, f.counts(), f.info, f.list, f.hub, f.hub_cap, f.doff, m_dlist,
                            m_doff, f.dchunk, f.dchunk_cap);
     OG_HIP(hipGetLastError());
     if (to_host) OG_HIP(hipMemcpyAsync(f.h_info, f.info, sizeof(FrontierInfo), hipMemcpyDeviceToHost, s));
@@ -2380,7 +2382,7 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
         const uint64_t lanes = d->d_deg ? (V + 3) / 4 : V;
         const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(2048, (lanes + kBlock - 1) / kBlock));
         hipLaunchKernelGGL(k_reach_stats, dim3(grid), dim3(kBlock), 0, s, d->d_val, d->d_ptr, d->d_deg,
-                           d->d_deg && d->deg_split ? d->d_deg + d->Vpad : (const uint32_t *)nullptr, V,
+                           d->deg_heavy(), V,
                            d->fr.counters + 2);
     }
     OG_HIP(hipGetLastError());
@@ -3000,11 +3002,12 @@ static void noalloc_bfs(const uint64_t *d_ptr, const uint32_t *d_cols, uint64_t 
     hipLaunchKernelGGL(k_sb_init, dim3(1), dim3(64), 0, s, bc.sbt, nsb);
     auto compact = [&] {
         hipLaunchKernelGGL(k_frontier_count<1>, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, V,
-                           d_ptr, (const uint64_t *)nullptr, (const uint32_t *)nullptr, tn, (const uint32_t *)nullptr,
-                           ctl, bc);
+                           d_ptr, (const uint64_t *)nullptr, (const uint32_t *)nullptr, (const uint32_t *)nullptr, tn,
+                           (const uint32_t *)nullptr, ctl, bc);
         hipLaunchKernelGGL(k_frontier_scan, dim3(1), dim3(kWave), 0, s, bc, info, doff, ctl, (uint64_t *)nullptr);
         hipLaunchKernelGGL(k_frontier_write<1>, dim3(nblk), dim3(kBlock), 0, s, flag, (uint8_t *)nullptr, Vpad, d_ptr,
-                           (const uint64_t *)nullptr, (const uint32_t *)nullptr, tn, bc, info, list, hub,
+                           (const uint64_t *)nullptr, (const uint32_t *)nullptr, (const uint32_t *)nullptr, tn, bc,
+                           info, list, hub,
                            kNoAllocHubCap, doff, (uint32_t *)nullptr, (uint64_t *)nullptr, (uint32_t *)nullptr,
                            (uint64_t)0);
     };

@@ -97,7 +97,9 @@ __global__ void k_srv_seed(SeedList sl, uint64_t *next, uint8_t *flag, unsigned 
 constexpr int kCommitCopies = 16;
 constexpr int kCommitStride = 65;
 __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ flag, uint64_t V,
-                                                       const uint64_t *__restrict__ ptr, uint64_t *__restrict__ visit,
+                                                       const uint64_t *__restrict__ ptr,
+                                                       const uint32_t *__restrict__ deg,
+                                                       const uint32_t *__restrict__ hdeg, uint64_t *__restrict__ visit,
                                                        uint64_t *__restrict__ front, uint64_t *__restrict__ next,
                                                        SlotLevels L, KeepList keep, unsigned long long *acc,
                                                        unsigned long long *fstat, unsigned long long *alive) {
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
             const uint64_t vis = visit[v];
             const uint64_t nw = next[v] & ~vis;
             next[v] = 0;
-            const uint64_t deg = ptr[v + 1] - ptr[v];
+            const uint64_t dv = round_deg(deg, hdeg, ptr, nullptr, false, v);
             if (nw) {
                 // each slot that reaches v: digest term, one more reached
                 // vertex, deg more traversed edges (= its frontier edges)
@@ -137,11 +139,11 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
                     const uint32_t lv = slv[j];
                     atomicAdd(&sdig[cp + j], (unsigned long long)srv_mix64((v << 32) | lv));
                     atomicAdd(&sfc[cp + j], 1ull);
-                    if (deg) atomicAdd(&sfe[cp + j], (unsigned long long)deg);
+                    if (dv) atomicAdd(&sfe[cp + j], (unsigned long long)dv);
                     if ((kmask >> j) & 1) skeep[j][v] = lv;
                 }
             }
-            const uint64_t fr = deg ? nw : 0;
+            const uint64_t fr = dv ? nw : 0;
             if (fr) {
                 front[v] = fr;
                 al |= fr;
@@ -990,7 +992,8 @@ void BfsEngine::step() {
     SlotLevels L{};
     for (uint32_t j = 0; j < kSlots; ++j) L.lv[j] = round - slots[j].start_round + 1;
     hipLaunchKernelGGL(k_srv_commit, dim3(sweep_grid(V, 4096)), dim3(kBlock), 0, s, f.flag, V,
-                       d->d_ptr, visit, front, next, L, keep_list(used), d_acc, d_fstat, d_alive);
+                       d->d_ptr, d->d_deg, d->deg_heavy(), visit, front, next, L, keep_list(used), d_acc, d_fstat,
+                       d_alive);
     OG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_srv_direction, dim3(1), dim3(64), 0, s, used, V, d->E, dir_opt ? 1 : 0, d_fstat,
                        d_explored, d_acc, d_pull);
