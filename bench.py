@@ -69,15 +69,25 @@ def kernel_table(st):
 
 
 def cpu_info():
-    model = ""
+    """CPU model name plus the identifying fields of the first processor
+    entry (a virtualised host may report a generic model name only)."""
+    want = {"model name": "model", "cpu family": "family", "model": "model_id", "stepping": "stepping",
+            "cpu MHz": "mhz", "cache size": "cache"}
+    out = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                if not line.strip():
+                    break  # end of the first processor entry
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in want:
+                    out[want[k]] = v.strip()
     except OSError:
         pass
+    model = out.pop("model", "")
+    if out:
+        model += " (" + ", ".join(f"{k} {v}" for k, v in out.items()) + ")"
     return model
 
 
@@ -825,6 +835,14 @@ def main():
     # one checked run: distances must equal the oracle's (bit-exact)
     dist, st_chk = g.sssp(src)
     ms_per_step = 1e3 * wall / args.steps
+    # the timed steps leave the distances in HBM; the reference's sssp_opt
+    # copies them back inside its total time, so the same steps with the
+    # device-to-host copy of the 4V-byte distance array are timed beside them
+    cb_steps = max(1, min(args.steps, 5))
+    t = time.perf_counter()
+    for _ in range(cb_steps):
+        g.sssp(src)
+    ms_with_copyback = 1e3 * (time.perf_counter() - t) / cb_steps
     trav = stats[-1]["traversed_edges"]
     gteps = trav / (ms_per_step * 1e-3) / 1e9
 
@@ -889,6 +907,11 @@ def main():
             "vsplit": plan["vsplit"], "loader": args.loader, "delta": args.delta, "split_weight": args.split_weight,
             "edge_format": edge_format,
         },
+        "result_copy_back": {"in_value": False, "bytes": 4 * V, "ms_per_step_with_copy_back": round(ms_with_copyback, 3),
+                             "gteps_with_copy_back": round(trav / (ms_with_copyback * 1e-3) / 1e9, 4),
+                             "steps": cb_steps,
+                             "note": "value times steps whose distances stay in HBM; these steps also copy them "
+                                     "to a host array, as the reference's total time does"},
         "roofline": roof,
         "static_region": static,
         "host_link": link,
