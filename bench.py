@@ -484,10 +484,14 @@ def wl_serve(args):
 
     # per-query state (og_server_create's admission rule): per graph the BFS
     # engine (40 V bytes), per weighted graph the 16-slot SSSP engine (92 V + 8
-    # + the relax's chunk table)
-    full = sum(40 * g.num_vertices + (92 * g.num_vertices + 8 + 4 * (16 * g.num_edges // 1024 + 2)
-                                      if g.info()["kind"] == og.KIND_WEIGHTED else 0) for g in graphs)
-    budget = int(args.serve_budget_gib * (1 << 30)) if args.serve_budget_gib else full
+    # + the relax's chunk table), then as many single-source SSSP lanes (up to
+    # 2 per weighted graph) as the rest holds; by default the budget holds
+    # the engines and both lanes
+    engines = sum(40 * g.num_vertices + (92 * g.num_vertices + 8 + 4 * (16 * g.num_edges // 1024 + 2)
+                                         if g.info()["kind"] == og.KIND_WEIGHTED else 0) for g in graphs)
+    lanes = sum(2 * 19 * g.num_vertices for g in graphs if g.info()["kind"] == og.KIND_WEIGHTED)
+    budget = int(args.serve_budget_gib * (1 << 30)) if args.serve_budget_gib else engines + lanes
+    srv_flags = og.SERVER_NO_SSSP_LANES if args.no_sssp_lanes else 0
 
     def replay(t_arr, gis, kinds, svc, seed):
         """FIFO one-at-a-time service of the trace with sampled service times."""
@@ -504,7 +508,7 @@ def wl_serve(args):
         t_arr, gis, kinds, srcs = trace(rate, seed, bfs_share)
         plain_qps, plat = replay(t_arr, gis, kinds, service, seed + 4)
         td_qps, _ = replay(t_arr, gis, kinds, service_td, seed + 4)
-        srv = og.Server(graphs, state_budget_bytes=budget, sssp_delta=args.serve_sssp_delta)
+        srv = og.Server(graphs, state_budget_bytes=budget, sssp_delta=args.serve_sssp_delta, flags=srv_flags)
         info0 = srv.info()
         lat, done, submit_at = [], {}, {}
         t0 = time.perf_counter()
@@ -590,6 +594,7 @@ def wl_serve(args):
                        "rates": rates, "mixes": mixes, "duration_s": args.duration, "state_budget_bytes": budget,
                        "p99_target_ms": args.p99_target_ms, "sssp_delta": args.delta,
                        "served_sssp_delta": args.serve_sssp_delta or args.delta,
+                       "sssp_lanes": 0 if args.no_sssp_lanes else 2,
                        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                        "split_weight": args.split_weight})
     out.update({"serve": per_mix,
@@ -629,7 +634,7 @@ def wl_sssp_batch(args):
     rows = []
     for split, sd in ((0, 64), (8, 64), (8, 128), (8, 256), (8, 512)):
         g.prepare("ascetic", edge_budget_bytes=E * 8, advisek=1.0, split_weight=split)
-        with og.Server([g], sssp_delta=sd) as srv:
+        with og.Server([g], sssp_delta=sd, flags=og.SERVER_NO_SSSP_LANES) as srv:
             first = srv.submit(0, og.QUERY_SSSP, srcs[0])
             while not srv.poll(16, timeout_ms=100):
                 pass
@@ -712,6 +717,8 @@ def main():
     ap.add_argument("--duration", type=float, default=3.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--serve-scales", default="22,24,26", help="serve: R-MAT scales of the graphs")
     ap.add_argument("--serve-budget-gib", type=float, default=0.0, help="serve: per-query state budget")
+    ap.add_argument("--no-sssp-lanes", action="store_true",
+                    help="serve: no single-source SSSP lanes (every SSSP query in the batched engine)")
     ap.add_argument("--verify", type=int, default=100, help="serve: served results re-run on the plain path")
     ap.add_argument("--serve-mix", default="0.7,1.0", help="serve: BFS shares of the query mixes to run")
     ap.add_argument("--p99-target-ms", type=float, default=50.0, help="serve: p99 latency target")

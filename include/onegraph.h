@@ -429,6 +429,11 @@ typedef struct og_server_params {
                                      light / heavy pending masks, active list
                                      and work prefix (92 V + 8 bytes) and
                                      chunk table (4 (16 E / 1024 + 2) bytes);
+                                     then, per weighted graph and as far as
+                                     the rest of the budget holds them, up to
+                                     2 single-source SSSP lanes (values,
+                                     frontier flags, bin lists and degree
+                                     prefix: about 17-18 V bytes each);
                                      0 = a quarter of free HBM                 */
     uint32_t max_pending;         /* queued queries before submit returns OG_EBUSY; 0 = 2^20 */
     uint32_t flags;               /* OG_SERVER_* bits                       */
@@ -436,13 +441,23 @@ typedef struct og_server_params {
                                      this many ms of its submit completes with
                                      OG_ETIMEDOUT instead (0: no deadline);
                                      admitted queries always run to the end */
-    uint32_t sssp_delta;          /* bucket width of the SSSP slots' near-far
-                                     rounds; 0 = the graph's tuning delta.
+    uint32_t sssp_delta;          /* bucket width of the SSSP slots' and
+                                     lanes' near-far rounds; 0 = the graph's
+                                     tuning delta.
                                      On graphs prepared with split_weight the
                                      slots run light/heavy rounds (wide
                                      buckets pay: 256 with weights in [1, 63]),
                                      else they relax whole rows (16-64)       */
 } og_server_params;
+
+/* SSSP queries of a weighted graph go first to an idle single-source lane
+ * (og_sssp's device-driven rounds on the lane's own state); the batched
+ * 16-slot engine admits queries only while every lane is busy and either it
+ * is already running queries or 8 or more wait.  This flag creates no lanes,
+ * so every SSSP query runs in the batched engine. */
+#define OG_SERVER_NO_SSSP_LANES 0x1u
+#define OG_PATH_BATCH 0u
+#define OG_PATH_LANE 1u
 
 /* Testing only: the server's SSSP engines stop at their first round with
  * OG_EHIP, as after a device error, without touching the GPU (the failure
@@ -457,13 +472,16 @@ typedef struct og_query_result {
     double queue_ms;              /* submit -> admitted into a slot      */
     double latency_ms;            /* submit -> result ready              */
     uint32_t rounds;              /* engine rounds the query took part in */
-    uint32_t reserved;
+    uint32_t path;                /* what ran it: OG_PATH_BATCH (a slot of a
+                                     batch engine) or OG_PATH_LANE (an SSSP
+                                     single-source lane)                     */
 } og_query_result;
 
 typedef struct og_server_info {
     uint32_t num_graphs;
     uint32_t bfs_slots[OG_SERVER_MAX_GRAPHS];   /* BFS queries in flight at once per graph */
-    uint32_t sssp_slots[OG_SERVER_MAX_GRAPHS];  /* SSSP lanes per graph                    */
+    uint32_t sssp_slots[OG_SERVER_MAX_GRAPHS];  /* SSSP queries in flight at once per graph
+                                                   (batched slots + single-source lanes)   */
     uint64_t state_bytes;         /* HBM held for per-query state */
     uint64_t submitted, completed, rejected, pending, in_flight, rounds;
 } og_server_info;

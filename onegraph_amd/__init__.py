@@ -89,6 +89,8 @@ class PrStats(C.Structure):
 SERVER_MAX_GRAPHS = 8
 QUERY_BFS, QUERY_SSSP = 0, 1
 QUERY_KEEP_VALUES = 1
+PATH_BATCH, PATH_LANE = 0, 1  # og_query_result.path: a batch engine's slot / a single-source SSSP lane
+SERVER_NO_SSSP_LANES = 0x1  # every SSSP query runs in the batched engine (og.h OG_SERVER_NO_SSSP_LANES)
 SERVER_TEST_FAIL_SSSP = 0x80000000  # testing: SSSP engines stop at their first round (og.h OG_SERVER_TEST_FAIL_SSSP)
 
 
@@ -101,7 +103,7 @@ class QueryResult(C.Structure):
     _fields_ = [("id", C.c_uint64), ("graph", C.c_uint32), ("kind", C.c_uint32), ("source", C.c_uint32),
                 ("status", C.c_uint32), ("reached", C.c_uint64), ("traversed_edges", C.c_uint64),
                 ("checksum", C.c_uint64), ("queue_ms", C.c_double), ("latency_ms", C.c_double),
-                ("rounds", C.c_uint32), ("reserved", C.c_uint32)]
+                ("rounds", C.c_uint32), ("path", C.c_uint32)]
 
 
 class ServerInfo(C.Structure):

@@ -515,6 +515,20 @@ inline double sum_ms(const TimedClass &c) {
 // graph and the current tuning; freed with the DeviceState.
 void frontier_alloc(DeviceState *d, const og_graph *g, Frontier &f, MemPool &m);
 
+// A single-source lane over a prepared, wholly device-resident graph: it
+// shares the graph's device arrays (row pointers, edges, degree table, light
+// / heavy split, in-edges) and owns values, frontier buffers, streams and
+// captured rounds, so several lanes run single-source traversals side by
+// side on their own streams.  `delta` (0: the graph's tuning) is the lane's
+// near-far width.  Freed with device_state_free, which releases only what
+// the lane allocated.
+DeviceState *lane_state_create(og_graph *g, uint32_t delta);
+// Device bytes lane_state_create allocates for graph g (values, flags, bin
+// lists, degree prefix, block counts, hub tiles).
+uint64_t lane_state_bytes(const og_graph *g);
+// og_sssp from src on a lane; out (V values) may be null.
+void lane_run_sssp(og_graph *g, DeviceState *lane, uint32_t src, uint32_t *out, og_run_stats *st);
+
 // Resets f.ctl for a new traversal on stream s (a kernel, so it may sit in a
 // captured graph): statistics zeroed, near-far width `delta` with the first
 // threshold 1 + delta (the source's value is 1), or none for delta = 0;

@@ -2473,6 +2473,65 @@ static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t
     if (stats) *stats = st;
 }
 
+DeviceState *lane_state_create(og_graph *g, uint32_t delta) {
+    const DeviceState *d = g->dev;
+    if (d->plan.vsplit < d->V || d->managed) raise(OG_EUNSUPPORTED, "a lane needs the whole graph in device memory");
+    std::unique_ptr<DeviceState, void (*)(DeviceState *)> l(new DeviceState, device_state_free);
+    // the graph as prepared (read only: every array below belongs to g->dev)
+    l->approach = d->approach;
+    l->device = d->device;
+    l->V = d->V;
+    l->E = d->E;
+    l->Vpad = d->Vpad;
+    l->eb = d->eb;
+    l->deb = d->deb;
+    l->plan = d->plan;
+    l->tune = d->tune;
+    if (delta) l->tune.delta = delta;
+    l->host_threads = d->host_threads;
+    l->flags = d->flags;
+    l->d_ptr = d->d_ptr;
+    l->d_pmid = d->d_pmid;
+    l->d_deg = d->d_deg;
+    l->deg_split = d->deg_split;
+    l->split_weight = d->split_weight;
+    l->d_static = d->d_static;
+    l->static_used = d->static_used;
+    l->packed = d->packed;
+    l->wbits = d->wbits;
+    l->d_in_ptr = d->d_in_ptr;
+    l->d_in_cols = d->d_in_cols;
+    // the lane's own run state
+    OG_HIP(hipSetDevice(d->device));
+    OG_HIP(hipStreamCreateWithFlags(&l->s_compute, hipStreamNonBlocking));
+    OG_HIP(hipStreamCreateWithFlags(&l->s_copy, hipStreamNonBlocking));
+    OG_HIP(hipStreamCreateWithFlags(&l->s_static, hipStreamNonBlocking));
+    OG_HIP(hipEventCreateWithFlags(&l->ev_static, hipEventDisableTiming));
+    OG_HIP(hipEventCreateWithFlags(&l->ev_fork, hipEventDisableTiming));
+    l->d_val = (uint32_t *)l->dalloc(l->V * sizeof(uint32_t));
+    frontier_alloc(l.get(), g, l->fr, l->mem);
+    OG_HIP(hipDeviceSynchronize());
+    return l.release();
+}
+
+uint64_t lane_state_bytes(const og_graph *g) {
+    const DeviceState *d = g->dev;
+    const uint64_t V = d->V, Vpad = d->Vpad;
+    const uint64_t nblk = (Vpad + verts_per_block(frontier_fv(V)) - 1) / verts_per_block(frontier_fv(V));
+    const uint64_t nsb = (nblk + kSbBlocks - 1) / kSbBlocks;
+    const uint64_t hub = std::max<uint64_t>(hub_capacity(g, d->plan.vsplit, d->tune), 1);
+    // what lane_state_create and frontier_alloc request: values, flags (and
+    // the heavy-round flags with the split), bin lists, degree prefix, the
+    // (empty) dynamic chunk table, block counts, super-block sums and
+    // prefixes, round summary, control block, counters and hub tiles
+    return 4 * V + Vpad * (d->d_pmid ? 2 : 1) + 4 * V + 8 * (V + 1) + 4 * 2 + 4ull * kBcntRows * nblk + 8 * nblk +
+           8ull * kSbVals * nsb + 24 * nsb + sizeof(FrontierInfo) + sizeof(RoundCtl) + 16 * 8 + 8 * hub;
+}
+
+void lane_run_sssp(og_graph *g, DeviceState *lane, uint32_t src, uint32_t *out, og_run_stats *st) {
+    run_frontier<ALGO_SSSP, true>(g, src, out, st, lane);
+}
+
 static og_status run_algo(og_graph *g, int algo, uint32_t src, uint32_t *out, og_run_stats *stats) {
     clear_error();
     if (!g) return fail(OG_EINVAL, "graph is NULL");

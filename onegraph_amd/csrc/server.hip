@@ -30,7 +30,13 @@
 // one 64-byte line, every slot runs near-far rounds with its own threshold,
 // and the slots near at a vertex share the reads of its out-edges.  Queries
 // join at round boundaries like BFS queries.  (A 64-slot distance matrix,
-// 256 bytes per vertex, was tried first and lost to the caches.)
+// 256 bytes per vertex, was tried first and lost to the caches.)  Beside the
+// batched engine every weighted graph has up to kSsLanes single-source lanes
+// (og_sssp's captured device-driven rounds on state of their own), and an
+// SSSP query goes to an idle lane first: a lone query takes a few rounds of
+// a single-source traversal there, against tens of rounds of full-width
+// sweeps in the batched engine, which admits queries only while every lane
+// is busy and it already runs queries or kSsBatchMin of them wait.
 #include <algorithm>
 #include <condition_variable>
 #include <deque>
@@ -811,6 +817,27 @@ struct SsspEngine {
     void step();
     void finalize(uint32_t done);
     void fail_held(og_status st, const std::string &msg);
+    bool may_admit() const;
+};
+
+constexpr uint32_t kSsLanes = 2;           // single-source SSSP lanes per weighted graph (fewer if the budget is short)
+constexpr uint32_t kSsBatchMin = kSs / 2;  // waiting SSSP queries (every lane busy) that start a batch
+
+// A single-source SSSP lane: takes one query at a time from its graph's SSSP
+// queue and runs og_sssp's traversal on its own DeviceState (values,
+// frontier, streams, captured rounds) over the shared device-resident graph.
+struct SsspLane {
+    og_server *srv = nullptr;
+    og_graph *g = nullptr;
+    DeviceState *lane = nullptr;
+    uint32_t gi = 0;
+    std::deque<Query> *pending = nullptr;
+    EngineFailure *failure = nullptr;
+    bool busy = false;  // under srv->mu; stays true once the lane has failed
+    std::atomic<uint64_t> rounds_run{0};
+    std::thread th;
+    ~SsspLane() { device_state_free(lane); }
+    void loop();
 };
 
 }  // namespace og
@@ -821,6 +848,7 @@ struct og_server {
     std::vector<og::EngineFailure> failures;            // [graph * 2 + kind]
     std::vector<std::unique_ptr<og::BfsEngine>> bfs;    // [graph]
     std::vector<std::unique_ptr<og::SsspEngine>> sssp;  // [graph], null when unweighted
+    std::vector<std::vector<std::unique_ptr<og::SsspLane>>> lanes;  // [graph], empty when unweighted
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     bool stop = false;
@@ -1139,17 +1167,17 @@ void SsspEngine::step() {
     SsSeeds sl{};
     {
         std::unique_lock<std::mutex> lk(srv->mu);
-        srv->cv_work.wait(lk, [&] { return srv->stop || used || !pending->empty(); });
+        srv->cv_work.wait(lk, [&] { return srv->stop || used || may_admit(); });
         if (srv->stop) return;
         const double t = now_ms(srv);
         expire_queued(srv, gi, pending, t);
         const uint32_t clean = kSs - (uint32_t)__builtin_popcount(used | dirty);
-        if (dirty && (pending->size() > clean || __builtin_popcount(dirty) >= (int)kSs / 2)) {
+        if (dirty && may_admit() && (pending->size() > clean || __builtin_popcount(dirty) >= (int)kSs / 2)) {
             hipLaunchKernelGGL(k_ss_clear, dim3(sweep_grid(d->V, 16384)), dim3(kBlock), 0, s, dist, d->V, dirty);
             OG_HIP(hipGetLastError());
             dirty = 0;  // stream order: the reset runs before this round's seeds
         }
-        while (!pending->empty() && (uint32_t)__builtin_popcount(used | dirty) < kSs) {
+        while (may_admit() && (uint32_t)__builtin_popcount(used | dirty) < kSs) {
             uint32_t j = 0;
             while (((used | dirty) >> j) & 1u) ++j;
             slots[j].used = true;
@@ -1207,6 +1235,18 @@ void SsspEngine::step() {
     }
 }
 
+// Lanes first: with lanes, queued queries are admitted only while every
+// lane is busy and the engine already runs queries or kSsBatchMin wait (a
+// lone query is answered sooner by a single-source lane).  Caller holds
+// srv->mu.
+bool SsspEngine::may_admit() const {
+    if (pending->empty()) return false;
+    const auto &ls = srv->lanes[gi];
+    for (const auto &l : ls)
+        if (!l->busy) return false;
+    return ls.empty() || used != 0 || pending->size() >= kSsBatchMin;
+}
+
 void SsspEngine::fail_held(og_status st, const std::string &msg) {
     std::lock_guard<std::mutex> lk(srv->mu);
     fail_engine(srv, gi, pending, failure, slots, kSs, st, msg);
@@ -1230,6 +1270,75 @@ void SsspEngine::loop() {
     }
 }
 
+void SsspLane::loop() {
+    Query q{};
+    try {
+        OG_HIP(hipSetDevice(lane->device));
+        for (;;) {
+            double t_admit = 0;
+            {
+                std::unique_lock<std::mutex> lk(srv->mu);
+                srv->cv_work.wait(lk, [&] { return srv->stop || !pending->empty(); });
+                if (srv->stop) return;
+                t_admit = now_ms(srv);
+                expire_queued(srv, gi, pending, t_admit);
+                if (pending->empty()) continue;
+                q = pending->front();
+                pending->pop_front();
+                srv->pending--;
+                srv->in_flight++;
+                busy = true;
+            }
+            srv->cv_work.notify_all();  // with every lane busy the batched engine may admit the rest
+            if (srv->params.flags & OG_SERVER_TEST_FAIL_SSSP)
+                raise(OG_EHIP, "forced failure (OG_SERVER_TEST_FAIL_SSSP)");
+            std::vector<uint32_t> vals;
+            if (q.flags & OG_QUERY_KEEP_VALUES) vals.resize(lane->V);
+            og_run_stats st{};
+            lane_run_sssp(g, lane, q.source, vals.empty() ? nullptr : vals.data(), &st);
+            rounds_run.fetch_add(st.iterations, std::memory_order_relaxed);
+            const double t = now_ms(srv);
+            std::lock_guard<std::mutex> lk(srv->mu);
+            og_query_result r{};
+            r.id = q.id;
+            r.graph = gi;
+            r.kind = OG_QUERY_SSSP;
+            r.source = q.source;
+            r.status = OG_OK;
+            r.checksum = st.checksum;
+            r.reached = st.reached;
+            r.traversed_edges = st.traversed_edges;
+            r.rounds = st.iterations;
+            r.path = OG_PATH_LANE;
+            r.queue_ms = t_admit - q.t_submit;
+            r.latency_ms = t - q.t_submit;
+            srv->done.push_back(r);
+            if (!vals.empty()) srv->kept[q.id] = std::move(vals);
+            srv->completed++;
+            srv->in_flight--;
+            busy = false;
+            srv->cv_done.notify_all();
+        }
+    } catch (const Error &e) {
+        // the query in hand and the graph's SSSP queue fail, as when the
+        // batched engine stops; the lane stays busy, so it takes no more
+        std::lock_guard<std::mutex> lk(srv->mu);
+        SlotState held[1];
+        held[0].used = busy;
+        held[0].q = q;
+        fail_engine(srv, gi, pending, failure, held, 1, e.st, e.msg);
+        busy = true;
+    } catch (...) {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        SlotState held[1];
+        held[0].used = busy;
+        held[0].q = q;
+        fail_engine(srv, gi, pending, failure, held, 1, OG_ENOMEM_HOST, "SSSP lane stopped: host allocation failed");
+        busy = true;
+    }
+    srv->cv_work.notify_all();
+}
+
 static void server_free(og_server *srv) {
     {
         std::lock_guard<std::mutex> lk(srv->mu);
@@ -1240,6 +1349,10 @@ static void server_free(og_server *srv) {
         if (e && e->th.joinable()) e->th.join();
     for (auto &e : srv->sssp)
         if (e && e->th.joinable()) e->th.join();
+    for (auto &ls : srv->lanes)
+        for (auto &l : ls)
+            if (l->th.joinable()) l->th.join();
+    srv->lanes.clear();
     srv->bfs.clear();
     srv->sssp.clear();
     for (og_graph *g : srv->graphs) g->busy.store(0);
@@ -1292,10 +1405,18 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
         if (need > budget)
             raise(OG_EBUDGET, "state budget %llu bytes cannot hold the engines' query state (%llu bytes)",
                   (unsigned long long)budget, (unsigned long long)need);
+        // single-source SSSP lanes per weighted graph, as many (up to
+        // kSsLanes) as the rest of the budget holds for every weighted graph
+        uint64_t per_lane_set = 0;
+        for (og_graph *g : srv->graphs)
+            if (g->kind == OG_KIND_WEIGHTED) per_lane_set += lane_state_bytes(g);
+        uint32_t nlanes = (srv->params.flags & OG_SERVER_NO_SSSP_LANES) || !per_lane_set ? 0 : kSsLanes;
+        while (nlanes && need + nlanes * per_lane_set > budget) --nlanes;
         srv->queues.resize(2 * num_graphs);
         srv->failures.resize(2 * num_graphs);
         srv->bfs.resize(num_graphs);
         srv->sssp.resize(num_graphs);
+        srv->lanes.resize(num_graphs);
         for (uint32_t gi = 0; gi < num_graphs; ++gi) {
             og_graph *g = srv->graphs[gi];
             const uint64_t V = g->V;
@@ -1341,11 +1462,24 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             x->setup();
             srv->state_bytes += sssp_state_bytes(V, g->E);
             srv->sssp[gi] = std::move(x);
+            for (uint32_t k = 0; k < nlanes; ++k) {
+                std::unique_ptr<SsspLane> l(new SsspLane);
+                l->srv = srv;
+                l->g = g;
+                l->gi = gi;
+                l->pending = &srv->queues[2 * gi + 1];
+                l->failure = &srv->failures[2 * gi + 1];
+                l->lane = lane_state_create(g, srv->params.sssp_delta);
+                srv->state_bytes += l->lane->device_bytes();
+                srv->lanes[gi].push_back(std::move(l));
+            }
         }
         OG_HIP(hipDeviceSynchronize());
         for (auto &e : srv->bfs) e->th = std::thread(&BfsEngine::loop, e.get());
         for (auto &x : srv->sssp)
             if (x) x->th = std::thread(&SsspEngine::loop, x.get());
+        for (auto &ls : srv->lanes)
+            for (auto &l : ls) l->th = std::thread(&SsspLane::loop, l.get());
         return OG_OK;
     });
     if (st != OG_OK) {
@@ -1425,9 +1559,10 @@ extern "C" og_status og_server_get_info(og_server *srv, og_server_info *info) {
     info->num_graphs = (uint32_t)srv->graphs.size();
     for (uint32_t gi = 0; gi < srv->graphs.size(); ++gi) {
         info->bfs_slots[gi] = kSlots;
-        info->sssp_slots[gi] = srv->sssp[gi] ? kSs : 0;
+        info->sssp_slots[gi] = srv->sssp[gi] ? kSs + (uint32_t)srv->lanes[gi].size() : 0;
         info->rounds += srv->bfs[gi]->rounds_run.load(std::memory_order_relaxed);
         if (srv->sssp[gi]) info->rounds += srv->sssp[gi]->rounds_run.load(std::memory_order_relaxed);
+        for (auto &l : srv->lanes[gi]) info->rounds += l->rounds_run.load(std::memory_order_relaxed);
     }
     info->state_bytes = srv->state_bytes;
     info->submitted = srv->submitted;

@@ -238,15 +238,16 @@ def test_sssp_batch_bucket_widths(gpu, delta):
     ptr, to, w, _ = g.export()
     cand = np.flatnonzero(np.diff(np.append(ptr.astype(np.int64), len(to))) > 0)
     srcs = [int(x) for x in np.random.default_rng(63).choice(cand, 20, replace=False)]
-    with og.Server([g]) as srv:
+    lanes_off = og.SERVER_NO_SSSP_LANES  # the batched engine takes every query
+    with og.Server([g], flags=lanes_off) as srv:
         ids = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs]
         res = _wait_all(srv, ids)
-    with og.Server([g], sssp_delta=7) as srv:  # the server's own width over the graph's
+    with og.Server([g], sssp_delta=7, flags=lanes_off) as srv:  # the server's own width over the graph's
         ids2 = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs]
         res2 = _wait_all(srv, ids2)
     g.prepare("ascetic", edge_budget_bytes=g.num_edges * 8, advisek=1.0, split_weight=8)
     g.set_tuning(delta=delta)
-    with og.Server([g], sssp_delta=256 if delta == 32 else 0) as srv:  # light / heavy rounds
+    with og.Server([g], sssp_delta=256 if delta == 32 else 0, flags=lanes_off) as srv:  # light / heavy rounds
         ids3 = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs[:10]]
         time.sleep(0.002)
         ids3 += [srv.submit(0, og.QUERY_SSSP, s) for s in srcs[10:]]  # joining running slots
@@ -290,12 +291,61 @@ def test_sssp_batch_many_sweep_blocks(gpu, split):
     cand = np.flatnonzero(np.diff(np.append(ptr.astype(np.int64), len(to))) > 0)
     srcs = [int(x) for x in np.random.default_rng(73).choice(cand, 24, replace=False)]
     _say("sssp batch rmat16 split", split)
-    with og.Server([g], sssp_delta=256 if split else 64) as srv:
+    with og.Server([g], sssp_delta=256 if split else 64, flags=og.SERVER_NO_SSSP_LANES) as srv:
         ids = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs]
         res = _wait_all(srv, ids)
     for n, s in enumerate(srcs):
         assert res[ids[n]]["checksum"] == og.levels_checksum(oracle.sssp(ptr, to, w, s)), (split, s)
     g.close()
+
+
+def test_sssp_lanes_first(gpu, graphs):
+    """SSSP queries go to an idle single-source lane first: a lone query runs
+    as og_sssp does (same rounds); a burst fills both lanes and then the
+    batched engine (8 or more waiting); fewer waiting queries wait for a lane.
+    The budget decides how many lanes a weighted graph gets."""
+    g = graphs[0]
+    ptr, to, w, _ = g.export()
+    deg = np.diff(np.append(ptr.astype(np.int64), len(to)))
+    cand = np.flatnonzero(deg > 0)
+    srcs = [int(x) for x in np.random.default_rng(29).choice(cand, 40, replace=False)]
+    _, st0 = g.sssp(srcs[0], want_values=False)
+    with og.Server([g]) as srv:
+        assert srv.info()["sssp_slots"] == [16 + 2]
+        i = srv.submit(0, og.QUERY_SSSP, srcs[0], keep=True)
+        r = _wait_all(srv, [i])[i]
+        assert r["status"] == 0 and r["path"] == og.PATH_LANE
+        assert np.array_equal(srv.take_values(i, 0), oracle.sssp(ptr, to, w, srcs[0]))
+        assert r["checksum"] == st0["checksum"]
+        for burst in (srcs[1:5], srcs[5:40]):  # fewer than 8 past the lanes, then a full batch
+            _say("lanes: burst of", len(burst))
+            ids = [srv.submit(0, og.QUERY_SSSP, s, keep=True) for s in burst]
+            res = _wait_all(srv, ids)
+            paths = [res[i]["path"] for i in ids]
+            if len(burst) < 2 + 8:  # never 8 waiting: lanes only
+                assert set(paths) == {og.PATH_LANE}, paths
+            else:  # 33 waiting behind the lanes: batches
+                assert og.PATH_BATCH in paths and paths.count(og.PATH_LANE) >= 2, paths
+            for n, s in enumerate(burst):
+                ref = oracle.sssp(ptr, to, w, s)
+                assert res[ids[n]]["status"] == 0
+                assert np.array_equal(srv.take_values(ids[n], 0), ref), s
+                assert res[ids[n]]["checksum"] == og.levels_checksum(ref)
+                assert res[ids[n]]["traversed_edges"] == int(deg[ref != og.UNREACHED].sum())
+        info = srv.info()
+        assert info["in_flight"] == 0 and info["pending"] == 0 and info["completed"] == 40
+        lanes2 = info["state_bytes"] - _state_bytes([g])
+    # the exact engine state plus one lane's bytes: one lane
+    with og.Server([g], state_budget_bytes=_state_bytes([g]) + lanes2 // 2) as srv:
+        assert srv.info()["sssp_slots"] == [16 + 1]
+        assert srv.info()["state_bytes"] == _state_bytes([g]) + lanes2 // 2
+        ids = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs[:12]]
+        res = _wait_all(srv, ids)
+        for n, s in enumerate(srcs[:12]):
+            assert res[ids[n]]["checksum"] == og.levels_checksum(oracle.sssp(ptr, to, w, s))
+    with og.Server([g], flags=og.SERVER_NO_SSSP_LANES) as srv:
+        assert srv.info()["sssp_slots"] == [16]
+        assert srv.info()["state_bytes"] == _state_bytes([g])
 
 
 def test_deadline_expires_queued_queries(gpu, graphs):
