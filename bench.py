@@ -725,17 +725,19 @@ def main():
     ap.add_argument("--serve-sssp-delta", type=int, default=0,
                     help="serve: bucket width of the server's batched SSSP slots (0: the graphs' tuning delta, "
                          "256 with the light/heavy split)")
-    ap.add_argument("--hw-queues", type=int, default=16,
-                    help="serve: GPU_MAX_HW_QUEUES for the process (each engine's stream gets its own hardware "
-                         "queue instead of sharing one of HIP's default 4; at most 32)")
+    ap.add_argument("--hw-queues", type=int, default=24,
+                    help="serve: GPU_MAX_HW_QUEUES for the process (each engine's and lane's stream gets its own "
+                         "hardware queue instead of sharing one of HIP's default 4; at most 32)")
     args = ap.parse_args()
     if args.scale is None:
         args.scale = {"sssp-ooc": 24, "bfs-hbm": 20, "cc-ooc": 25, "pr-ooc": 25, "msbfs": 26, "serve": 0,
                       "sssp-batch": 24}[args.workload]
     if args.workload == "serve" and args.hw_queues:
-        # before the first HIP call: the server's six engine streams (BFS and
-        # SSSP per graph) each get a hardware queue, so one engine's kernels do
-        # not wait behind another's in a shared queue
+        # before the first HIP call: the graphs' own streams (3 each), the
+        # server's engine streams (BFS and batched SSSP per graph) and its
+        # SSSP lanes' streams (2 per weighted graph), 21 for three graphs,
+        # each get a hardware queue, so one engine's kernels do not wait
+        # behind another's in a shared queue
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, args.hw_queues))
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); --gpus {args.gpus} refused",

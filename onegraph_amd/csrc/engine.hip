@@ -1221,9 +1221,10 @@ void device_state_free(DeviceState *d) {
     for (hipEvent_t e : d->ev_h2d) (void)hipEventDestroy(e);
     for (hipEvent_t e : d->ev_relax) (void)hipEventDestroy(e);
     d->mem.release();
+    // (a lane runs everything on one stream: s_copy and s_static alias it)
+    if (d->s_copy && d->s_copy != d->s_compute) (void)hipStreamDestroy(d->s_copy);
+    if (d->s_static && d->s_static != d->s_compute) (void)hipStreamDestroy(d->s_static);
     if (d->s_compute) (void)hipStreamDestroy(d->s_compute);
-    if (d->s_copy) (void)hipStreamDestroy(d->s_copy);
-    if (d->s_static) (void)hipStreamDestroy(d->s_static);
     if (d->ev_static) (void)hipEventDestroy(d->ev_static);
     if (d->ev_fork) (void)hipEventDestroy(d->ev_fork);
     delete d;
@@ -2501,11 +2502,12 @@ DeviceState *lane_state_create(og_graph *g, uint32_t delta) {
     l->wbits = d->wbits;
     l->d_in_ptr = d->d_in_ptr;
     l->d_in_cols = d->d_in_cols;
-    // the lane's own run state
+    // the lane's own run state, on one stream: a graph wholly in HBM has no
+    // dynamic region to copy or overlap, and every stream of a process takes
+    // a hardware queue (or shares one) while the server runs
     OG_HIP(hipSetDevice(d->device));
     OG_HIP(hipStreamCreateWithFlags(&l->s_compute, hipStreamNonBlocking));
-    OG_HIP(hipStreamCreateWithFlags(&l->s_copy, hipStreamNonBlocking));
-    OG_HIP(hipStreamCreateWithFlags(&l->s_static, hipStreamNonBlocking));
+    l->s_copy = l->s_static = l->s_compute;
     OG_HIP(hipEventCreateWithFlags(&l->ev_static, hipEventDisableTiming));
     OG_HIP(hipEventCreateWithFlags(&l->ev_fork, hipEventDisableTiming));
     l->d_val = (uint32_t *)l->dalloc(l->V * sizeof(uint32_t));
