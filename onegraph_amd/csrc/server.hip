@@ -99,9 +99,14 @@ __global__ void k_srv_seed(SeedList sl, uint64_t *next, uint8_t *flag, unsigned 
 // address, which the LDS serialises; a stride of 65 words puts the copies of
 // a slot on distinct banks.  The slots' levels and kept-result pointers are
 // staged in LDS first: indexed per lane, the by-value kernel arguments would
-// be read from memory once per (vertex, slot).
+// be read from memory once per (vertex, slot).  A slot's reached-vertex
+// count and their out-degrees share one word (count << kFcShift | degrees),
+// so a (vertex, slot) costs two LDS atomics, not three: a workgroup commits
+// fewer than 2^24 vertices (at most 4096 workgroups sweep V < 2^36) whose
+// degrees sum below 2^40 (the host refuses graphs with E >= 2^40).
 constexpr int kCommitCopies = 16;
 constexpr int kCommitStride = 65;
+constexpr int kFcShift = 40;
 __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ flag, uint64_t V,
                                                        const uint64_t *__restrict__ ptr,
                                                        const uint32_t *__restrict__ deg,
@@ -109,12 +114,11 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
                                                        uint64_t *__restrict__ front, uint64_t *__restrict__ next,
                                                        SlotLevels L, KeepList keep, unsigned long long *acc,
                                                        unsigned long long *fstat, unsigned long long *alive) {
-    __shared__ unsigned long long sdig[kCommitCopies * kCommitStride], sfc[kCommitCopies * kCommitStride],
-        sfe[kCommitCopies * kCommitStride];
+    __shared__ unsigned long long sdig[kCommitCopies * kCommitStride], sfe[kCommitCopies * kCommitStride];
     __shared__ uint32_t slv[64];
     __shared__ uint32_t *skeep[64];
     __shared__ unsigned long long skm;
-    for (int i = threadIdx.x; i < kCommitCopies * kCommitStride; i += kBlock) sdig[i] = sfc[i] = sfe[i] = 0;
+    for (int i = threadIdx.x; i < kCommitCopies * kCommitStride; i += kBlock) sdig[i] = sfe[i] = 0;
     if (threadIdx.x < 64) {
         slv[threadIdx.x] = L.lv[threadIdx.x];
         skeep[threadIdx.x] = keep.p[threadIdx.x];
@@ -144,8 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
                     m &= m - 1;
                     const uint32_t lv = slv[j];
                     atomicAdd(&sdig[cp + j], (unsigned long long)srv_mix64((v << 32) | lv));
-                    atomicAdd(&sfc[cp + j], 1ull);
-                    if (dv) atomicAdd(&sfe[cp + j], (unsigned long long)dv);
+                    atomicAdd(&sfe[cp + j], (1ull << kFcShift) | (unsigned long long)dv);
                     if ((kmask >> j) & 1) skeep[j][v] = lv;
                 }
             }
@@ -162,12 +165,12 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
     if ((threadIdx.x & (kWave - 1)) == 0 && al) atomicOr(alive, (unsigned long long)al);
     __syncthreads();
     if (threadIdx.x < 64) {
-        unsigned long long dg = 0, fc = 0, fe = 0;
+        unsigned long long dg = 0, x = 0;
         for (int c = 0; c < kCommitCopies; ++c) {
             dg += sdig[c * kCommitStride + threadIdx.x];
-            fc += sfc[c * kCommitStride + threadIdx.x];
-            fe += sfe[c * kCommitStride + threadIdx.x];
+            x += sfe[c * kCommitStride + threadIdx.x];
         }
+        const unsigned long long fc = x >> kFcShift, fe = x & ((1ull << kFcShift) - 1);
         if (dg) atomicAdd(acc + threadIdx.x, dg);
         if (fc) {
             atomicAdd(fstat + threadIdx.x, fc);
@@ -179,9 +182,14 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
 // End of the commit: the round's newly reached vertices and edges of each
 // slot join its reached / traversed totals (acc[64 + j], acc[128 + j]), and,
 // with dir_opt, every slot picks its direction for the round (Beamer et al.,
-// SC'12, per slot): top-down -> bottom-up once its frontier edges exceed 1/14
-// of its unexplored edges, back when its frontier holds fewer than V / 24
-// vertices.  Bit j of *pull = slot j relaxes bottom-up.  Resets the stats.
+// SC'12, per slot): top-down -> bottom-up once its frontier edges exceed
+// 1/kSrvAlpha of its unexplored edges, back when its frontier holds fewer
+// than V / 24 vertices.  kSrvAlpha is twice the single-source BFS's 14: the
+// bottom-up pass sweeps V once for every pulling slot of the round, so in a
+// batch it pays off earlier (R-MAT 26, 333 queries/s: rounds 3.7 -> 3.3 ms,
+// p99 59 -> 45 ms; 56 gave the same as 28).  Bit j of *pull = slot j relaxes
+// bottom-up.  Resets the stats.
+constexpr uint64_t kSrvAlpha = 28;
 __global__ void k_srv_direction(uint64_t used, uint64_t V, uint64_t E, int dir_opt, unsigned long long *fstat,
                                 unsigned long long *explored, unsigned long long *acc, unsigned long long *pull) {
     const uint32_t j = threadIdx.x;
@@ -192,7 +200,7 @@ __global__ void k_srv_direction(uint64_t used, uint64_t V, uint64_t E, int dir_o
     const uint64_t unexplored = E > ex + fe ? E - ex - fe : 0;
     const bool was = (*pull >> j) & 1;
     bool now = false;
-    if (dir_opt && ((used >> j) & 1)) now = was ? fc >= V / 24 : fe > unexplored / 14;
+    if (dir_opt && ((used >> j) & 1)) now = was ? fc >= V / 24 : fe > unexplored / kSrvAlpha;
     explored[j] = ex + fe;
     fstat[j] = fstat[64 + j] = 0;
     const uint64_t b = __ballot(now);
@@ -1374,6 +1382,8 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
         if (!g) return fail(OG_EINVAL, "graph %u is NULL", i);
         if (!g->dev) return fail(OG_ESTATE, "graph %u has not been prepared", i);
         if (g->kind == OG_KIND_PULL) return fail(OG_EUNSUPPORTED, "graph %u is a pull-layout graph", i);
+        if (g->E >> kFcShift)  // the commit's per-workgroup degree sums
+            return fail(OG_EUNSUPPORTED, "graph %u has 2^%d edges or more", i, kFcShift);
         if (g->dev->plan.vsplit < g->V)
             return fail(OG_EUNSUPPORTED, "graph %u is not device resident (prepare it with the whole graph static)", i);
         for (uint32_t k = 0; k < i; ++k)
