@@ -24,6 +24,7 @@
 #include <atomic>
 #include <vector>
 #include <cstdio>
+#include <cstring>
 
 #include "device.h"
 
@@ -1227,7 +1228,45 @@ void device_state_free(DeviceState *d) {
     if (d->s_compute) (void)hipStreamDestroy(d->s_compute);
     if (d->ev_static) (void)hipEventDestroy(d->ev_static);
     if (d->ev_fork) (void)hipEventDestroy(d->ev_fork);
+    for (hipEvent_t e : d->ev_out)
+        if (e) (void)hipEventDestroy(e);
     delete d;
+}
+
+constexpr size_t kOutChunk = 8u << 20;  // bytes per pinned staging chunk of a result copy-back
+
+void copy_to_host(DeviceState *d, void *dst, const void *src, size_t n, hipStream_t s) {
+    if (n <= kOutChunk) {  // one chunk: nothing to overlap
+        OG_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s));
+        OG_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    if (!d->h_out[0]) {
+        for (auto &p : d->h_out) p = d->palloc(kOutChunk);
+        for (auto &e : d->ev_out) OG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    if (!d->pool) d->pool.reset(new WorkerPool(d->host_threads));
+    const size_t nch = (n + kOutChunk - 1) / kOutChunk;
+    auto issue = [&](size_t k) {
+        const size_t off = k * kOutChunk, len = std::min(kOutChunk, n - off);
+        OG_HIP(hipMemcpyAsync(d->h_out[k & 1], (const char *)src + off, len, hipMemcpyDeviceToHost, s));
+        OG_HIP(hipEventRecord(d->ev_out[k & 1], s));
+    };
+    issue(0);
+    for (size_t k = 0; k < nch; ++k) {
+        // chunk k + 1 reuses the buffer chunk k - 1 left: its host copy ended
+        // in the previous iteration
+        if (k + 1 < nch) issue(k + 1);
+        OG_HIP(hipEventSynchronize(d->ev_out[k & 1]));
+        const size_t off = k * kOutChunk, len = std::min(kOutChunk, n - off);
+        const char *from = (const char *)d->h_out[k & 1];
+        char *to = (char *)dst + off;
+        const unsigned T = d->pool->size();
+        d->pool->run([&](unsigned t) {
+            const size_t a = len * t / T, b = len * (t + 1) / T;
+            std::memcpy(to + a, from + a, b - a);
+        });
+    }
 }
 
 // Largest number of hub tiles any frontier can produce for the current tuning.
@@ -2388,7 +2427,7 @@ static void run_frontier(og_graph *g, uint32_t src, uint32_t *out, og_run_stats 
     }
     OG_HIP(hipGetLastError());
     OG_HIP(hipMemcpyAsync(d->fr.h_counters, d->fr.counters, 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    if (out) OG_HIP(hipMemcpyAsync(out, d->d_val, V * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (out) copy_to_host(d, out, d->d_val, V * sizeof(uint32_t), s);
     OG_HIP(hipStreamSynchronize(s));
     st.reached = d->fr.h_counters[2];
     st.traversed_edges = d->fr.h_counters[3];
@@ -2462,7 +2501,7 @@ static void run_msbfs(og_graph *g, const uint32_t *sources, uint32_t n, uint32_t
             for (uint32_t j = 0; j < nb; ++j) {
                 hipLaunchKernelGGL(k_ms_extract, dim3(fill_grid(V)), dim3(256), 0, s, d->d_mlvl, V, j, d->d_extract);
                 OG_HIP(hipGetLastError());
-                OG_HIP(hipMemcpyAsync(levels + (uint64_t)(b0 + j) * V, d->d_extract, V * 4, hipMemcpyDeviceToHost, s));
+                copy_to_host(d, levels + (uint64_t)(b0 + j) * V, d->d_extract, V * 4, s);
             }
             OG_HIP(hipStreamSynchronize(s));
         }

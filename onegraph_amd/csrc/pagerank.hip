@@ -389,7 +389,7 @@ static void run_pr(og_graph *g, const og_pr_params &prm, T *out, og_pr_stats *st
     }
     OG_HIP(hipStreamSynchronize(s));
     st.run.total_ms = total.ms();
-    if (out) OG_HIP(hipMemcpy(out, rank, V * sizeof(T), hipMemcpyDeviceToHost));
+    if (out) copy_to_host(d, out, rank, V * sizeof(T), s);
     // keep rank buffer order stable for the next call
     if (rank != (T *)d->d_rank[0]) std::swap(d->d_rank[0], d->d_rank[1]);
     st.iterations = it;
