@@ -485,11 +485,11 @@ def wl_serve(args):
     # per-query state (og_server_create's admission rule): per graph the BFS
     # engine (40 V bytes), per weighted graph the 16-slot SSSP engine (92 V + 8
     # + the relax's chunk table), then as many single-source SSSP lanes (up to
-    # 2 per weighted graph) as the rest holds; by default the budget holds
-    # the engines and both lanes
+    # --sssp-lanes per weighted graph) as the rest holds; by default the
+    # budget holds the engines and all the lanes
     engines = sum(40 * g.num_vertices + (92 * g.num_vertices + 8 + 4 * (16 * g.num_edges // 1024 + 2)
                                          if g.info()["kind"] == og.KIND_WEIGHTED else 0) for g in graphs)
-    lanes = sum(2 * 19 * g.num_vertices for g in graphs if g.info()["kind"] == og.KIND_WEIGHTED)
+    lanes = sum(args.sssp_lanes * 19 * g.num_vertices for g in graphs if g.info()["kind"] == og.KIND_WEIGHTED)
     budget = int(args.serve_budget_gib * (1 << 30)) if args.serve_budget_gib else engines + lanes
     srv_flags = og.SERVER_NO_SSSP_LANES if args.no_sssp_lanes else 0
 
@@ -508,7 +508,8 @@ def wl_serve(args):
         t_arr, gis, kinds, srcs = trace(rate, seed, bfs_share)
         plain_qps, plat = replay(t_arr, gis, kinds, service, seed + 4)
         td_qps, _ = replay(t_arr, gis, kinds, service_td, seed + 4)
-        srv = og.Server(graphs, state_budget_bytes=budget, sssp_delta=args.serve_sssp_delta, flags=srv_flags)
+        srv = og.Server(graphs, state_budget_bytes=budget, sssp_delta=args.serve_sssp_delta, flags=srv_flags,
+                        sssp_lanes=args.sssp_lanes)
         info0 = srv.info()
         lat, done, submit_at = [], {}, {}
         t0 = time.perf_counter()
@@ -594,7 +595,7 @@ def wl_serve(args):
                        "rates": rates, "mixes": mixes, "duration_s": args.duration, "state_budget_bytes": budget,
                        "p99_target_ms": args.p99_target_ms, "sssp_delta": args.delta,
                        "served_sssp_delta": args.serve_sssp_delta or args.delta,
-                       "sssp_lanes": 0 if args.no_sssp_lanes else 2,
+                       "sssp_lanes": 0 if args.no_sssp_lanes else args.sssp_lanes,
                        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                        "split_weight": args.split_weight})
     out.update({"serve": per_mix,
@@ -717,6 +718,7 @@ def main():
     ap.add_argument("--duration", type=float, default=3.0, help="serve: seconds of arrivals per rate")
     ap.add_argument("--serve-scales", default="22,24,26", help="serve: R-MAT scales of the graphs")
     ap.add_argument("--serve-budget-gib", type=float, default=0.0, help="serve: per-query state budget")
+    ap.add_argument("--sssp-lanes", type=int, default=2, help="serve: single-source SSSP lanes per weighted graph")
     ap.add_argument("--no-sssp-lanes", action="store_true",
                     help="serve: no single-source SSSP lanes (every SSSP query in the batched engine)")
     ap.add_argument("--verify", type=int, default=100, help="serve: served results re-run on the plain path")
@@ -725,20 +727,23 @@ def main():
     ap.add_argument("--serve-sssp-delta", type=int, default=0,
                     help="serve: bucket width of the server's batched SSSP slots (0: the graphs' tuning delta, "
                          "256 with the light/heavy split)")
-    ap.add_argument("--hw-queues", type=int, default=24,
+    ap.add_argument("--hw-queues", type=int, default=0,
                     help="serve: GPU_MAX_HW_QUEUES for the process (each engine's and lane's stream gets its own "
-                         "hardware queue instead of sharing one of HIP's default 4; at most 32)")
+                         "hardware queue instead of sharing one of HIP's default 4; at most 32; 0: one per stream "
+                         "of the three graphs, their engines and lanes)")
     args = ap.parse_args()
     if args.scale is None:
         args.scale = {"sssp-ooc": 24, "bfs-hbm": 20, "cc-ooc": 25, "pr-ooc": 25, "msbfs": 26, "serve": 0,
                       "sssp-batch": 24}[args.workload]
-    if args.workload == "serve" and args.hw_queues:
+    if args.workload == "serve":
         # before the first HIP call: the graphs' own streams (3 each), the
         # server's engine streams (BFS and batched SSSP per graph) and its
-        # SSSP lanes' streams (2 per weighted graph), 21 for three graphs,
-        # each get a hardware queue, so one engine's kernels do not wait
-        # behind another's in a shared queue
-        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, args.hw_queues))
+        # SSSP lanes' streams (one per lane), 21 for three graphs with two
+        # lanes each, each get a hardware queue, so one engine's kernels do
+        # not wait behind another's in a shared queue
+        ng = len(args.serve_scales.split(","))
+        hwq = args.hw_queues or 3 * ng + 2 * ng + ng * (0 if args.no_sssp_lanes else args.sssp_lanes)
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, hwq))
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); --gpus {args.gpus} refused",
               file=sys.stderr)

@@ -431,7 +431,7 @@ typedef struct og_server_params {
                                      chunk table (4 (16 E / 1024 + 2) bytes);
                                      then, per weighted graph and as far as
                                      the rest of the budget holds them, up to
-                                     2 single-source SSSP lanes (values,
+                                     sssp_lanes single-source SSSP lanes (values,
                                      frontier flags, bin lists and degree
                                      prefix: about 17-18 V bytes each);
                                      0 = a quarter of free HBM                 */
@@ -448,6 +448,9 @@ typedef struct og_server_params {
                                      slots run light/heavy rounds (wide
                                      buckets pay: 256 with weights in [1, 63]),
                                      else they relax whole rows (16-64)       */
+    uint32_t sssp_lanes;          /* single-source SSSP lanes per weighted
+                                     graph, 1..8 (0: 2), as far as the budget
+                                     holds them                                */
 } og_server_params;
 
 /* SSSP queries of a weighted graph go first to an idle single-source lane

@@ -31,7 +31,8 @@
 // and the slots near at a vertex share the reads of its out-edges.  Queries
 // join at round boundaries like BFS queries.  (A 64-slot distance matrix,
 // 256 bytes per vertex, was tried first and lost to the caches.)  Beside the
-// batched engine every weighted graph has up to kSsLanes single-source lanes
+// batched engine every weighted graph has single-source lanes (kSsLanes unless
+// og_server_params.sssp_lanes says otherwise)
 // (og_sssp's captured device-driven rounds on state of their own), and an
 // SSSP query goes to an idle lane first: a lone query takes a few rounds of
 // a single-source traversal there, against tens of rounds of full-width
@@ -828,7 +829,8 @@ struct SsspEngine {
     bool may_admit() const;
 };
 
-constexpr uint32_t kSsLanes = 2;           // single-source SSSP lanes per weighted graph (fewer if the budget is short)
+constexpr uint32_t kSsLanes = 2;           // default single-source SSSP lanes per weighted graph
+constexpr uint32_t kSsLanesMax = 8;        // og_server_params.sssp_lanes at most
 constexpr uint32_t kSsBatchMin = kSs / 2;  // waiting SSSP queries (every lane busy) that start a batch
 
 // A single-source SSSP lane: takes one query at a time from its graph's SSSP
@@ -1420,7 +1422,9 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
         uint64_t per_lane_set = 0;
         for (og_graph *g : srv->graphs)
             if (g->kind == OG_KIND_WEIGHTED) per_lane_set += lane_state_bytes(g);
-        uint32_t nlanes = (srv->params.flags & OG_SERVER_NO_SSSP_LANES) || !per_lane_set ? 0 : kSsLanes;
+        const uint32_t want = srv->params.sssp_lanes ? srv->params.sssp_lanes : kSsLanes;
+        if (want > kSsLanesMax) raise(OG_EINVAL, "sssp_lanes %u > %u", want, kSsLanesMax);
+        uint32_t nlanes = (srv->params.flags & OG_SERVER_NO_SSSP_LANES) || !per_lane_set ? 0 : want;
         while (nlanes && need + nlanes * per_lane_set > budget) --nlanes;
         srv->queues.resize(2 * num_graphs);
         srv->failures.resize(2 * num_graphs);
