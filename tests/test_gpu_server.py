@@ -346,6 +346,17 @@ def test_sssp_lanes_first(gpu, graphs):
     with og.Server([g], flags=og.SERVER_NO_SSSP_LANES) as srv:
         assert srv.info()["sssp_slots"] == [16]
         assert srv.info()["state_bytes"] == _state_bytes([g])
+    with og.Server([g], sssp_lanes=4) as srv:  # more lanes: a burst of 12 never reaches the batch
+        assert srv.info()["sssp_slots"] == [16 + 4]
+        assert srv.info()["state_bytes"] == _state_bytes([g]) + 2 * lanes2
+        ids = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs[:11]]
+        res = _wait_all(srv, ids)
+        assert {res[i]["path"] for i in ids} == {og.PATH_LANE}
+        for n, s in enumerate(srcs[:11]):
+            assert res[ids[n]]["checksum"] == og.levels_checksum(oracle.sssp(ptr, to, w, s))
+    with pytest.raises(og.OgError) as e:
+        og.Server([g], sssp_lanes=9)
+    assert e.value.name == "OG_EINVAL"
 
 
 def test_deadline_expires_queued_queries(gpu):
