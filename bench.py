@@ -483,15 +483,17 @@ def wl_serve(args):
     service = sample_plain("diropt", og.PLAN_DIRECTION_OPT)
 
     # per-query state (og_server_create's admission rule): per graph the BFS
-    # engine (40 V bytes), per weighted graph the 16-slot SSSP engine (92 V + 8
-    # + the relax's chunk table), then as many single-source SSSP lanes (up to
-    # --sssp-lanes per weighted graph) as the rest holds; by default the
-    # budget holds the engines and all the lanes
+    # engine (40 V bytes), per weighted graph with batching the 16-slot SSSP
+    # engine (92 V + 8 + the relax's chunk table), then as many single-source
+    # SSSP lanes (up to --sssp-lanes per weighted graph) as the rest holds;
+    # by default the budget holds the engines and all the lanes
+    batch = args.sssp_batch or args.no_sssp_lanes
     engines = sum(40 * g.num_vertices + (92 * g.num_vertices + 8 + 4 * (16 * g.num_edges // 1024 + 2)
-                                         if g.info()["kind"] == og.KIND_WEIGHTED else 0) for g in graphs)
+                                         if g.info()["kind"] == og.KIND_WEIGHTED and batch else 0) for g in graphs)
     lanes = sum(args.sssp_lanes * 19 * g.num_vertices for g in graphs if g.info()["kind"] == og.KIND_WEIGHTED)
     budget = int(args.serve_budget_gib * (1 << 30)) if args.serve_budget_gib else engines + lanes
-    srv_flags = og.SERVER_NO_SSSP_LANES if args.no_sssp_lanes else 0
+    srv_flags = (og.SERVER_NO_SSSP_LANES if args.no_sssp_lanes else 0) | (og.SERVER_SSSP_BATCH if args.sssp_batch
+                                                                         else 0)
 
     def replay(t_arr, gis, kinds, svc, seed):
         """FIFO one-at-a-time service of the trace with sampled service times."""
@@ -596,6 +598,7 @@ def wl_serve(args):
                        "p99_target_ms": args.p99_target_ms, "sssp_delta": args.delta,
                        "served_sssp_delta": args.serve_sssp_delta or args.delta,
                        "sssp_lanes": 0 if args.no_sssp_lanes else args.sssp_lanes,
+                       "sssp_batch": bool(args.sssp_batch or args.no_sssp_lanes),
                        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                        "split_weight": args.split_weight})
     out.update({"serve": per_mix,
@@ -719,6 +722,8 @@ def main():
     ap.add_argument("--serve-scales", default="22,24,26", help="serve: R-MAT scales of the graphs")
     ap.add_argument("--serve-budget-gib", type=float, default=0.0, help="serve: per-query state budget")
     ap.add_argument("--sssp-lanes", type=int, default=2, help="serve: single-source SSSP lanes per weighted graph")
+    ap.add_argument("--sssp-batch", action="store_true",
+                    help="serve: the batched SSSP engine beside the lanes (admits when the lanes are busy and 8 wait)")
     ap.add_argument("--no-sssp-lanes", action="store_true",
                     help="serve: no single-source SSSP lanes (every SSSP query in the batched engine)")
     ap.add_argument("--verify", type=int, default=100, help="serve: served results re-run on the plain path")
@@ -742,7 +747,8 @@ def main():
         # lanes each, each get a hardware queue, so one engine's kernels do
         # not wait behind another's in a shared queue
         ng = len(args.serve_scales.split(","))
-        hwq = args.hw_queues or 3 * ng + 2 * ng + ng * (0 if args.no_sssp_lanes else args.sssp_lanes)
+        batch = args.sssp_batch or args.no_sssp_lanes
+        hwq = args.hw_queues or 3 * ng + ng * (2 if batch else 1) + ng * (0 if args.no_sssp_lanes else args.sssp_lanes)
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, hwq))
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); --gpus {args.gpus} refused",

@@ -421,19 +421,22 @@ og_status og_pagerank(og_graph *g, const og_pr_params *params, float *ranks_f32,
 typedef struct og_server og_server;
 
 typedef struct og_server_params {
-    uint64_t state_budget_bytes;  /* HBM for per-query state, all of which
-                                     must fit (else OG_EBUDGET): per graph the
-                                     BFS engine's masks and 4 kept-levels
-                                     buffers (40 V bytes), per weighted graph
-                                     the SSSP engine's 16 distance columns,
-                                     light / heavy pending masks, active list
-                                     and work prefix (92 V + 8 bytes) and
-                                     chunk table (4 (16 E / 1024 + 2) bytes);
-                                     then, per weighted graph and as far as
-                                     the rest of the budget holds them, up to
-                                     sssp_lanes single-source SSSP lanes (values,
-                                     frontier flags, bin lists and degree
-                                     prefix: about 17-18 V bytes each);
+    uint64_t state_budget_bytes;  /* HBM for per-query state (else
+                                     OG_EBUDGET): per graph the BFS engine's
+                                     masks and 4 kept-levels buffers (40 V
+                                     bytes); per weighted graph with
+                                     OG_SERVER_SSSP_BATCH or
+                                     OG_SERVER_NO_SSSP_LANES the batched SSSP
+                                     engine's 16 distance columns, light /
+                                     heavy pending masks, active list and work
+                                     prefix (92 V + 8 bytes) and chunk table
+                                     (4 (16 E / 1024 + 2) bytes); then, unless
+                                     OG_SERVER_NO_SSSP_LANES, per weighted
+                                     graph as many single-source SSSP lanes
+                                     (values, frontier flags, bin lists and
+                                     degree prefix: about 17-18 V bytes each)
+                                     as the rest holds, up to sssp_lanes, at
+                                     least one;
                                      0 = a quarter of free HBM                 */
     uint32_t max_pending;         /* queued queries before submit returns OG_EBUSY; 0 = 2^20 */
     uint32_t flags;               /* OG_SERVER_* bits                       */
@@ -453,12 +456,15 @@ typedef struct og_server_params {
                                      holds them                                */
 } og_server_params;
 
-/* SSSP queries of a weighted graph go first to an idle single-source lane
- * (og_sssp's device-driven rounds on the lane's own state); the batched
- * 16-slot engine admits queries only while every lane is busy and either it
- * is already running queries or 8 or more wait.  This flag creates no lanes,
- * so every SSSP query runs in the batched engine. */
+/* SSSP queries of a weighted graph run on its single-source lanes
+ * (og_sssp's device-driven rounds on each lane's own state), one query per
+ * lane at a time.  OG_SERVER_SSSP_BATCH adds the batched 16-slot engine: it
+ * admits queries while every lane is busy and it already runs queries or 8
+ * or more wait.  OG_SERVER_NO_SSSP_LANES runs every SSSP query in the
+ * batched engine (no lanes).  The engine's state is allocated and charged
+ * only when one of the two flags asks for it. */
 #define OG_SERVER_NO_SSSP_LANES 0x1u
+#define OG_SERVER_SSSP_BATCH 0x2u
 #define OG_PATH_BATCH 0u
 #define OG_PATH_LANE 1u
 

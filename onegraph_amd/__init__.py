@@ -91,6 +91,7 @@ QUERY_BFS, QUERY_SSSP = 0, 1
 QUERY_KEEP_VALUES = 1
 PATH_BATCH, PATH_LANE = 0, 1  # og_query_result.path: a batch engine's slot / a single-source SSSP lane
 SERVER_NO_SSSP_LANES = 0x1  # every SSSP query runs in the batched engine (og.h OG_SERVER_NO_SSSP_LANES)
+SERVER_SSSP_BATCH = 0x2  # the batched SSSP engine beside the lanes (og.h OG_SERVER_SSSP_BATCH)
 SERVER_TEST_FAIL_SSSP = 0x80000000  # testing: SSSP engines stop at their first round (og.h OG_SERVER_TEST_FAIL_SSSP)
 
 

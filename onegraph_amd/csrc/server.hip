@@ -25,19 +25,21 @@
 // Queries that arrive while a round runs join at the next round boundary, so
 // their work overlaps with the queries already in flight.
 //
-// SSSP: every weighted graph has one batch engine with kSs = 16 query slots
-// (see "batched SSSP" below): each vertex keeps the 16 slots' distances in
-// one 64-byte line, every slot runs near-far rounds with its own threshold,
-// and the slots near at a vertex share the reads of its out-edges.  Queries
-// join at round boundaries like BFS queries.  (A 64-slot distance matrix,
-// 256 bytes per vertex, was tried first and lost to the caches.)  Beside the
-// batched engine every weighted graph has single-source lanes (kSsLanes unless
-// og_server_params.sssp_lanes says otherwise)
-// (og_sssp's captured device-driven rounds on state of their own), and an
-// SSSP query goes to an idle lane first: a lone query takes a few rounds of
-// a single-source traversal there, against tens of rounds of full-width
-// sweeps in the batched engine, which admits queries only while every lane
-// is busy and it already runs queries or kSsBatchMin of them wait.
+// SSSP: every weighted graph has single-source lanes (kSsLanes unless
+// og_server_params.sssp_lanes says otherwise), each running og_sssp's
+// captured device-driven rounds on state of its own, one query at a time.
+// On request (OG_SERVER_SSSP_BATCH) a batched engine with kSs = 16 query
+// slots joins them (see "batched SSSP" below): each vertex keeps the 16
+// slots' distances in one 64-byte line, every slot runs near-far rounds with
+// its own threshold, the slots near at a vertex share the reads of its
+// out-edges, and queries join at round boundaries like BFS queries.  It
+// admits queries only while every lane is busy and it already runs queries
+// or kSsBatchMin of them wait.  Lanes are the default because they measured
+// better in the mixed workload: a lone query takes ~25 single-source rounds
+// on a lane against ~30 rounds of full-width sweeps in the batched engine,
+// and a busy batched engine takes the GPU from the BFS engines.  (A 64-slot
+// distance matrix, 256 bytes per vertex, was tried first and lost to the
+// caches.)
 #include <algorithm>
 #include <condition_variable>
 #include <deque>
@@ -1401,12 +1403,16 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
                 raise(OG_EBUSY, "graph %u is running another call", locked);
             srv->graphs.push_back(graphs[locked]);
         }
-        // Admission: every graph's BFS engine state and every weighted
-        // graph's SSSP engine state must fit the budget.
+        // Admission: every graph's BFS engine state and, when batching is
+        // asked for, every weighted graph's batched SSSP engine state must
+        // fit the budget.
+        const uint32_t fl = srv->params.flags;
+        const bool batch = (fl & (OG_SERVER_SSSP_BATCH | OG_SERVER_NO_SSSP_LANES)) != 0;
+        const bool lanes_on = !(fl & OG_SERVER_NO_SSSP_LANES);
         uint64_t need = 0;
         for (og_graph *g : srv->graphs) {
             need += bfs_state_bytes(g->V);
-            if (g->kind == OG_KIND_WEIGHTED) need += sssp_state_bytes(g->V, g->E);
+            if (g->kind == OG_KIND_WEIGHTED && batch) need += sssp_state_bytes(g->V, g->E);
         }
         uint64_t budget = srv->params.state_budget_bytes;
         if (!budget) {
@@ -1424,8 +1430,11 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             if (g->kind == OG_KIND_WEIGHTED) per_lane_set += lane_state_bytes(g);
         const uint32_t want = srv->params.sssp_lanes ? srv->params.sssp_lanes : kSsLanes;
         if (want > kSsLanesMax) raise(OG_EINVAL, "sssp_lanes %u > %u", want, kSsLanesMax);
-        uint32_t nlanes = (srv->params.flags & OG_SERVER_NO_SSSP_LANES) || !per_lane_set ? 0 : want;
+        uint32_t nlanes = lanes_on && per_lane_set ? want : 0;
         while (nlanes && need + nlanes * per_lane_set > budget) --nlanes;
+        if (lanes_on && per_lane_set && !nlanes && !batch)
+            raise(OG_EBUDGET, "state budget %llu bytes holds no SSSP lane (%llu bytes of BFS state + %llu per lane set)",
+                  (unsigned long long)budget, (unsigned long long)need, (unsigned long long)per_lane_set);
         srv->queues.resize(2 * num_graphs);
         srv->failures.resize(2 * num_graphs);
         srv->bfs.resize(num_graphs);
@@ -1466,16 +1475,18 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             srv->state_bytes += bfs_state_bytes(V);
             srv->bfs[gi] = std::move(e);
             if (g->kind != OG_KIND_WEIGHTED) continue;
-            std::unique_ptr<SsspEngine> x(new SsspEngine);
-            x->srv = srv;
-            x->g = g;
-            x->d = g->dev;
-            x->gi = gi;
-            x->pending = &srv->queues[2 * gi + 1];
-            x->failure = &srv->failures[2 * gi + 1];
-            x->setup();
-            srv->state_bytes += sssp_state_bytes(V, g->E);
-            srv->sssp[gi] = std::move(x);
+            if (batch) {
+                std::unique_ptr<SsspEngine> x(new SsspEngine);
+                x->srv = srv;
+                x->g = g;
+                x->d = g->dev;
+                x->gi = gi;
+                x->pending = &srv->queues[2 * gi + 1];
+                x->failure = &srv->failures[2 * gi + 1];
+                x->setup();
+                srv->state_bytes += sssp_state_bytes(V, g->E);
+                srv->sssp[gi] = std::move(x);
+            }
             for (uint32_t k = 0; k < nlanes; ++k) {
                 std::unique_ptr<SsspLane> l(new SsspLane);
                 l->srv = srv;
@@ -1513,7 +1524,7 @@ extern "C" og_status og_server_submit(og_server *srv, uint32_t graph, uint32_t k
     if (!srv) return fail(OG_EINVAL, "server is NULL");
     if (graph >= srv->graphs.size()) return fail(OG_EINVAL, "graph index %u out of range", graph);
     if (kind > 1) return fail(OG_EINVAL, "kind must be OG_QUERY_BFS or OG_QUERY_SSSP");
-    if (kind == OG_QUERY_SSSP && !srv->sssp[graph])
+    if (kind == OG_QUERY_SSSP && !srv->sssp[graph] && srv->lanes[graph].empty())
         return fail(OG_EUNSUPPORTED, "graph %u has no weights: SSSP queries are not served", graph);
     if (source >= srv->graphs[graph]->V) return fail(OG_EINVAL, "source %u >= V", source);
     std::lock_guard<std::mutex> lk(srv->mu);
@@ -1573,7 +1584,7 @@ extern "C" og_status og_server_get_info(og_server *srv, og_server_info *info) {
     info->num_graphs = (uint32_t)srv->graphs.size();
     for (uint32_t gi = 0; gi < srv->graphs.size(); ++gi) {
         info->bfs_slots[gi] = kSlots;
-        info->sssp_slots[gi] = srv->sssp[gi] ? kSs + (uint32_t)srv->lanes[gi].size() : 0;
+        info->sssp_slots[gi] = (srv->sssp[gi] ? kSs : 0) + (uint32_t)srv->lanes[gi].size();
         info->rounds += srv->bfs[gi]->rounds_run.load(std::memory_order_relaxed);
         if (srv->sssp[gi]) info->rounds += srv->sssp[gi]->rounds_run.load(std::memory_order_relaxed);
         for (auto &l : srv->lanes[gi]) info->rounds += l->rounds_run.load(std::memory_order_relaxed);

@@ -1,7 +1,8 @@
 """Continuous-batching server: each query's result is the same alone and in
-the mix (BFS slots batched 64 to an engine, SSSP slots 16 to an engine with
-their own near-far thresholds), equals the single-query path and the oracle;
-admission, reservation and failure rules hold."""
+the mix (BFS slots batched 64 to an engine; SSSP queries on single-source
+lanes, or on request batched 16 to an engine with their own near-far
+thresholds), equals the single-query path and the oracle; admission,
+reservation and failure rules hold."""
 import sys
 import time
 
@@ -97,7 +98,8 @@ def test_alone_and_in_the_mix(gpu, graphs):
     _say("server mix of", len(qs))
     # the exact budget: queries join the BFS and SSSP batches mid-flight;
     # kept BFS levels use 4 buffers per graph, so those queries queue
-    with og.Server(graphs, state_budget_bytes=_state_bytes(graphs)) as srv:
+    # the batched engine alone (no lanes), at the exact budget
+    with og.Server(graphs, state_budget_bytes=_state_bytes(graphs), flags=og.SERVER_NO_SSSP_LANES) as srv:
         info = srv.info()
         assert info["bfs_slots"] == [64, 64, 64] and info["sssp_slots"] == [16, 16, 0]
         ids = [srv.submit(gi, k, s, keep=True) for gi, k, s in qs]
@@ -140,9 +142,12 @@ def test_admission_and_reservation(gpu, graphs):
         og.Server([g], state_budget_bytes=16)
     assert e.value.name == "OG_EBUDGET"
     with pytest.raises(og.OgError) as e:  # one byte short
-        og.Server([g], state_budget_bytes=_state_bytes([g]) - 1)
+        og.Server([g], state_budget_bytes=_state_bytes([g]) - 1, flags=og.SERVER_NO_SSSP_LANES)
     assert e.value.name == "OG_EBUDGET"
-    srv = og.Server([g], state_budget_bytes=_state_bytes([g]), max_pending=4)
+    with pytest.raises(og.OgError) as e:  # BFS state only: no SSSP lane fits
+        og.Server([g], state_budget_bytes=40 * g.num_vertices)
+    assert e.value.name == "OG_EBUDGET"
+    srv = og.Server([g], state_budget_bytes=_state_bytes([g]), max_pending=4, flags=og.SERVER_NO_SSSP_LANES)
     assert srv.info()["bfs_slots"] == [64] and srv.info()["sssp_slots"] == [16]
     assert srv.info()["state_bytes"] == _state_bytes([g])
     with pytest.raises(og.OgError) as e:  # reserved while served
@@ -203,7 +208,7 @@ def test_sssp_batches_alone_and_in_the_mix(gpu, graphs):
     cand = np.flatnonzero(deg > 0)
     srcs = [int(x) for x in np.random.default_rng(19).choice(cand, 36, replace=False)] + [int(cand[0])] * 3
     _say("sssp burst of", len(srcs), "on graph 0")
-    with og.Server([g, graphs[1]]) as srv:
+    with og.Server([g, graphs[1]], flags=og.SERVER_SSSP_BATCH) as srv:  # lanes, then batches
         ids = [srv.submit(0, og.QUERY_SSSP, s, keep=True) for s in srcs]
         ids += [srv.submit(1, og.QUERY_SSSP, 5, keep=True)]  # the other engine runs beside it
         res = _wait_all(srv, ids)
@@ -266,17 +271,18 @@ def test_failed_engine_refuses_its_queue(gpu, graphs):
     as after a device error, without touching the GPU) fails the queries it
     held and every later submit to it; the BFS engines keep serving."""
     g = graphs[0]
-    with og.Server([g], flags=og.SERVER_TEST_FAIL_SSSP) as srv:
-        i = srv.submit(0, og.QUERY_SSSP, 0)
-        r = _wait_all(srv, [i])[i]
-        assert og.STATUS[r["status"]] == "OG_EHIP"
-        with pytest.raises(og.OgError) as e:
-            srv.submit(0, og.QUERY_SSSP, 1)
-        assert e.value.name == "OG_EHIP" and "stopped" in str(e.value)
-        j = srv.submit(0, og.QUERY_BFS, 0)
-        rb = _wait_all(srv, [j])[j]
-        assert rb["status"] == 0
-        assert srv.info()["in_flight"] == 0 and srv.info()["pending"] == 0
+    for extra in (0, og.SERVER_NO_SSSP_LANES):  # a lane fails; the batched engine fails
+        with og.Server([g], flags=og.SERVER_TEST_FAIL_SSSP | extra) as srv:
+            i = srv.submit(0, og.QUERY_SSSP, 0)
+            r = _wait_all(srv, [i])[i]
+            assert og.STATUS[r["status"]] == "OG_EHIP"
+            with pytest.raises(og.OgError) as e:
+                srv.submit(0, og.QUERY_SSSP, 1)
+            assert e.value.name == "OG_EHIP" and "stopped" in str(e.value)
+            j = srv.submit(0, og.QUERY_BFS, 0)
+            rb = _wait_all(srv, [j])[j]
+            assert rb["status"] == 0
+            assert srv.info()["in_flight"] == 0 and srv.info()["pending"] == 0
     _, st = g.bfs(0, want_values=False)
     assert st["checksum"] == rb["checksum"]
 
@@ -300,60 +306,60 @@ def test_sssp_batch_many_sweep_blocks(gpu, split):
 
 
 def test_sssp_lanes_first(gpu, graphs):
-    """SSSP queries go to an idle single-source lane first: a lone query runs
-    as og_sssp does (same rounds); a burst fills both lanes and then the
-    batched engine (8 or more waiting); fewer waiting queries wait for a lane.
-    The budget decides how many lanes a weighted graph gets."""
+    """SSSP queries run on single-source lanes by default (a lone query and a
+    burst alike); with og.SERVER_SSSP_BATCH a burst fills both lanes and then
+    the batched engine (8 or more waiting), while fewer waiting queries wait
+    for a lane.  The budget decides how many lanes a weighted graph gets."""
     g = graphs[0]
     ptr, to, w, _ = g.export()
     deg = np.diff(np.append(ptr.astype(np.int64), len(to)))
     cand = np.flatnonzero(deg > 0)
     srcs = [int(x) for x in np.random.default_rng(29).choice(cand, 40, replace=False)]
     _, st0 = g.sssp(srcs[0], want_values=False)
-    with og.Server([g]) as srv:
-        assert srv.info()["sssp_slots"] == [16 + 2]
+    bfs_bytes = 40 * g.num_vertices
+
+    def check(srv, burst, paths_want):
+        ids = [srv.submit(0, og.QUERY_SSSP, s, keep=True) for s in burst]
+        res = _wait_all(srv, ids)
+        paths = [res[i]["path"] for i in ids]
+        assert paths_want(paths), paths
+        for n, s in enumerate(burst):
+            ref = oracle.sssp(ptr, to, w, s)
+            assert res[ids[n]]["status"] == 0
+            assert np.array_equal(srv.take_values(ids[n], 0), ref), s
+            assert res[ids[n]]["checksum"] == og.levels_checksum(ref)
+            assert res[ids[n]]["traversed_edges"] == int(deg[ref != og.UNREACHED].sum())
+
+    with og.Server([g]) as srv:  # default: two lanes, no batched engine
+        assert srv.info()["sssp_slots"] == [2]
+        lanes2 = srv.info()["state_bytes"] - bfs_bytes
         i = srv.submit(0, og.QUERY_SSSP, srcs[0], keep=True)
         r = _wait_all(srv, [i])[i]
         assert r["status"] == 0 and r["path"] == og.PATH_LANE
         assert np.array_equal(srv.take_values(i, 0), oracle.sssp(ptr, to, w, srcs[0]))
         assert r["checksum"] == st0["checksum"]
-        for burst in (srcs[1:5], srcs[5:40]):  # fewer than 8 past the lanes, then a full batch
-            _say("lanes: burst of", len(burst))
-            ids = [srv.submit(0, og.QUERY_SSSP, s, keep=True) for s in burst]
-            res = _wait_all(srv, ids)
-            paths = [res[i]["path"] for i in ids]
-            if len(burst) < 2 + 8:  # never 8 waiting: lanes only
-                assert set(paths) == {og.PATH_LANE}, paths
-            else:  # 33 waiting behind the lanes: batches
-                assert og.PATH_BATCH in paths and paths.count(og.PATH_LANE) >= 2, paths
-            for n, s in enumerate(burst):
-                ref = oracle.sssp(ptr, to, w, s)
-                assert res[ids[n]]["status"] == 0
-                assert np.array_equal(srv.take_values(ids[n], 0), ref), s
-                assert res[ids[n]]["checksum"] == og.levels_checksum(ref)
-                assert res[ids[n]]["traversed_edges"] == int(deg[ref != og.UNREACHED].sum())
+        _say("lanes only: burst of 35")
+        check(srv, srcs[5:40], lambda p: set(p) == {og.PATH_LANE})
         info = srv.info()
-        assert info["in_flight"] == 0 and info["pending"] == 0 and info["completed"] == 40
-        lanes2 = info["state_bytes"] - _state_bytes([g])
-    # the exact engine state plus one lane's bytes: one lane
-    with og.Server([g], state_budget_bytes=_state_bytes([g]) + lanes2 // 2) as srv:
-        assert srv.info()["sssp_slots"] == [16 + 1]
-        assert srv.info()["state_bytes"] == _state_bytes([g]) + lanes2 // 2
-        ids = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs[:12]]
-        res = _wait_all(srv, ids)
-        for n, s in enumerate(srcs[:12]):
-            assert res[ids[n]]["checksum"] == og.levels_checksum(oracle.sssp(ptr, to, w, s))
+        assert info["in_flight"] == 0 and info["pending"] == 0 and info["completed"] == 36
+    with og.Server([g], flags=og.SERVER_SSSP_BATCH) as srv:  # lanes, then the batched engine
+        assert srv.info()["sssp_slots"] == [16 + 2]
+        assert srv.info()["state_bytes"] == _state_bytes([g]) + lanes2
+        _say("lanes + batch: bursts of 4 and 35")
+        check(srv, srcs[1:5], lambda p: set(p) == {og.PATH_LANE})  # never 8 waiting
+        check(srv, srcs[5:40], lambda p: og.PATH_BATCH in p and p.count(og.PATH_LANE) >= 2)
+    # the BFS state plus one lane's bytes: one lane
+    with og.Server([g], state_budget_bytes=bfs_bytes + lanes2 // 2) as srv:
+        assert srv.info()["sssp_slots"] == [1]
+        assert srv.info()["state_bytes"] == bfs_bytes + lanes2 // 2
+        check(srv, srcs[:12], lambda p: set(p) == {og.PATH_LANE})
     with og.Server([g], flags=og.SERVER_NO_SSSP_LANES) as srv:
         assert srv.info()["sssp_slots"] == [16]
         assert srv.info()["state_bytes"] == _state_bytes([g])
-    with og.Server([g], sssp_lanes=4) as srv:  # more lanes: a burst of 12 never reaches the batch
-        assert srv.info()["sssp_slots"] == [16 + 4]
-        assert srv.info()["state_bytes"] == _state_bytes([g]) + 2 * lanes2
-        ids = [srv.submit(0, og.QUERY_SSSP, s) for s in srcs[:11]]
-        res = _wait_all(srv, ids)
-        assert {res[i]["path"] for i in ids} == {og.PATH_LANE}
-        for n, s in enumerate(srcs[:11]):
-            assert res[ids[n]]["checksum"] == og.levels_checksum(oracle.sssp(ptr, to, w, s))
+    with og.Server([g], sssp_lanes=4) as srv:
+        assert srv.info()["sssp_slots"] == [4]
+        assert srv.info()["state_bytes"] == bfs_bytes + 2 * lanes2
+        check(srv, srcs[:11], lambda p: set(p) == {og.PATH_LANE})
     with pytest.raises(og.OgError) as e:
         og.Server([g], sssp_lanes=9)
     assert e.value.name == "OG_EINVAL"

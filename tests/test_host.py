@@ -327,6 +327,6 @@ def test_plan_flags_match_header():
         assert getattr(og, "PLAN_" + name) == int(value), name
     assert len(set(defs.values())) == len(defs)  # distinct bits
     sdefs = dict(re.findall(r"#define OG_SERVER_([A-Z_]+) (0x[0-9a-f]+)u", hdr))
-    assert set(sdefs) == {"NO_SSSP_LANES", "TEST_FAIL_SSSP"}
+    assert set(sdefs) == {"NO_SSSP_LANES", "SSSP_BATCH", "TEST_FAIL_SSSP"}
     for name, value in sdefs.items():
         assert getattr(og, "SERVER_" + name) == int(value, 16), name
