@@ -511,7 +511,7 @@ def wl_serve(args):
         plain_qps, plat = replay(t_arr, gis, kinds, service, seed + 4)
         td_qps, _ = replay(t_arr, gis, kinds, service_td, seed + 4)
         srv = og.Server(graphs, state_budget_bytes=budget, sssp_delta=args.serve_sssp_delta, flags=srv_flags,
-                        sssp_lanes=args.sssp_lanes)
+                        sssp_lanes=args.sssp_lanes, sssp_running_max=args.sssp_running_max)
         info0 = srv.info()
         lat, done, submit_at = [], {}, {}
         t0 = time.perf_counter()
@@ -599,6 +599,7 @@ def wl_serve(args):
                        "served_sssp_delta": args.serve_sssp_delta or args.delta,
                        "sssp_lanes": 0 if args.no_sssp_lanes else args.sssp_lanes,
                        "sssp_batch": bool(args.sssp_batch or args.no_sssp_lanes),
+                       "sssp_running_max": args.sssp_running_max,
                        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                        "split_weight": args.split_weight})
     out.update({"serve": per_mix,
@@ -722,6 +723,8 @@ def main():
     ap.add_argument("--serve-scales", default="22,24,26", help="serve: R-MAT scales of the graphs")
     ap.add_argument("--serve-budget-gib", type=float, default=0.0, help="serve: per-query state budget")
     ap.add_argument("--sssp-lanes", type=int, default=2, help="serve: single-source SSSP lanes per weighted graph")
+    ap.add_argument("--sssp-running-max", type=int, default=0,
+                    help="serve: SSSP lanes running at once over all graphs (0: no cap)")
     ap.add_argument("--sssp-batch", action="store_true",
                     help="serve: the batched SSSP engine beside the lanes (admits when the lanes are busy and 8 wait)")
     ap.add_argument("--no-sssp-lanes", action="store_true",

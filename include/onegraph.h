@@ -454,6 +454,10 @@ typedef struct og_server_params {
     uint32_t sssp_lanes;          /* single-source SSSP lanes per weighted
                                      graph, 1..8 (0: 2), as far as the budget
                                      holds them                                */
+    uint32_t sssp_running_max;    /* SSSP lanes running queries at once over
+                                     all graphs (0: no cap): fewer concurrent
+                                     SSSP traversals leave more of the GPU to
+                                     the BFS engines under load                */
 } og_server_params;
 
 /* SSSP queries of a weighted graph run on its single-source lanes

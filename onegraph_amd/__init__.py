@@ -97,7 +97,8 @@ SERVER_TEST_FAIL_SSSP = 0x80000000  # testing: SSSP engines stop at their first 
 
 class ServerParams(C.Structure):
     _fields_ = [("state_budget_bytes", C.c_uint64), ("max_pending", C.c_uint32), ("flags", C.c_uint32),
-                ("deadline_ms", C.c_uint32), ("sssp_delta", C.c_uint32), ("sssp_lanes", C.c_uint32)]
+                ("deadline_ms", C.c_uint32), ("sssp_delta", C.c_uint32), ("sssp_lanes", C.c_uint32),
+                ("sssp_running_max", C.c_uint32)]
 
 
 class QueryResult(C.Structure):
@@ -452,10 +453,10 @@ class Server:
     """Continuous-batching query server over device-resident graphs."""
 
     def __init__(self, graphs, state_budget_bytes=0, max_pending=0, flags=0, sssp_delta=0, deadline_ms=0,
-                 sssp_lanes=0):
+                 sssp_lanes=0, sssp_running_max=0):
         self._graphs = list(graphs)
         arr = (_P * len(self._graphs))(*[g._h.value for g in self._graphs])
-        p = ServerParams(state_budget_bytes, max_pending, flags, deadline_ms, sssp_delta, sssp_lanes)
+        p = ServerParams(state_budget_bytes, max_pending, flags, deadline_ms, sssp_delta, sssp_lanes, sssp_running_max)
         h = _P()
         _check(lib().og_server_create(arr, len(self._graphs), C.byref(p), C.byref(h)))
         self._h = h

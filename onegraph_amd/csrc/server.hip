@@ -850,6 +850,7 @@ struct SsspLane {
     std::thread th;
     ~SsspLane() { device_state_free(lane); }
     void loop();
+    void fail_lane(const Query &q, og_status st, const std::string &msg);
 };
 
 }  // namespace og
@@ -869,6 +870,7 @@ struct og_server {
     std::unordered_map<uint64_t, std::vector<uint32_t>> kept;
     og_server_params params{};
     uint64_t submitted = 0, completed = 0, rejected = 0, pending = 0, in_flight = 0;
+    uint32_t lanes_running = 0;  // SSSP lanes running a query (og_server_params.sssp_running_max)
     uint64_t state_bytes = 0;
     og::Stopwatch clock;
 };
@@ -1290,7 +1292,10 @@ void SsspLane::loop() {
             double t_admit = 0;
             {
                 std::unique_lock<std::mutex> lk(srv->mu);
-                srv->cv_work.wait(lk, [&] { return srv->stop || !pending->empty(); });
+                const uint32_t cap = srv->params.sssp_running_max;
+                srv->cv_work.wait(lk, [&] {
+                    return srv->stop || (!pending->empty() && (!cap || srv->lanes_running < cap));
+                });
                 if (srv->stop) return;
                 t_admit = now_ms(srv);
                 expire_queued(srv, gi, pending, t_admit);
@@ -1299,6 +1304,7 @@ void SsspLane::loop() {
                 pending->pop_front();
                 srv->pending--;
                 srv->in_flight++;
+                srv->lanes_running++;
                 busy = true;
             }
             srv->cv_work.notify_all();  // with every lane busy the batched engine may admit the rest
@@ -1328,27 +1334,30 @@ void SsspLane::loop() {
             if (!vals.empty()) srv->kept[q.id] = std::move(vals);
             srv->completed++;
             srv->in_flight--;
+            srv->lanes_running--;
             busy = false;
             srv->cv_done.notify_all();
+            srv->cv_work.notify_all();  // a lane of another graph may wait for the running cap
         }
     } catch (const Error &e) {
-        // the query in hand and the graph's SSSP queue fail, as when the
-        // batched engine stops; the lane stays busy, so it takes no more
-        std::lock_guard<std::mutex> lk(srv->mu);
-        SlotState held[1];
-        held[0].used = busy;
-        held[0].q = q;
-        fail_engine(srv, gi, pending, failure, held, 1, e.st, e.msg);
-        busy = true;
+        fail_lane(q, e.st, e.msg);
     } catch (...) {
-        std::lock_guard<std::mutex> lk(srv->mu);
-        SlotState held[1];
-        held[0].used = busy;
-        held[0].q = q;
-        fail_engine(srv, gi, pending, failure, held, 1, OG_ENOMEM_HOST, "SSSP lane stopped: host allocation failed");
-        busy = true;
+        fail_lane(q, OG_ENOMEM_HOST, "SSSP lane stopped: host allocation failed");
     }
     srv->cv_work.notify_all();
+}
+
+// The query in hand and the graph's SSSP queue fail, as when the batched
+// engine stops; the lane stays busy, so it takes no more, and gives back its
+// place under the running cap.
+void SsspLane::fail_lane(const Query &q, og_status st, const std::string &msg) {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    SlotState held[1];
+    held[0].used = busy;
+    held[0].q = q;
+    if (busy) srv->lanes_running--;
+    fail_engine(srv, gi, pending, failure, held, 1, st, msg);
+    busy = true;
 }
 
 static void server_free(og_server *srv) {

@@ -365,6 +365,31 @@ def test_sssp_lanes_first(gpu, graphs):
     assert e.value.name == "OG_EINVAL"
 
 
+def test_sssp_running_cap(gpu, graphs):
+    """With at most one SSSP lane running at once over both weighted graphs,
+    bursts on both still all complete with the oracle's distances, and BFS
+    queries beside them are served."""
+    rng = np.random.default_rng(31)
+    qs = []
+    for gi in (0, 1):
+        ptr, to, w, _ = graphs[gi].export()
+        cand = np.flatnonzero(np.diff(np.append(ptr.astype(np.int64), len(to))) > 0)
+        qs += [(gi, int(s)) for s in rng.choice(cand, 10, replace=False)]
+    with og.Server([graphs[0], graphs[1]], sssp_running_max=1) as srv:
+        assert srv.info()["sssp_slots"] == [2, 2]
+        ids = [srv.submit(gi, og.QUERY_SSSP, s) for gi, s in qs]
+        bfs = [srv.submit(gi, og.QUERY_BFS, s) for gi, s in qs[::4]]
+        res = _wait_all(srv, ids + bfs)
+        assert srv.info()["in_flight"] == 0 and srv.info()["pending"] == 0
+    for n, (gi, s) in enumerate(qs):
+        ptr, to, w, _ = graphs[gi].export()
+        assert res[ids[n]]["checksum"] == og.levels_checksum(oracle.sssp(ptr, to, w, s)), (gi, s)
+        assert res[ids[n]]["path"] == og.PATH_LANE
+    for n, (gi, s) in enumerate(qs[::4]):
+        _, st = graphs[gi].bfs(s, want_values=False)
+        assert res[bfs[n]]["checksum"] == st["checksum"]
+
+
 def test_deadline_expires_queued_queries(gpu):
     """With a 1 ms admission deadline, a burst of 3000 BFS queries (64 slots)
     leaves most of them queued past it: they complete with OG_ETIMEDOUT, the
