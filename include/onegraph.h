@@ -463,8 +463,8 @@ typedef struct og_server_params {
 /* SSSP queries of a weighted graph run on its single-source lanes
  * (og_sssp's device-driven rounds on each lane's own state), one query per
  * lane at a time.  OG_SERVER_SSSP_BATCH adds the batched 16-slot engine: it
- * admits queries while every lane is busy and it already runs queries or 8
- * or more wait.  OG_SERVER_NO_SSSP_LANES runs every SSSP query in the
+ * admits queries while no lane can take them (all busy, or sssp_running_max
+ * reached) and it already runs queries or 8 or more wait.  OG_SERVER_NO_SSSP_LANES runs every SSSP query in the
  * batched engine (no lanes).  The engine's state is allocated and charged
  * only when one of the two flags asks for it. */
 #define OG_SERVER_NO_SSSP_LANES 0x1u

@@ -1249,15 +1249,17 @@ void SsspEngine::step() {
     }
 }
 
-// Lanes first: with lanes, queued queries are admitted only while every
-// lane is busy and the engine already runs queries or kSsBatchMin wait (a
-// lone query is answered sooner by a single-source lane).  Caller holds
-// srv->mu.
+// Lanes first: with lanes, queued queries are admitted only while no lane
+// can take them (every lane busy, or the running cap reached) and the engine
+// already runs queries or kSsBatchMin wait (a lone query is answered sooner
+// by a single-source lane).  Caller holds srv->mu.
 bool SsspEngine::may_admit() const {
     if (pending->empty()) return false;
     const auto &ls = srv->lanes[gi];
-    for (const auto &l : ls)
-        if (!l->busy) return false;
+    const uint32_t cap = srv->params.sssp_running_max;
+    if (!(cap && srv->lanes_running >= cap))
+        for (const auto &l : ls)
+            if (!l->busy) return false;
     return ls.empty() || used != 0 || pending->size() >= kSsBatchMin;
 }
 
@@ -1433,7 +1435,7 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             raise(OG_EBUDGET, "state budget %llu bytes cannot hold the engines' query state (%llu bytes)",
                   (unsigned long long)budget, (unsigned long long)need);
         // single-source SSSP lanes per weighted graph, as many (up to
-        // kSsLanes) as the rest of the budget holds for every weighted graph
+        // sssp_lanes) as the rest of the budget holds for every weighted graph
         uint64_t per_lane_set = 0;
         for (og_graph *g : srv->graphs)
             if (g->kind == OG_KIND_WEIGHTED) per_lane_set += lane_state_bytes(g);
