@@ -2197,7 +2197,7 @@ constexpr uint64_t edge_bytes_alg() {
 }
 
 constexpr int kRoundBatch = 4;     // device-driven rounds enqueued per host check
-constexpr int kRoundBatchMax = 64;  // largest first batch (sized from the previous run; a multiple of kRoundBatch)
+constexpr int kRoundBatchMax = 64;  // largest first batch (sized from the previous run)
 
 // Iterates compaction -> static relax || dynamic relax -> after() until no
 // vertex is active or held back.  Accumulates into st / tm.
@@ -2281,8 +2281,11 @@ static void frontier_loop(og_graph *g, DeviceState *d, og_run_stats &st, RunTime
             OG_HIP(hipMemcpyAsync(d->fr.h_ctl, d->fr.ctl, sizeof(RoundCtl), hipMemcpyDeviceToHost, s));
         };
         if (!(d->flags & OG_PLAN_NO_GRAPH)) {
+            // first batch: exactly the pairs the previous run took (a 6-round
+            // BFS replayed 8 rounds when this was rounded up to a multiple of
+            // kRoundBatch, two empty relax + compaction pairs per run)
             const uint32_t guess = std::min<uint32_t>(d->last_pairs[ALGO], kRoundBatchMax);
-            int rounds = std::max<int>(kRoundBatch, (int)(guess + kRoundBatch - 1) / kRoundBatch * kRoundBatch);
+            int rounds = guess ? (int)guess : kRoundBatch;
             for (;;) {
                 const uint64_t key = (uint64_t)ALGO | (uint64_t)W << 4 | (uint64_t)(delta != 0) << 5 |
                                      (uint64_t)has_dyn << 6 | (uint64_t)dir_opt << 7 | (uint64_t)split << 16 |
