@@ -78,13 +78,18 @@ __device__ __forceinline__ uint64_t wave_or(uint64_t x) {
 }
 
 // Seeds: the slot's bit goes into next[src]; the commit admits it.  The
-// slot's explored-edge count restarts.
-__global__ void k_srv_seed(SeedList sl, uint64_t *next, uint8_t *flag, unsigned long long *explored) {
+// slot's explored-edge count and its sums (digest, reached, traversed in
+// acc[j], acc[64 + j], acc[128 + j]) restart: a finished slot keeps its last
+// sums until it takes another query, and nothing adds to an unused slot's.
+__global__ void k_srv_seed(SeedList sl, uint64_t *next, uint8_t *flag, unsigned long long *explored,
+                           unsigned long long *acc) {
     const uint32_t i = threadIdx.x;
     if (i < sl.n) {
-        atomicOr((unsigned long long *)(next + sl.src[i]), 1ull << sl.slot[i]);
+        const uint32_t j = sl.slot[i];
+        atomicOr((unsigned long long *)(next + sl.src[i]), 1ull << j);
         flag[sl.src[i]] = 1;
-        explored[sl.slot[i]] = 0;
+        explored[j] = 0;
+        acc[j] = acc[64 + j] = acc[128 + j] = 0;
     }
 }
 
@@ -191,10 +196,14 @@ __global__ __launch_bounds__(kBlock) void k_srv_commit(uint8_t *__restrict__ fla
 // bottom-up pass sweeps V once for every pulling slot of the round, so in a
 // batch it pays off earlier (R-MAT 26, 333 queries/s: rounds 3.7 -> 3.3 ms,
 // p99 59 -> 45 ms; 56 gave the same as 28).  Bit j of *pull = slot j relaxes
-// bottom-up.  Resets the stats.
+// bottom-up.  Resets the stats, and moves the commit's alive mask from
+// sum[0] (which the next commit ORs into) to sum[2], next to the pull mask
+// sum[1] and the slot sums acc = sum + 3, so one copy brings the round's
+// summary to the host and no memset clears sum[0] before the next commit.
 constexpr uint64_t kSrvAlpha = 28;
 __global__ void k_srv_direction(uint64_t used, uint64_t V, uint64_t E, int dir_opt, unsigned long long *fstat,
-                                unsigned long long *explored, unsigned long long *acc, unsigned long long *pull) {
+                                unsigned long long *explored, unsigned long long *sum) {
+    unsigned long long *const pull = sum + 1, *const acc = sum + 3;
     const uint32_t j = threadIdx.x;
     const uint64_t fc = fstat[j], fe = fstat[64 + j];
     acc[64 + j] += fc;
@@ -207,7 +216,11 @@ __global__ void k_srv_direction(uint64_t used, uint64_t V, uint64_t E, int dir_o
     explored[j] = ex + fe;
     fstat[j] = fstat[64 + j] = 0;
     const uint64_t b = __ballot(now);
-    if (j == 0) *pull = b;
+    if (j == 0) {
+        *pull = b;
+        sum[2] = sum[0];
+        sum[0] = 0;
+    }
 }
 
 // After the relax: the masks of the round's listed vertices (bins 0/1 and
@@ -229,10 +242,6 @@ __global__ __launch_bounds__(kBlock) void k_srv_clear_visit(uint64_t V, uint64_t
     }
 }
 
-__global__ void k_srv_zero_acc(uint64_t done, unsigned long long *acc) {
-    const uint32_t j = threadIdx.x;
-    if (j < 64 && ((done >> j) & 1)) acc[j] = acc[64 + j] = acc[128 + j] = 0;
-}
 
 // ---------------------------------------------------- batched SSSP (server)
 //
@@ -759,7 +768,11 @@ struct BfsEngine {
     hipStream_t s = nullptr;
     hipEvent_t ev_summary = nullptr;  // the round's alive mask and slot sums are on the host
     uint64_t *visit = nullptr, *front = nullptr, *next = nullptr;
-    unsigned long long *d_alive = nullptr, *h_alive = nullptr, *d_acc = nullptr, *h_acc = nullptr;
+    // round summary: [0] alive slots (the commit ORs into it), [1] pulling
+    // slots, [2] the round's alive slots, [3, 195) slot sums (acc); the host
+    // copy h_sum receives [1, 195) each round
+    unsigned long long *d_sum = nullptr, *h_sum = nullptr;
+    unsigned long long *d_alive = nullptr, *d_acc = nullptr, *h_acc = nullptr;
     // direction-optimising rounds (graphs prepared with their in-edges)
     unsigned long long *d_fstat = nullptr;     // [2][64] frontier vertices / edges per slot
     unsigned long long *d_explored = nullptr;  // [64] edges of the slot's past frontiers
@@ -948,8 +961,6 @@ void BfsEngine::finalize(uint64_t done) {
             OG_HIP(hipMemcpyAsync(vals.back().second.data(), kl.p[j], V * 4, hipMemcpyDeviceToHost, s));
         }
     }
-    hipLaunchKernelGGL(k_srv_zero_acc, dim3(1), dim3(64), 0, s, done, d_acc);
-    OG_HIP(hipGetLastError());
     if (!vals.empty()) OG_HIP(hipStreamSynchronize(s));
     dirty |= done;
     const double t = now_ms(srv);
@@ -1026,11 +1037,10 @@ void BfsEngine::step() {
         if (kb >= 0) OG_HIP(hipMemsetAsync(keep_buf[kb], 0xFF, V * 4, s));
     }
     if (sl.n) {
-        hipLaunchKernelGGL(k_srv_seed, dim3(1), dim3(64), 0, s, sl, next, f.flag, d_explored);
+        hipLaunchKernelGGL(k_srv_seed, dim3(1), dim3(64), 0, s, sl, next, f.flag, d_explored, d_acc);
         OG_HIP(hipGetLastError());
     }
     const bool dir_opt = d->d_in_ptr != nullptr;
-    OG_HIP(hipMemsetAsync(d_alive, 0, 8, s));
     SlotLevels L{};
     for (uint32_t j = 0; j < kSlots; ++j) L.lv[j] = round - slots[j].start_round + 1;
     hipLaunchKernelGGL(k_srv_commit, dim3(sweep_grid(V, 4096)), dim3(kBlock), 0, s, f.flag, V,
@@ -1038,10 +1048,10 @@ void BfsEngine::step() {
                        d_alive);
     OG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_srv_direction, dim3(1), dim3(64), 0, s, used, V, d->E, dir_opt ? 1 : 0, d_fstat,
-                       d_explored, d_acc, d_pull);
-    OG_HIP(hipMemcpyAsync(h_alive, d_alive, 16, hipMemcpyDeviceToHost, s));
-    // results of the slots that finish this round are complete after the commit
-    OG_HIP(hipMemcpyAsync(h_acc, d_acc, 3 * 64 * 8, hipMemcpyDeviceToHost, s));
+                       d_explored, d_sum);
+    // pull mask, alive mask and the slot sums (results of the slots that
+    // finish this round are complete after the commit) in one copy
+    OG_HIP(hipMemcpyAsync(h_sum + 1, d_sum + 1, (2 + 3 * 64) * 8, hipMemcpyDeviceToHost, s));
     OG_HIP(hipEventRecord(ev_summary, s));
     // Counts stay on the device: the compaction, relax and clear kernels read
     // them from f.info, so the round needs no host round trip before its end.
@@ -1058,9 +1068,9 @@ void BfsEngine::step() {
     OG_HIP(hipEventSynchronize(ev_summary));
     ++round;
     rounds_run.fetch_add(1, std::memory_order_relaxed);
-    if (h_alive[1]) pull_rounds.fetch_add(1, std::memory_order_relaxed);
+    if (h_sum[1]) pull_rounds.fetch_add(1, std::memory_order_relaxed);
     // a slot is done when the round's commit left it no frontier vertex
-    const uint64_t doneset = used & ~(uint64_t)*h_alive;
+    const uint64_t doneset = used & ~(uint64_t)h_sum[2];
     if (doneset) finalize(doneset);
 }
 
@@ -1472,17 +1482,17 @@ extern "C" og_status og_server_create(og_graph *const *graphs, uint32_t num_grap
             OG_HIP(hipMemset(e->visit, 0, V * 8));
             OG_HIP(hipMemset(e->front, 0, V * 8));
             OG_HIP(hipMemset(e->next, 0, V * 8));
-            e->d_alive = (unsigned long long *)e->mem.dalloc(16);  // [0] alive slots, [1] pulling slots
-            e->h_alive = (unsigned long long *)e->mem.palloc(16);
-            e->d_pull = e->d_alive + 1;
+            e->d_sum = (unsigned long long *)e->mem.dalloc((3 + 3 * 64) * 8);
+            e->h_sum = (unsigned long long *)e->mem.palloc((3 + 3 * 64) * 8);
+            OG_HIP(hipMemset(e->d_sum, 0, (3 + 3 * 64) * 8));
+            e->d_alive = e->d_sum;
+            e->d_pull = e->d_sum + 1;
+            e->d_acc = e->d_sum + 3;
+            e->h_acc = e->h_sum + 3;
             e->d_fstat = (unsigned long long *)e->mem.dalloc(2 * 64 * 8);
             e->d_explored = (unsigned long long *)e->mem.dalloc(64 * 8);
-            OG_HIP(hipMemset(e->d_alive, 0, 16));
             OG_HIP(hipMemset(e->d_fstat, 0, 2 * 64 * 8));
             OG_HIP(hipMemset(e->d_explored, 0, 64 * 8));
-            e->d_acc = (unsigned long long *)e->mem.dalloc(3 * 64 * 8);
-            e->h_acc = (unsigned long long *)e->mem.palloc(3 * 64 * 8);
-            OG_HIP(hipMemset(e->d_acc, 0, 3 * 64 * 8));
             srv->state_bytes += bfs_state_bytes(V);
             srv->bfs[gi] = std::move(e);
             if (g->kind != OG_KIND_WEIGHTED) continue;
