@@ -754,7 +754,7 @@ def main():
         hwq = args.hw_queues or 3 * ng + ng * (2 if batch else 1) + ng * (0 if args.no_sssp_lanes else args.sssp_lanes)
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, hwq))
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (this build has no multi-GPU path); --gpus {args.gpus} refused",
+        print(f"bench.py measures one GPU only (the engine runs on one GPU; sharding is not implemented); --gpus {args.gpus} refused",
               file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration_mib:
