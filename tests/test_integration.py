@@ -102,3 +102,17 @@ def test_c_caller_sssp_matches_oracle(gpu, callers, wgraph, tmp_path):
     r = subprocess.run([c, "sssp", f, str(src), out], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert np.array_equal(np.fromfile(out, np.uint32), oracle.sssp(ptr, to, w, src))
+
+
+@pytest.mark.gpu
+def test_c_caller_serves_like_the_library(gpu, callers, wgraph):
+    """The server through the C ABI from a C program: its SSSP (on a lane)
+    and BFS answers carry the checksums of og_sssp / og_bfs from the same
+    source."""
+    c, _ = callers
+    f, g = wgraph
+    ptr, to, _, _ = g.export()
+    src = int(np.flatnonzero(np.diff(np.append(ptr.astype(np.int64), len(to))) > 0)[3])
+    r = subprocess.run([c, "serve", f, str(src), "-"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("ok sssp_slots 2"), r.stdout
